@@ -1,0 +1,134 @@
+#!/usr/bin/env python3
+"""Benchmark: plain 3D SPH dam break, 1,048,576 particles, on one MI355X.
+
+Prints one JSON line (particle-steps/s over the timed steps, device-resident
+state, synthetic seeded input), with the HBM roofline of the dominant kernel
+from HIP-event kernel timings and the single-core CPU oracle timed on a
+bounded sample of the same scene.
+
+    python bench.py [--gpus 1] [--steps 500] [--warmup 20]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/sec at 1M-particle 3D SPH; surrogate vs plain at ≤1% density err"
+HBM_PEAK_GBS = 8000.0
+# Compulsory bytes per particle of each pass (f32 SoA; every array read and
+# written once; neighbour re-reads not counted).
+ALG_BYTES = {"cell_key": 20, "scan": 0, "scatter": 16, "rank_gather": 68, "density": 20,
+             "force_integrate": 56}
+B_ALG_STEP = 185
+
+
+def _args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="CPU oracle sample length (steps)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_force_integrate.json"),
+                    help="rocprofv3 FETCH/WRITE summary for the dominant kernel (optional)")
+    return ap.parse_args()
+
+
+def main():
+    a = _args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU form of this "
+              "path is design-only", file=sys.stderr)
+        sys.exit(2)
+    from phi_eng_amd import scenes
+    from phi_eng_amd.ffi import last_error
+
+    sc = scenes.c2_dam_break_1m()
+    w = scenes.make_world(sc)
+    n = sc.n
+    for _ in range(a.warmup):
+        w.step(sc.dt)
+    w.sync()
+
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        w.step(sc.dt)
+    w.sync()
+    t1 = time.perf_counter()
+    err = last_error()
+    if err:
+        raise RuntimeError(err)
+    wall = t1 - t0
+    value = n * a.steps / wall
+    ms_per_step = wall * 1e3 / a.steps
+
+    # Per-kernel HIP-event timings over a second run of the same length.
+    w.set_timing(True)
+    t2 = time.perf_counter()
+    for _ in range(a.steps):
+        w.step(sc.dt)
+    w.sync()
+    t3 = time.perf_counter()
+    timing = w.get_timing()
+    w.set_timing(False)
+    kernels = {k: {"avg_us": 1e3 * ms / cnt, "launches": int(cnt)} for k, (ms, cnt) in timing.items()}
+    dom = max((k for k in kernels if k in ALG_BYTES and ALG_BYTES[k]), key=lambda k: kernels[k]["avg_us"])
+    avg_s = kernels[dom]["avg_us"] * 1e-6
+    achieved = ALG_BYTES[dom] * n / avg_s / 1e9
+    traffic = None
+    if os.path.exists(a.traffic):
+        with open(a.traffic) as f:
+            tr = json.load(f)
+        if tr.get("kernel") == dom:
+            traffic = tr.get("bytes_per_launch")
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "alg_bytes_per_particle": ALG_BYTES[dom], "avg_launch_us": round(kernels[dom]["avg_us"], 2)}
+    step_frac = B_ALG_STEP * n / (wall / a.steps) / (HBM_PEAK_GBS * 1e9)
+
+    cpu = None
+    if not a.no_cpu and a.cpu_steps > 0:
+        from oracle import oracle as O
+        bodies = O.bodies_from_boxes(sc.boxes)
+        pos, vel = w.get_state()
+        c0 = time.perf_counter()
+        O.run_f32(pos, vel, sc.config, sc.gravity, bodies, sc.dt, a.cpu_steps, jacobi=False)
+        c1 = time.perf_counter()
+        cpu = {"value": round(n * a.cpu_steps / (c1 - c0), 1), "unit": "particle-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"{a.cpu_steps} steps of the same {n}-particle scene (state after the GPU run), "
+                         "f32 hash-grid Gauss-Seidel restatement of fluid3.rs, 1 thread"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded jittered lattice dam break",
+        "config": {"workload": "dam_break_3d_1m", "particles": n, "lattice": [64, 128, 128],
+                   "tank": [12.8, 6.4, 6.4], "h": 0.1, "spacing": 0.05, "dt": sc.dt,
+                   "stiffness": sc.config.stiffness, "gamma": 7.0, "path": "plain"},
+        "roofline": roof,
+        "step_roofline_frac": round(step_frac, 5),
+        "kernels": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
+        "timed_with_events_ms_per_step": round((t3 - t2) * 1e3 / a.steps, 4),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,148 @@
+/*
+ * phieng_fluid3.h — C ABI of the MI355X-native PhiEng 3D SPH fluid step.
+ *
+ * Every function in the first block keeps the name, argument meaning and
+ * error behaviour of the reference export it replaces (cited as
+ * crates/engine/src/ffi.rs:<line>), so an existing caller (the C# P/Invoke
+ * layer, a C host) links against libphieng_hip.so unchanged.  The second
+ * block holds additions with no reference counterpart: capacity lift,
+ * parity readback, batching, timing and the learned pressure surrogate.
+ *
+ * Conventions (ffi.rs:1-19): opaque world pointers, caller-owned buffers,
+ * null tolerated everywhere (no-op / 0 / -1), no exception or HIP error
+ * crosses the boundary.  One world must not be used from two threads at
+ * once; different worlds are independent.
+ */
+#ifndef PHIENG_FLUID3_H
+#define PHIENG_FLUID3_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct World3 World3;
+
+/* Shape tags of phieng_world3_add_body (ffi.rs:3193-3195). */
+#define PHIENG_SHAPE3_SPHERE 0u
+#define PHIENG_SHAPE3_BOX 1u
+#define PHIENG_SHAPE3_CAPSULE 2u
+
+/* #[repr(C)] FluidConfig3Ffi (ffi.rs:743-767); 112 bytes, align 4.
+ * Field semantics follow FluidConfig3 (crates/dynamics/src/fluid3.rs:51-135). */
+typedef struct FluidConfig3Ffi {
+    float particle_mass;          /*   0 */
+    float rest_density;           /*   4 */
+    float smoothing_length;       /*   8 */
+    float stiffness;              /*  12 */
+    float gamma;                  /*  16 */
+    float artificial_viscosity;   /*  20 */
+    float surface_tension;        /*  24 */
+    float cfl;                    /*  28 */
+    uint32_t max_substeps;        /*  32 */
+    uint8_t adaptive_resolution;  /*  36  (Rust bool) */
+    float split_density_ratio;    /*  40 */
+    float merge_density_ratio;    /*  44 */
+    uint8_t boundary_particles;   /*  48  (Rust bool) */
+    float viscosity;              /*  52 */
+    float drag;                   /*  56 */
+    float wall_friction;          /*  60 */
+    uint8_t phase_count;          /*  64 */
+    float phase_rest_density[4];  /*  68 */
+    float phase_mass[4];          /*  84 */
+    float max_velocity;           /* 100 */
+    float particle_radius;        /* 104 */
+    uint8_t enabled;              /* 108  (Rust bool) */
+} FluidConfig3Ffi;
+
+/* ---------------------------------------------------------------------
+ * Reference exports (drop-in).
+ * ------------------------------------------------------------------- */
+
+/* ffi.rs:3199-3201 — never null; release with phieng_world3_free. */
+World3 *phieng_world3_new(void);
+/* ffi.rs:3209-3213 — null is a no-op. */
+void phieng_world3_free(World3 *world);
+/* ffi.rs:3221-3226 — null, non-finite or dt <= 0 is a no-op; dt is
+ * clamped to 1/15 s. Runs the fluid phase of World3::step
+ * (crates/dynamics/src/world.rs:2670-2697, 2809-2812) on the GPU. */
+void phieng_world3_step(World3 *world, float dt);
+/* ffi.rs:3234-3244 — non-finite components are ignored. */
+void phieng_world3_set_gravity(World3 *world, float gx, float gy, float gz);
+/* ffi.rs:3703-3760 — returns the body id or -1.  Static bodies only:
+ * a dynamic body (is_static == 0) returns -1 because two-way coupling
+ * with moving bodies is not implemented on the GPU path. */
+int32_t phieng_world3_add_body(World3 *world, float pos_x, float pos_y, float pos_z,
+                               uint32_t shape_tag, float a, float b, float c, float density,
+                               uint8_t is_static, float friction, float restitution);
+/* ffi.rs:1032-1044 — 1 on success, 0 on null or invalid input (world
+ * unchanged); validation rules ffi.rs:857-914.  Also returns 0 for
+ * adaptive_resolution or boundary_particles set, which the GPU path
+ * does not implement. */
+int32_t phieng_world3_set_fluid_config(World3 *world, const FluidConfig3Ffi *cfg);
+/* ffi.rs:1050-1059 — 1 on success, 0 on null. */
+int32_t phieng_world3_get_fluid_config(const World3 *world, FluidConfig3Ffi *out);
+/* ffi.rs:3579-3600 — interleaved xyz; count capped at 1,000,000 per call;
+ * any non-finite value rejects the whole call (returns 0).  Returns the
+ * number added (total capped at the fluid capacity, 4096 by default as
+ * MAX_PARTICLES_3D, fluid3.rs:28). */
+uint32_t phieng_world3_fluid_add(World3 *world, const float *positions, uint32_t count);
+/* ffi.rs:3609-3635 — as fluid_add with a phase index per particle. */
+uint32_t phieng_world3_fluid_add_phase(World3 *world, const float *positions,
+                                       const uint8_t *phases, uint32_t count);
+/* ffi.rs:3643-3648 */
+uint32_t phieng_world3_fluid_count(const World3 *world);
+/* ffi.rs:3656-3662 — 1 on success, 0 on null. */
+int32_t phieng_world3_fluid_clear(World3 *world);
+/* ffi.rs:3672-3697 — writes [x,y,z,vx,vy,vz] per particle in insertion
+ * order, up to max; returns the number written. */
+uint32_t phieng_world3_fluid_get(const World3 *world, float *out, uint32_t max);
+
+/* ---------------------------------------------------------------------
+ * Extensions (no reference counterpart).
+ * ------------------------------------------------------------------- */
+
+/* Lift the per-fluid particle cap (MAX_PARTICLES_3D = 4096, fluid3.rs:28)
+ * to `capacity` (at most 16,777,216).  Returns the capacity in force. */
+uint32_t phieng_world3_fluid_reserve(World3 *world, uint32_t capacity);
+
+/* Overwrite velocities (insertion order, vx,vy,vz per particle) — used to
+ * start parity checks from an arbitrary state.  Returns the number set,
+ * 0 on null or any non-finite value. */
+uint32_t phieng_world3_fluid_set_velocity(World3 *world, const float *vel, uint32_t count);
+
+/* Record per-particle intermediates of the next steps (bit flags below). */
+#define PHIENG_RECORD_CELLS 1u   /* cell key triples + sorted order   */
+#define PHIENG_RECORD_DENSITY 2u /* density, pressure, neighbour count */
+#define PHIENG_RECORD_ACCEL 4u   /* a = g + F/rho                      */
+int32_t phieng_world3_fluid_set_record(World3 *world, uint32_t flags);
+/* Read what the most recent substep recorded, in insertion order (cell3:
+ * 3 x int32 per particle; sorted_ids: original index of the particle at
+ * each sorted slot; accel3: 3 floats).  Null outputs are skipped.
+ * Returns the particle count, 0 if nothing was recorded. */
+uint32_t phieng_world3_fluid_get_record(const World3 *world, int32_t *cell3,
+                                        uint32_t *sorted_ids, float *density,
+                                        float *pressure, float *accel3,
+                                        uint32_t *nbr_count, uint32_t max);
+
+/* Kernel timing: when enabled, every launch of the step is bracketed by
+ * HIP events on the world's stream.  get_timing fills per-kernel totals
+ * (ms) and launch counts for up to `max` kernels and returns the number
+ * of kernel kinds; names via phieng_kernel_name. */
+int32_t phieng_world3_set_timing(World3 *world, int32_t enable);
+int32_t phieng_world3_get_timing(const World3 *world, double *total_ms, uint64_t *launches,
+                                 int32_t max);
+const char *phieng_kernel_name(int32_t kind);
+
+/* Block until all queued GPU work of the world has finished.  Returns 1,
+ * or 0 on null / a HIP error (then phieng_last_error names it). */
+int32_t phieng_world3_sync(World3 *world);
+/* Text of the last error seen by any entry point of this thread ("" if none). */
+const char *phieng_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHIENG_FLUID3_H */

@@ -1,0 +1,158 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+use this module, and only as the checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle_sph.so")
+
+
+class OracleBody3(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_uint32), ("px", ctypes.c_float), ("py", ctypes.c_float),
+                ("pz", ctypes.c_float), ("a", ctypes.c_float), ("b", ctypes.c_float),
+                ("c", ctypes.c_float)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            from . import build as _b  # noqa
+        _lib = ctypes.CDLL(LIB)
+        fp = ctypes.POINTER(ctypes.c_float)
+        _lib.oracle3_poly6.restype = ctypes.c_float
+        _lib.oracle3_poly6.argtypes = [ctypes.c_float, ctypes.c_float]
+        _lib.oracle3_poly6_f64.restype = ctypes.c_double
+        _lib.oracle3_poly6_f64.argtypes = [ctypes.c_double, ctypes.c_double]
+        _lib.oracle3_pressure.restype = ctypes.c_float
+        _lib.oracle3_pressure.argtypes = [ctypes.c_float] * 4
+        _lib.oracle3_spiky_grad.restype = None
+        _lib.oracle3_spiky_grad.argtypes = [fp, ctypes.c_float, ctypes.c_float, fp]
+        _lib.oracle3_cfl_substeps.restype = ctypes.c_uint32
+        _lib.oracle3_cfl_substeps.argtypes = [ctypes.c_uint32, fp, ctypes.c_void_p, ctypes.c_float]
+    return _lib
+
+
+def _p(a, t=ctypes.c_float):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def bodies_from_boxes(boxes, spheres=(), capsules=()):
+    arr = (OracleBody3 * max(1, len(boxes) + len(spheres) + len(capsules)))()
+    k = 0
+    for c, he in boxes:
+        arr[k] = OracleBody3(1, *c, *he)
+        k += 1
+    for c, r in spheres:
+        arr[k] = OracleBody3(0, *c, r, 0.0, 0.0)
+        k += 1
+    for c, r, hl in capsules:
+        arr[k] = OracleBody3(2, *c, r, hl, 0.0)
+        k += 1
+    return arr, k
+
+
+def masses(cfg, n, phases=None):
+    """Per-particle mass at insertion (fluid3.rs:205-207)."""
+    pm = np.array([max(cfg.phase_mass[k], 0.0) for k in range(4)], dtype=np.float32)
+    if phases is None:
+        return np.full(n, pm[0], dtype=np.float32)
+    return pm[np.minimum(np.asarray(phases, dtype=np.int64), 3)]
+
+
+def substep_f32(pos, vel, cfg, gravity, bodies, dt, jacobi=True, phases=None, mass=None):
+    """One f32 substep; returns dict(pos, vel, density, pressure, accel, count, scale)."""
+    n = len(pos)
+    pos = np.ascontiguousarray(pos, dtype=np.float32).copy()
+    vel = np.ascontiguousarray(vel, dtype=np.float32).copy()
+    rho = np.zeros(n, dtype=np.float32)
+    p = np.zeros(n, dtype=np.float32)
+    acc = np.zeros((n, 3), dtype=np.float32)
+    cnt = np.zeros(n, dtype=np.uint32)
+    scale = np.zeros(n, dtype=np.float64)
+    m = mass if mass is not None else masses(cfg, n, phases)
+    ph = None if phases is None else np.ascontiguousarray(phases, dtype=np.uint8)
+    g = np.asarray(gravity, dtype=np.float32)
+    arr, nb = bodies
+    lib().oracle3_substep_f32(
+        ctypes.c_uint32(n), _p(pos), _p(vel), _p(rho), _p(p), _p(m), _p(ph, ctypes.c_uint8),
+        ctypes.byref(cfg), _p(g), arr, ctypes.c_uint32(nb), ctypes.c_float(dt), ctypes.c_int(int(jacobi)),
+        _p(acc), _p(cnt, ctypes.c_uint32), _p(scale, ctypes.c_double))
+    return dict(pos=pos, vel=vel, density=rho, pressure=p, accel=acc, count=cnt, scale=scale)
+
+
+def substep_f64(pos, vel, cfg, gravity, bodies, dt, phases=None, mass=None):
+    n = len(pos)
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    vel = np.ascontiguousarray(vel, dtype=np.float32)
+    m = mass if mass is not None else masses(cfg, n, phases)
+    ph = None if phases is None else np.ascontiguousarray(phases, dtype=np.uint8)
+    g = np.asarray(gravity, dtype=np.float32)
+    rho = np.zeros(n)
+    p = np.zeros(n)
+    acc = np.zeros((n, 3))
+    opos = np.zeros((n, 3))
+    ovel = np.zeros((n, 3))
+    arr, nb = bodies
+    d = ctypes.c_double
+    lib().oracle3_substep_f64(
+        ctypes.c_uint32(n), _p(pos), _p(vel), _p(m), _p(ph, ctypes.c_uint8), ctypes.byref(cfg), _p(g), arr,
+        ctypes.c_uint32(nb), ctypes.c_float(dt), _p(rho, d), _p(p, d), _p(acc, d), _p(opos, d), _p(ovel, d))
+    return dict(pos=opos, vel=ovel, density=rho, pressure=p, accel=acc)
+
+
+def run_f32(pos, vel, cfg, gravity, bodies, dt, steps, jacobi=False, phases=None):
+    """`steps` world steps of dt (step_fluid incl. CFL substeps); returns (pos, vel, substeps)."""
+    n = len(pos)
+    pos = np.ascontiguousarray(pos, dtype=np.float32).copy()
+    vel = np.ascontiguousarray(vel, dtype=np.float32).copy()
+    m = masses(cfg, n, phases)
+    ph = None if phases is None else np.ascontiguousarray(phases, dtype=np.uint8)
+    g = np.asarray(gravity, dtype=np.float32)
+    arr, nb = bodies
+    sub = lib().oracle3_run_f32(
+        ctypes.c_uint32(n), _p(pos), _p(vel), _p(m), _p(ph, ctypes.c_uint8), ctypes.byref(cfg), _p(g), arr,
+        ctypes.c_uint32(nb), ctypes.c_float(dt), ctypes.c_uint32(steps), ctypes.c_int(int(jacobi)))
+    return pos, vel, sub
+
+
+def cells(pos, h):
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    out = np.zeros((len(pos), 3), dtype=np.int32)
+    lib().oracle3_cells(ctypes.c_uint32(len(pos)), _p(pos), ctypes.c_float(h), _p(out, ctypes.c_int32))
+    return out
+
+
+def sorted_ids(pos, h):
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    out = np.zeros(len(pos), dtype=np.uint32)
+    lib().oracle3_sorted_ids(ctypes.c_uint32(len(pos)), _p(pos), ctypes.c_float(h), _p(out, ctypes.c_uint32))
+    return out
+
+
+def poly6(r2, h):
+    return lib().oracle3_poly6(r2, h)
+
+
+def spiky_grad(rv, r, h):
+    rv = np.asarray(rv, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    lib().oracle3_spiky_grad(_p(rv), ctypes.c_float(r), ctypes.c_float(h), _p(out))
+    return out
+
+
+def pressure(rho, rest, k, gamma):
+    return lib().oracle3_pressure(rho, rest, k, gamma)
+
+
+def cfl_substeps(vel, cfg, dt):
+    vel = np.ascontiguousarray(vel, dtype=np.float32)
+    return lib().oracle3_cfl_substeps(ctypes.c_uint32(len(vel)), _p(vel), ctypes.byref(cfg), ctypes.c_float(dt))

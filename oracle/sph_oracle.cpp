@@ -1,0 +1,615 @@
+// sph_oracle.cpp — CPU restatement of the PhiEng 3D SPH fluid step.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the GPU path:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// it.  The product library (phi_eng_amd/libphieng_hip.so) never links it.
+//
+// It follows crates/dynamics/src/fluid3.rs line by line, in f32 with the
+// reference's operation order (compiled with -ffp-contract=off), plus an
+// FP64 twin of the same algorithm for error reporting.  Citations are
+// file:line in the reference tree.
+//
+// What pins each function (tests/test_oracle.py):
+//   poly6 / spiky_grad / pressure  — closed forms, kernel zeros and signs
+//       (fluid3.rs:1296-1311, 1418-1455), the bit-level linear-EOS known
+//       answer (fluid.rs:1326-1329), exact-lattice kernel sums.
+//   cells / sorted order           — saturating-cast definition, brute force,
+//       std::stable_sort equivalence.
+//   density / neighbour counts     — O(N^2) brute force on small inputs,
+//       lattice closed form (interior rho = m * sum W).
+//   force / acceleration           — two-particle closed form, momentum
+//       antisymmetry sum_i m_i F_i = 0, FP64 twin agreement.
+//   integrate + boundary clamp     — containment, count preserved,
+//       run-twice determinism, hydrostatic band (fluid3.rs:1313-1347,
+//       1398-1416, 1560-1605), single-particle closed forms.
+//   Parity unpinned: glam 0.29.3 Vec3 association order and libm powf
+//   (the Tait power is taken in f64 and rounded to f32 here and on the GPU).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <algorithm>
+#include <unordered_map>
+#include <vector>
+
+#include "../include/phieng_fluid3.h"
+
+namespace {
+
+// crates/math/src/math.rs:3-4
+constexpr float kEps = 1e-6f;
+constexpr float kEpsSq = 1e-12f;
+constexpr float kPiF = 3.14159265358979323846f;  // std::f32::consts::PI
+
+struct Body {  // static body, identity rotation (the C ABI cannot rotate)
+    uint32_t shape;
+    float px, py, pz;
+    float a, b, c;
+};
+
+// Rust `f32::max`: NaN operands lose; ties keep the second operand.
+template <class T>
+inline T rmax(T a, T b) {
+    if (a != a) return b;
+    if (b != b) return a;
+    return a > b ? a : b;
+}
+// Rust `f32::clamp` (fluid3.rs:649): NaN passes through.
+template <class T>
+inline T rclamp(T v, T lo, T hi) {
+    if (v < lo) v = lo;
+    if (v > hi) v = hi;
+    return v;
+}
+// Rust `f32 as i32`: saturating, NaN -> 0.
+inline int32_t sat_i32(double v) {
+    if (v != v) return 0;
+    if (v >= 2147483648.0) return INT32_MAX;
+    if (v <= -2147483649.0) return INT32_MIN;
+    return (int32_t)v;
+}
+
+template <class T>
+struct V3 {
+    T x, y, z;
+};
+template <class T>
+inline V3<T> sub(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T>
+inline V3<T> add(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T>
+inline V3<T> muls(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T>
+inline V3<T> divs(V3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+// glam Vec3::dot: (x*x' + y*y') + z*z' (parity unpinned, see header).
+template <class T>
+inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T>
+inline T len(V3<T> a) { return std::sqrt(dot(a, a)); }
+template <class T>
+inline bool finite3(V3<T> a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
+
+struct Key {
+    int32_t x, y, z;
+    bool operator==(const Key &o) const { return x == o.x && y == o.y && z == o.z; }
+};
+struct KeyHash {
+    size_t operator()(const Key &k) const {
+        uint64_t h = (uint64_t)(uint32_t)k.x * 0x9E3779B97F4A7C15ull;
+        h ^= (uint64_t)(uint32_t)k.y * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+        h ^= (uint64_t)(uint32_t)k.z * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
+        return (size_t)h;
+    }
+};
+
+template <class T>
+struct Consts;
+template <>
+struct Consts<float> {
+    static float pi() { return kPiF; }
+};
+template <>
+struct Consts<double> {
+    static double pi() { return 3.14159265358979323846; }
+};
+
+template <class T>
+struct Fluid {
+    // FluidConfig3 (fluid3.rs:51-135) as the kernels read it.
+    const FluidConfig3Ffi &cfg;
+    uint32_t n;
+    std::vector<V3<T>> pos, vel;
+    std::vector<T> density, pressure, mass;
+    std::vector<uint8_t> phase;
+    std::unordered_map<Key, std::vector<uint32_t>, KeyHash> grid;
+    std::vector<V3<T>> forces;
+
+    explicit Fluid(const FluidConfig3Ffi &c) : cfg(c), n(0) {}
+
+    T h_kernel() const { return rmax<T>((T)cfg.smoothing_length, (T)kEps); }
+
+    // fluid3.rs:220-228
+    static T poly6(T r2, T h) {
+        if (r2 >= h * h || r2 < (T)0) return (T)0;
+        T h2 = h * h;
+        T t = h2 - r2;
+        return (T)315.0 / ((T)64.0 * Consts<T>::pi() * h2 * h2 * h2 * h2 * h) * t * t * t;
+    }
+    // fluid3.rs:232-239
+    static V3<T> spiky_grad(V3<T> r_vec, T r, T h) {
+        if (r >= h || r < (T)kEps) return {0, 0, 0};
+        T s = (T)-45.0 / (Consts<T>::pi() * h * h * h * h * h * h) * (h - r) * (h - r);
+        return muls(divs(r_vec, r), s);
+    }
+    // fluid3.rs:244-251 (Tait power in f64, rounded for f32)
+    static T eos(T density, T rest, T k, T gamma) {
+        if (gamma == (T)1.0) return rmax<T>(k * (density - rest), (T)0);
+        T rho0 = rmax<T>(rest, (T)kEps);
+        T pw = (T)std::pow((double)(density / rho0), (double)gamma);
+        return rmax<T>(k * rho0 * (pw - (T)1.0), (T)0);
+    }
+    // fluid3.rs:254-269
+    static T artificial_pressure(V3<T> delta, V3<T> v_rel, T r_sq, T h, T rho_bar, T c, T alpha) {
+        T vd = dot(v_rel, delta);
+        if (vd >= (T)0) return (T)0;
+        T mu = h * vd / (r_sq + (T)0.01 * h * h);
+        return -alpha * c * mu / rmax<T>(rho_bar, (T)kEps);
+    }
+    // fluid3.rs:272-278
+    static T cohesion(T dist, T h) { return dist >= h ? (T)0 : (T)1.0 - dist / h; }
+
+    Key cell_of(V3<T> p) const {
+        T h = (T)cfg.smoothing_length;
+        return {sat_i32((double)std::floor(p.x / h)), sat_i32((double)std::floor(p.y / h)),
+                sat_i32((double)std::floor(p.z / h))};
+    }
+    // fluid3.rs:282-315 (non-finite positions skipped, index order kept)
+    void rebuild_grid() {
+        for (auto &kv : grid) kv.second.clear();
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!finite3(pos[i])) continue;
+            grid[cell_of(pos[i])].push_back(i);
+        }
+    }
+    // fluid3.rs:319-340: cells oz -> oy -> ox, ascending index inside a cell,
+    // raw smoothing_length for the cut-off, self included.
+    void neighbors(uint32_t i, std::vector<uint32_t> &out) const {
+        out.clear();
+        T h = (T)cfg.smoothing_length;
+        V3<T> p = pos[i];
+        Key c = cell_of(p);
+        for (int oz = -1; oz <= 1; ++oz)
+            for (int oy = -1; oy <= 1; ++oy)
+                for (int ox = -1; ox <= 1; ++ox) {
+                    // i32 wrap cannot be reached from the C ABI (fluid_add
+                    // rejects non-finite input); use 64-bit sums to stay defined.
+                    int64_t kx = (int64_t)c.x + ox, ky = (int64_t)c.y + oy, kz = (int64_t)c.z + oz;
+                    if (kx > INT32_MAX || kx < INT32_MIN || ky > INT32_MAX || ky < INT32_MIN ||
+                        kz > INT32_MAX || kz < INT32_MIN)
+                        continue;
+                    auto it = grid.find(Key{(int32_t)kx, (int32_t)ky, (int32_t)kz});
+                    if (it == grid.end()) continue;
+                    for (uint32_t j : it->second) {
+                        T d2 = dot(sub(pos[j], p), sub(pos[j], p));
+                        if (d2 < h * h) out.push_back(j);
+                    }
+                }
+    }
+    // fluid3.rs:639-655
+    void compute_density_pressure(T h, uint32_t *out_count) {
+        std::vector<uint32_t> nb;
+        for (uint32_t i = 0; i < n; ++i) {
+            neighbors(i, nb);
+            V3<T> pi = pos[i];
+            T rho = 0;
+            for (uint32_t j : nb) {
+                T r2 = dot(sub(pos[j], pi), sub(pos[j], pi));
+                rho += mass[j] * poly6(r2, h);
+            }
+            if (out_count) out_count[i] = (uint32_t)nb.size();
+            density[i] = rclamp<T>(rho, (T)0, (T)3.0 * rmax<T>((T)cfg.rest_density, (T)kEps));
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            T rho0 = (T)cfg.phase_rest_density[std::min<uint32_t>(phase[i], 3u)];
+            pressure[i] = eos(density[i], rho0, (T)cfg.stiffness, (T)cfg.gamma);
+        }
+    }
+    // fluid3.rs:1029-1086 (ghost term omitted: boundary_particles rejected).
+    // Also returns the cancellation-aware scale of the acceleration.
+    void compute_forces(T h, double *scale_out) {
+        forces.assign(n, V3<T>{0, 0, 0});
+        T c_s = std::sqrt(rmax<T>((T)cfg.stiffness, (T)0));
+        std::vector<uint32_t> nb;
+        for (uint32_t i = 0; i < n; ++i) {
+            neighbors(i, nb);
+            V3<T> pi = pos[i], vi = vel[i];
+            T rho_i = rmax<T>(density[i], (T)kEps);
+            T p_i = pressure[i];
+            V3<T> f{0, 0, 0};
+            double mag = 0;
+            for (uint32_t j : nb) {
+                if (j == i) continue;
+                T rho_j = rmax<T>(density[j], (T)kEps);
+                V3<T> delta = sub(pi, pos[j]);
+                T dist = len(delta);
+                V3<T> grad = spiky_grad(delta, dist, h);
+                T term = p_i / (rho_i * rho_i) + pressure[j] / (rho_j * rho_j);
+                if ((T)cfg.artificial_viscosity > (T)0 && dist > (T)kEps) {
+                    T rho_bar = (T)0.5 * (rho_i + rho_j);
+                    term += artificial_pressure(delta, sub(vi, vel[j]), dist * dist, h, rho_bar, c_s,
+                                                (T)cfg.artificial_viscosity);
+                }
+                T s = mass[j] * term;
+                f = sub(f, muls(grad, s));
+                mag += std::sqrt((double)dot(grad, grad)) * std::fabs((double)s);
+                if ((T)cfg.surface_tension > (T)0 && dist > (T)kEps) {
+                    T coh = cohesion(dist, h);
+                    if (coh > (T)0) {
+                        T cs = (T)cfg.surface_tension * mass[i] * mass[j] * coh;
+                        f = sub(f, muls(divs(delta, dist), cs));
+                        mag += std::fabs((double)cs);
+                    }
+                }
+            }
+            forces[i] = f;
+            if (scale_out) scale_out[i] = mag / (double)rho_i;
+        }
+    }
+
+    // fluid3.rs:471-585 for the shapes the C ABI creates (identity rotation,
+    // so glam's quaternion products reduce to the identity up to zero signs).
+    bool collide(V3<T> p, T pr, const Body &b, V3<T> &normal, T &depth) const {
+        V3<T> c{(T)b.px, (T)b.py, (T)b.pz};
+        if (b.shape == PHIENG_SHAPE3_SPHERE) {
+            T radius = (T)b.a;
+            V3<T> delta = sub(p, c);
+            T dist = len(delta);
+            if (dist >= radius + pr) return false;
+            normal = dist > (T)kEps ? divs(delta, dist) : V3<T>{1, 0, 0};
+            depth = radius + pr - dist;
+            return true;
+        }
+        if (b.shape == PHIENG_SHAPE3_BOX) {
+            V3<T> he{(T)b.a, (T)b.b, (T)b.c};
+            V3<T> local = sub(p, c);
+            V3<T> cl{rclamp(local.x, -he.x, he.x), rclamp(local.y, -he.y, he.y),
+                     rclamp(local.z, -he.z, he.z)};
+            V3<T> diff = sub(local, cl);
+            if (dot(diff, diff) > (T)kEpsSq) {
+                T dist = len(diff);
+                if (dist >= pr) return false;
+                normal = divs(diff, dist);
+                depth = pr - dist;
+                return true;
+            }
+            // Centre inside: eject along the least-penetration axis; the
+            // reference's `signum().copysign(1.0)` is always +1 (fluid3.rs:514-520).
+            T dx = he.x - std::fabs(local.x);
+            T dy = he.y - std::fabs(local.y);
+            T dz = he.z - std::fabs(local.z);
+            if (dx <= dy && dx <= dz) {
+                normal = {1, 0, 0};
+                depth = pr + dx;
+            } else if (dy <= dz) {
+                normal = {0, 1, 0};
+                depth = pr + dy;
+            } else {
+                normal = {0, 0, 1};
+                depth = pr + dz;
+            }
+            return true;
+        }
+        if (b.shape == PHIENG_SHAPE3_CAPSULE) {
+            T r = (T)b.a, hl = (T)b.b;
+            V3<T> local = sub(p, c);
+            T sy = rclamp(local.y, -hl, hl);
+            V3<T> diff{local.x - (T)0, local.y - sy, local.z - (T)0};
+            if (dot(diff, diff) > (T)kEpsSq) {
+                T dist = len(diff);
+                if (dist >= r + pr) return false;
+                normal = divs(diff, dist);
+                depth = r + pr - dist;
+                return true;
+            }
+            V3<T> radial{local.x, 0, local.z};
+            if (dot(radial, radial) > (T)kEpsSq) {
+                // glam normalize_or_zero: v * (1 / length)
+                T rcp = (T)1.0 / len(radial);
+                normal = (std::isfinite(rcp) && rcp > (T)0) ? muls(radial, rcp) : V3<T>{0, 0, 0};
+            } else {
+                normal = {1, 0, 0};
+            }
+            depth = pr + r - len(diff);
+            return true;
+        }
+        return false;
+    }
+
+    // fluid3.rs:1141-1179 for one particle (static bodies: impulses are
+    // no-ops, body.rs:1349-1352; static body velocity is zero).
+    void clamp_particle(uint32_t i, const Body *bodies, uint32_t nb) {
+        T pr = rmax<T>((T)cfg.particle_radius, (T)0);
+        for (uint32_t k = 0; k < nb; ++k) {
+            V3<T> ppos = pos[i];
+            V3<T> normal;
+            T depth;
+            if (!collide(ppos, pr, bodies[k], normal, depth)) continue;
+            if (depth <= (T)0) continue;
+            pos[i] = add(ppos, muls(normal, depth));
+            T vn = dot(vel[i], normal);
+            if (vn < (T)0) {
+                vel[i] = sub(vel[i], muls(normal, vn));
+                if ((T)cfg.wall_friction > (T)0) {
+                    T mu = rclamp<T>((T)cfg.wall_friction, (T)0, (T)1);
+                    V3<T> v_t = vel[i];
+                    V3<T> wall_v{0, 0, 0};
+                    V3<T> wall_vt = sub(wall_v, muls(normal, dot(wall_v, normal)));
+                    V3<T> delta_v = muls(sub(wall_vt, v_t), mu);
+                    vel[i] = add(vel[i], delta_v);
+                }
+            }
+        }
+    }
+
+    // One integrate update of particle i from the given neighbour list,
+    // reading neighbours from (rpos, rvel, density) (fluid3.rs:1092-1139).
+    void integrate_one(uint32_t i, const std::vector<uint32_t> &nb, const std::vector<V3<T>> &rpos,
+                       const std::vector<V3<T>> &rvel, V3<T> g, T dt, T h, V3<T> &out_v, V3<T> &out_x,
+                       V3<T> *acc_out) const {
+        T rho_i = rmax<T>(density[i], (T)kEps);
+        V3<T> acc = add(g, divs(forces[i], rho_i));
+        if (acc_out) *acc_out = acc;
+        V3<T> vi = rvel[i];
+        V3<T> v = add(vi, muls(acc, dt));
+        if ((T)cfg.viscosity > (T)0) {
+            V3<T> pi = rpos[i];
+            V3<T> smooth{0, 0, 0};
+            for (uint32_t j : nb) {
+                if (j == i) continue;
+                T rho_bar = (T)0.5 * (rho_i + rmax<T>(density[j], (T)kEps));
+                T w = poly6(dot(sub(rpos[j], pi), sub(rpos[j], pi)), h);
+                smooth = add(smooth, muls(muls(sub(rvel[j], vi), mass[j] / rho_bar), w));
+            }
+            v = add(v, muls(smooth, (T)cfg.viscosity));
+        }
+        T speed = len(v);
+        if (speed > (T)cfg.max_velocity) v = muls(v, (T)cfg.max_velocity / speed);
+        if (!finite3(v)) v = {0, 0, 0};
+        V3<T> x = add(rpos[i], muls(v, dt));
+        if (!finite3(x)) x = rpos[i];
+        out_v = v;
+        out_x = x;
+    }
+
+    // fluid3.rs:990-1184.  jacobi = false replays the reference's in-place
+    // Gauss-Seidel integrate (neighbours j < i already advanced, stale grid);
+    // jacobi = true reads only pre-step state (the GPU contract).
+    void substep(V3<T> g, T dt, bool jacobi, const Body *bodies, uint32_t nbodies, T *acc_out,
+                 uint32_t *count_out, double *scale_out) {
+        T h = h_kernel();
+        rebuild_grid();
+        compute_density_pressure(h, count_out);
+        compute_forces(h, scale_out);
+        std::vector<uint32_t> nb;
+        if (jacobi) {
+            std::vector<V3<T>> npos(n), nvel(n);
+            for (uint32_t i = 0; i < n; ++i) {
+                neighbors(i, nb);
+                V3<T> a;
+                integrate_one(i, nb, pos, vel, g, dt, h, nvel[i], npos[i], &a);
+                if (acc_out) {
+                    acc_out[3 * i] = a.x;
+                    acc_out[3 * i + 1] = a.y;
+                    acc_out[3 * i + 2] = a.z;
+                }
+            }
+            pos.swap(npos);
+            vel.swap(nvel);
+        } else {
+            for (uint32_t i = 0; i < n; ++i) {
+                neighbors(i, nb);
+                V3<T> a, v, x;
+                integrate_one(i, nb, pos, vel, g, dt, h, v, x, &a);
+                vel[i] = v;
+                pos[i] = x;
+                if (acc_out) {
+                    acc_out[3 * i] = a.x;
+                    acc_out[3 * i + 1] = a.y;
+                    acc_out[3 * i + 2] = a.z;
+                }
+            }
+        }
+        for (uint32_t i = 0; i < n; ++i) clamp_particle(i, bodies, nbodies);
+        // fluid3.rs:1184-1213: the second grid rebuild only feeds buoyancy of
+        // dynamic bodies, which this path does not have.
+    }
+
+    // fluid3.rs:610-636
+    uint32_t cfl_substeps(T dt) const {
+        uint32_t mx = std::max<uint32_t>(cfg.max_substeps, 1u);
+        if ((T)cfg.cfl <= (T)0 || mx <= 1) return 1;
+        T h = rmax<T>((T)cfg.smoothing_length, (T)kEps);
+        T rho0 = rmax<T>((T)cfg.rest_density, (T)kEps);
+        T c0 = (T)cfg.gamma == (T)1.0 ? std::sqrt(rmax<T>((T)cfg.stiffness, (T)0))
+                                      : std::sqrt((T)cfg.gamma * rmax<T>((T)cfg.stiffness, (T)0) * rho0);
+        T vmax = 0;
+        for (uint32_t i = 0; i < n; ++i) vmax = rmax<T>(vmax, len(vel[i]));
+        T speed = c0 + vmax;
+        if (!std::isfinite(speed) || speed <= (T)kEps) return 1;
+        T dt_max = (T)cfg.cfl * h / speed;
+        if (!std::isfinite(dt_max) || dt_max <= (T)0) return 1;
+        double q = std::ceil((double)(dt / dt_max));
+        int64_t s;
+        if (q != q) s = 0;
+        else if (q >= 9.2233720368547758e18) s = INT64_MAX;
+        else if (q <= -9.2233720368547758e18) s = INT64_MIN;
+        else s = (int64_t)q;
+        return (uint32_t)std::min<int64_t>(std::max<int64_t>(s, 1), (int64_t)mx);
+    }
+
+    void load(uint32_t count, const float *p, const float *v, const float *m, const uint8_t *ph,
+              const float *rho) {
+        n = count;
+        pos.resize(n);
+        vel.resize(n);
+        density.resize(n);
+        pressure.assign(n, 0);
+        mass.resize(n);
+        phase.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            pos[i] = {(T)p[3 * i], (T)p[3 * i + 1], (T)p[3 * i + 2]};
+            vel[i] = v ? V3<T>{(T)v[3 * i], (T)v[3 * i + 1], (T)v[3 * i + 2]} : V3<T>{0, 0, 0};
+            mass[i] = (T)m[i];
+            phase[i] = ph ? ph[i] : 0;
+            density[i] = rho ? (T)rho[i] : (T)cfg.phase_rest_density[std::min<uint32_t>(phase[i], 3u)];
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+typedef struct OracleBody3 {
+    uint32_t shape;
+    float px, py, pz;
+    float a, b, c;
+} OracleBody3;
+
+// One f32 substep in place (fluid3.rs:990-1184).  jacobi = 0 reproduces the
+// reference's Gauss-Seidel integrate.  Optional outputs (null to skip):
+// acc3 = g + F/rho per particle, count = neighbour count incl. self,
+// scale = cancellation-aware magnitude sum(|m_j term grad W|)/rho_i.
+int oracle3_substep_f32(uint32_t n, float *pos, float *vel, float *density, float *pressure,
+                        const float *mass, const uint8_t *phase, const FluidConfig3Ffi *cfg,
+                        const float *g3, const OracleBody3 *bodies, uint32_t nbodies, float dt,
+                        int jacobi, float *acc3, uint32_t *count, double *scale) {
+    Fluid<float> f(*cfg);
+    f.load(n, pos, vel, mass, phase, density);
+    f.substep({g3[0], g3[1], g3[2]}, dt, jacobi != 0, reinterpret_cast<const Body *>(bodies), nbodies,
+              acc3, count, scale);
+    for (uint32_t i = 0; i < n; ++i) {
+        pos[3 * i] = f.pos[i].x;
+        pos[3 * i + 1] = f.pos[i].y;
+        pos[3 * i + 2] = f.pos[i].z;
+        vel[3 * i] = f.vel[i].x;
+        vel[3 * i + 1] = f.vel[i].y;
+        vel[3 * i + 2] = f.vel[i].z;
+        if (density) density[i] = f.density[i];
+        if (pressure) pressure[i] = f.pressure[i];
+    }
+    return 1;
+}
+
+// FluidState3::step_fluid (fluid3.rs:591-607) for `steps` world steps of dt
+// (dt already clamped by the caller as phieng_world3_step does,
+// ffi.rs:3221-3226).  State is carried across steps in a persistent Fluid
+// so the hash grid is reused as in the reference.  Returns the number of
+// substeps taken by the last step.
+int oracle3_run_f32(uint32_t n, float *pos, float *vel, const float *mass, const uint8_t *phase,
+                    const FluidConfig3Ffi *cfg, const float *g3, const OracleBody3 *bodies,
+                    uint32_t nbodies, float dt, uint32_t steps, int jacobi) {
+    Fluid<float> f(*cfg);
+    f.load(n, pos, vel, mass, phase, nullptr);
+    uint32_t sub = 0;
+    for (uint32_t s = 0; s < steps; ++s) {
+        if (!cfg->enabled || n == 0 || !(dt > 0.0f)) break;
+        sub = f.cfl_substeps(dt);
+        float sdt = dt / (float)sub;
+        for (uint32_t k = 0; k < sub; ++k)
+            f.substep({g3[0], g3[1], g3[2]}, sdt, jacobi != 0, reinterpret_cast<const Body *>(bodies),
+                      nbodies, nullptr, nullptr, nullptr);
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        pos[3 * i] = f.pos[i].x;
+        pos[3 * i + 1] = f.pos[i].y;
+        pos[3 * i + 2] = f.pos[i].z;
+        vel[3 * i] = f.vel[i].x;
+        vel[3 * i + 1] = f.vel[i].y;
+        vel[3 * i + 2] = f.vel[i].z;
+    }
+    return (int)sub;
+}
+
+// FP64 twin of one Jacobi substep from an f32 state.  Outputs in f64.
+int oracle3_substep_f64(uint32_t n, const float *pos, const float *vel, const float *mass,
+                        const uint8_t *phase, const FluidConfig3Ffi *cfg, const float *g3,
+                        const OracleBody3 *bodies, uint32_t nbodies, float dt, double *out_rho,
+                        double *out_p, double *out_acc3, double *out_pos3, double *out_vel3) {
+    Fluid<double> f(*cfg);
+    f.load(n, pos, vel, mass, phase, nullptr);
+    std::vector<double> acc(3 * (size_t)n);
+    f.substep({g3[0], g3[1], g3[2]}, (double)dt, true, reinterpret_cast<const Body *>(bodies), nbodies,
+              acc.data(), nullptr, nullptr);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (out_rho) out_rho[i] = f.density[i];
+        if (out_p) out_p[i] = f.pressure[i];
+        if (out_acc3) {
+            out_acc3[3 * i] = acc[3 * i];
+            out_acc3[3 * i + 1] = acc[3 * i + 1];
+            out_acc3[3 * i + 2] = acc[3 * i + 2];
+        }
+        if (out_pos3) {
+            out_pos3[3 * i] = f.pos[i].x;
+            out_pos3[3 * i + 1] = f.pos[i].y;
+            out_pos3[3 * i + 2] = f.pos[i].z;
+        }
+        if (out_vel3) {
+            out_vel3[3 * i] = f.vel[i].x;
+            out_vel3[3 * i + 1] = f.vel[i].y;
+            out_vel3[3 * i + 2] = f.vel[i].z;
+        }
+    }
+    return 1;
+}
+
+// Cell key triples (fluid3.rs:305-309) with Rust's saturating cast.
+void oracle3_cells(uint32_t n, const float *pos, float h, int32_t *cells3) {
+    for (uint32_t i = 0; i < n; ++i) {
+        cells3[3 * i] = sat_i32((double)std::floor(pos[3 * i] / h));
+        cells3[3 * i + 1] = sat_i32((double)std::floor(pos[3 * i + 1] / h));
+        cells3[3 * i + 2] = sat_i32((double)std::floor(pos[3 * i + 2] / h));
+    }
+}
+
+// The grid's bucket contents laid out in z-major cell order: cells sorted
+// by (cz, cy, cx), members in ascending index (fluid3.rs:301-314);
+// non-finite positions, which the reference leaves out of the grid, last
+// in index order.  Equal to a stable sort of 0..n by that key.
+void oracle3_sorted_ids(uint32_t n, const float *pos, float h, uint32_t *ids) {
+    std::vector<int32_t> c(3 * (size_t)n);
+    oracle3_cells(n, pos, h, c.data());
+    std::vector<uint8_t> fin(n);
+    for (uint32_t i = 0; i < n; ++i)
+        fin[i] = std::isfinite(pos[3 * i]) && std::isfinite(pos[3 * i + 1]) && std::isfinite(pos[3 * i + 2]);
+    std::vector<uint32_t> idx(n);
+    for (uint32_t i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        if (fin[a] != fin[b]) return fin[a] > fin[b];
+        if (!fin[a]) return false;
+        if (c[3 * a + 2] != c[3 * b + 2]) return c[3 * a + 2] < c[3 * b + 2];
+        if (c[3 * a + 1] != c[3 * b + 1]) return c[3 * a + 1] < c[3 * b + 1];
+        return c[3 * a] < c[3 * b];
+    });
+    std::memcpy(ids, idx.data(), sizeof(uint32_t) * n);
+}
+
+// Scalar kernels for closed-form tests (fluid3.rs:220-251).
+float oracle3_poly6(float r2, float h) { return Fluid<float>::poly6(r2, h); }
+void oracle3_spiky_grad(const float *r_vec3, float r, float h, float *out3) {
+    V3<float> g = Fluid<float>::spiky_grad({r_vec3[0], r_vec3[1], r_vec3[2]}, r, h);
+    out3[0] = g.x;
+    out3[1] = g.y;
+    out3[2] = g.z;
+}
+float oracle3_pressure(float density, float rest, float k, float gamma) {
+    return Fluid<float>::eos(density, rest, k, gamma);
+}
+double oracle3_poly6_f64(double r2, double h) { return Fluid<double>::poly6(r2, h); }
+uint32_t oracle3_cfl_substeps(uint32_t n, const float *vel, const FluidConfig3Ffi *cfg, float dt) {
+    Fluid<float> f(*cfg);
+    std::vector<float> p(3 * (size_t)n, 0.0f), m(n, 1.0f);
+    f.load(n, p.data(), vel, m.data(), nullptr, nullptr);
+    return f.cfl_substeps(dt);
+}
+
+}  // extern "C"

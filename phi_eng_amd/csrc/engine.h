@@ -1,0 +1,182 @@
+// engine.h — device-resident state and step orchestration shared by a
+// single World3 (one scene) and a Batch3 (many scenes).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/phieng_fluid3.h"
+#include "sph_device.h"
+
+namespace ph {
+
+enum Kind : int {
+    KIND_CELL_KEY = 0,
+    KIND_SCAN_REDUCE,
+    KIND_SCAN_PARTIALS,
+    KIND_SCAN_APPLY,
+    KIND_SCATTER,
+    KIND_RANK_GATHER,
+    KIND_DENSITY,
+    KIND_FORCE,
+    KIND_VMAX,
+    KIND_EXPORT,
+    KIND_MONITOR,
+    KIND_APPEND,
+    KIND_BBOX,
+    KIND_FILL,
+    KIND_SETVEL,
+    KIND_PERSISTENT,
+    KIND_COUNT
+};
+const char *kind_name(int k);
+
+struct HipError : std::runtime_error {
+    HipError(const char *what, hipError_t e)
+        : std::runtime_error(std::string(what) + ": " + hipGetErrorString(e)) {}
+};
+#define PH_CK(x)                                   \
+    do {                                           \
+        hipError_t e_ = (x);                       \
+        if (e_ != hipSuccess) throw ph::HipError(#x, e_); \
+    } while (0)
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Grow to at least m elements; contents are not preserved.
+    void ensure(size_t m) {
+        if (m <= n && p) return;
+        release();
+        size_t want = m ? m : 1;
+        PH_CK(hipMalloc((void **)&p, want * sizeof(T)));
+        n = want;
+    }
+};
+
+// Host-side description of one scene (one fluid + its static bodies).
+struct SceneHost {
+    FluidConfig3Ffi cfg;
+    float g[3];
+    float dt;
+    std::vector<BodyDev> bodies;
+};
+
+// f32 constants of one scene, computed with the reference's operation
+// order (fluid3.rs:220-251, 649, 1028) so device results match it.
+void fill_scene_physics(SceneDev &s, const SceneHost &h);
+// Pick a grid domain (cell box) for a scene from the particle cell box and
+// its bodies.  The domain only affects speed: particles outside it land in
+// the scene's outlier bucket and are still handled exactly.
+void choose_domain(SceneDev &s, const int32_t *pbox, uint32_t nparticles, const std::vector<BodyDev> &bodies);
+
+struct Engine {
+    int device = -1;
+    hipStream_t stream = nullptr;
+
+    // particle arrays (two generations; cs/ct select the current ones)
+    DBuf<float4> posm[2], vel[2];
+    DBuf<uint32_t> tag[2];
+    int cs = 0, ct = 0;
+    size_t alloc_particles = 0;
+    DBuf<uint32_t> keys, slots, tmp;
+    DBuf<float2> rq;
+    // bucket tables
+    DBuf<uint32_t> counts, start, partial;
+    size_t alloc_buckets = 0;
+    // scene / tile / body tables
+    DBuf<SceneDev> d_scenes;
+    DBuf<TileDev> d_tiles;
+    DBuf<BodyDev> d_bodies;
+    DBuf<int32_t> d_bbox, d_monitor;
+    DBuf<uint32_t> d_vmax;
+    DBuf<float> staging_f, export_buf;
+    DBuf<uint8_t> staging_u8;
+    // recording of intermediates (parity readback)
+    uint32_t rec_flags = 0;
+    bool rec_valid = false;
+    DBuf<int32_t> rec_cells;
+    DBuf<uint32_t> rec_sorted, rec_count;
+    DBuf<float> rec_rho, rec_p, rec_acc;
+
+    std::vector<SceneDev> scenes;
+    std::vector<TileDev> tiles;
+    std::vector<BodyDev> bodies;
+    uint32_t n_total = 0;
+    uint32_t nbuckets = 0;
+    bool scenes_dirty = true, tiles_dirty = true, bodies_dirty = true;
+
+    // domain monitor (asynchronous readback of outlier counts + boxes)
+    int32_t *h_monitor = nullptr;
+    hipEvent_t mon_event = nullptr;
+    bool mon_pending = false;
+    uint64_t substeps_done = 0;
+    uint32_t monitor_every = 16;
+
+    // kernel timing
+    bool timing = false;
+    struct Ev {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> pending_ev;
+    std::vector<hipEvent_t> ev_pool;
+    double total_ms[KIND_COUNT] = {};
+    uint64_t launches[KIND_COUNT] = {};
+
+    Engine() = default;
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+    ~Engine();
+
+    void init_device();
+    // Make room for `cap` particles, keeping the current state.
+    void ensure_particles(uint32_t cap);
+    void build_tiles();
+    void assign_buckets();
+    void upload_tables();
+    void ensure_record();
+    // One substep of every scene (dt per scene in SceneDev::dt).
+    void substep();
+    // Asynchronous domain monitor: returns true if some scene changed domain.
+    bool poll_monitor(uint32_t nparticles_hint);
+    // Cell box of the current particles of scene k (synchronous).
+    void particle_cell_box(uint32_t k, int32_t out[6]);
+    float max_speed(uint32_t begin, uint32_t end);
+
+    template <class Fn>
+    void run(int kind, Fn &&fn) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (timing) {
+            a = get_event();
+            b = get_event();
+            PH_CK(hipEventRecord(a, stream));
+        }
+        hipError_t e = fn();
+        if (e != hipSuccess) throw HipError(kind_name(kind), e);
+        if (timing) {
+            PH_CK(hipEventRecord(b, stream));
+            pending_ev.push_back({kind, a, b});
+            if (pending_ev.size() > 8192) harvest_timing();
+        }
+    }
+    hipEvent_t get_event();
+    void harvest_timing();
+    void reset_timing();
+};
+
+}  // namespace ph

@@ -1,0 +1,273 @@
+// sph_device.h — device-side types and arithmetic of the SPH fluid step.
+//
+// Every floating-point expression here keeps the operand order of the
+// reference (crates/dynamics/src/fluid3.rs) and the library is built with
+// -ffp-contract=off and IEEE division/sqrt, so a particle's sums come out
+// bit-identical to the f32 CPU path when neighbours are visited in the
+// reference order (cells oz -> oy -> ox, ascending original index).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PH_EPS 1e-6f    // crates/math/src/math.rs:3
+#define PH_EPS_SQ 1e-12f  // crates/math/src/math.rs:4
+#define PH_ID_MASK 0x00FFFFFFu  // low 24 bits of a tag: original index
+#define PH_TILE 256             // particles per tile (= threads per block)
+#define PH_MAX_BODIES 64
+
+// Static collision body (identity rotation; the C ABI cannot rotate one).
+struct BodyDev {
+    uint32_t shape;  // 0 sphere, 1 box, 2 capsule (ffi.rs:3193-3195)
+    float px, py, pz;
+    float a, b, c;
+};
+
+// Per-scene parameters: grid domain, bucket range, particle range and the
+// f32 constants the kernels read.  One scene = one World3 fluid.
+struct SceneDev {
+    int32_t x0, y0, z0;  // domain origin (cell coordinates)
+    int32_t nx, ny, nz;  // domain extent (cells)
+    uint32_t bucket_base;  // first global bucket; ncells cells, then outlier, then non-finite
+    uint32_t ncells;
+    uint32_t part_begin, part_end;  // particle slots of this scene
+    float h_grid;   // raw smoothing_length: cells and the d2 < h*h cut (fluid3.rs:321)
+    float h_grid2;  // h_grid * h_grid
+    float h;        // smoothing_length.max(EPS): kernels (fluid3.rs:999)
+    float h2;       // h * h
+    float poly6_c;  // 315 / (64 pi h2 h2 h2 h2 h)      (fluid3.rs:227)
+    float spiky_c;  // -45 / (pi h h h h h h)           (fluid3.rs:237)
+    float rho_max;  // 3 * rest_density.max(EPS)        (fluid3.rs:649)
+    float stiffness, gamma;
+    float phase_rho0[4];
+    float av_alpha, c_s, kappa;
+    float viscosity, max_velocity, particle_radius, wall_mu, wall_friction;
+    float gx, gy, gz;
+    float dt;
+    uint32_t body_begin, body_count;
+    uint32_t pad[2];
+};
+
+struct TileDev {
+    uint32_t scene;
+    uint32_t first;  // first particle slot; the tile covers up to PH_TILE slots
+};
+
+// Rust f32::max: a NaN operand loses.
+__device__ __forceinline__ float rmaxf(float a, float b) {
+    if (a != a) return b;
+    if (b != b) return a;
+    return a > b ? a : b;
+}
+// Rust f32::clamp: NaN passes through.
+__device__ __forceinline__ float rclampf(float v, float lo, float hi) {
+    if (v < lo) v = lo;
+    if (v > hi) v = hi;
+    return v;
+}
+// Rust `f32 as i32`: saturating, NaN -> 0.
+__device__ __forceinline__ int32_t sat_i32(float v) {
+    if (v != v) return 0;
+    if (v >= 2147483648.0f) return INT32_MAX;
+    if (v < -2147483648.0f) return INT32_MIN;
+    return (int32_t)v;
+}
+__device__ __forceinline__ bool finite3(float x, float y, float z) {
+    return isfinite(x) && isfinite(y) && isfinite(z);
+}
+// glam Vec3::length_squared: (x*x + y*y) + z*z.
+__device__ __forceinline__ float dot3(float x, float y, float z) { return x * x + y * y + z * z; }
+
+// fluid3.rs:220-228 (coefficient precomputed on the host in f32).
+__device__ __forceinline__ float poly6(float r2, const SceneDev &s) {
+    if (r2 >= s.h2 || r2 < 0.0f) return 0.0f;
+    float t = s.h2 - r2;
+    return s.poly6_c * t * t * t;
+}
+
+// fluid3.rs:244-251; the Tait power is taken in f64 and rounded.
+__device__ __forceinline__ float eos(float density, float rest, float k, float gamma) {
+    if (gamma == 1.0f) return rmaxf(k * (density - rest), 0.0f);
+    float rho0 = rmaxf(rest, PH_EPS);
+    float pw = (float)pow((double)(density / rho0), (double)gamma);
+    return rmaxf(k * rho0 * (pw - 1.0f), 0.0f);
+}
+
+__device__ __forceinline__ void cell_of(float x, float y, float z, float h, int32_t &cx, int32_t &cy,
+                                        int32_t &cz) {
+    cx = sat_i32(floorf(x / h));
+    cy = sat_i32(floorf(y / h));
+    cz = sat_i32(floorf(z / h));
+}
+
+__device__ __forceinline__ bool in_domain(const SceneDev &s, int64_t x, int64_t y, int64_t z) {
+    return x >= s.x0 && x < (int64_t)s.x0 + s.nx && y >= s.y0 && y < (int64_t)s.y0 + s.ny &&
+           z >= s.z0 && z < (int64_t)s.z0 + s.nz;
+}
+
+// Visit every grid candidate of a particle in the reference's order
+// (fluid3.rs:326-338): cells oz, oy, ox in -1..=1, members by ascending
+// original index.  With a z-major bucket key the three cells ox = -1..1 of
+// one (oy, oz) row are adjacent buckets, so a row is one contiguous slot
+// range of the sorted arrays.  Particles whose 27 cells leave the domain
+// while the scene has outliers take the per-cell path, which also scans
+// the outlier bucket (kept sorted by (cell z, y, x), then index).
+template <class F>
+__device__ __forceinline__ void for_each_candidate(const SceneDev &s, const uint32_t *__restrict__ start,
+                                                   const float4 *__restrict__ posm, int32_t cx,
+                                                   int32_t cy, int32_t cz, F &&f) {
+    const uint32_t base = s.bucket_base;
+    const uint32_t ob = start[base + s.ncells], oe = start[base + s.ncells + 1];
+    bool inside = in_domain(s, (int64_t)cx - 1, (int64_t)cy - 1, (int64_t)cz - 1) &&
+                  in_domain(s, (int64_t)cx + 1, (int64_t)cy + 1, (int64_t)cz + 1);
+    if (inside || ob == oe) {
+        // Fast path: at most 9 contiguous row ranges.  Cells outside the
+        // domain are empty when the scene has no outliers.
+        const int64_t xlo = max((int64_t)cx - 1, (int64_t)s.x0);
+        const int64_t xhi = min((int64_t)cx + 1, (int64_t)s.x0 + s.nx - 1);
+        if (xlo > xhi) return;
+        for (int oz = -1; oz <= 1; ++oz) {
+            const int64_t z = (int64_t)cz + oz;
+            if (z < s.z0 || z >= (int64_t)s.z0 + s.nz) continue;
+            for (int oy = -1; oy <= 1; ++oy) {
+                const int64_t y = (int64_t)cy + oy;
+                if (y < s.y0 || y >= (int64_t)s.y0 + s.ny) continue;
+                const uint32_t row = (uint32_t)(((z - s.z0) * s.ny + (y - s.y0)) * s.nx);
+                const uint32_t b = start[base + row + (uint32_t)(xlo - s.x0)];
+                const uint32_t e = start[base + row + (uint32_t)(xhi - s.x0) + 1];
+                for (uint32_t j = b; j < e; ++j) f(j);
+            }
+        }
+        return;
+    }
+    // Slow path: cell by cell, outlier cells from the outlier bucket.
+    for (int oz = -1; oz <= 1; ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int64_t x = (int64_t)cx + ox, y = (int64_t)cy + oy, z = (int64_t)cz + oz;
+                if (in_domain(s, x, y, z)) {
+                    const uint32_t k = (uint32_t)(((z - s.z0) * s.ny + (y - s.y0)) * s.nx + (x - s.x0));
+                    const uint32_t b = start[base + k], e = start[base + k + 1];
+                    for (uint32_t j = b; j < e; ++j) f(j);
+                } else {
+                    for (uint32_t j = ob; j < oe; ++j) {
+                        const float4 q = posm[j];
+                        int32_t qx, qy, qz;
+                        cell_of(q.x, q.y, q.z, s.h_grid, qx, qy, qz);
+                        if (qx == x && qy == y && qz == z) f(j);
+                    }
+                }
+            }
+}
+
+// fluid3.rs:471-585 for identity-rotation shapes.  Returns false when
+// separated.
+__device__ __forceinline__ bool collide(float px, float py, float pz, float pr, const BodyDev &b,
+                                        float &nx, float &ny, float &nz, float &depth) {
+    if (b.shape == 0u) {  // sphere
+        const float dx = px - b.px, dy = py - b.py, dz = pz - b.pz;
+        const float dist = sqrtf(dot3(dx, dy, dz));
+        if (dist >= b.a + pr) return false;
+        if (dist > PH_EPS) {
+            nx = dx / dist;
+            ny = dy / dist;
+            nz = dz / dist;
+        } else {
+            nx = 1.0f;
+            ny = 0.0f;
+            nz = 0.0f;
+        }
+        depth = b.a + pr - dist;
+        return true;
+    }
+    if (b.shape == 1u) {  // box
+        const float lx = px - b.px, ly = py - b.py, lz = pz - b.pz;
+        const float cxl = rclampf(lx, -b.a, b.a), cyl = rclampf(ly, -b.b, b.b), czl = rclampf(lz, -b.c, b.c);
+        const float dx = lx - cxl, dy = ly - cyl, dz = lz - czl;
+        if (dot3(dx, dy, dz) > PH_EPS_SQ) {
+            const float dist = sqrtf(dot3(dx, dy, dz));
+            if (dist >= pr) return false;
+            nx = dx / dist;
+            ny = dy / dist;
+            nz = dz / dist;
+            depth = pr - dist;
+            return true;
+        }
+        // Centre inside: least-penetration axis, always +axis (fluid3.rs:514-520).
+        const float ex = b.a - fabsf(lx), ey = b.b - fabsf(ly), ez = b.c - fabsf(lz);
+        nx = ny = nz = 0.0f;
+        if (ex <= ey && ex <= ez) {
+            nx = 1.0f;
+            depth = pr + ex;
+        } else if (ey <= ez) {
+            ny = 1.0f;
+            depth = pr + ey;
+        } else {
+            nz = 1.0f;
+            depth = pr + ez;
+        }
+        return true;
+    }
+    if (b.shape == 2u) {  // capsule along local y
+        const float lx = px - b.px, ly = py - b.py, lz = pz - b.pz;
+        const float sy = rclampf(ly, -b.b, b.b);
+        const float dx = lx - 0.0f, dy = ly - sy, dz = lz - 0.0f;
+        if (dot3(dx, dy, dz) > PH_EPS_SQ) {
+            const float dist = sqrtf(dot3(dx, dy, dz));
+            if (dist >= b.a + pr) return false;
+            nx = dx / dist;
+            ny = dy / dist;
+            nz = dz / dist;
+            depth = b.a + pr - dist;
+            return true;
+        }
+        if (dot3(lx, 0.0f, lz) > PH_EPS_SQ) {
+            const float rcp = 1.0f / sqrtf(dot3(lx, 0.0f, lz));  // glam normalize_or_zero
+            if (isfinite(rcp) && rcp > 0.0f) {
+                nx = lx * rcp;
+                ny = 0.0f * rcp;
+                nz = lz * rcp;
+            } else {
+                nx = ny = nz = 0.0f;
+            }
+        } else {
+            nx = 1.0f;
+            ny = 0.0f;
+            nz = 0.0f;
+        }
+        depth = pr + b.a - sqrtf(dot3(dx, dy, dz));
+        return true;
+    }
+    return false;
+}
+
+// fluid3.rs:1141-1179 for one particle against the scene's static bodies.
+__device__ __forceinline__ void clamp_to_bodies(const SceneDev &s, const BodyDev *__restrict__ bodies,
+                                                float &x, float &y, float &z, float &vx, float &vy,
+                                                float &vz) {
+    const float pr = rmaxf(s.particle_radius, 0.0f);
+    for (uint32_t k = 0; k < s.body_count; ++k) {
+        const BodyDev b = bodies[s.body_begin + k];
+        float nx, ny, nz, depth;
+        if (!collide(x, y, z, pr, b, nx, ny, nz, depth)) continue;
+        if (depth <= 0.0f) continue;
+        x = x + nx * depth;
+        y = y + ny * depth;
+        z = z + nz * depth;
+        const float vn = vx * nx + vy * ny + vz * nz;
+        if (vn < 0.0f) {
+            vx = vx - nx * vn;
+            vy = vy - ny * vn;
+            vz = vz - nz * vn;
+            if (s.wall_friction > 0.0f) {
+                const float mu = s.wall_mu;
+                // Static wall: velocity zero, so wall_vt = 0 - n * (0 . n).
+                const float wd = 0.0f * nx + 0.0f * ny + 0.0f * nz;
+                const float wtx = 0.0f - nx * wd, wty = 0.0f - ny * wd, wtz = 0.0f - nz * wd;
+                const float dvx = (wtx - vx) * mu, dvy = (wty - vy) * mu, dvz = (wtz - vz) * mu;
+                vx = vx + dvx;
+                vy = vy + dvy;
+                vz = vz + dvz;
+            }
+        }
+    }
+}

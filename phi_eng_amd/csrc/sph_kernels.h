@@ -1,0 +1,44 @@
+// sph_kernels.h — host launchers of the SPH kernels (sph_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sph_device.h"
+
+namespace ph {
+
+uint32_t scan_partials_needed(uint32_t nbuckets);
+
+hipError_t launch_cell_key(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                           const float4 *posm, const uint32_t *tag, uint32_t *keys, uint32_t *slots,
+                           uint32_t *counts, int32_t *rec_cells);
+hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t nb, uint32_t *partial);
+hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start,
+                                int32_t *bbox, uint32_t nscenes);
+hipError_t launch_scan_apply(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
+                             uint32_t *start);
+hipError_t launch_scatter(hipStream_t st, uint32_t n, const uint32_t *keys, const uint32_t *slots,
+                          const uint32_t *start, uint32_t *tmp);
+hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                              const uint32_t *tmp, const uint32_t *keys, const uint32_t *start,
+                              const float4 *posm_in, const float4 *vel_in, const uint32_t *tag_in,
+                              float4 *posm_out, float4 *vel_out, uint32_t *tag_out, uint32_t *rec_sorted);
+hipError_t launch_density(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                          const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
+                          float *rec_rho, float *rec_p, uint32_t *rec_count);
+hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                                  const BodyDev *bodies, const uint32_t *start, const float4 *posm,
+                                  const float4 *vel, const float2 *rq, const uint32_t *tag, float4 *posm_out,
+                                  float4 *vel_out, float *rec_acc, int32_t *bbox);
+hipError_t launch_vmax(hipStream_t st, uint32_t n, const float4 *vel, uint32_t *out_bits);
+hipError_t launch_export(hipStream_t st, uint32_t n, const float4 *posm, const float4 *vel, const uint32_t *tag,
+                         uint32_t id_base, float *out6);
+hipError_t launch_set_velocity(hipStream_t st, uint32_t n, const uint32_t *tag, const float *vel3,
+                               float4 *vel);
+hipError_t launch_append(hipStream_t st, uint32_t n0, uint32_t k, const float *xyz, const uint8_t *phase,
+                         float4 masses, float4 *posm, float4 *vel, uint32_t *tag);
+hipError_t launch_cell_bbox(hipStream_t st, uint32_t n, const float4 *posm, float h, int32_t *bb);
+hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
+hipError_t launch_monitor(hipStream_t st, const SceneDev *scenes, uint32_t nscenes, const uint32_t *start,
+                          const int32_t *bbox, int32_t *out);
+
+}  // namespace ph

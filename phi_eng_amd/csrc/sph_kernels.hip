@@ -1,0 +1,591 @@
+// sph_kernels.hip — HIP kernels of the SPH fluid step for gfx950 (CDNA4).
+//
+// One substep (crates/dynamics/src/fluid3.rs:990-1184) is
+//   K1 cell_key      cell triple -> z-major bucket, slot = atomic count
+//   K2 scan (3)      exclusive scan of bucket counts -> bucket start table
+//   K3 scatter       slot table: sorted position (unranked) -> source slot
+//   K3b rank_gather  rank inside each bucket by original index (reference
+//                    bucket order, fluid3.rs:301-314), gather state
+//   K4 density       rho, p, p/rho^2 per particle (fluid3.rs:639-655)
+//   K5 force_integ.  pressure force + XSPH + integrate + boundary clamp
+//                    (fluid3.rs:1029-1179, Jacobi: reads pre-step state)
+// All per-particle kernels run over tiles of PH_TILE consecutive slots of
+// one scene, so a batch of scenes uses the same kernels as one world.
+#include "sph_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace {
+
+constexpr int kScanItems = 8;
+constexpr int kScanBlock = 256;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan across a 256-thread block; returns the thread's prefix
+// and the block total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &total) {
+    __shared__ uint32_t wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+    for (int k = 0; k < w; ++k) off += wsum[k];
+    total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return off + inc - v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- K1
+__global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict__ tiles,
+                                                      const SceneDev *__restrict__ scenes,
+                                                      const float4 *__restrict__ posm,
+                                                      const uint32_t *__restrict__ tag,
+                                                      uint32_t *__restrict__ keys,
+                                                      uint32_t *__restrict__ slots,
+                                                      uint32_t *__restrict__ counts,
+                                                      int32_t *__restrict__ rec_cells) {
+    const TileDev t = tiles[blockIdx.x];
+    const SceneDev &s = scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    if (i >= s.part_end) return;
+    const float4 p = posm[i];
+    int32_t cx, cy, cz;
+    cell_of(p.x, p.y, p.z, s.h_grid, cx, cy, cz);
+    uint32_t lk;
+    if (!finite3(p.x, p.y, p.z)) {
+        lk = s.ncells + 1;  // left out of the grid (fluid3.rs:302-304)
+    } else if (in_domain(s, cx, cy, cz)) {
+        lk = (uint32_t)((((int64_t)cz - s.z0) * s.ny + ((int64_t)cy - s.y0)) * s.nx + ((int64_t)cx - s.x0));
+    } else {
+        lk = s.ncells;  // outlier bucket
+    }
+    const uint32_t key = s.bucket_base + lk;
+    keys[i] = key;
+    slots[i] = atomicAdd(&counts[key], 1u);
+    if (rec_cells) {
+        const uint32_t id = tag[i] & PH_ID_MASK;
+        rec_cells[3 * id] = cx;
+        rec_cells[3 * id + 1] = cy;
+        rec_cells[3 * id + 2] = cz;
+    }
+}
+
+// ---------------------------------------------------------------- K2
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__restrict__ counts,
+                                                            uint32_t nb, uint32_t *__restrict__ partial) {
+    const uint32_t base = blockIdx.x * (kScanBlock * kScanItems) + threadIdx.x * kScanItems;
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < nb) v += counts[base + k];
+    uint32_t tot;
+    block_excl_scan(v, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// Single block: exclusive scan of the block partials, start[nb] = total.
+// Also resets the per-scene cell bounding boxes the integrate pass fills.
+__global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t *__restrict__ partial, uint32_t np,
+                                                              uint32_t *__restrict__ start, uint32_t nb,
+                                                              int32_t *__restrict__ bbox, uint32_t nscenes) {
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < np; c += kScanBlock) {
+        const uint32_t k = c + threadIdx.x;
+        const uint32_t v = k < np ? partial[k] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, tot);
+        if (k < np) partial[k] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) start[nb] = carry;
+    for (uint32_t k = threadIdx.x; k < nscenes; k += kScanBlock) {
+        bbox[6 * k + 0] = INT32_MAX;
+        bbox[6 * k + 1] = INT32_MAX;
+        bbox[6 * k + 2] = INT32_MAX;
+        bbox[6 * k + 3] = INT32_MIN;
+        bbox[6 * k + 4] = INT32_MIN;
+        bbox[6 * k + 5] = INT32_MIN;
+    }
+}
+
+// Bucket start = block offset + in-block exclusive scan; counts are zeroed
+// for the next substep's atomics.
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t *__restrict__ counts, uint32_t nb,
+                                                           const uint32_t *__restrict__ partial,
+                                                           uint32_t *__restrict__ start) {
+    const uint32_t base = blockIdx.x * (kScanBlock * kScanItems) + threadIdx.x * kScanItems;
+    uint32_t c[kScanItems];
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        c[k] = base + k < nb ? counts[base + k] : 0u;
+        v += c[k];
+    }
+    uint32_t tot;
+    uint32_t off = block_excl_scan(v, tot) + partial[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < nb) {
+            start[base + k] = off;
+            counts[base + k] = 0u;
+        }
+        off += c[k];
+    }
+}
+
+// ---------------------------------------------------------------- K3
+__global__ __launch_bounds__(PH_TILE) void k_scatter(uint32_t n, const uint32_t *__restrict__ keys,
+                                                     const uint32_t *__restrict__ slots,
+                                                     const uint32_t *__restrict__ start,
+                                                     uint32_t *__restrict__ tmp) {
+    const uint32_t i = blockIdx.x * PH_TILE + threadIdx.x;
+    if (i >= n) return;
+    tmp[start[keys[i]] + slots[i]] = i;
+}
+
+// Final slot = bucket start + rank of the particle inside its bucket.  In
+// a grid cell the rank orders by original index, which is the order the
+// reference's bucket Vec holds (fluid3.rs:301-314); in the outlier bucket
+// it orders by (cell z, y, x) first so every outlier cell is contiguous.
+__global__ __launch_bounds__(PH_TILE) void k_rank_gather(
+    const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const uint32_t *__restrict__ tmp,
+    const uint32_t *__restrict__ keys, const uint32_t *__restrict__ start, const float4 *__restrict__ posm_in,
+    const float4 *__restrict__ vel_in, const uint32_t *__restrict__ tag_in, float4 *__restrict__ posm_out,
+    float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted) {
+    const TileDev t = tiles[blockIdx.x];
+    const SceneDev &s = scenes[t.scene];
+    const uint32_t q = t.first + threadIdx.x;
+    if (q >= s.part_end) return;
+    const uint32_t src = tmp[q];
+    const uint32_t key = keys[src];
+    const uint32_t b = start[key], e = start[key + 1];
+    const uint32_t mytag = tag_in[src];
+    const uint32_t myid = mytag & PH_ID_MASK;
+    uint32_t rank = 0;
+    if (key - s.bucket_base == s.ncells) {
+        const float4 mp = posm_in[src];
+        int32_t mx, my, mz;
+        cell_of(mp.x, mp.y, mp.z, s.h_grid, mx, my, mz);
+        for (uint32_t k = b; k < e; ++k) {
+            const uint32_t o = tmp[k];
+            const float4 op = posm_in[o];
+            int32_t ox, oy, oz;
+            cell_of(op.x, op.y, op.z, s.h_grid, ox, oy, oz);
+            const uint32_t oid = tag_in[o] & PH_ID_MASK;
+            const bool less = oz != mz ? oz < mz : (oy != my ? oy < my : (ox != mx ? ox < mx : oid < myid));
+            rank += less ? 1u : 0u;
+        }
+    } else {
+        for (uint32_t k = b; k < e; ++k) rank += (tag_in[tmp[k]] & PH_ID_MASK) < myid ? 1u : 0u;
+    }
+    const uint32_t d = b + rank;
+    posm_out[d] = posm_in[src];
+    vel_out[d] = vel_in[src];
+    tag_out[d] = mytag;
+    if (rec_sorted) rec_sorted[d] = myid;
+}
+
+// ---------------------------------------------------------------- K4
+// rq[i] = (rho_i, p_i / (max(rho_i, EPS))^2): the pressure quotient the
+// force pass needs (fluid3.rs:1046-1051), computed once per particle.
+__global__ __launch_bounds__(PH_TILE) void k_density(const TileDev *__restrict__ tiles,
+                                                     const SceneDev *__restrict__ scenes,
+                                                     const uint32_t *__restrict__ start,
+                                                     const float4 *__restrict__ posm,
+                                                     const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
+                                                     float *__restrict__ rec_rho, float *__restrict__ rec_p,
+                                                     uint32_t *__restrict__ rec_count) {
+    const TileDev t = tiles[blockIdx.x];
+    const SceneDev &s = scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    if (i >= s.part_end) return;
+    const float4 pi = posm[i];
+    float rho = 0.0f;
+    uint32_t cnt = 0;
+    if (finite3(pi.x, pi.y, pi.z)) {
+        int32_t cx, cy, cz;
+        cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
+        const float hg2 = s.h_grid2;
+        for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
+            const float4 pj = posm[j];
+            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+            if (r2 < hg2) {
+                ++cnt;
+                rho += pj.w * poly6(r2, s);
+            }
+        });
+    }
+    rho = rclampf(rho, 0.0f, s.rho_max);
+    const uint32_t tg = tag[i];
+    const float rho0 = s.phase_rho0[min(tg >> 24, 3u)];
+    const float p = eos(rho, rho0, s.stiffness, s.gamma);
+    const float rc = rmaxf(rho, PH_EPS);
+    rq[i] = make_float2(rho, p / (rc * rc));
+    if (rec_rho) {
+        const uint32_t id = tg & PH_ID_MASK;
+        rec_rho[id] = rho;
+        rec_p[id] = p;
+        rec_count[id] = cnt;
+    }
+}
+
+// ---------------------------------------------------------------- K5
+// Force (fluid3.rs:1035-1086) and XSPH (fluid3.rs:1110-1124) share one
+// neighbour sweep over the pre-step state; then integrate, speed cap,
+// finite guards and the boundary clamp of this particle.
+__global__ __launch_bounds__(PH_TILE) void k_force_integrate(
+    const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const BodyDev *__restrict__ bodies,
+    const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
+    const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, float4 *__restrict__ posm_out,
+    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox) {
+    const TileDev t = tiles[blockIdx.x];
+    const SceneDev &s = scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    const bool active = i < s.part_end;
+    int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bz0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN, bz1 = INT32_MIN;
+    if (active) {
+        const float4 pi = posm[i];
+        const float4 vi = vel[i];
+        const float2 rqi = rq[i];
+        const float rho_i = rmaxf(rqi.x, PH_EPS);
+        const float q_i = rqi.y;
+        const float h = s.h;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        if (finite3(pi.x, pi.y, pi.z)) {
+            int32_t cx, cy, cz;
+            cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
+            const float hg2 = s.h_grid2;
+            for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
+                if (j == i) return;
+                const float4 pj = posm[j];
+                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                const float r2 = dot3(dx, dy, dz);
+                if (!(r2 < hg2)) return;
+                const float2 rqj = rq[j];
+                const float rho_j = rmaxf(rqj.x, PH_EPS);
+                // delta = x_i - x_j = -(dx, dy, dz); |delta|^2 == r2 exactly.
+                const float ex = -dx, ey = -dy, ez = -dz;
+                const float dist = sqrtf(r2);
+                float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                if (!(dist >= h || dist < PH_EPS)) {
+                    const float sc = s.spiky_c * (h - dist) * (h - dist);
+                    gx = ex / dist * sc;
+                    gy = ey / dist * sc;
+                    gz = ez / dist * sc;
+                }
+                float term = q_i + rqj.y;
+                if (s.av_alpha > 0.0f && dist > PH_EPS) {
+                    // fluid3.rs:254-269 (artificial_pressure)
+                    const float4 vj = vel[j];
+                    const float rho_bar = 0.5f * (rho_i + rho_j);
+                    const float vd = (vi.x - vj.x) * ex + (vi.y - vj.y) * ey + (vi.z - vj.z) * ez;
+                    float ap = 0.0f;
+                    if (!(vd >= 0.0f)) {
+                        const float mu = h * vd / (dist * dist + 0.01f * h * h);
+                        ap = -s.av_alpha * s.c_s * mu / rmaxf(rho_bar, PH_EPS);
+                    }
+                    term = term + ap;
+                }
+                const float sm = pj.w * term;
+                fx = fx - gx * sm;
+                fy = fy - gy * sm;
+                fz = fz - gz * sm;
+                if (s.kappa > 0.0f && dist > PH_EPS) {
+                    const float coh = dist >= h ? 0.0f : 1.0f - dist / h;
+                    if (coh > 0.0f) {
+                        const float cs = s.kappa * pi.w * pj.w * coh;
+                        fx = fx - ex / dist * cs;
+                        fy = fy - ey / dist * cs;
+                        fz = fz - ez / dist * cs;
+                    }
+                }
+                if (s.viscosity > 0.0f) {
+                    const float4 vj = vel[j];
+                    const float rho_bar = 0.5f * (rho_i + rho_j);
+                    const float qm = pj.w / rho_bar;
+                    const float w = poly6(r2, s);
+                    sx = sx + (vj.x - vi.x) * qm * w;
+                    sy = sy + (vj.y - vi.y) * qm * w;
+                    sz = sz + (vj.z - vi.z) * qm * w;
+                }
+            });
+        }
+        // fluid3.rs:1095 acc = g + F / rho (the second division by rho).
+        const float ax = s.gx + fx / rho_i, ay = s.gy + fy / rho_i, az = s.gz + fz / rho_i;
+        float ux = vi.x + ax * s.dt, uy = vi.y + ay * s.dt, uz = vi.z + az * s.dt;
+        if (s.viscosity > 0.0f) {
+            ux = ux + sx * s.viscosity;
+            uy = uy + sy * s.viscosity;
+            uz = uz + sz * s.viscosity;
+        }
+        const float speed = sqrtf(dot3(ux, uy, uz));
+        if (speed > s.max_velocity) {
+            const float r = s.max_velocity / speed;
+            ux = ux * r;
+            uy = uy * r;
+            uz = uz * r;
+        }
+        if (!finite3(ux, uy, uz)) ux = uy = uz = 0.0f;
+        float x = pi.x + ux * s.dt, y = pi.y + uy * s.dt, z = pi.z + uz * s.dt;
+        if (!finite3(x, y, z)) {
+            x = pi.x;
+            y = pi.y;
+            z = pi.z;
+        }
+        clamp_to_bodies(s, bodies, x, y, z, ux, uy, uz);
+        posm_out[i] = make_float4(x, y, z, pi.w);
+        vel_out[i] = make_float4(ux, uy, uz, 0.0f);
+        if (rec_acc) {
+            const uint32_t id = tag[i] & PH_ID_MASK;
+            rec_acc[3 * id] = ax;
+            rec_acc[3 * id + 1] = ay;
+            rec_acc[3 * id + 2] = az;
+        }
+        if (finite3(x, y, z)) {
+            cell_of(x, y, z, s.h_grid, bx0, by0, bz0);
+            bx1 = bx0;
+            by1 = by0;
+            bz1 = bz0;
+        }
+    }
+    // Cell bounding box of the advanced particles (domain monitor): wave
+    // min/max, one atomic per wave.
+    for (int d = 32; d >= 1; d >>= 1) {
+        bx0 = min(bx0, __shfl_xor(bx0, d, 64));
+        by0 = min(by0, __shfl_xor(by0, d, 64));
+        bz0 = min(bz0, __shfl_xor(bz0, d, 64));
+        bx1 = max(bx1, __shfl_xor(bx1, d, 64));
+        by1 = max(by1, __shfl_xor(by1, d, 64));
+        bz1 = max(bz1, __shfl_xor(bz1, d, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && bx0 != INT32_MAX) {
+        int32_t *bb = bbox + 6 * t.scene;
+        atomicMin(bb + 0, bx0);
+        atomicMin(bb + 1, by0);
+        atomicMin(bb + 2, bz0);
+        atomicMax(bb + 3, bx1);
+        atomicMax(bb + 4, by1);
+        atomicMax(bb + 5, bz1);
+    }
+}
+
+// ---------------------------------------------------------------- misc
+// Largest |v| (fluid3.rs:627-629) as float bits (non-negative floats
+// order like their bit patterns); NaN speeds are skipped as f32::max does.
+__global__ __launch_bounds__(PH_TILE) void k_vmax(uint32_t n, const float4 *__restrict__ vel,
+                                                  uint32_t *__restrict__ out_bits) {
+    const uint32_t i = blockIdx.x * PH_TILE + threadIdx.x;
+    uint32_t b = 0;
+    if (i < n) {
+        const float4 v = vel[i];
+        const float sp = sqrtf(dot3(v.x, v.y, v.z));
+        if (sp == sp) b = __float_as_uint(sp);
+    }
+    for (int d = 32; d >= 1; d >>= 1) b = max(b, (uint32_t)__shfl_xor((int)b, d, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, b);
+}
+
+// State in insertion order: out6[id] = (x, y, z, vx, vy, vz).
+__global__ __launch_bounds__(PH_TILE) void k_export(uint32_t n, const float4 *__restrict__ posm,
+                                                    const float4 *__restrict__ vel,
+                                                    const uint32_t *__restrict__ tag, uint32_t id_base,
+                                                    float *__restrict__ out6) {
+    const uint32_t i = blockIdx.x * PH_TILE + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = posm[i], v = vel[i];
+    const uint32_t id = (tag[i] & PH_ID_MASK) - id_base;
+    float *o = out6 + 6 * (size_t)id;
+    o[0] = p.x;
+    o[1] = p.y;
+    o[2] = p.z;
+    o[3] = v.x;
+    o[4] = v.y;
+    o[5] = v.z;
+}
+
+// Overwrite velocities from insertion-ordered vel3.
+__global__ __launch_bounds__(PH_TILE) void k_set_velocity(uint32_t n, const uint32_t *__restrict__ tag,
+                                                          const float *__restrict__ vel3,
+                                                          float4 *__restrict__ vel) {
+    const uint32_t i = blockIdx.x * PH_TILE + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = tag[i] & PH_ID_MASK;
+    vel[i] = make_float4(vel3[3 * id], vel3[3 * id + 1], vel3[3 * id + 2], 0.0f);
+}
+
+// Append particles [n0, n0 + k): zero velocity, mass and tag from the
+// phase tables (fluid3.rs:196-212).
+__global__ __launch_bounds__(PH_TILE) void k_append(uint32_t n0, uint32_t k, const float *__restrict__ xyz,
+                                                    const uint8_t *__restrict__ phase, float4 m4lo,
+                                                    float4 *__restrict__ posm, float4 *__restrict__ vel,
+                                                    uint32_t *__restrict__ tag) {
+    const uint32_t i = blockIdx.x * PH_TILE + threadIdx.x;
+    if (i >= k) return;
+    const uint32_t ph = phase ? phase[i] : 0u;
+    const uint32_t pc = min(ph, 3u);
+    const float m = pc == 0 ? m4lo.x : (pc == 1 ? m4lo.y : (pc == 2 ? m4lo.z : m4lo.w));
+    posm[n0 + i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], m);
+    vel[n0 + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    tag[n0 + i] = (ph << 24) | (n0 + i);
+}
+
+// Cell bounding box of finite particles (domain set-up).
+__global__ __launch_bounds__(PH_TILE) void k_cell_bbox(uint32_t n, const float4 *__restrict__ posm, float h,
+                                                       int32_t *__restrict__ bb) {
+    const uint32_t i = blockIdx.x * PH_TILE + threadIdx.x;
+    int32_t x0 = INT32_MAX, y0 = INT32_MAX, z0 = INT32_MAX, x1 = INT32_MIN, y1 = INT32_MIN, z1 = INT32_MIN;
+    if (i < n) {
+        const float4 p = posm[i];
+        if (finite3(p.x, p.y, p.z)) {
+            cell_of(p.x, p.y, p.z, h, x0, y0, z0);
+            x1 = x0;
+            y1 = y0;
+            z1 = z0;
+        }
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        x0 = min(x0, __shfl_xor(x0, d, 64));
+        y0 = min(y0, __shfl_xor(y0, d, 64));
+        z0 = min(z0, __shfl_xor(z0, d, 64));
+        x1 = max(x1, __shfl_xor(x1, d, 64));
+        y1 = max(y1, __shfl_xor(y1, d, 64));
+        z1 = max(z1, __shfl_xor(z1, d, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && x0 != INT32_MAX) {
+        atomicMin(bb + 0, x0);
+        atomicMin(bb + 1, y0);
+        atomicMin(bb + 2, z0);
+        atomicMax(bb + 3, x1);
+        atomicMax(bb + 4, y1);
+        atomicMax(bb + 5, z1);
+    }
+}
+
+__global__ __launch_bounds__(PH_TILE) void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * PH_TILE + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Per-scene outlier counts + bounding boxes into one small array the
+// host copies back asynchronously (domain monitor).
+__global__ void k_monitor(const SceneDev *__restrict__ scenes, uint32_t nscenes,
+                          const uint32_t *__restrict__ start, const int32_t *__restrict__ bbox,
+                          int32_t *__restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nscenes) return;
+    const SceneDev &s = scenes[k];
+    out[8 * k] = (int32_t)(start[s.bucket_base + s.ncells + 1] - start[s.bucket_base + s.ncells]);
+    out[8 * k + 1] = (int32_t)(start[s.bucket_base + s.ncells + 2] - start[s.bucket_base + s.ncells + 1]);
+    for (int c = 0; c < 6; ++c) out[8 * k + 2 + c] = bbox[6 * k + c];
+}
+
+// ---------------------------------------------------------------- launchers
+namespace ph {
+
+static inline uint32_t blocks_for(uint32_t n) { return (n + PH_TILE - 1) / PH_TILE; }
+
+hipError_t launch_cell_key(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                           const float4 *posm, const uint32_t *tag, uint32_t *keys, uint32_t *slots,
+                           uint32_t *counts, int32_t *rec_cells) {
+    hipLaunchKernelGGL(k_cell_key, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, posm, tag, keys, slots,
+                       counts, rec_cells);
+    return hipGetLastError();
+}
+
+uint32_t scan_partials_needed(uint32_t nbuckets) {
+    return (nbuckets + kScanBlock * kScanItems - 1) / (kScanBlock * kScanItems);
+}
+
+hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t nb, uint32_t *partial) {
+    hipLaunchKernelGGL(k_scan_reduce, dim3(scan_partials_needed(nb)), dim3(kScanBlock), 0, st, counts, nb,
+                       partial);
+    return hipGetLastError();
+}
+hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start,
+                                int32_t *bbox, uint32_t nscenes) {
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlock), 0, st, partial, scan_partials_needed(nb),
+                       start, nb, bbox, nscenes);
+    return hipGetLastError();
+}
+hipError_t launch_scan_apply(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
+                             uint32_t *start) {
+    hipLaunchKernelGGL(k_scan_apply, dim3(scan_partials_needed(nb)), dim3(kScanBlock), 0, st, counts, nb,
+                       partial, start);
+    return hipGetLastError();
+}
+hipError_t launch_scatter(hipStream_t st, uint32_t n, const uint32_t *keys, const uint32_t *slots,
+                          const uint32_t *start, uint32_t *tmp) {
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, n, keys, slots, start, tmp);
+    return hipGetLastError();
+}
+hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                              const uint32_t *tmp, const uint32_t *keys, const uint32_t *start,
+                              const float4 *posm_in, const float4 *vel_in, const uint32_t *tag_in,
+                              float4 *posm_out, float4 *vel_out, uint32_t *tag_out, uint32_t *rec_sorted) {
+    hipLaunchKernelGGL(k_rank_gather, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, tmp, keys, start,
+                       posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted);
+    return hipGetLastError();
+}
+hipError_t launch_density(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                          const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
+                          float *rec_rho, float *rec_p, uint32_t *rec_count) {
+    hipLaunchKernelGGL(k_density, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, rq,
+                       rec_rho, rec_p, rec_count);
+    return hipGetLastError();
+}
+hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                                  const BodyDev *bodies, const uint32_t *start, const float4 *posm,
+                                  const float4 *vel, const float2 *rq, const uint32_t *tag, float4 *posm_out,
+                                  float4 *vel_out, float *rec_acc, int32_t *bbox) {
+    hipLaunchKernelGGL(k_force_integrate, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm,
+                       vel, rq, tag, posm_out, vel_out, rec_acc, bbox);
+    return hipGetLastError();
+}
+hipError_t launch_vmax(hipStream_t st, uint32_t n, const float4 *vel, uint32_t *out_bits) {
+    hipLaunchKernelGGL(k_vmax, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, n, vel, out_bits);
+    return hipGetLastError();
+}
+hipError_t launch_export(hipStream_t st, uint32_t n, const float4 *posm, const float4 *vel, const uint32_t *tag,
+                         uint32_t id_base, float *out6) {
+    hipLaunchKernelGGL(k_export, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, n, posm, vel, tag, id_base, out6);
+    return hipGetLastError();
+}
+hipError_t launch_set_velocity(hipStream_t st, uint32_t n, const uint32_t *tag, const float *vel3,
+                               float4 *vel) {
+    hipLaunchKernelGGL(k_set_velocity, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, n, tag, vel3, vel);
+    return hipGetLastError();
+}
+hipError_t launch_append(hipStream_t st, uint32_t n0, uint32_t k, const float *xyz, const uint8_t *phase,
+                         float4 masses, float4 *posm, float4 *vel, uint32_t *tag) {
+    hipLaunchKernelGGL(k_append, dim3(blocks_for(k)), dim3(PH_TILE), 0, st, n0, k, xyz, phase, masses, posm,
+                       vel, tag);
+    return hipGetLastError();
+}
+hipError_t launch_cell_bbox(hipStream_t st, uint32_t n, const float4 *posm, float h, int32_t *bb) {
+    hipLaunchKernelGGL(k_cell_bbox, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, n, posm, h, bb);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_monitor(hipStream_t st, const SceneDev *scenes, uint32_t nscenes, const uint32_t *start,
+                          const int32_t *bbox, int32_t *out) {
+    hipLaunchKernelGGL(k_monitor, dim3((nscenes + 63) / 64), dim3(64), 0, st, scenes, nscenes, start, bbox, out);
+    return hipGetLastError();
+}
+
+}  // namespace ph

@@ -1,0 +1,501 @@
+// world3.hip — the PhiEng World3 fluid C ABI over the GPU engine.
+//
+// Each entry point keeps the reference's argument meaning and error
+// behaviour (crates/engine/src/ffi.rs, cited per function); nothing throws
+// across the boundary: HIP and allocation failures become the same return
+// codes the reference uses, with the message kept for phieng_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/phieng_fluid3.h"
+#include "engine.h"
+#include "sph_kernels.h"
+
+static_assert(sizeof(FluidConfig3Ffi) == 112, "FluidConfig3Ffi must match ffi.rs:743-767");
+static_assert(offsetof(FluidConfig3Ffi, max_substeps) == 32, "layout");
+static_assert(offsetof(FluidConfig3Ffi, adaptive_resolution) == 36, "layout");
+static_assert(offsetof(FluidConfig3Ffi, boundary_particles) == 48, "layout");
+static_assert(offsetof(FluidConfig3Ffi, phase_count) == 64, "layout");
+static_assert(offsetof(FluidConfig3Ffi, phase_rest_density) == 68, "layout");
+static_assert(offsetof(FluidConfig3Ffi, phase_mass) == 84, "layout");
+static_assert(offsetof(FluidConfig3Ffi, max_velocity) == 100, "layout");
+static_assert(offsetof(FluidConfig3Ffi, enabled) == 108, "layout");
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const char *where, const char *what) {
+    g_last_error = std::string(where) + ": " + what;
+}
+
+constexpr uint32_t kDefaultCapacity = 4096;        // MAX_PARTICLES_3D, fluid3.rs:28
+constexpr uint32_t kMaxPerCall = 1000000;          // ffi.rs:3587
+constexpr uint32_t kMaxCapacity = PH_ID_MASK + 1;  // 24-bit original index
+
+// FluidConfig3::default (fluid3.rs:105-134).
+FluidConfig3Ffi default_config() {
+    FluidConfig3Ffi c;
+    std::memset(&c, 0, sizeof(c));
+    c.particle_mass = 0.0625f;
+    c.rest_density = 1.0f;
+    c.smoothing_length = 0.5f;
+    c.stiffness = 100.0f;
+    c.gamma = 1.0f;
+    c.artificial_viscosity = 0.0f;
+    c.surface_tension = 0.0f;
+    c.cfl = 0.0f;
+    c.max_substeps = 1;
+    c.adaptive_resolution = 0;
+    c.split_density_ratio = 0.5f;
+    c.merge_density_ratio = 1.6f;
+    c.boundary_particles = 0;
+    c.viscosity = 0.3f;
+    c.drag = 4.0f;
+    c.wall_friction = 0.0f;
+    c.phase_count = 1;
+    for (int k = 0; k < 4; ++k) {
+        c.phase_rest_density[k] = 1.0f;
+        c.phase_mass[k] = 0.0625f;
+    }
+    c.max_velocity = 30.0f;
+    c.particle_radius = 0.125f;
+    c.enabled = 1;
+    return c;
+}
+
+// fluid3_from_ffi (ffi.rs:857-914).
+bool config_valid(const FluidConfig3Ffi &c) {
+    const float f[] = {c.particle_mass, c.rest_density, c.smoothing_length, c.stiffness, c.gamma,
+                       c.artificial_viscosity, c.surface_tension, c.cfl, c.split_density_ratio,
+                       c.merge_density_ratio, c.viscosity, c.drag, c.wall_friction, c.max_velocity,
+                       c.particle_radius, c.phase_rest_density[0], c.phase_rest_density[1],
+                       c.phase_rest_density[2], c.phase_rest_density[3], c.phase_mass[0], c.phase_mass[1],
+                       c.phase_mass[2], c.phase_mass[3]};
+    for (float v : f)
+        if (!std::isfinite(v)) return false;
+    if (c.particle_mass <= 0.0f || c.rest_density <= 0.0f || c.smoothing_length <= 0.0f || c.stiffness < 0.0f ||
+        c.gamma < 1.0f || c.artificial_viscosity < 0.0f || c.artificial_viscosity > 2.0f ||
+        c.surface_tension < 0.0f || c.cfl < 0.0f || c.cfl > 1.0f || c.max_substeps < 1 ||
+        c.split_density_ratio <= 0.0f || c.merge_density_ratio <= 1.0f || c.viscosity < 0.0f ||
+        c.viscosity > 1.0f || c.drag < 0.0f || c.wall_friction < 0.0f || c.wall_friction > 1.0f ||
+        c.phase_count < 1 || c.phase_count > 4 || c.max_velocity < 0.0f || c.particle_radius < 0.0f)
+        return false;
+    for (int k = 0; k < 4; ++k)
+        if (c.phase_rest_density[k] <= 0.0f || c.phase_mass[k] <= 0.0f) return false;
+    return true;
+}
+
+}  // namespace
+
+struct World3 {
+    FluidConfig3Ffi cfg = default_config();
+    float g[3] = {0.0f, -9.81f, 0.0f};  // World3::new (world.rs)
+    std::vector<BodyDev> bodies;
+    int32_t next_body_id = 0;
+    float time = 0.0f;
+    uint32_t capacity = kDefaultCapacity;
+    uint32_t n = 0;
+    bool domain_dirty = true;
+    ph::Engine eng;
+
+    ph::SceneHost scene_host(float dt) const {
+        ph::SceneHost h;
+        h.cfg = cfg;
+        h.g[0] = g[0];
+        h.g[1] = g[1];
+        h.g[2] = g[2];
+        h.dt = dt;
+        h.bodies = bodies;
+        return h;
+    }
+
+    // Bring the engine's single scene in line with the host state.
+    void sync_scene(float dt) {
+        ph::Engine &e = eng;
+        if (e.scenes.size() != 1) {
+            e.scenes.assign(1, SceneDev{});
+            domain_dirty = true;
+        }
+        SceneDev &s = e.scenes[0];
+        SceneDev before = s;
+        ph::fill_scene_physics(s, scene_host(dt));
+        s.part_begin = 0;
+        s.part_end = n;
+        s.body_begin = 0;
+        s.body_count = (uint32_t)bodies.size();
+        if (e.bodies.size() != bodies.size() ||
+            (bodies.size() && std::memcmp(e.bodies.data(), bodies.data(), sizeof(BodyDev) * bodies.size()))) {
+            e.bodies = bodies;
+            e.bodies_dirty = true;
+            domain_dirty = true;
+        }
+        if (before.h_grid != s.h_grid) domain_dirty = true;
+        if (e.n_total != n || e.tiles.empty()) {
+            e.n_total = n;
+            e.build_tiles();
+        }
+        if (domain_dirty) {
+            int32_t box[6];
+            e.scenes_dirty = true;
+            e.upload_tables();
+            e.particle_cell_box(0, box);
+            ph::choose_domain(s, box, n, bodies);
+            e.assign_buckets();
+            domain_dirty = false;
+        } else if (std::memcmp(&before, &s, sizeof(SceneDev)) != 0) {
+            e.scenes_dirty = true;
+        }
+        e.upload_tables();
+    }
+
+    uint32_t cfl_substeps(float dt) {
+        // fluid3.rs:610-636
+        const uint32_t mx = std::max<uint32_t>(cfg.max_substeps, 1u);
+        if (cfg.cfl <= 0.0f || mx <= 1) return 1;
+        auto rmax = [](float a, float b) { return a != a ? b : (b != b ? a : (a > b ? a : b)); };
+        const float h = rmax(cfg.smoothing_length, 1e-6f);
+        const float rho0 = rmax(cfg.rest_density, 1e-6f);
+        const float c0 = cfg.gamma == 1.0f ? std::sqrt(rmax(cfg.stiffness, 0.0f))
+                                           : std::sqrt(cfg.gamma * rmax(cfg.stiffness, 0.0f) * rho0);
+        const float vmax = eng.max_speed(0, n);
+        const float speed = c0 + vmax;
+        if (!std::isfinite(speed) || speed <= 1e-6f) return 1;
+        const float dt_max = cfg.cfl * h / speed;
+        if (!std::isfinite(dt_max) || dt_max <= 0.0f) return 1;
+        const float q = std::ceil(dt / dt_max);
+        int64_t s;
+        if (q != q) s = 0;
+        else if (q >= 9.2233720368547758e18f) s = INT64_MAX;
+        else if (q <= -9.2233720368547758e18f) s = INT64_MIN;
+        else s = (int64_t)q;
+        return (uint32_t)std::min<int64_t>(std::max<int64_t>(s, 1), (int64_t)mx);
+    }
+
+    // World3::step fluid phase (world.rs:2670-2697, 2809-2812) and
+    // FluidState3::step_fluid (fluid3.rs:591-607).
+    void step(float dt) {
+        if (cfg.enabled && n > 0 && dt > 0.0f) {
+            eng.init_device();
+            if (!domain_dirty) eng.poll_monitor(n);
+            sync_scene(dt);
+            const uint32_t sub = cfl_substeps(dt);
+            const float sdt = dt / (float)sub;
+            if (sub != 1) sync_scene(sdt);
+            for (uint32_t k = 0; k < sub; ++k) eng.substep();
+        }
+        time += dt;
+    }
+
+    uint32_t add(const float *xyz, const uint8_t *phases, uint32_t count) {
+        const uint32_t m = std::min(count, kMaxPerCall);
+        for (size_t i = 0; i < 3 * (size_t)m; ++i)
+            if (!std::isfinite(xyz[i])) return 0;
+        const uint32_t room = capacity > n ? capacity - n : 0;
+        const uint32_t k = std::min(room, m);
+        if (k == 0) return 0;
+        eng.init_device();
+        eng.n_total = n;
+        eng.ensure_particles(n + k);
+        eng.staging_f.ensure(3 * (size_t)k);
+        PH_CK(hipMemcpyAsync(eng.staging_f.p, xyz, sizeof(float) * 3 * k, hipMemcpyHostToDevice, eng.stream));
+        const uint8_t *dph = nullptr;
+        if (phases) {
+            eng.staging_u8.ensure(k);
+            PH_CK(hipMemcpyAsync(eng.staging_u8.p, phases, k, hipMemcpyHostToDevice, eng.stream));
+            dph = eng.staging_u8.p;
+        }
+        auto pm = [&](int p) { return std::max(cfg.phase_mass[p], 0.0f); };
+        const float4 masses = make_float4(pm(0), pm(1), pm(2), pm(3));
+        const uint32_t n0 = n;
+        eng.run(ph::KIND_APPEND, [&] {
+            return ph::launch_append(eng.stream, n0, k, eng.staging_f.p, dph, masses, eng.posm[eng.cs].p,
+                                     eng.vel[eng.cs].p, eng.tag[eng.ct].p);
+        });
+        PH_CK(hipStreamSynchronize(eng.stream));
+        n += k;
+        eng.n_total = n;
+        eng.build_tiles();
+        eng.rec_valid = false;
+        domain_dirty = true;
+        return k;
+    }
+
+    uint32_t get(float *out, uint32_t max) {
+        const uint32_t m = std::min(max, n);
+        if (m == 0) return 0;
+        eng.init_device();
+        eng.export_buf.ensure(6 * (size_t)n);
+        eng.run(ph::KIND_EXPORT, [&] {
+            return ph::launch_export(eng.stream, n, eng.posm[eng.cs].p, eng.vel[eng.cs].p, eng.tag[eng.ct].p, 0u,
+                                     eng.export_buf.p);
+        });
+        PH_CK(hipMemcpyAsync(out, eng.export_buf.p, sizeof(float) * 6 * m, hipMemcpyDeviceToHost, eng.stream));
+        PH_CK(hipStreamSynchronize(eng.stream));
+        return m;
+    }
+};
+
+extern "C" {
+
+const char *phieng_last_error(void) { return g_last_error.c_str(); }
+
+World3 *phieng_world3_new(void) {
+    try {
+        return new World3();
+    } catch (...) {
+        return nullptr;  // allocation of the host object itself failed
+    }
+}
+
+void phieng_world3_free(World3 *world) {
+    if (!world) return;
+    try {
+        delete world;
+    } catch (...) {
+    }
+}
+
+void phieng_world3_step(World3 *world, float dt) {
+    if (!world || !std::isfinite(dt) || dt <= 0.0f) return;
+    try {
+        world->step(std::min(dt, 1.0f / 15.0f));
+    } catch (const std::exception &e) {
+        set_error("phieng_world3_step", e.what());
+    }
+}
+
+void phieng_world3_set_gravity(World3 *world, float gx, float gy, float gz) {
+    if (!world || !std::isfinite(gx) || !std::isfinite(gy) || !std::isfinite(gz)) return;
+    world->g[0] = gx;
+    world->g[1] = gy;
+    world->g[2] = gz;
+}
+
+int32_t phieng_world3_add_body(World3 *world, float px, float py, float pz, uint32_t shape_tag, float a, float b,
+                               float c, float density, uint8_t is_static, float friction, float restitution) {
+    (void)friction;
+    (void)restitution;
+    if (!world || !std::isfinite(px) || !std::isfinite(py) || !std::isfinite(pz)) return -1;
+    BodyDev bd{shape_tag, px, py, pz, a, b, c};
+    switch (shape_tag) {
+        case PHIENG_SHAPE3_SPHERE:
+            if (!(a > 0.0f && std::isfinite(a))) return -1;
+            bd.b = bd.c = 0.0f;
+            break;
+        case PHIENG_SHAPE3_BOX:
+            if (!(a > 0.0f && b > 0.0f && c > 0.0f && std::isfinite(a) && std::isfinite(b) && std::isfinite(c)))
+                return -1;
+            break;
+        case PHIENG_SHAPE3_CAPSULE:
+            if (!(a > 0.0f && b > 0.0f && std::isfinite(a) && std::isfinite(b))) return -1;
+            bd.c = 0.0f;
+            break;
+        default:
+            return -1;
+    }
+    if (!std::isfinite(density) || density <= 0.0f) return -1;
+    if (!is_static) {
+        set_error("phieng_world3_add_body", "dynamic bodies are not supported by the GPU fluid path");
+        return -1;
+    }
+    if (world->bodies.size() >= PH_MAX_BODIES) {
+        set_error("phieng_world3_add_body", "too many bodies");
+        return -1;
+    }
+    world->bodies.push_back(bd);
+    return world->next_body_id++;
+}
+
+int32_t phieng_world3_set_fluid_config(World3 *world, const FluidConfig3Ffi *cfg) {
+    if (!world || !cfg) return 0;
+    if (!config_valid(*cfg)) return 0;
+    if (cfg->adaptive_resolution || cfg->boundary_particles) {
+        set_error("phieng_world3_set_fluid_config",
+                  "adaptive_resolution and boundary_particles are not implemented on the GPU path");
+        return 0;
+    }
+    FluidConfig3Ffi c = *cfg;
+    // Rust bools read back as 0/1.
+    c.adaptive_resolution = 0;
+    c.boundary_particles = 0;
+    c.enabled = cfg->enabled ? 1 : 0;
+    world->cfg = c;
+    return 1;
+}
+
+int32_t phieng_world3_get_fluid_config(const World3 *world, FluidConfig3Ffi *out) {
+    if (!world || !out) return 0;
+    *out = world->cfg;
+    return 1;
+}
+
+uint32_t phieng_world3_fluid_add(World3 *world, const float *positions, uint32_t count) {
+    if (!world || !positions) return 0;
+    try {
+        return world->add(positions, nullptr, count);
+    } catch (const std::exception &e) {
+        set_error("phieng_world3_fluid_add", e.what());
+        return 0;
+    }
+}
+
+uint32_t phieng_world3_fluid_add_phase(World3 *world, const float *positions, const uint8_t *phases,
+                                       uint32_t count) {
+    if (!world || !positions || !phases) return 0;
+    try {
+        return world->add(positions, phases, count);
+    } catch (const std::exception &e) {
+        set_error("phieng_world3_fluid_add_phase", e.what());
+        return 0;
+    }
+}
+
+uint32_t phieng_world3_fluid_count(const World3 *world) { return world ? world->n : 0; }
+
+int32_t phieng_world3_fluid_clear(World3 *world) {
+    if (!world) return 0;
+    world->n = 0;
+    world->eng.n_total = 0;
+    world->eng.tiles.clear();
+    world->eng.tiles_dirty = true;
+    world->eng.rec_valid = false;
+    world->domain_dirty = true;
+    return 1;
+}
+
+uint32_t phieng_world3_fluid_get(const World3 *world, float *out, uint32_t max) {
+    if (!world || !out) return 0;
+    try {
+        return const_cast<World3 *>(world)->get(out, max);
+    } catch (const std::exception &e) {
+        set_error("phieng_world3_fluid_get", e.what());
+        return 0;
+    }
+}
+
+uint32_t phieng_world3_fluid_reserve(World3 *world, uint32_t capacity) {
+    if (!world) return 0;
+    world->capacity = std::max(world->n, std::min(std::max(capacity, kDefaultCapacity), kMaxCapacity));
+    return world->capacity;
+}
+
+uint32_t phieng_world3_fluid_set_velocity(World3 *world, const float *vel, uint32_t count) {
+    if (!world || !vel) return 0;
+    const uint32_t m = std::min(count, world->n);
+    for (size_t i = 0; i < 3 * (size_t)m; ++i)
+        if (!std::isfinite(vel[i])) return 0;
+    if (m == 0) return 0;
+    try {
+        ph::Engine &e = world->eng;
+        e.init_device();
+        // Particles beyond `count` keep their velocity: start from the current one.
+        std::vector<float> full(6 * (size_t)world->n);
+        world->get(full.data(), world->n);
+        std::vector<float> v3(3 * (size_t)world->n);
+        for (uint32_t i = 0; i < world->n; ++i)
+            for (int c = 0; c < 3; ++c) v3[3 * i + c] = i < m ? vel[3 * i + c] : full[6 * i + 3 + c];
+        e.staging_f.ensure(v3.size());
+        PH_CK(hipMemcpyAsync(e.staging_f.p, v3.data(), sizeof(float) * v3.size(), hipMemcpyHostToDevice, e.stream));
+        e.run(ph::KIND_SETVEL, [&] {
+            return ph::launch_set_velocity(e.stream, world->n, e.tag[e.ct].p, e.staging_f.p, e.vel[e.cs].p);
+        });
+        PH_CK(hipStreamSynchronize(e.stream));
+        return m;
+    } catch (const std::exception &ex) {
+        set_error("phieng_world3_fluid_set_velocity", ex.what());
+        return 0;
+    }
+}
+
+int32_t phieng_world3_fluid_set_record(World3 *world, uint32_t flags) {
+    if (!world) return 0;
+    try {
+        world->eng.rec_flags = flags & (PHIENG_RECORD_CELLS | PHIENG_RECORD_DENSITY | PHIENG_RECORD_ACCEL);
+        world->eng.rec_valid = false;
+        if (world->eng.rec_flags && world->eng.alloc_particles) world->eng.ensure_record();
+        return 1;
+    } catch (const std::exception &e) {
+        set_error("phieng_world3_fluid_set_record", e.what());
+        return 0;
+    }
+}
+
+uint32_t phieng_world3_fluid_get_record(const World3 *world, int32_t *cell3, uint32_t *sorted_ids, float *density,
+                                        float *pressure, float *accel3, uint32_t *nbr_count, uint32_t max) {
+    if (!world || !world->eng.rec_valid) return 0;
+    try {
+        const ph::Engine &e = world->eng;
+        const uint32_t m = std::min(max, world->n);
+        const uint32_t f = e.rec_flags;
+        hipStream_t st = e.stream;
+        if (cell3 && (f & PHIENG_RECORD_CELLS))
+            PH_CK(hipMemcpyAsync(cell3, e.rec_cells.p, sizeof(int32_t) * 3 * m, hipMemcpyDeviceToHost, st));
+        if (sorted_ids && (f & PHIENG_RECORD_CELLS))
+            PH_CK(hipMemcpyAsync(sorted_ids, e.rec_sorted.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, st));
+        if (density && (f & PHIENG_RECORD_DENSITY))
+            PH_CK(hipMemcpyAsync(density, e.rec_rho.p, sizeof(float) * m, hipMemcpyDeviceToHost, st));
+        if (pressure && (f & PHIENG_RECORD_DENSITY))
+            PH_CK(hipMemcpyAsync(pressure, e.rec_p.p, sizeof(float) * m, hipMemcpyDeviceToHost, st));
+        if (nbr_count && (f & PHIENG_RECORD_DENSITY))
+            PH_CK(hipMemcpyAsync(nbr_count, e.rec_count.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, st));
+        if (accel3 && (f & PHIENG_RECORD_ACCEL))
+            PH_CK(hipMemcpyAsync(accel3, e.rec_acc.p, sizeof(float) * 3 * m, hipMemcpyDeviceToHost, st));
+        PH_CK(hipStreamSynchronize(st));
+        return m;
+    } catch (const std::exception &ex) {
+        set_error("phieng_world3_fluid_get_record", ex.what());
+        return 0;
+    }
+}
+
+int32_t phieng_world3_set_timing(World3 *world, int32_t enable) {
+    if (!world) return 0;
+    try {
+        if (world->eng.device >= 0) world->eng.reset_timing();
+        world->eng.timing = enable != 0;
+        return 1;
+    } catch (const std::exception &e) {
+        set_error("phieng_world3_set_timing", e.what());
+        return 0;
+    }
+}
+
+int32_t phieng_world3_get_timing(const World3 *world, double *total_ms, uint64_t *launches, int32_t max) {
+    if (!world) return 0;
+    try {
+        ph::Engine &e = const_cast<World3 *>(world)->eng;
+        if (e.device >= 0) e.harvest_timing();
+        const int32_t m = std::min<int32_t>(max, ph::KIND_COUNT);
+        for (int32_t k = 0; k < m; ++k) {
+            if (total_ms) total_ms[k] = e.total_ms[k];
+            if (launches) launches[k] = e.launches[k];
+        }
+        return ph::KIND_COUNT;
+    } catch (const std::exception &ex) {
+        set_error("phieng_world3_get_timing", ex.what());
+        return 0;
+    }
+}
+
+const char *phieng_kernel_name(int32_t kind) { return ph::kind_name(kind); }
+
+int32_t phieng_world3_sync(World3 *world) {
+    if (!world) return 0;
+    if (world->eng.device < 0) return 1;
+    try {
+        PH_CK(hipSetDevice(world->eng.device));
+        PH_CK(hipStreamSynchronize(world->eng.stream));
+        return 1;
+    } catch (const std::exception &e) {
+        set_error("phieng_world3_sync", e.what());
+        return 0;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+"""ctypes binding of ``libphieng_hip.so`` (include/phieng_fluid3.h).
+
+Loading fails loudly when the library has not been built: there is no
+fallback implementation of the fluid step in Python.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "libphieng_hip.so")
+
+
+class FluidConfig3Ffi(ctypes.Structure):
+    """``#[repr(C)] FluidConfig3Ffi`` (crates/engine/src/ffi.rs:743-767), 112 bytes."""
+
+    _fields_ = [
+        ("particle_mass", ctypes.c_float),
+        ("rest_density", ctypes.c_float),
+        ("smoothing_length", ctypes.c_float),
+        ("stiffness", ctypes.c_float),
+        ("gamma", ctypes.c_float),
+        ("artificial_viscosity", ctypes.c_float),
+        ("surface_tension", ctypes.c_float),
+        ("cfl", ctypes.c_float),
+        ("max_substeps", ctypes.c_uint32),
+        ("adaptive_resolution", ctypes.c_uint8),
+        ("split_density_ratio", ctypes.c_float),
+        ("merge_density_ratio", ctypes.c_float),
+        ("boundary_particles", ctypes.c_uint8),
+        ("viscosity", ctypes.c_float),
+        ("drag", ctypes.c_float),
+        ("wall_friction", ctypes.c_float),
+        ("phase_count", ctypes.c_uint8),
+        ("phase_rest_density", ctypes.c_float * 4),
+        ("phase_mass", ctypes.c_float * 4),
+        ("max_velocity", ctypes.c_float),
+        ("particle_radius", ctypes.c_float),
+        ("enabled", ctypes.c_uint8),
+    ]
+
+    @classmethod
+    def default(cls):
+        """FluidConfig3::default (crates/dynamics/src/fluid3.rs:105-134)."""
+        c = cls()
+        c.particle_mass = 0.0625
+        c.rest_density = 1.0
+        c.smoothing_length = 0.5
+        c.stiffness = 100.0
+        c.gamma = 1.0
+        c.artificial_viscosity = 0.0
+        c.surface_tension = 0.0
+        c.cfl = 0.0
+        c.max_substeps = 1
+        c.adaptive_resolution = 0
+        c.split_density_ratio = 0.5
+        c.merge_density_ratio = 1.6
+        c.boundary_particles = 0
+        c.viscosity = 0.3
+        c.drag = 4.0
+        c.wall_friction = 0.0
+        c.phase_count = 1
+        for k in range(4):
+            c.phase_rest_density[k] = 1.0
+            c.phase_mass[k] = 0.0625
+        c.max_velocity = 30.0
+        c.particle_radius = 0.125
+        c.enabled = 1
+        return c
+
+    def copy(self):
+        c = FluidConfig3Ffi()
+        ctypes.memmove(ctypes.byref(c), ctypes.byref(self), ctypes.sizeof(self))
+        return c
+
+
+assert ctypes.sizeof(FluidConfig3Ffi) == 112
+
+_P = ctypes.c_void_p
+_F = ctypes.c_float
+_U32 = ctypes.c_uint32
+_I32 = ctypes.c_int32
+_fp = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_dp = ctypes.POINTER(ctypes.c_double)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_cfgp = ctypes.POINTER(FluidConfig3Ffi)
+
+# name -> (restype, argtypes); every symbol include/phieng_fluid3.h declares.
+SIGNATURES = {
+    "phieng_world3_new": (_P, []),
+    "phieng_world3_free": (None, [_P]),
+    "phieng_world3_step": (None, [_P, _F]),
+    "phieng_world3_set_gravity": (None, [_P, _F, _F, _F]),
+    "phieng_world3_add_body": (_I32, [_P, _F, _F, _F, _U32, _F, _F, _F, _F, ctypes.c_uint8, _F, _F]),
+    "phieng_world3_set_fluid_config": (_I32, [_P, _cfgp]),
+    "phieng_world3_get_fluid_config": (_I32, [_P, _cfgp]),
+    "phieng_world3_fluid_add": (_U32, [_P, _fp, _U32]),
+    "phieng_world3_fluid_add_phase": (_U32, [_P, _fp, _u8p, _U32]),
+    "phieng_world3_fluid_count": (_U32, [_P]),
+    "phieng_world3_fluid_clear": (_I32, [_P]),
+    "phieng_world3_fluid_get": (_U32, [_P, _fp, _U32]),
+    "phieng_world3_fluid_reserve": (_U32, [_P, _U32]),
+    "phieng_world3_fluid_set_velocity": (_U32, [_P, _fp, _U32]),
+    "phieng_world3_fluid_set_record": (_I32, [_P, _U32]),
+    "phieng_world3_fluid_get_record": (_U32, [_P, _i32p, _u32p, _fp, _fp, _fp, _u32p, _U32]),
+    "phieng_world3_set_timing": (_I32, [_P, _I32]),
+    "phieng_world3_get_timing": (_I32, [_P, _dp, _u64p, _I32]),
+    "phieng_kernel_name": (ctypes.c_char_p, [_I32]),
+    "phieng_world3_sync": (_I32, [_P]),
+    "phieng_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(lib_path):
+            raise RuntimeError(
+                f"{lib_path} is missing: build it with `python -m phi_eng_amd.build` "
+                "(the fluid step has no non-HIP implementation)")
+        l = ctypes.CDLL(lib_path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def last_error():
+    return lib().phieng_last_error().decode()

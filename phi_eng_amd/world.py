@@ -1,0 +1,152 @@
+"""Python mirror of the reference World3 fluid surface over the C ABI.
+
+Method names follow the reference (``World3::step``, ``fluid.add_particles``,
+``phieng_world3_*`` in crates/engine/src/ffi.rs); arrays are numpy.
+"""
+import ctypes
+
+import numpy as np
+
+from .ffi import FluidConfig3Ffi, last_error, lib
+
+SHAPE3_SPHERE, SHAPE3_BOX, SHAPE3_CAPSULE = 0, 1, 2
+RECORD_CELLS, RECORD_DENSITY, RECORD_ACCEL = 1, 2, 4
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class World3:
+    """Owns one ``World3*`` from ``phieng_world3_new`` (ffi.rs:3199)."""
+
+    def __init__(self, capacity=None):
+        self._l = lib()
+        self.ptr = self._l.phieng_world3_new()
+        if not self.ptr:
+            raise MemoryError("phieng_world3_new returned null")
+        if capacity is not None:
+            self.reserve(capacity)
+
+    def close(self):
+        if self.ptr:
+            self._l.phieng_world3_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- configuration --------------------------------------------------
+    def set_gravity(self, g):
+        self._l.phieng_world3_set_gravity(self.ptr, float(g[0]), float(g[1]), float(g[2]))
+
+    def set_fluid_config(self, cfg: FluidConfig3Ffi) -> bool:
+        return bool(self._l.phieng_world3_set_fluid_config(self.ptr, ctypes.byref(cfg)))
+
+    def get_fluid_config(self) -> FluidConfig3Ffi:
+        c = FluidConfig3Ffi()
+        if not self._l.phieng_world3_get_fluid_config(self.ptr, ctypes.byref(c)):
+            raise RuntimeError("get_fluid_config failed")
+        return c
+
+    def add_static_box(self, center, half_extents):
+        return self._l.phieng_world3_add_body(self.ptr, *map(float, center), SHAPE3_BOX,
+                                              *map(float, half_extents), 1.0, 1, 0.5, 0.0)
+
+    def add_body(self, center, shape, a, b=0.0, c=0.0, density=1.0, is_static=1):
+        return self._l.phieng_world3_add_body(self.ptr, *map(float, center), int(shape), float(a),
+                                              float(b), float(c), float(density), int(is_static),
+                                              0.5, 0.0)
+
+    def reserve(self, capacity):
+        return self._l.phieng_world3_fluid_reserve(self.ptr, int(capacity))
+
+    # -- particles -------------------------------------------------------
+    def add_particles(self, positions, phases=None):
+        p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        total = 0
+        for s in range(0, len(p), 1_000_000):  # ffi.rs:3587 caps each call
+            chunk = np.ascontiguousarray(p[s:s + 1_000_000])
+            if phases is None:
+                got = self._l.phieng_world3_fluid_add(self.ptr, _fptr(chunk), len(chunk))
+            else:
+                ph = np.ascontiguousarray(np.asarray(phases, dtype=np.uint8)[s:s + 1_000_000])
+                got = self._l.phieng_world3_fluid_add_phase(
+                    self.ptr, _fptr(chunk), ph.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                    len(chunk))
+            total += got
+            if got < len(chunk):
+                break
+        return total
+
+    def count(self):
+        return self._l.phieng_world3_fluid_count(self.ptr)
+
+    def clear(self):
+        return self._l.phieng_world3_fluid_clear(self.ptr)
+
+    def get_state(self):
+        """(positions, velocities) in insertion order, each (N, 3) float32."""
+        n = self.count()
+        out = np.empty((n, 6), dtype=np.float32)
+        got = self._l.phieng_world3_fluid_get(self.ptr, _fptr(out), n)
+        if got != n:
+            raise RuntimeError(f"fluid_get returned {got} of {n}: {last_error()}")
+        return out[:, :3].copy(), out[:, 3:].copy()
+
+    def set_velocity(self, vel):
+        v = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 3)
+        return self._l.phieng_world3_fluid_set_velocity(self.ptr, _fptr(v), len(v))
+
+    # -- stepping --------------------------------------------------------
+    def step(self, dt):
+        self._l.phieng_world3_step(self.ptr, float(dt))
+
+    def sync(self):
+        if not self._l.phieng_world3_sync(self.ptr):
+            raise RuntimeError(last_error())
+
+    # -- parity readback ---------------------------------------------------
+    def set_record(self, flags):
+        if not self._l.phieng_world3_fluid_set_record(self.ptr, int(flags)):
+            raise RuntimeError(last_error())
+
+    def get_record(self):
+        n = self.count()
+        cells = np.zeros((n, 3), dtype=np.int32)
+        order = np.zeros(n, dtype=np.uint32)
+        rho = np.zeros(n, dtype=np.float32)
+        p = np.zeros(n, dtype=np.float32)
+        acc = np.zeros((n, 3), dtype=np.float32)
+        cnt = np.zeros(n, dtype=np.uint32)
+        got = self._l.phieng_world3_fluid_get_record(
+            self.ptr, cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            order.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _fptr(rho), _fptr(p), _fptr(acc),
+            cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n)
+        if got != n:
+            raise RuntimeError(f"get_record returned {got}: {last_error()}")
+        return dict(cells=cells, order=order, density=rho, pressure=p, accel=acc, count=cnt)
+
+    # -- timing -----------------------------------------------------------
+    def set_timing(self, on):
+        self._l.phieng_world3_set_timing(self.ptr, 1 if on else 0)
+
+    def get_timing(self):
+        k = 32
+        ms = (ctypes.c_double * k)()
+        cnt = (ctypes.c_uint64 * k)()
+        m = self._l.phieng_world3_get_timing(self.ptr, ms, cnt, k)
+        out = {}
+        for i in range(min(m, k)):
+            if cnt[i]:
+                out[self._l.phieng_kernel_name(i).decode()] = (ms[i], cnt[i])
+        return out
