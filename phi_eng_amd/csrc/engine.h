@@ -95,6 +95,7 @@ struct Engine {
     size_t alloc_particles = 0;
     DBuf<uint32_t> keys, slots, tmp;
     DBuf<float2> rq;
+    DBuf<uint32_t> masks, flags;  // per-row neighbour bit masks (9 x N) + row flags
     // bucket tables
     DBuf<uint32_t> counts, start, partial;
     size_t alloc_buckets = 0;
@@ -102,7 +103,7 @@ struct Engine {
     DBuf<SceneDev> d_scenes;
     DBuf<TileDev> d_tiles;
     DBuf<BodyDev> d_bodies;
-    DBuf<int32_t> d_bbox, d_monitor;
+    DBuf<int32_t> d_bbox, d_monitor, d_box6;  // per-tile cell boxes, monitor output, set-up box
     DBuf<uint32_t> d_vmax;
     DBuf<float> staging_f, export_buf;
     DBuf<uint8_t> staging_u8;

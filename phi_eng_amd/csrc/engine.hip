@@ -202,6 +202,8 @@ void Engine::ensure_particles(uint32_t cap) {
     slots.ensure(want);
     tmp.ensure(want);
     rq.ensure(want);
+    masks.ensure(9 * want);
+    flags.ensure(want);
     alloc_particles = want;
     if (rec_flags) ensure_record();
 }
@@ -218,9 +220,13 @@ void Engine::ensure_record() {
 
 void Engine::build_tiles() {
     tiles.clear();
-    for (uint32_t k = 0; k < scenes.size(); ++k)
+    for (uint32_t k = 0; k < scenes.size(); ++k) {
+        scenes[k].tile_begin = (uint32_t)tiles.size();
         for (uint32_t f = scenes[k].part_begin; f < scenes[k].part_end; f += PH_TILE) tiles.push_back({k, f});
+        scenes[k].tile_count = (uint32_t)tiles.size() - scenes[k].tile_begin;
+    }
     tiles_dirty = true;
+    scenes_dirty = true;
 }
 
 void Engine::assign_buckets() {
@@ -246,12 +252,12 @@ void Engine::upload_tables() {
         d_scenes.ensure(scenes.size());
         PH_CK(hipMemcpyAsync(d_scenes.p, scenes.data(), sizeof(SceneDev) * scenes.size(), hipMemcpyHostToDevice,
                              stream));
-        d_bbox.ensure(6 * scenes.size());
         d_monitor.ensure(8 * scenes.size());
         scenes_dirty = false;
     }
     if (tiles_dirty) {
         d_tiles.ensure(tiles.size());
+        d_bbox.ensure(6 * std::max<size_t>(tiles.size(), 1));
         if (!tiles.empty())
             PH_CK(hipMemcpyAsync(d_tiles.p, tiles.data(), sizeof(TileDev) * tiles.size(), hipMemcpyHostToDevice,
                                  stream));
@@ -291,13 +297,14 @@ void Engine::substep() {
                                   tag[ct].p, posm[ns].p, vel[ns].p, tag[nt_].p, rc ? rec_sorted.p : nullptr);
     });
     run(KIND_DENSITY, [&] {
-        return launch_density(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p,
-                              rd ? rec_rho.p : nullptr, rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr);
+        return launch_density(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p, masks.p,
+                              flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr, rd ? rec_p.p : nullptr,
+                              rd ? rec_count.p : nullptr);
     });
     run(KIND_FORCE, [&] {
         return launch_force_integrate(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p,
-                                      vel[ns].p, rq.p, tag[nt_].p, posm[cs].p, vel[cs].p, ra ? rec_acc.p : nullptr,
-                                      d_bbox.p);
+                                      vel[ns].p, rq.p, tag[nt_].p, masks.p, flags.p, (uint32_t)alloc_particles,
+                                      posm[cs].p, vel[cs].p, ra ? rec_acc.p : nullptr, d_bbox.p);
     });
     ct = nt_;
     rec_valid = rec_flags != 0;
@@ -337,12 +344,12 @@ bool Engine::poll_monitor(uint32_t) {
 void Engine::particle_cell_box(uint32_t k, int32_t out[6]) {
     const SceneDev &s = scenes[k];
     int32_t init[6] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
-    d_bbox.ensure(std::max<size_t>(6 * scenes.size(), 6));
-    PH_CK(hipMemcpyAsync(d_bbox.p, init, sizeof(init), hipMemcpyHostToDevice, stream));
+    d_box6.ensure(6);
+    PH_CK(hipMemcpyAsync(d_box6.p, init, sizeof(init), hipMemcpyHostToDevice, stream));
     const uint32_t n = s.part_end - s.part_begin;
     if (n)
-        run(KIND_BBOX, [&] { return launch_cell_bbox(stream, n, posm[cs].p + s.part_begin, s.h_grid, d_bbox.p); });
-    PH_CK(hipMemcpyAsync(out, d_bbox.p, sizeof(init), hipMemcpyDeviceToHost, stream));
+        run(KIND_BBOX, [&] { return launch_cell_bbox(stream, n, posm[cs].p + s.part_begin, s.h_grid, d_box6.p); });
+    PH_CK(hipMemcpyAsync(out, d_box6.p, sizeof(init), hipMemcpyDeviceToHost, stream));
     PH_CK(hipStreamSynchronize(stream));
 }
 

@@ -44,7 +44,7 @@ struct SceneDev {
     float gx, gy, gz;
     float dt;
     uint32_t body_begin, body_count;
-    uint32_t pad[2];
+    uint32_t tile_begin, tile_count;  // this scene's run of the tile table
 };
 
 struct TileDev {

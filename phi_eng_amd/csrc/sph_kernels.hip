@@ -47,7 +47,19 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &total)
 
 }  // namespace
 
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8),
+// each with its own L2.  Give every XCD one contiguous run of tiles, so
+// the neighbour rows a tile reads were mostly fetched by tiles on the same
+// XCD.  A bijection on [0, nt).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nt) {
+    const uint32_t x = b & 7u, k = b >> 3, per = nt >> 3, rem = nt & 7u;
+    return x * per + min(x, rem) + k;
+}
+
 // ---------------------------------------------------------------- K1
+// Bucket key per particle and its slot inside the bucket.  Particles come
+// in last substep's sorted order, so neighbouring lanes mostly share a
+// bucket: one atomic per run of equal keys in a wave, not one per lane.
 __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict__ tiles,
                                                       const SceneDev *__restrict__ scenes,
                                                       const float4 *__restrict__ posm,
@@ -56,24 +68,39 @@ __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict_
                                                       uint32_t *__restrict__ slots,
                                                       uint32_t *__restrict__ counts,
                                                       int32_t *__restrict__ rec_cells) {
-    const TileDev t = tiles[blockIdx.x];
-    const SceneDev &s = scenes[t.scene];
+    const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
-    if (i >= s.part_end) return;
-    const float4 p = posm[i];
-    int32_t cx, cy, cz;
-    cell_of(p.x, p.y, p.z, s.h_grid, cx, cy, cz);
-    uint32_t lk;
-    if (!finite3(p.x, p.y, p.z)) {
-        lk = s.ncells + 1;  // left out of the grid (fluid3.rs:302-304)
-    } else if (in_domain(s, cx, cy, cz)) {
-        lk = (uint32_t)((((int64_t)cz - s.z0) * s.ny + ((int64_t)cy - s.y0)) * s.nx + ((int64_t)cx - s.x0));
-    } else {
-        lk = s.ncells;  // outlier bucket
+    const bool active = i < s.part_end;
+    const int lane = threadIdx.x & 63;
+    uint32_t key = 0xFFFFFFFFu;
+    int32_t cx = 0, cy = 0, cz = 0;
+    if (active) {
+        const float4 p = posm[i];
+        cell_of(p.x, p.y, p.z, s.h_grid, cx, cy, cz);
+        uint32_t lk;
+        if (!finite3(p.x, p.y, p.z)) {
+            lk = s.ncells + 1;  // left out of the grid (fluid3.rs:302-304)
+        } else if (in_domain(s, cx, cy, cz)) {
+            lk = (uint32_t)((((int64_t)cz - s.z0) * s.ny + ((int64_t)cy - s.y0)) * s.nx + ((int64_t)cx - s.x0));
+        } else {
+            lk = s.ncells;  // outlier bucket
+        }
+        key = s.bucket_base + lk;
     }
-    const uint32_t key = s.bucket_base + lk;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || key != prev;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = ~0ull >> (63 - lane);
+    const int run0 = 63 - __clzll(heads & upto);
+    const unsigned long long after = heads & ~upto;
+    const int run1 = after ? __ffsll((long long)after) - 1 : 64;
+    uint32_t base = 0;
+    if (active && head) base = atomicAdd(&counts[key], (uint32_t)(run1 - run0));
+    base = __shfl(base, run0, 64);
+    if (!active) return;
     keys[i] = key;
-    slots[i] = atomicAdd(&counts[key], 1u);
+    slots[i] = base + (uint32_t)(lane - run0);
     if (rec_cells) {
         const uint32_t id = tag[i] & PH_ID_MASK;
         rec_cells[3 * id] = cx;
@@ -96,10 +123,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__re
 }
 
 // Single block: exclusive scan of the block partials, start[nb] = total.
-// Also resets the per-scene cell bounding boxes the integrate pass fills.
 __global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t *__restrict__ partial, uint32_t np,
-                                                              uint32_t *__restrict__ start, uint32_t nb,
-                                                              int32_t *__restrict__ bbox, uint32_t nscenes) {
+                                                              uint32_t *__restrict__ start, uint32_t nb) {
     uint32_t carry = 0;
     for (uint32_t c = 0; c < np; c += kScanBlock) {
         const uint32_t k = c + threadIdx.x;
@@ -110,14 +135,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t *__restri
         carry += tot;
     }
     if (threadIdx.x == 0) start[nb] = carry;
-    for (uint32_t k = threadIdx.x; k < nscenes; k += kScanBlock) {
-        bbox[6 * k + 0] = INT32_MAX;
-        bbox[6 * k + 1] = INT32_MAX;
-        bbox[6 * k + 2] = INT32_MAX;
-        bbox[6 * k + 3] = INT32_MIN;
-        bbox[6 * k + 4] = INT32_MIN;
-        bbox[6 * k + 5] = INT32_MIN;
-    }
 }
 
 // Bucket start = block offset + in-block exclusive scan; counts are zeroed
@@ -164,8 +181,8 @@ __global__ __launch_bounds__(PH_TILE) void k_rank_gather(
     const uint32_t *__restrict__ keys, const uint32_t *__restrict__ start, const float4 *__restrict__ posm_in,
     const float4 *__restrict__ vel_in, const uint32_t *__restrict__ tag_in, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted) {
-    const TileDev t = tiles[blockIdx.x];
-    const SceneDev &s = scenes[t.scene];
+    const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneDev s = scenes[t.scene];
     const uint32_t q = t.first + threadIdx.x;
     if (q >= s.part_end) return;
     const uint32_t src = tmp[q];
@@ -197,39 +214,121 @@ __global__ __launch_bounds__(PH_TILE) void k_rank_gather(
     if (rec_sorted) rec_sorted[d] = myid;
 }
 
+// ---------------------------------------------------------------- rows
+// True when the particle's neighbourhood leaves the domain while the scene
+// has outliers (then the per-cell path of for_each_candidate is needed).
+__device__ __forceinline__ bool needs_cell_path(const SceneDev &s, const uint32_t *__restrict__ start, int32_t cx,
+                                                int32_t cy, int32_t cz) {
+    const bool inside = in_domain(s, (int64_t)cx - 1, (int64_t)cy - 1, (int64_t)cz - 1) &&
+                        in_domain(s, (int64_t)cx + 1, (int64_t)cy + 1, (int64_t)cz + 1);
+    return !inside && start[s.bucket_base + s.ncells] != start[s.bucket_base + s.ncells + 1];
+}
+
+// Slot range [b, e) of row r = 3 (oz + 1) + (oy + 1) (empty outside the domain).
+__device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__restrict__ start, int32_t cx,
+                                          int32_t cy, int32_t cz, int r, uint32_t &b, uint32_t &e) {
+    const int64_t xlo = max((int64_t)cx - 1, (int64_t)s.x0);
+    const int64_t xhi = min((int64_t)cx + 1, (int64_t)s.x0 + s.nx - 1);
+    const int64_t z = (int64_t)cz + (r / 3) - 1, y = (int64_t)cy + (r % 3) - 1;
+    b = e = 0;
+    if (xlo > xhi || z < s.z0 || z >= (int64_t)s.z0 + s.nz || y < s.y0 || y >= (int64_t)s.y0 + s.ny) return;
+    const uint32_t row = s.bucket_base + (uint32_t)(((z - s.z0) * s.ny + (y - s.y0)) * s.nx);
+    b = start[row + (uint32_t)(xlo - s.x0)];
+    e = start[row + (uint32_t)(xhi - s.x0) + 1];
+}
+
+#define PH_ROW_BITS 32u
+#define PH_SLOW_PATH 0x80000000u
+
 // ---------------------------------------------------------------- K4
-// rq[i] = (rho_i, p_i / (max(rho_i, EPS))^2): the pressure quotient the
-// force pass needs (fluid3.rs:1046-1051), computed once per particle.
+// Density (fluid3.rs:639-655) in two phases per row: the cut-off test over
+// the row's candidates builds a bit mask (stored for the force pass), then
+// the set bits are summed in ascending slot order, i.e. the reference's
+// neighbour order.  Rows with more than 32 candidates finish their tail
+// inline and flag it.  rq[i] = (rho_i, p_i / max(rho_i, EPS)^2), the
+// pressure quotient of the force pass (fluid3.rs:1046-1051).
 __global__ __launch_bounds__(PH_TILE) void k_density(const TileDev *__restrict__ tiles,
                                                      const SceneDev *__restrict__ scenes,
                                                      const uint32_t *__restrict__ start,
                                                      const float4 *__restrict__ posm,
                                                      const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
-                                                     float *__restrict__ rec_rho, float *__restrict__ rec_p,
-                                                     uint32_t *__restrict__ rec_count) {
-    const TileDev t = tiles[blockIdx.x];
-    const SceneDev &s = scenes[t.scene];
+                                                     uint32_t *__restrict__ masks, uint32_t *__restrict__ flags,
+                                                     uint32_t mstride, float *__restrict__ rec_rho,
+                                                     float *__restrict__ rec_p, uint32_t *__restrict__ rec_count) {
+    const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     if (i >= s.part_end) return;
     const float4 pi = posm[i];
     float rho = 0.0f;
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, fl = 0;
     if (finite3(pi.x, pi.y, pi.z)) {
         int32_t cx, cy, cz;
         cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
         const float hg2 = s.h_grid2;
-        for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
-            const float4 pj = posm[j];
-            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-            if (r2 < hg2) {
-                ++cnt;
-                rho += pj.w * poly6(r2, s);
+        if (!needs_cell_path(s, start, cx, cy, cz)) {
+#pragma unroll 1
+            for (int r = 0; r < 9; ++r) {
+                uint32_t br, er;
+                row_range(s, start, cx, cy, cz, r, br, er);
+                const uint32_t lim = min(er, br + PH_ROW_BITS);
+                uint32_t m = 0;
+                uint32_t j = br;
+                // Four independent loads in flight per lane.
+                for (; j + 4 <= lim; j += 4) {
+                    const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
+                    const uint32_t o = j - br;
+                    m |= (dot3(q0.x - pi.x, q0.y - pi.y, q0.z - pi.z) < hg2 ? 1u : 0u) << o;
+                    m |= (dot3(q1.x - pi.x, q1.y - pi.y, q1.z - pi.z) < hg2 ? 2u : 0u) << o;
+                    m |= (dot3(q2.x - pi.x, q2.y - pi.y, q2.z - pi.z) < hg2 ? 4u : 0u) << o;
+                    m |= (dot3(q3.x - pi.x, q3.y - pi.y, q3.z - pi.z) < hg2 ? 8u : 0u) << o;
+                }
+                for (; j < lim; ++j) {
+                    const float4 pj = posm[j];
+                    const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                    m |= (r2 < hg2 ? 1u : 0u) << (j - br);
+                }
+                masks[(size_t)r * mstride + i] = m;
+                while (m) {
+                    const uint32_t k = __ffs(m) - 1;
+                    m &= m - 1;
+                    const float4 pj = posm[br + k];
+                    const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                    ++cnt;
+                    rho += pj.w * poly6(r2, s);
+                }
+                if (er > lim) {
+                    fl |= 1u << r;
+                    for (uint32_t j = lim; j < er; ++j) {
+                        const float4 pj = posm[j];
+                        const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                        if (r2 < hg2) {
+                            ++cnt;
+                            rho += pj.w * poly6(r2, s);
+                        }
+                    }
+                }
             }
-        });
+        } else {
+            fl = PH_SLOW_PATH;
+            for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
+                const float4 pj = posm[j];
+                const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                if (r2 < hg2) {
+                    ++cnt;
+                    rho += pj.w * poly6(r2, s);
+                }
+            });
+        }
+    } else {
+        fl = PH_SLOW_PATH;  // no neighbours: the force pass skips the sweep
     }
+    flags[i] = fl;
     rho = rclampf(rho, 0.0f, s.rho_max);
     const uint32_t tg = tag[i];
-    const float rho0 = s.phase_rho0[min(tg >> 24, 3u)];
+    // Indexed through global memory: a lane-varying index into the scene
+    // copy would put the whole copy in scratch.
+    const float rho0 = scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
     const float p = eos(rho, rho0, s.stiffness, s.gamma);
     const float rc = rmaxf(rho, PH_EPS);
     rq[i] = make_float2(rho, p / (rc * rc));
@@ -242,16 +341,74 @@ __global__ __launch_bounds__(PH_TILE) void k_density(const TileDev *__restrict__
 }
 
 // ---------------------------------------------------------------- K5
-// Force (fluid3.rs:1035-1086) and XSPH (fluid3.rs:1110-1124) share one
-// neighbour sweep over the pre-step state; then integrate, speed cap,
-// finite guards and the boundary clamp of this particle.
+// Per-pair force (fluid3.rs:1042-1085) and XSPH (fluid3.rs:1114-1123) terms,
+// accumulated in the reference's operation order.
+struct PairAcc {
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;  // force
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;  // XSPH smoothing
+};
+
+__device__ __forceinline__ void pair_terms(const SceneDev &s, const float4 &pi, const float4 &vi, float rho_i,
+                                           float q_i, const float4 &pj, const float4 &vj, const float2 &rqj,
+                                           float r2, PairAcc &a) {
+    const float h = s.h;
+    const float rho_j = rmaxf(rqj.x, PH_EPS);
+    // delta = x_i - x_j; |delta|^2 equals the cut-off test's r2 exactly.
+    const float ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+    const float dist = sqrtf(r2);
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    if (!(dist >= h || dist < PH_EPS)) {  // spiky_grad, fluid3.rs:232-239
+        const float sc = s.spiky_c * (h - dist) * (h - dist);
+        gx = ex / dist * sc;
+        gy = ey / dist * sc;
+        gz = ez / dist * sc;
+    }
+    float term = q_i + rqj.y;
+    if (s.av_alpha > 0.0f && dist > PH_EPS) {  // artificial_pressure, fluid3.rs:254-269
+        const float rho_bar = 0.5f * (rho_i + rho_j);
+        const float vd = (vi.x - vj.x) * ex + (vi.y - vj.y) * ey + (vi.z - vj.z) * ez;
+        float ap = 0.0f;
+        if (!(vd >= 0.0f)) {
+            const float mu = h * vd / (dist * dist + 0.01f * h * h);
+            ap = -s.av_alpha * s.c_s * mu / rmaxf(rho_bar, PH_EPS);
+        }
+        term = term + ap;
+    }
+    const float sm = pj.w * term;
+    a.fx = a.fx - gx * sm;
+    a.fy = a.fy - gy * sm;
+    a.fz = a.fz - gz * sm;
+    if (s.kappa > 0.0f && dist > PH_EPS) {  // cohesion, fluid3.rs:272-278, 1062-1070
+        const float coh = dist >= h ? 0.0f : 1.0f - dist / h;
+        if (coh > 0.0f) {
+            const float cs = s.kappa * pi.w * pj.w * coh;
+            a.fx = a.fx - ex / dist * cs;
+            a.fy = a.fy - ey / dist * cs;
+            a.fz = a.fz - ez / dist * cs;
+        }
+    }
+    if (s.viscosity > 0.0f) {
+        const float rho_bar = 0.5f * (rho_i + rho_j);
+        const float qm = pj.w / rho_bar;
+        const float w = poly6(r2, s);
+        a.sx = a.sx + (vj.x - vi.x) * qm * w;
+        a.sy = a.sy + (vj.y - vi.y) * qm * w;
+        a.sz = a.sz + (vj.z - vi.z) * qm * w;
+    }
+}
+
+// Force + XSPH over the neighbours the density pass marked (same state,
+// same test), then integrate, speed cap, finite guards and the boundary
+// clamp of this particle (fluid3.rs:1088-1179, Jacobi).
 __global__ __launch_bounds__(PH_TILE) void k_force_integrate(
     const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const BodyDev *__restrict__ bodies,
     const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
-    const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, float4 *__restrict__ posm_out,
+    const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ masks,
+    const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox) {
-    const TileDev t = tiles[blockIdx.x];
-    const SceneDev &s = scenes[t.scene];
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const TileDev t = tiles[tile];
+    const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     const bool active = i < s.part_end;
     int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bz0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN, bz1 = INT32_MIN;
@@ -261,75 +418,69 @@ __global__ __launch_bounds__(PH_TILE) void k_force_integrate(
         const float2 rqi = rq[i];
         const float rho_i = rmaxf(rqi.x, PH_EPS);
         const float q_i = rqi.y;
-        const float h = s.h;
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        PairAcc acc;
+        const uint32_t fl = flags[i];
+        const float hg2 = s.h_grid2;
         if (finite3(pi.x, pi.y, pi.z)) {
             int32_t cx, cy, cz;
             cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
-            const float hg2 = s.h_grid2;
-            for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
-                if (j == i) return;
-                const float4 pj = posm[j];
-                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                const float r2 = dot3(dx, dy, dz);
-                if (!(r2 < hg2)) return;
-                const float2 rqj = rq[j];
-                const float rho_j = rmaxf(rqj.x, PH_EPS);
-                // delta = x_i - x_j = -(dx, dy, dz); |delta|^2 == r2 exactly.
-                const float ex = -dx, ey = -dy, ez = -dz;
-                const float dist = sqrtf(r2);
-                float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-                if (!(dist >= h || dist < PH_EPS)) {
-                    const float sc = s.spiky_c * (h - dist) * (h - dist);
-                    gx = ex / dist * sc;
-                    gy = ey / dist * sc;
-                    gz = ez / dist * sc;
-                }
-                float term = q_i + rqj.y;
-                if (s.av_alpha > 0.0f && dist > PH_EPS) {
-                    // fluid3.rs:254-269 (artificial_pressure)
-                    const float4 vj = vel[j];
-                    const float rho_bar = 0.5f * (rho_i + rho_j);
-                    const float vd = (vi.x - vj.x) * ex + (vi.y - vj.y) * ey + (vi.z - vj.z) * ez;
-                    float ap = 0.0f;
-                    if (!(vd >= 0.0f)) {
-                        const float mu = h * vd / (dist * dist + 0.01f * h * h);
-                        ap = -s.av_alpha * s.c_s * mu / rmaxf(rho_bar, PH_EPS);
+            if (!(fl & PH_SLOW_PATH)) {
+#pragma unroll 1
+                for (int r = 0; r < 9; ++r) {
+                    uint32_t br, er;
+                    row_range(s, start, cx, cy, cz, r, br, er);
+                    uint32_t m = masks[(size_t)r * mstride + i];
+                    m &= ~(i - br < PH_ROW_BITS ? 1u << (i - br) : 0u);  // no self pair
+                    if (m) {
+                        // Software pipeline: the next neighbour's loads are in
+                        // flight while this pair is evaluated.
+                        uint32_t j = br + __ffs(m) - 1;
+                        m &= m - 1;
+                        float4 pj = posm[j], vj = vel[j];
+                        float2 qj = rq[j];
+                        while (true) {
+                            const bool more = m != 0;
+                            uint32_t jn = j;
+                            if (more) {
+                                jn = br + __ffs(m) - 1;
+                                m &= m - 1;
+                            }
+                            const float4 pn = posm[jn], vn = vel[jn];
+                            const float2 qn = rq[jn];
+                            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                            pair_terms(s, pi, vi, rho_i, q_i, pj, vj, qj, r2, acc);
+                            if (!more) break;
+                            j = jn;
+                            pj = pn;
+                            vj = vn;
+                            qj = qn;
+                        }
                     }
-                    term = term + ap;
-                }
-                const float sm = pj.w * term;
-                fx = fx - gx * sm;
-                fy = fy - gy * sm;
-                fz = fz - gz * sm;
-                if (s.kappa > 0.0f && dist > PH_EPS) {
-                    const float coh = dist >= h ? 0.0f : 1.0f - dist / h;
-                    if (coh > 0.0f) {
-                        const float cs = s.kappa * pi.w * pj.w * coh;
-                        fx = fx - ex / dist * cs;
-                        fy = fy - ey / dist * cs;
-                        fz = fz - ez / dist * cs;
+                    if (fl & (1u << r)) {
+                        for (uint32_t j = br + PH_ROW_BITS; j < er; ++j) {
+                            if (j == i) continue;
+                            const float4 pj = posm[j];
+                            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                            if (r2 < hg2) pair_terms(s, pi, vi, rho_i, q_i, pj, vel[j], rq[j], r2, acc);
+                        }
                     }
                 }
-                if (s.viscosity > 0.0f) {
-                    const float4 vj = vel[j];
-                    const float rho_bar = 0.5f * (rho_i + rho_j);
-                    const float qm = pj.w / rho_bar;
-                    const float w = poly6(r2, s);
-                    sx = sx + (vj.x - vi.x) * qm * w;
-                    sy = sy + (vj.y - vi.y) * qm * w;
-                    sz = sz + (vj.z - vi.z) * qm * w;
-                }
-            });
+            } else {
+                for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
+                    if (j == i) return;
+                    const float4 pj = posm[j];
+                    const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                    if (r2 < hg2) pair_terms(s, pi, vi, rho_i, q_i, pj, vel[j], rq[j], r2, acc);
+                });
+            }
         }
         // fluid3.rs:1095 acc = g + F / rho (the second division by rho).
-        const float ax = s.gx + fx / rho_i, ay = s.gy + fy / rho_i, az = s.gz + fz / rho_i;
+        const float ax = s.gx + acc.fx / rho_i, ay = s.gy + acc.fy / rho_i, az = s.gz + acc.fz / rho_i;
         float ux = vi.x + ax * s.dt, uy = vi.y + ay * s.dt, uz = vi.z + az * s.dt;
         if (s.viscosity > 0.0f) {
-            ux = ux + sx * s.viscosity;
-            uy = uy + sy * s.viscosity;
-            uz = uz + sz * s.viscosity;
+            ux = ux + acc.sx * s.viscosity;
+            uy = uy + acc.sy * s.viscosity;
+            uz = uz + acc.sz * s.viscosity;
         }
         const float speed = sqrtf(dot3(ux, uy, uz));
         if (speed > s.max_velocity) {
@@ -361,8 +512,9 @@ __global__ __launch_bounds__(PH_TILE) void k_force_integrate(
             bz1 = bz0;
         }
     }
-    // Cell bounding box of the advanced particles (domain monitor): wave
-    // min/max, one atomic per wave.
+    // Cell bounding box of the advanced particles for the domain monitor:
+    // one plain store per tile (same-address atomics from every wave would
+    // serialize across the XCDs).
     for (int d = 32; d >= 1; d >>= 1) {
         bx0 = min(bx0, __shfl_xor(bx0, d, 64));
         by0 = min(by0, __shfl_xor(by0, d, 64));
@@ -371,16 +523,25 @@ __global__ __launch_bounds__(PH_TILE) void k_force_integrate(
         by1 = max(by1, __shfl_xor(by1, d, 64));
         bz1 = max(bz1, __shfl_xor(bz1, d, 64));
     }
-    if ((threadIdx.x & 63) == 0 && bx0 != INT32_MAX) {
-        int32_t *bb = bbox + 6 * t.scene;
-        atomicMin(bb + 0, bx0);
-        atomicMin(bb + 1, by0);
-        atomicMin(bb + 2, bz0);
-        atomicMax(bb + 3, bx1);
-        atomicMax(bb + 4, by1);
-        atomicMax(bb + 5, bz1);
+    __shared__ int32_t wb[4][6];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wb[w][0] = bx0;
+        wb[w][1] = by0;
+        wb[w][2] = bz0;
+        wb[w][3] = bx1;
+        wb[w][4] = by1;
+        wb[w][5] = bz1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int c = threadIdx.x;
+        int32_t v = wb[0][c];
+        for (int k = 1; k < PH_TILE / 64; ++k) v = c < 3 ? min(v, wb[k][c]) : max(v, wb[k][c]);
+        bbox[6 * (size_t)tile + c] = v;
     }
 }
+
 
 // ---------------------------------------------------------------- misc
 // Largest |v| (fluid3.rs:627-629) as float bits (non-negative floats
@@ -479,17 +640,35 @@ __global__ __launch_bounds__(PH_TILE) void k_fill_u32(uint32_t *__restrict__ p, 
     if (i < n) p[i] = v;
 }
 
-// Per-scene outlier counts + bounding boxes into one small array the
-// host copies back asynchronously (domain monitor).
-__global__ void k_monitor(const SceneDev *__restrict__ scenes, uint32_t nscenes,
-                          const uint32_t *__restrict__ start, const int32_t *__restrict__ bbox,
-                          int32_t *__restrict__ out) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nscenes) return;
+// Per-scene outlier / non-finite counts and the cell bounding box reduced
+// over the scene's tiles, into one small array the host copies back
+// asynchronously (domain monitor).  One block per scene.
+__global__ __launch_bounds__(256) void k_monitor(const SceneDev *__restrict__ scenes,
+                                                 const uint32_t *__restrict__ start,
+                                                 const int32_t *__restrict__ tile_bbox,
+                                                 int32_t *__restrict__ out) {
+    const uint32_t k = blockIdx.x;
     const SceneDev &s = scenes[k];
-    out[8 * k] = (int32_t)(start[s.bucket_base + s.ncells + 1] - start[s.bucket_base + s.ncells]);
-    out[8 * k + 1] = (int32_t)(start[s.bucket_base + s.ncells + 2] - start[s.bucket_base + s.ncells + 1]);
-    for (int c = 0; c < 6; ++c) out[8 * k + 2 + c] = bbox[6 * k + c];
+    int32_t v[6] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
+    for (uint32_t t = s.tile_begin + threadIdx.x; t < s.tile_begin + s.tile_count; t += 256) {
+        for (int c = 0; c < 3; ++c) v[c] = min(v[c], tile_bbox[6 * (size_t)t + c]);
+        for (int c = 3; c < 6; ++c) v[c] = max(v[c], tile_bbox[6 * (size_t)t + c]);
+    }
+    __shared__ int32_t red[256][6];
+    for (int c = 0; c < 6; ++c) red[threadIdx.x][c] = v[c];
+    __syncthreads();
+    for (int d = 128; d >= 1; d >>= 1) {
+        if ((int)threadIdx.x < d)
+            for (int c = 0; c < 6; ++c)
+                red[threadIdx.x][c] = c < 3 ? min(red[threadIdx.x][c], red[threadIdx.x + d][c])
+                                            : max(red[threadIdx.x][c], red[threadIdx.x + d][c]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[8 * k] = (int32_t)(start[s.bucket_base + s.ncells + 1] - start[s.bucket_base + s.ncells]);
+        out[8 * k + 1] = (int32_t)(start[s.bucket_base + s.ncells + 2] - start[s.bucket_base + s.ncells + 1]);
+        for (int c = 0; c < 6; ++c) out[8 * k + 2 + c] = red[0][c];
+    }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -516,8 +695,10 @@ hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t n
 }
 hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start,
                                 int32_t *bbox, uint32_t nscenes) {
+    (void)bbox;
+    (void)nscenes;
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlock), 0, st, partial, scan_partials_needed(nb),
-                       start, nb, bbox, nscenes);
+                       start, nb);
     return hipGetLastError();
 }
 hipError_t launch_scan_apply(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
@@ -541,17 +722,19 @@ hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *ti
 }
 hipError_t launch_density(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                           const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
-                          float *rec_rho, float *rec_p, uint32_t *rec_count) {
-    hipLaunchKernelGGL(k_density, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, rq,
-                       rec_rho, rec_p, rec_count);
+                          uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
+                          uint32_t *rec_count) {
+    hipLaunchKernelGGL(k_density, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, rq, masks,
+                       flags, mstride, rec_rho, rec_p, rec_count);
     return hipGetLastError();
 }
 hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                                   const BodyDev *bodies, const uint32_t *start, const float4 *posm,
-                                  const float4 *vel, const float2 *rq, const uint32_t *tag, float4 *posm_out,
-                                  float4 *vel_out, float *rec_acc, int32_t *bbox) {
+                                  const float4 *vel, const float2 *rq, const uint32_t *tag, const uint32_t *masks,
+                                  const uint32_t *flags, uint32_t mstride, float4 *posm_out, float4 *vel_out,
+                                  float *rec_acc, int32_t *bbox) {
     hipLaunchKernelGGL(k_force_integrate, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm,
-                       vel, rq, tag, posm_out, vel_out, rec_acc, bbox);
+                       vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox);
     return hipGetLastError();
 }
 hipError_t launch_vmax(hipStream_t st, uint32_t n, const float4 *vel, uint32_t *out_bits) {
@@ -583,8 +766,8 @@ hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v) 
     return hipGetLastError();
 }
 hipError_t launch_monitor(hipStream_t st, const SceneDev *scenes, uint32_t nscenes, const uint32_t *start,
-                          const int32_t *bbox, int32_t *out) {
-    hipLaunchKernelGGL(k_monitor, dim3((nscenes + 63) / 64), dim3(64), 0, st, scenes, nscenes, start, bbox, out);
+                          const int32_t *tile_bbox, int32_t *out) {
+    hipLaunchKernelGGL(k_monitor, dim3(nscenes), dim3(256), 0, st, scenes, start, tile_bbox, out);
     return hipGetLastError();
 }
 
