@@ -1,0 +1,161 @@
+"""C ABI of libphieng_hip.so without a GPU: exports, layouts and the host
+logic of the reference's FFI (crates/engine/src/ffi.rs) — validation,
+null tolerance, return codes.  No compute call runs here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from phi_eng_amd import ffi
+from phi_eng_amd.ffi import FluidConfig3Ffi
+from phi_eng_amd.world import World3
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", f)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names |= set(re.findall(r"\b(phieng_\w+)\s*\(", src))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ffi.lib()
+    declared = _declared_functions()
+    assert len(declared) >= 20
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert declared == set(ffi.SIGNATURES), declared ^ set(ffi.SIGNATURES)
+
+
+def test_fluid_config_layout_matches_ffi_rs():
+    """#[repr(C)] FluidConfig3Ffi (ffi.rs:743-767): 112 bytes."""
+    assert ctypes.sizeof(FluidConfig3Ffi) == 112
+    off = {n: getattr(FluidConfig3Ffi, n).offset for n, _ in FluidConfig3Ffi._fields_}
+    assert off["max_substeps"] == 32 and off["adaptive_resolution"] == 36
+    assert off["split_density_ratio"] == 40 and off["boundary_particles"] == 48
+    assert off["viscosity"] == 52 and off["phase_count"] == 64
+    assert off["phase_rest_density"] == 68 and off["phase_mass"] == 84
+    assert off["max_velocity"] == 100 and off["particle_radius"] == 104 and off["enabled"] == 108
+
+
+def test_default_config_round_trip():
+    """FluidConfig3::default (fluid3.rs:105-134) and get/set round trip
+    (ffi.rs:5841-5900)."""
+    with World3() as w:
+        c = w.get_fluid_config()
+        d = FluidConfig3Ffi.default()
+        assert bytes(c) == bytes(d)
+        d.stiffness = 123.0
+        d.gamma = 7.0
+        assert w.set_fluid_config(d)
+        assert w.get_fluid_config().stiffness == np.float32(123.0)
+
+
+@pytest.mark.parametrize("field,value", [
+    ("particle_mass", 0.0), ("rest_density", -1.0), ("smoothing_length", 0.0), ("stiffness", -1.0),
+    ("gamma", 0.5), ("artificial_viscosity", 2.5), ("surface_tension", -0.1), ("cfl", 1.5),
+    ("max_substeps", 0), ("split_density_ratio", 0.0), ("merge_density_ratio", 1.0), ("viscosity", 1.5),
+    ("drag", -1.0), ("wall_friction", 1.5), ("phase_count", 0), ("phase_count", 5), ("max_velocity", -1.0),
+    ("particle_radius", -0.1), ("stiffness", float("nan")), ("viscosity", float("inf")),
+])
+def test_invalid_config_rejected_world_unchanged(field, value):
+    """fluid3_from_ffi rules (ffi.rs:857-914): 0 returned, config unchanged."""
+    with World3() as w:
+        before = bytes(w.get_fluid_config())
+        c = FluidConfig3Ffi.default()
+        setattr(c, field, value)
+        assert not w.set_fluid_config(c)
+        assert bytes(w.get_fluid_config()) == before
+
+
+def test_phase_tables_validated():
+    with World3() as w:
+        c = FluidConfig3Ffi.default()
+        c.phase_mass[2] = 0.0
+        assert not w.set_fluid_config(c)
+        c = FluidConfig3Ffi.default()
+        c.phase_rest_density[3] = float("nan")
+        assert not w.set_fluid_config(c)
+
+
+def test_unimplemented_options_rejected():
+    with World3() as w:
+        for f in ("adaptive_resolution", "boundary_particles"):
+            c = FluidConfig3Ffi.default()
+            setattr(c, f, 1)
+            assert not w.set_fluid_config(c)
+            assert f in ffi.last_error()
+
+
+def test_null_tolerance():
+    """Every entry point tolerates a null world (ffi.rs:1-19, 7255-7300)."""
+    lib = ffi.lib()
+    buf = np.zeros(6, dtype=np.float32)
+    fp = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    lib.phieng_world3_free(None)
+    lib.phieng_world3_step(None, 0.016)
+    lib.phieng_world3_set_gravity(None, 0.0, 0.0, 0.0)
+    assert lib.phieng_world3_add_body(None, 0, 0, 0, 1, 1, 1, 1, 1, 1, 0.5, 0) == -1
+    assert lib.phieng_world3_set_fluid_config(None, None) == 0
+    assert lib.phieng_world3_get_fluid_config(None, None) == 0
+    assert lib.phieng_world3_fluid_add(None, fp, 1) == 0
+    assert lib.phieng_world3_fluid_add_phase(None, fp, None, 1) == 0
+    assert lib.phieng_world3_fluid_count(None) == 0
+    assert lib.phieng_world3_fluid_clear(None) == 0
+    assert lib.phieng_world3_fluid_get(None, fp, 1) == 0
+    assert lib.phieng_world3_fluid_reserve(None, 10) == 0
+    assert lib.phieng_world3_sync(None) == 0
+    with World3() as w:
+        assert lib.phieng_world3_fluid_add(w.ptr, None, 1) == 0
+        assert lib.phieng_world3_fluid_get(w.ptr, None, 1) == 0
+        assert lib.phieng_world3_get_fluid_config(w.ptr, None) == 0
+        lib.phieng_world3_step(w.ptr, float("nan"))
+        lib.phieng_world3_step(w.ptr, -1.0)
+        lib.phieng_world3_step(w.ptr, 0.016)  # empty fluid: no-op, no device needed
+
+
+def test_add_body_rules():
+    """ffi.rs:3703-3760: shape validation, -1 on bad input, ids count up."""
+    with World3() as w:
+        assert w.add_body((0, 0, 0), 1, 1, 1, 1) == 0
+        assert w.add_body((0, 0, 0), 0, 0.5) == 1
+        assert w.add_body((0, 0, 0), 2, 0.2, 0.5) == 2
+        assert w.add_body((0, 0, 0), 1, 1, -1, 1) == -1
+        assert w.add_body((0, 0, 0), 0, 0.0) == -1
+        assert w.add_body((0, 0, 0), 7, 1.0) == -1
+        assert w.add_body((float("nan"), 0, 0), 0, 1.0) == -1
+        assert w.add_body((0, 0, 0), 0, 1.0, density=0.0) == -1
+        # moving bodies are not part of the GPU fluid path: refused, not ignored
+        assert w.add_body((0, 0, 0), 0, 1.0, is_static=0) == -1
+        assert "dynamic" in ffi.last_error()
+
+
+def test_fluid_add_rejects_non_finite_atomically():
+    """ffi.rs:3592-3594: any non-finite value -> 0 added, nothing changes
+    (checked before any device work)."""
+    with World3() as w:
+        pts = np.zeros((4, 3), dtype=np.float32)
+        pts[2, 1] = np.nan
+        assert w.add_particles(pts) == 0
+        assert w.count() == 0
+
+
+def test_capacity_defaults_to_reference_cap():
+    with World3() as w:
+        assert w.reserve(0) == 4096  # MAX_PARTICLES_3D (fluid3.rs:28)
+        assert w.reserve(1 << 20) == 1 << 20
+        assert w.reserve(1 << 30) == 1 << 24
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(ffi, "_lib", None)
+    monkeypatch.setattr(ffi, "lib_path", str(tmp_path / "missing.so"))
+    with pytest.raises(RuntimeError, match="missing"):
+        ffi.lib()

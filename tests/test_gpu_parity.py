@@ -107,3 +107,67 @@ def test_run_twice_bit_identical(gpu):
         w.close()
     assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
     assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
+
+
+def _world_from_fixture(g):
+    from phi_eng_amd.world import World3
+    from tests.golden.make_golden import config_from
+    w = World3(capacity=max(4096, len(g["pos_in"])))
+    assert w.set_fluid_config(config_from(g))
+    w.set_gravity(g["gravity"])
+    for row in g["bodies"]:
+        assert w.add_body(row[1:4], int(row[0]), row[4], row[5], row[6]) >= 0
+    assert w.add_particles(g["pos_in"], phases=g["phase"]) == len(g["pos_in"])
+    assert w.set_velocity(g["vel_in"]) == len(g["pos_in"])
+    return w
+
+
+@pytest.mark.parametrize("name", ["dam8_rest", "dam6_options", "edge_default_cfg"])
+def test_golden_fixture(gpu, name):
+    """One GPU substep from each committed fixture equals the fixture's
+    oracle outputs (counts exactly; floats within 1e-5, bit-equality reported)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"{name}.npz"))
+    w = _world_from_fixture(g)
+    w.set_record(RECORD_CELLS | RECORD_DENSITY | RECORD_ACCEL)
+    w.step(float(g["dt"]))
+    rec = w.get_record()
+    pos1, vel1 = w.get_state()
+    print(f"[{name}] n={len(pos1)}", flush=True)
+    assert np.array_equal(rec["count"], g["count"])
+    assert _rel(rec["density"], g["density"]).max() <= REL
+    s = np.maximum(np.linalg.norm(g["accel"], axis=1), np.linalg.norm(g["gravity"]) + g["scale"])
+    assert (np.abs(rec["accel"].astype(np.float64) - g["accel"]).max(axis=1) / s).max() <= REL
+    assert np.abs(pos1.astype(np.float64) - g["pos_out"]).max() <= REL * max(1.0, np.abs(g["pos_out"]).max())
+    assert np.abs(vel1.astype(np.float64) - g["vel_out"]).max() <= REL * max(1.0, np.abs(g["vel_out"]).max())
+    bits = np.mean(np.concatenate([rec["density"].view(np.uint32) == g["density"].view(np.uint32),
+                                   (rec["accel"].view(np.uint32) == g["accel"].view(np.uint32)).ravel(),
+                                   (pos1.view(np.uint32) == g["pos_out"].view(np.uint32)).ravel()]))
+    print(f"[{name}] bit-equal fraction {bits:.4f}", flush=True)
+
+
+def test_full_size_1m_step(gpu, oracle_lib):
+    """The 1,048,576-particle dam break: one step against the oracle at full
+    size (cells, order, counts exact; floats within 1e-5), after 20 GPU steps."""
+    sc = scenes.c2_dam_break_1m()
+    w = scenes.make_world(sc)
+    for _ in range(20):
+        w.step(sc.dt)
+    _compare_step(w, sc, oracle_lib, "1M after 20 steps")
+
+
+def test_full_size_invariants(gpu):
+    """Size-independent properties at 1M over 100 steps: count preserved,
+    finite, contained by the tank, mass conserved, bounded speed."""
+    sc = scenes.c2_dam_break_1m()
+    w = scenes.make_world(sc)
+    for _ in range(100):
+        w.step(sc.dt)
+    pos, vel = w.get_state()
+    assert len(pos) == sc.n == w.count()
+    assert np.isfinite(pos).all() and np.isfinite(vel).all()
+    pr = sc.config.particle_radius
+    assert (pos[:, 0] >= pr - 1e-4).all() and (pos[:, 0] <= 12.8 - pr + 1e-4).all()
+    assert (pos[:, 1] >= pr - 1e-4).all()
+    assert (pos[:, 2] >= pr - 1e-4).all() and (pos[:, 2] <= 6.4 - pr + 1e-4).all()
+    assert np.linalg.norm(vel, axis=1).max() <= sc.config.max_velocity * (1 + 1e-6)
