@@ -35,13 +35,34 @@ def _args():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=2, help="CPU oracle sample length (steps)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_force_integrate.json"),
-                    help="rocprofv3 FETCH/WRITE summary for the dominant kernel (optional)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="HBM bytes per launch per kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE (optional)")
+    ap.add_argument("--calibrate", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (rocprof counter scale)")
     return ap.parse_args()
+
+
+def calibrate(mib):
+    """A plain copy of known size through the HIP runtime (its copy kernel
+    reads and writes mib MiB), for scaling FETCH_SIZE / WRITE_SIZE."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(5):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"calibrate_copy_bytes": n, "copies": 5}))
 
 
 def main():
     a = _args()
+    if a.calibrate:
+        calibrate(a.calibrate)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU form of this "
               "path is design-only", file=sys.stderr)
@@ -85,8 +106,7 @@ def main():
     if os.path.exists(a.traffic):
         with open(a.traffic) as f:
             tr = json.load(f)
-        if tr.get("kernel") == dom:
-            traffic = tr.get("bytes_per_launch")
+        traffic = tr.get("bytes_per_launch", {}).get(dom)
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "alg_bytes_per_particle": ALG_BYTES[dom], "avg_launch_us": round(kernels[dom]["avg_us"], 2)}
