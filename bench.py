@@ -37,6 +37,8 @@ def _args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="HBM bytes per launch per kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE (optional)")
+    ap.add_argument("--workload", default="plain", choices=["plain", "batch"],
+                    help="plain: the 1M dam break (the headline); batch: 256 uneven scenes in one launch per step")
     ap.add_argument("--calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (rocprof counter scale)")
     return ap.parse_args()
@@ -67,6 +69,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU form of this "
               "path is design-only", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "batch":
+        return bench_batch(a)
     from phi_eng_amd import scenes
     from phi_eng_amd.ffi import last_error
 
@@ -147,6 +151,46 @@ def main():
         "timed_with_events_ms_per_step": round((t3 - t2) * 1e3 / a.steps, 4),
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out))
+
+
+def bench_batch(a):
+    """256 independent dam scenes (counts log-uniform in [2048, 65536], seed 3),
+    one persistent launch per step from a hipGraph, launch shape autotuned."""
+    from phi_eng_amd import scenes
+    from phi_eng_amd.world import Batch3
+    scs = scenes.batch_scenes(count=256, seed=3)
+    worlds = [scenes.make_world(sc) for sc in scs]
+    b = Batch3(worlds)
+    n = b.count()
+    dt = 1e-4
+    cache = os.path.join(ROOT, "profiles", "batch_tune_cache.txt")
+    mode, bpc, tuned_ms = b.autotune(dt, trial_steps=10, cache_path=cache)
+    for _ in range(a.warmup):
+        b.step(dt)
+    b.sync()
+    steps = a.steps if a.steps != 500 else 200
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step(dt)
+    b.sync()
+    wall = time.perf_counter() - t0
+    b.set_timing(True)
+    for _ in range(20):
+        b.step(dt)
+    b.sync()
+    timing = b.get_timing()
+    kern = {k: round(1e3 * ms / c, 2) for k, (ms, c) in timing.items()}
+    out = {"metric": "particle-steps/sec, batched 3D SPH (256 scenes)", "value": round(n * steps / wall, 1),
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+           "ms_per_step": round(wall * 1e3 / steps, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic: 256 seeded jittered dam breaks",
+           "config": {"workload": "batch_256_scenes", "particles": n, "scenes": len(scs),
+                      "counts": [int(min(sc.n for sc in scs)), int(max(sc.n for sc in scs))], "dt": dt,
+                      "mode": {0: "kernels", 1: "persistent", 2: "persistent+hipGraph"}[mode],
+                      "blocks_per_cu": bpc, "autotune_ms_per_step": round(tuned_ms, 4)},
+           "step_roofline_frac": round(B_ALG_STEP * n / (wall / steps) / (HBM_PEAK_GBS * 1e9), 5),
+           "kernels": kern}
     print(json.dumps(out))
 
 

@@ -141,6 +141,41 @@ int32_t phieng_world3_sync(World3 *world);
 /* Text of the last error seen by any entry point of this thread ("" if none). */
 const char *phieng_last_error(void);
 
+/* ---------------------------------------------------------------------
+ * Batched stepping (no reference counterpart): many independent worlds
+ * advance together.  phieng_batch3_new copies the worlds' particles into
+ * one concatenated device state (each world keeps its config, gravity,
+ * static bodies and grid); the worlds are not touched again until
+ * phieng_batch3_sync_worlds writes the state back.  Every scene's
+ * trajectory is bit-identical to its world stepped alone.
+ * ------------------------------------------------------------------- */
+typedef struct Batch3 Batch3;
+#define PHIENG_BATCH_KERNELS 0           /* the eight launches of a single world   */
+#define PHIENG_BATCH_PERSISTENT 1        /* one persistent launch per substep      */
+#define PHIENG_BATCH_PERSISTENT_GRAPH 2  /* the persistent launch from a hipGraph  */
+/* Null (with phieng_batch3_last_error) if a world is null, disabled, uses
+ * CFL substepping, or the total exceeds 2^24 particles. */
+Batch3 *phieng_batch3_new(World3 *const *worlds, uint32_t count);
+void phieng_batch3_free(Batch3 *batch);
+/* Launch configuration; blocks_per_cu = 0 uses the occupancy limit. */
+int32_t phieng_batch3_set_mode(Batch3 *batch, int32_t mode, uint32_t blocks_per_cu);
+/* Advance every world by dt (same rules as phieng_world3_step). 1 / 0. */
+int32_t phieng_batch3_step(Batch3 *batch, float dt);
+/* Wait for the queued steps; 0 if a grid barrier timed out. */
+int32_t phieng_batch3_sync(Batch3 *batch);
+/* Write the batch state back into its worlds. 1 / 0. */
+int32_t phieng_batch3_sync_worlds(Batch3 *batch);
+uint32_t phieng_batch3_count(const Batch3 *batch);
+/* Pick the fastest launch configuration by timing `trial_steps` substeps
+ * of each from the same saved state (restored afterwards); the choice is
+ * appended to the text file cache_path (may be null) and reused from it.
+ * Returns the chosen mode or -1. */
+int32_t phieng_batch3_autotune(Batch3 *batch, float dt, uint32_t trial_steps, const char *cache_path,
+                               uint32_t *out_blocks_per_cu, double *out_ms_per_step);
+int32_t phieng_batch3_set_timing(Batch3 *batch, int32_t enable);
+int32_t phieng_batch3_get_timing(Batch3 *batch, double *total_ms, uint64_t *launches, int32_t max);
+const char *phieng_batch3_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,6 +6,27 @@
 
 namespace ph {
 
+// Arguments of the persistent one-launch substep (all scenes).
+struct PersistArgs {
+    const TileDev *tiles;
+    uint32_t ntiles;
+    const SceneDev *scenes;
+    const BodyDev *bodies;
+    float4 *posm_cur, *vel_cur, *posm_alt, *vel_alt;
+    uint32_t *tag_cur, *tag_alt;
+    uint32_t *keys, *slots, *tmp, *counts, *start, *partial;
+    float2 *rq;
+    uint32_t *masks, *flags;
+    uint32_t mstride;
+    int32_t *tile_bbox;
+    uint32_t n, nbuckets;
+    uint32_t *bar;  // [arrivals, generation, abort]
+    uint32_t spin_limit;
+    int32_t *rec_cells;
+    uint32_t *rec_sorted, *rec_count;
+    float *rec_rho, *rec_p, *rec_acc;
+};
+
 uint32_t scan_partials_needed(uint32_t nbuckets);
 
 hipError_t launch_cell_key(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
@@ -42,5 +63,9 @@ hipError_t launch_cell_bbox(hipStream_t st, uint32_t n, const float4 *posm, floa
 hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
 hipError_t launch_monitor(hipStream_t st, const SceneDev *scenes, uint32_t nscenes, const uint32_t *start,
                           const int32_t *bbox, int32_t *out);
+
+hipError_t launch_step_persistent(hipStream_t st, uint32_t blocks, const PersistArgs &a);
+hipError_t persistent_max_blocks_per_cu(int *out);
+hipError_t launch_tag_shift(hipStream_t st, uint32_t n, const uint32_t *src, uint32_t *dst, int32_t delta);
 
 }  // namespace ph

@@ -60,7 +60,7 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nt) {
 // Bucket key per particle and its slot inside the bucket.  Particles come
 // in last substep's sorted order, so neighbouring lanes mostly share a
 // bucket: one atomic per run of equal keys in a wave, not one per lane.
-__global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict__ tiles,
+__device__ __forceinline__ void dev_cell_key(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles,
                                                       const SceneDev *__restrict__ scenes,
                                                       const float4 *__restrict__ posm,
                                                       const uint32_t *__restrict__ tag,
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict_
                                                       uint32_t *__restrict__ slots,
                                                       uint32_t *__restrict__ counts,
                                                       int32_t *__restrict__ rec_cells) {
-    const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = tiles[xcd_tile(bid, nbl)];
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     const bool active = i < s.part_end;
@@ -109,21 +109,37 @@ __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict_
     }
 }
 
+__global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict__ tiles,
+                                                      const SceneDev *__restrict__ scenes,
+                                                      const float4 *__restrict__ posm,
+                                                      const uint32_t *__restrict__ tag,
+                                                      uint32_t *__restrict__ keys,
+                                                      uint32_t *__restrict__ slots,
+                                                      uint32_t *__restrict__ counts,
+                                                      int32_t *__restrict__ rec_cells) {
+    dev_cell_key(blockIdx.x, gridDim.x, tiles, scenes, posm, tag, keys, slots, counts, rec_cells);
+}
+
 // ---------------------------------------------------------------- K2
-__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__restrict__ counts,
+__device__ __forceinline__ void dev_scan_reduce(uint32_t bid, uint32_t nbl, const uint32_t *__restrict__ counts,
                                                             uint32_t nb, uint32_t *__restrict__ partial) {
-    const uint32_t base = blockIdx.x * (kScanBlock * kScanItems) + threadIdx.x * kScanItems;
+    const uint32_t base = bid * (kScanBlock * kScanItems) + threadIdx.x * kScanItems;
     uint32_t v = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
         if (base + k < nb) v += counts[base + k];
     uint32_t tot;
     block_excl_scan(v, tot);
-    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+    if (threadIdx.x == 0) partial[bid] = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__restrict__ counts,
+                                                            uint32_t nb, uint32_t *__restrict__ partial) {
+    dev_scan_reduce(blockIdx.x, gridDim.x, counts, nb, partial);
 }
 
 // Single block: exclusive scan of the block partials, start[nb] = total.
-__global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t *__restrict__ partial, uint32_t np,
+__device__ __forceinline__ void dev_scan_partials(uint32_t bid, uint32_t nbl, uint32_t *__restrict__ partial, uint32_t np,
                                                               uint32_t *__restrict__ start, uint32_t nb) {
     uint32_t carry = 0;
     for (uint32_t c = 0; c < np; c += kScanBlock) {
@@ -137,12 +153,17 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t *__restri
     if (threadIdx.x == 0) start[nb] = carry;
 }
 
+__global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t *__restrict__ partial, uint32_t np,
+                                                              uint32_t *__restrict__ start, uint32_t nb) {
+    dev_scan_partials(blockIdx.x, gridDim.x, partial, np, start, nb);
+}
+
 // Bucket start = block offset + in-block exclusive scan; counts are zeroed
 // for the next substep's atomics.
-__global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t *__restrict__ counts, uint32_t nb,
+__device__ __forceinline__ void dev_scan_apply(uint32_t bid, uint32_t nbl, uint32_t *__restrict__ counts, uint32_t nb,
                                                            const uint32_t *__restrict__ partial,
                                                            uint32_t *__restrict__ start) {
-    const uint32_t base = blockIdx.x * (kScanBlock * kScanItems) + threadIdx.x * kScanItems;
+    const uint32_t base = bid * (kScanBlock * kScanItems) + threadIdx.x * kScanItems;
     uint32_t c[kScanItems];
     uint32_t v = 0;
 #pragma unroll
@@ -151,7 +172,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t *__restrict_
         v += c[k];
     }
     uint32_t tot;
-    uint32_t off = block_excl_scan(v, tot) + partial[blockIdx.x];
+    uint32_t off = block_excl_scan(v, tot) + partial[bid];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (base + k < nb) {
@@ -162,26 +183,38 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t *__restrict_
     }
 }
 
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t *__restrict__ counts, uint32_t nb,
+                                                           const uint32_t *__restrict__ partial,
+                                                           uint32_t *__restrict__ start) {
+    dev_scan_apply(blockIdx.x, gridDim.x, counts, nb, partial, start);
+}
+
 // ---------------------------------------------------------------- K3
+__device__ __forceinline__ void dev_scatter(uint32_t bid, uint32_t nbl, uint32_t n, const uint32_t *__restrict__ keys,
+                                                     const uint32_t *__restrict__ slots,
+                                                     const uint32_t *__restrict__ start,
+                                                     uint32_t *__restrict__ tmp) {
+    const uint32_t i = bid * PH_TILE + threadIdx.x;
+    if (i >= n) return;
+    tmp[start[keys[i]] + slots[i]] = i;
+}
+
 __global__ __launch_bounds__(PH_TILE) void k_scatter(uint32_t n, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ slots,
                                                      const uint32_t *__restrict__ start,
                                                      uint32_t *__restrict__ tmp) {
-    const uint32_t i = blockIdx.x * PH_TILE + threadIdx.x;
-    if (i >= n) return;
-    tmp[start[keys[i]] + slots[i]] = i;
+    dev_scatter(blockIdx.x, gridDim.x, n, keys, slots, start, tmp);
 }
 
 // Final slot = bucket start + rank of the particle inside its bucket.  In
 // a grid cell the rank orders by original index, which is the order the
 // reference's bucket Vec holds (fluid3.rs:301-314); in the outlier bucket
 // it orders by (cell z, y, x) first so every outlier cell is contiguous.
-__global__ __launch_bounds__(PH_TILE) void k_rank_gather(
-    const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const uint32_t *__restrict__ tmp,
+__device__ __forceinline__ void dev_rank_gather(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const uint32_t *__restrict__ tmp,
     const uint32_t *__restrict__ keys, const uint32_t *__restrict__ start, const float4 *__restrict__ posm_in,
     const float4 *__restrict__ vel_in, const uint32_t *__restrict__ tag_in, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted) {
-    const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = tiles[xcd_tile(bid, nbl)];
     const SceneDev s = scenes[t.scene];
     const uint32_t q = t.first + threadIdx.x;
     if (q >= s.part_end) return;
@@ -212,6 +245,13 @@ __global__ __launch_bounds__(PH_TILE) void k_rank_gather(
     vel_out[d] = vel_in[src];
     tag_out[d] = mytag;
     if (rec_sorted) rec_sorted[d] = myid;
+}
+
+__global__ __launch_bounds__(PH_TILE) void k_rank_gather(const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const uint32_t *__restrict__ tmp,
+    const uint32_t *__restrict__ keys, const uint32_t *__restrict__ start, const float4 *__restrict__ posm_in,
+    const float4 *__restrict__ vel_in, const uint32_t *__restrict__ tag_in, float4 *__restrict__ posm_out,
+    float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted) {
+    dev_rank_gather(blockIdx.x, gridDim.x, tiles, scenes, tmp, keys, start, posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted);
 }
 
 // ---------------------------------------------------------------- rows
@@ -247,7 +287,7 @@ __device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__r
 // neighbour order.  Rows with more than 32 candidates finish their tail
 // inline and flag it.  rq[i] = (rho_i, p_i / max(rho_i, EPS)^2), the
 // pressure quotient of the force pass (fluid3.rs:1046-1051).
-__global__ __launch_bounds__(PH_TILE) void k_density(const TileDev *__restrict__ tiles,
+__device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles,
                                                      const SceneDev *__restrict__ scenes,
                                                      const uint32_t *__restrict__ start,
                                                      const float4 *__restrict__ posm,
@@ -255,7 +295,7 @@ __global__ __launch_bounds__(PH_TILE) void k_density(const TileDev *__restrict__
                                                      uint32_t *__restrict__ masks, uint32_t *__restrict__ flags,
                                                      uint32_t mstride, float *__restrict__ rec_rho,
                                                      float *__restrict__ rec_p, uint32_t *__restrict__ rec_count) {
-    const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = tiles[xcd_tile(bid, nbl)];
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     if (i >= s.part_end) return;
@@ -340,6 +380,17 @@ __global__ __launch_bounds__(PH_TILE) void k_density(const TileDev *__restrict__
     }
 }
 
+__global__ __launch_bounds__(PH_TILE) void k_density(const TileDev *__restrict__ tiles,
+                                                     const SceneDev *__restrict__ scenes,
+                                                     const uint32_t *__restrict__ start,
+                                                     const float4 *__restrict__ posm,
+                                                     const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
+                                                     uint32_t *__restrict__ masks, uint32_t *__restrict__ flags,
+                                                     uint32_t mstride, float *__restrict__ rec_rho,
+                                                     float *__restrict__ rec_p, uint32_t *__restrict__ rec_count) {
+    dev_density(blockIdx.x, gridDim.x, tiles, scenes, start, posm, tag, rq, masks, flags, mstride, rec_rho, rec_p, rec_count);
+}
+
 // ---------------------------------------------------------------- K5
 // Per-pair force (fluid3.rs:1042-1085) and XSPH (fluid3.rs:1114-1123) terms,
 // accumulated in the reference's operation order.
@@ -400,13 +451,12 @@ __device__ __forceinline__ void pair_terms(const SceneDev &s, const float4 &pi, 
 // Force + XSPH over the neighbours the density pass marked (same state,
 // same test), then integrate, speed cap, finite guards and the boundary
 // clamp of this particle (fluid3.rs:1088-1179, Jacobi).
-__global__ __launch_bounds__(PH_TILE) void k_force_integrate(
-    const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const BodyDev *__restrict__ bodies,
+__device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const BodyDev *__restrict__ bodies,
     const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
     const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ masks,
     const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox) {
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t tile = xcd_tile(bid, nbl);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
@@ -542,6 +592,101 @@ __global__ __launch_bounds__(PH_TILE) void k_force_integrate(
     }
 }
 
+__global__ __launch_bounds__(PH_TILE) void k_force_integrate(const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const BodyDev *__restrict__ bodies,
+    const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
+    const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ masks,
+    const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
+    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox) {
+    dev_force_integrate(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox);
+}
+
+
+// ---------------------------------------------------------------- persistent
+// Grid-wide barrier for the persistent step.  bar[0] counts arrivals,
+// bar[1] is the generation, bar[2] an abort flag.  Agent-scope release on
+// arrival and acquire on departure make every block's writes of the phase
+// visible to all XCDs.  The wait is bounded: a block that spins past
+// spin_limit raises the abort flag and every block leaves at its next
+// barrier, so the launch always ends (the host then reports the error).
+__device__ __forceinline__ bool grid_barrier(uint32_t *bar, uint32_t nblocks, uint32_t spin_limit) {
+    __shared__ uint32_t ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t good = 1;
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                    ++spins > spin_limit) {
+                    __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    good = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+// One substep of every scene in a single launch: the eight phases of the
+// multi-kernel path, separated by grid barriers.  Each block walks the work
+// items of a phase with stride gridDim.x (a multiple of 8, so item k stays
+// on XCD k mod 8 as in the multi-kernel launches).
+__global__ __launch_bounds__(PH_TILE) void k_step_persistent(ph::PersistArgs a) {
+    const uint32_t G = gridDim.x;
+    const uint32_t np = (a.nbuckets + kScanBlock * kScanItems - 1) / (kScanBlock * kScanItems);
+    const uint32_t nsb = (a.n + PH_TILE - 1) / PH_TILE;
+    for (uint32_t k = blockIdx.x; k < a.ntiles; k += G) {
+        dev_cell_key(k, a.ntiles, a.tiles, a.scenes, a.posm_cur, a.tag_cur, a.keys, a.slots, a.counts, a.rec_cells);
+        __syncthreads();
+    }
+    if (!grid_barrier(a.bar, G, a.spin_limit)) return;
+    for (uint32_t k = blockIdx.x; k < np; k += G) {
+        dev_scan_reduce(k, np, a.counts, a.nbuckets, a.partial);
+        __syncthreads();
+    }
+    if (!grid_barrier(a.bar, G, a.spin_limit)) return;
+    if (blockIdx.x == 0) dev_scan_partials(0, 1, a.partial, np, a.start, a.nbuckets);
+    if (!grid_barrier(a.bar, G, a.spin_limit)) return;
+    for (uint32_t k = blockIdx.x; k < np; k += G) {
+        dev_scan_apply(k, np, a.counts, a.nbuckets, a.partial, a.start);
+        __syncthreads();
+    }
+    if (!grid_barrier(a.bar, G, a.spin_limit)) return;
+    for (uint32_t k = blockIdx.x; k < nsb; k += G) dev_scatter(k, nsb, a.n, a.keys, a.slots, a.start, a.tmp);
+    if (!grid_barrier(a.bar, G, a.spin_limit)) return;
+    for (uint32_t k = blockIdx.x; k < a.ntiles; k += G)
+        dev_rank_gather(k, a.ntiles, a.tiles, a.scenes, a.tmp, a.keys, a.start, a.posm_cur, a.vel_cur, a.tag_cur,
+                        a.posm_alt, a.vel_alt, a.tag_alt, a.rec_sorted);
+    if (!grid_barrier(a.bar, G, a.spin_limit)) return;
+    for (uint32_t k = blockIdx.x; k < a.ntiles; k += G)
+        dev_density(k, a.ntiles, a.tiles, a.scenes, a.start, a.posm_alt, a.tag_alt, a.rq, a.masks, a.flags,
+                    a.mstride, a.rec_rho, a.rec_p, a.rec_count);
+    if (!grid_barrier(a.bar, G, a.spin_limit)) return;
+    for (uint32_t k = blockIdx.x; k < a.ntiles; k += G) {
+        dev_force_integrate(k, a.ntiles, a.tiles, a.scenes, a.bodies, a.start, a.posm_alt, a.vel_alt, a.rq,
+                            a.tag_alt, a.masks, a.flags, a.mstride, a.posm_cur, a.vel_cur, a.rec_acc, a.tile_bbox);
+        __syncthreads();
+    }
+}
+
+// Tags of a world's particles moved into (delta > 0) or out of (delta < 0)
+// a batch: original index shifted by the world's offset, phase kept.
+__global__ __launch_bounds__(PH_TILE) void k_tag_shift(uint32_t n, const uint32_t *__restrict__ src,
+                                                       uint32_t *__restrict__ dst, int32_t delta) {
+    const uint32_t i = blockIdx.x * PH_TILE + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t t = src[i];
+    dst[i] = (t & ~PH_ID_MASK) | ((uint32_t)((int32_t)(t & PH_ID_MASK) + delta) & PH_ID_MASK);
+}
 
 // ---------------------------------------------------------------- misc
 // Largest |v| (fluid3.rs:627-629) as float bits (non-negative floats
@@ -768,6 +913,18 @@ hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v) 
 hipError_t launch_monitor(hipStream_t st, const SceneDev *scenes, uint32_t nscenes, const uint32_t *start,
                           const int32_t *tile_bbox, int32_t *out) {
     hipLaunchKernelGGL(k_monitor, dim3(nscenes), dim3(256), 0, st, scenes, start, tile_bbox, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_persistent(hipStream_t st, uint32_t blocks, const PersistArgs &a) {
+    hipLaunchKernelGGL(k_step_persistent, dim3(blocks), dim3(PH_TILE), 0, st, a);
+    return hipGetLastError();
+}
+hipError_t persistent_max_blocks_per_cu(int *out) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_step_persistent, PH_TILE, 0);
+}
+hipError_t launch_tag_shift(hipStream_t st, uint32_t n, const uint32_t *src, uint32_t *dst, int32_t delta) {
+    hipLaunchKernelGGL(k_tag_shift, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, n, src, dst, delta);
     return hipGetLastError();
 }
 

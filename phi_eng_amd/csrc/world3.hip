@@ -15,6 +15,7 @@
 #include "../../include/phieng_fluid3.h"
 #include "engine.h"
 #include "sph_kernels.h"
+#include "world3_internal.h"
 
 static_assert(sizeof(FluidConfig3Ffi) == 112, "FluidConfig3Ffi must match ffi.rs:743-767");
 static_assert(offsetof(FluidConfig3Ffi, max_substeps) == 32, "layout");
@@ -240,6 +241,28 @@ struct World3 {
         return m;
     }
 };
+
+uint32_t world3_count(const World3 *w) { return w->n; }
+const FluidConfig3Ffi &world3_config(const World3 *w) { return w->cfg; }
+const std::vector<BodyDev> &world3_bodies(const World3 *w) { return w->bodies; }
+ph::SceneHost world3_scene_host(const World3 *w, float dt) { return w->scene_host(dt); }
+void world3_device_state(const World3 *w, const float4 **posm, const float4 **vel, const uint32_t **tag,
+                         hipStream_t *stream) {
+    const ph::Engine &e = w->eng;
+    *posm = e.posm[e.cs].p;
+    *vel = e.vel[e.cs].p;
+    *tag = e.tag[e.ct].p;
+    *stream = e.stream;
+}
+void world3_device_state_mut(World3 *w, float4 **posm, float4 **vel, uint32_t **tag, hipStream_t *stream) {
+    ph::Engine &e = w->eng;
+    *posm = e.posm[e.cs].p;
+    *vel = e.vel[e.cs].p;
+    *tag = e.tag[e.ct].p;
+    *stream = e.stream;
+    w->domain_dirty = true;
+    e.rec_valid = false;
+}
 
 extern "C" {
 

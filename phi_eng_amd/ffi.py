@@ -110,6 +110,17 @@ SIGNATURES = {
     "phieng_kernel_name": (ctypes.c_char_p, [_I32]),
     "phieng_world3_sync": (_I32, [_P]),
     "phieng_last_error": (ctypes.c_char_p, []),
+    "phieng_batch3_new": (_P, [ctypes.POINTER(_P), _U32]),
+    "phieng_batch3_free": (None, [_P]),
+    "phieng_batch3_set_mode": (_I32, [_P, _I32, _U32]),
+    "phieng_batch3_step": (_I32, [_P, _F]),
+    "phieng_batch3_sync": (_I32, [_P]),
+    "phieng_batch3_sync_worlds": (_I32, [_P]),
+    "phieng_batch3_count": (_U32, [_P]),
+    "phieng_batch3_autotune": (_I32, [_P, _F, _U32, ctypes.c_char_p, _u32p, _dp]),
+    "phieng_batch3_set_timing": (_I32, [_P, _I32]),
+    "phieng_batch3_get_timing": (_I32, [_P, _dp, _u64p, _I32]),
+    "phieng_batch3_last_error": (ctypes.c_char_p, []),
 }
 
 _lib = None

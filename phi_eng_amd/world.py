@@ -150,3 +150,68 @@ class World3:
             if cnt[i]:
                 out[self._l.phieng_kernel_name(i).decode()] = (ms[i], cnt[i])
         return out
+
+
+BATCH_KERNELS, BATCH_PERSISTENT, BATCH_PERSISTENT_GRAPH = 0, 1, 2
+
+
+class Batch3:
+    """Many worlds advanced together (``phieng_batch3_*``)."""
+
+    def __init__(self, worlds):
+        self._l = lib()
+        self.worlds = list(worlds)
+        arr = (ctypes.c_void_p * len(self.worlds))(*[w.ptr for w in self.worlds])
+        self.ptr = self._l.phieng_batch3_new(arr, len(self.worlds))
+        if not self.ptr:
+            raise RuntimeError(self._l.phieng_batch3_last_error().decode())
+
+    def close(self):
+        if self.ptr:
+            self._l.phieng_batch3_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, ok):
+        if not ok:
+            raise RuntimeError(self._l.phieng_batch3_last_error().decode())
+
+    def set_mode(self, mode, blocks_per_cu=0):
+        self._ck(self._l.phieng_batch3_set_mode(self.ptr, int(mode), int(blocks_per_cu)))
+
+    def step(self, dt):
+        self._ck(self._l.phieng_batch3_step(self.ptr, float(dt)))
+
+    def sync(self):
+        self._ck(self._l.phieng_batch3_sync(self.ptr))
+
+    def sync_worlds(self):
+        self._ck(self._l.phieng_batch3_sync_worlds(self.ptr))
+
+    def count(self):
+        return self._l.phieng_batch3_count(self.ptr)
+
+    def autotune(self, dt, trial_steps=10, cache_path=None):
+        bpc = ctypes.c_uint32(0)
+        ms = ctypes.c_double(0)
+        m = self._l.phieng_batch3_autotune(self.ptr, float(dt), int(trial_steps),
+                                           cache_path.encode() if cache_path else None,
+                                           ctypes.byref(bpc), ctypes.byref(ms))
+        if m < 0:
+            raise RuntimeError(self._l.phieng_batch3_last_error().decode())
+        return m, bpc.value, ms.value
+
+    def set_timing(self, on):
+        self._l.phieng_batch3_set_timing(self.ptr, 1 if on else 0)
+
+    def get_timing(self):
+        k = 32
+        ms = (ctypes.c_double * k)()
+        cnt = (ctypes.c_uint64 * k)()
+        m = self._l.phieng_batch3_get_timing(self.ptr, ms, cnt, k)
+        return {self._l.phieng_kernel_name(i).decode(): (ms[i], cnt[i]) for i in range(min(m, k)) if cnt[i]}

@@ -1,0 +1,85 @@
+"""Batched stepping (no reference counterpart): every scene of a batch must
+be bit-identical to the same world stepped alone through the plain path,
+in every launch mode, and autotuning must not change results."""
+import numpy as np
+import pytest
+
+from phi_eng_amd import scenes
+from phi_eng_amd.world import BATCH_KERNELS, BATCH_PERSISTENT, BATCH_PERSISTENT_GRAPH, Batch3
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 12
+DT = 2e-4
+
+
+def _scenes(k=6, seed=41):
+    return scenes.batch_scenes(count=k, seed=seed, nmin=300, nmax=6000)
+
+
+def _alone(scs):
+    out = []
+    for sc in scs:
+        w = scenes.make_world(sc)
+        for _ in range(STEPS):
+            w.step(DT)
+        out.append(w.get_state())
+        w.close()
+    return out
+
+
+def _batched(scs, mode, bpc=0):
+    ws = [scenes.make_world(sc) for sc in scs]
+    b = Batch3(ws)
+    b.set_mode(mode, bpc)
+    for _ in range(STEPS):
+        b.step(DT)
+    b.sync()
+    b.sync_worlds()
+    out = [w.get_state() for w in ws]
+    b.close()
+    return out
+
+
+def _same(a, b):
+    return all(np.array_equal(x[0].view(np.uint32), y[0].view(np.uint32)) and
+               np.array_equal(x[1].view(np.uint32), y[1].view(np.uint32)) for x, y in zip(a, b))
+
+
+def test_batch_matches_worlds_alone_all_modes(gpu):
+    scs = _scenes()
+    ref = _alone(scs)
+    for mode, bpc in [(BATCH_KERNELS, 0), (BATCH_PERSISTENT, 0), (BATCH_PERSISTENT, 1),
+                      (BATCH_PERSISTENT_GRAPH, 0), (BATCH_PERSISTENT_GRAPH, 2)]:
+        got = _batched(scs, mode, bpc)
+        print(f"mode {mode} blocks/CU {bpc}: n={sum(len(g[0]) for g in got)}", flush=True)
+        assert _same(got, ref), (mode, bpc)
+
+
+def test_autotune_keeps_state_and_results(gpu, tmp_path):
+    scs = _scenes(5, seed=43)
+    ref = _alone(scs)
+    ws = [scenes.make_world(sc) for sc in scs]
+    b = Batch3(ws)
+    cache = str(tmp_path / "tune.txt")
+    mode, bpc, ms = b.autotune(DT, trial_steps=4, cache_path=cache)
+    print(f"autotune picked mode {mode} blocks/CU {bpc} ({ms:.3f} ms/step)", flush=True)
+    assert mode in (BATCH_KERNELS, BATCH_PERSISTENT_GRAPH)
+    for _ in range(STEPS):
+        b.step(DT)
+    b.sync()
+    b.sync_worlds()
+    assert _same([w.get_state() for w in ws], ref)
+    # the cache answers the same shape without trials
+    m2, bpc2, _ = b.autotune(DT, trial_steps=4, cache_path=cache)
+    assert (m2, bpc2) == (mode, bpc)
+    b.close()
+
+
+def test_batch_rejects_cfl_worlds(gpu):
+    sc = _scenes(1)[0]
+    sc.config.cfl = 0.4
+    sc.config.max_substeps = 4
+    w = scenes.make_world(sc)
+    with pytest.raises(RuntimeError, match="CFL"):
+        Batch3([w])
