@@ -176,6 +176,39 @@ int32_t phieng_batch3_set_timing(Batch3 *batch, int32_t enable);
 int32_t phieng_batch3_get_timing(Batch3 *batch, double *total_ms, uint64_t *launches, int32_t max);
 const char *phieng_batch3_last_error(void);
 
+/* ---------------------------------------------------------------------
+ * Learned pressure surrogate (no reference counterpart).  A 2-layer MLP,
+ * 16 inputs -> 128 (tanh-GELU) -> 16 outputs, BF16 weights on MFMA with
+ * FP32 accumulation, trained on the GPU by the library's own forward and
+ * backward kernels (Adam).  Parameters are an fp32 vector of
+ * phieng_surrogate3_num_params() values: W1 [16][128], b1 [128],
+ * W2 [128][16], b2 [16].  A training sample is 32 floats: MLP input x[16],
+ * three basis vectors b_k (9), the network-free acceleration a0 (3), the
+ * target acceleration a_t (3) and a scale s; its loss is
+ * |a0 + sum_k y_k b_k - a_t|^2 / s^2 with y the first three outputs.
+ * ------------------------------------------------------------------- */
+typedef struct Surrogate3 Surrogate3;
+Surrogate3 *phieng_surrogate3_new(uint32_t seed);
+void phieng_surrogate3_free(Surrogate3 *s);
+uint32_t phieng_surrogate3_num_params(void);
+int32_t phieng_surrogate3_get_params(Surrogate3 *s, float *out, uint32_t n);
+int32_t phieng_surrogate3_set_params(Surrogate3 *s, const float *in, uint32_t n);
+/* y = MLP(x) for `rows` rows (x: rows x 16, y: rows x 16). */
+int32_t phieng_surrogate3_mlp_forward(Surrogate3 *s, const float *x, uint32_t rows, float *y);
+/* grad = d(sum dy . MLP(x)) / d params. */
+int32_t phieng_surrogate3_mlp_backward(Surrogate3 *s, const float *x, const float *dy, uint32_t rows,
+                                       float *grad);
+int32_t phieng_surrogate3_add_samples(Surrogate3 *s, const float *samples, uint32_t count);
+uint32_t phieng_surrogate3_num_samples(const Surrogate3 *s);
+int32_t phieng_surrogate3_clear_samples(Surrogate3 *s);
+int32_t phieng_surrogate3_get_samples(Surrogate3 *s, float *out, uint32_t first, uint32_t count);
+/* Adam over shuffled batches; mean sample loss of the first and last epoch
+ * and of a0 alone (the method without the network). */
+int32_t phieng_surrogate3_train(Surrogate3 *s, uint32_t epochs, uint32_t batch, float lr, uint32_t seed,
+                                float *out_loss_first, float *out_loss_last, float *out_loss_base);
+int32_t phieng_surrogate3_eval(Surrogate3 *s, float *out_loss, float *out_loss_base);
+const char *phieng_surrogate3_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

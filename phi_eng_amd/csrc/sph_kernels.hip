@@ -254,31 +254,6 @@ __global__ __launch_bounds__(PH_TILE) void k_rank_gather(const TileDev *__restri
     dev_rank_gather(blockIdx.x, gridDim.x, tiles, scenes, tmp, keys, start, posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted);
 }
 
-// ---------------------------------------------------------------- rows
-// True when the particle's neighbourhood leaves the domain while the scene
-// has outliers (then the per-cell path of for_each_candidate is needed).
-__device__ __forceinline__ bool needs_cell_path(const SceneDev &s, const uint32_t *__restrict__ start, int32_t cx,
-                                                int32_t cy, int32_t cz) {
-    const bool inside = in_domain(s, (int64_t)cx - 1, (int64_t)cy - 1, (int64_t)cz - 1) &&
-                        in_domain(s, (int64_t)cx + 1, (int64_t)cy + 1, (int64_t)cz + 1);
-    return !inside && start[s.bucket_base + s.ncells] != start[s.bucket_base + s.ncells + 1];
-}
-
-// Slot range [b, e) of row r = 3 (oz + 1) + (oy + 1) (empty outside the domain).
-__device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__restrict__ start, int32_t cx,
-                                          int32_t cy, int32_t cz, int r, uint32_t &b, uint32_t &e) {
-    const int64_t xlo = max((int64_t)cx - 1, (int64_t)s.x0);
-    const int64_t xhi = min((int64_t)cx + 1, (int64_t)s.x0 + s.nx - 1);
-    const int64_t z = (int64_t)cz + (r / 3) - 1, y = (int64_t)cy + (r % 3) - 1;
-    b = e = 0;
-    if (xlo > xhi || z < s.z0 || z >= (int64_t)s.z0 + s.nz || y < s.y0 || y >= (int64_t)s.y0 + s.ny) return;
-    const uint32_t row = s.bucket_base + (uint32_t)(((z - s.z0) * s.ny + (y - s.y0)) * s.nx);
-    b = start[row + (uint32_t)(xlo - s.x0)];
-    e = start[row + (uint32_t)(xhi - s.x0) + 1];
-}
-
-#define PH_ROW_BITS 32u
-#define PH_SLOW_PATH 0x80000000u
 
 // ---------------------------------------------------------------- K4
 // Density (fluid3.rs:639-655) in two phases per row: the cut-off test over

@@ -121,6 +121,20 @@ SIGNATURES = {
     "phieng_batch3_set_timing": (_I32, [_P, _I32]),
     "phieng_batch3_get_timing": (_I32, [_P, _dp, _u64p, _I32]),
     "phieng_batch3_last_error": (ctypes.c_char_p, []),
+    "phieng_surrogate3_new": (_P, [_U32]),
+    "phieng_surrogate3_free": (None, [_P]),
+    "phieng_surrogate3_num_params": (_U32, []),
+    "phieng_surrogate3_get_params": (_I32, [_P, _fp, _U32]),
+    "phieng_surrogate3_set_params": (_I32, [_P, _fp, _U32]),
+    "phieng_surrogate3_mlp_forward": (_I32, [_P, _fp, _U32, _fp]),
+    "phieng_surrogate3_mlp_backward": (_I32, [_P, _fp, _fp, _U32, _fp]),
+    "phieng_surrogate3_add_samples": (_I32, [_P, _fp, _U32]),
+    "phieng_surrogate3_num_samples": (_U32, [_P]),
+    "phieng_surrogate3_clear_samples": (_I32, [_P]),
+    "phieng_surrogate3_get_samples": (_I32, [_P, _fp, _U32, _U32]),
+    "phieng_surrogate3_train": (_I32, [_P, _U32, _U32, _F, _U32, _fp, _fp, _fp]),
+    "phieng_surrogate3_eval": (_I32, [_P, _fp, _fp]),
+    "phieng_surrogate3_last_error": (ctypes.c_char_p, []),
 }
 
 _lib = None

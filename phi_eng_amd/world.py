@@ -215,3 +215,80 @@ class Batch3:
         cnt = (ctypes.c_uint64 * k)()
         m = self._l.phieng_batch3_get_timing(self.ptr, ms, cnt, k)
         return {self._l.phieng_kernel_name(i).decode(): (ms[i], cnt[i]) for i in range(min(m, k)) if cnt[i]}
+
+
+class Surrogate3:
+    """The learned pressure term's network and training state (``phieng_surrogate3_*``)."""
+
+    SAMPLE = 32
+
+    def __init__(self, seed=0):
+        self._l = lib()
+        self.ptr = self._l.phieng_surrogate3_new(int(seed))
+        if not self.ptr:
+            raise RuntimeError(self._l.phieng_surrogate3_last_error().decode())
+        self.nparam = self._l.phieng_surrogate3_num_params()
+
+    def close(self):
+        if self.ptr:
+            self._l.phieng_surrogate3_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, ok):
+        if not ok:
+            raise RuntimeError(self._l.phieng_surrogate3_last_error().decode())
+
+    def params(self):
+        out = np.zeros(self.nparam, dtype=np.float32)
+        self._ck(self._l.phieng_surrogate3_get_params(self.ptr, _fptr(out), self.nparam))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._ck(self._l.phieng_surrogate3_set_params(self.ptr, _fptr(p), len(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 16)
+        y = np.zeros((len(x), 16), dtype=np.float32)
+        self._ck(self._l.phieng_surrogate3_mlp_forward(self.ptr, _fptr(x), len(x), _fptr(y)))
+        return y
+
+    def backward(self, x, dy):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 16)
+        dy = np.ascontiguousarray(dy, dtype=np.float32).reshape(-1, 16)
+        g = np.zeros(self.nparam, dtype=np.float32)
+        self._ck(self._l.phieng_surrogate3_mlp_backward(self.ptr, _fptr(x), _fptr(dy), len(x), _fptr(g)))
+        return g
+
+    def add_samples(self, s):
+        s = np.ascontiguousarray(s, dtype=np.float32).reshape(-1, self.SAMPLE)
+        self._ck(self._l.phieng_surrogate3_add_samples(self.ptr, _fptr(s), len(s)))
+
+    def num_samples(self):
+        return self._l.phieng_surrogate3_num_samples(self.ptr)
+
+    def clear_samples(self):
+        self._l.phieng_surrogate3_clear_samples(self.ptr)
+
+    def samples(self, first=0, count=None):
+        count = self.num_samples() - first if count is None else count
+        out = np.zeros((count, self.SAMPLE), dtype=np.float32)
+        self._ck(self._l.phieng_surrogate3_get_samples(self.ptr, _fptr(out), first, count))
+        return out
+
+    def train(self, epochs, batch=4096, lr=1e-3, seed=0):
+        a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        self._ck(self._l.phieng_surrogate3_train(self.ptr, epochs, batch, lr, seed, ctypes.byref(a),
+                                                 ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def evaluate(self):
+        a, b = ctypes.c_float(), ctypes.c_float()
+        self._ck(self._l.phieng_surrogate3_eval(self.ptr, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
