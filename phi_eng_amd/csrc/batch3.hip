@@ -118,6 +118,10 @@ struct Batch3 {
             e.substep();
             return;
         }
+        // The host cannot see whether a persistent launch finished its scan
+        // (a timed-out barrier ends it early), so the multi-kernel path
+        // clears the bucket counts before its next sort.
+        e.counts_dirty = true;
         if (mode == PHIENG_BATCH_PERSISTENT || e.timing || e.rec_flags) {
             launch_persistent();
         } else {
@@ -141,25 +145,14 @@ struct Batch3 {
             }
             PH_CK(hipGraphLaunch(exec[k], e.stream));
         }
-        // bookkeeping identical to Engine::substep
-        e.ct = 1 - e.ct;
-        e.rec_valid = e.rec_flags != 0;
-        ++e.substeps_done;
-        if (!e.mon_pending && (e.substeps_done % e.monitor_every) == 0 && e.scenes.size() <= 4096) {
-            e.run(ph::KIND_MONITOR, [&] {
-                return ph::launch_monitor(e.stream, e.d_scenes.p, (uint32_t)e.scenes.size(), e.start.p, e.d_bbox.p,
-                                          e.d_monitor.p);
-            });
-            PH_CK(hipMemcpyAsync(e.h_monitor, e.d_monitor.p, 8 * sizeof(int32_t) * e.scenes.size(),
-                                 hipMemcpyDeviceToHost, e.stream));
-            PH_CK(hipEventRecord(e.mon_event, e.stream));
-            e.mon_pending = true;
-        }
+        e.gen_open = true;
+        e.finish_substep();
     }
 
     void step(float dt) {
         ph::Engine &e = eng;
         if (e.n_total == 0) return;
+        (void)hipGetLastError();  // a failed call before this step has already reported
         if (e.poll_monitor(e.n_total)) {
             e.upload_tables();
             drop_graphs();
@@ -303,6 +296,7 @@ int32_t phieng_batch3_step(Batch3 *b, float dt) {
         b->step(std::min(dt, 1.0f / 15.0f));
         return 1;
     } catch (const std::exception &ex) {
+        b->eng.recover();
         set_err("phieng_batch3_step", ex.what());
         return 0;
     }

@@ -31,6 +31,8 @@ enum Kind : int {
     KIND_FILL,
     KIND_SETVEL,
     KIND_PERSISTENT,
+    KIND_SUR_SWEEP,
+    KIND_SUR_AUX,
     KIND_COUNT
 };
 const char *kind_name(int k);
@@ -151,8 +153,25 @@ struct Engine {
     void assign_buckets();
     void upload_tables();
     void ensure_record();
-    // One substep of every scene (dt per scene in SceneDev::dt).
+    // One substep of every scene (dt per scene in SceneDev::dt):
+    // sort_phase + plain_phase + finish_substep.
     void substep();
+    void sort_phase(uint32_t *perm_out);
+    void plain_phase(float4 *acc_sorted);
+    void finish_substep();
+    // After an exception inside a substep: drop a stale launch error and
+    // make the generation selectors match what the device holds.  Counts
+    // left non-zero by an interrupted sort are cleared by the next sort.
+    void recover();
+    bool counts_dirty = false;  // a sort started and did not clear counts
+    bool gen_open = false;      // generation cs rewritten, ct not flipped yet
+    // surrogate history (previous rho, P = p/rho^2, its change) per slot
+    DBuf<float4> aux[2];
+    bool aux_on = false;
+    DBuf<uint32_t> perm;
+    DBuf<float4> acc_sorted;
+    DBuf<float2> tile_err;
+    DBuf<float> err_out;
     // Asynchronous domain monitor: returns true if some scene changed domain.
     bool poll_monitor(uint32_t nparticles_hint);
     // Cell box of the current particles of scene k (synchronous).

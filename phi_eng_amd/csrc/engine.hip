@@ -34,7 +34,7 @@ const char *kind_name(int k) {
     static const char *names[KIND_COUNT] = {
         "cell_key", "scan_reduce", "scan_partials", "scan_apply", "scatter", "rank_gather",
         "density",  "force_integrate", "vmax", "export", "monitor", "append", "cell_bbox",
-        "fill", "set_velocity", "batch_persistent"};
+        "fill", "set_velocity", "batch_persistent", "surrogate_sweep", "surrogate_history"};
     return (k >= 0 && k < KIND_COUNT) ? names[k] : "";
 }
 
@@ -276,12 +276,23 @@ void Engine::upload_tables() {
 }
 
 void Engine::substep() {
+    if (n_total == 0 || tiles.empty()) return;
+    sort_phase(nullptr);
+    plain_phase(nullptr);
+    finish_substep();
+}
+
+// K1 .. K3: keys, bucket scan, scatter, rank + gather into generation 1-cs.
+void Engine::sort_phase(uint32_t *perm_out) {
     const uint32_t nt = (uint32_t)tiles.size();
     const uint32_t n = n_total;
-    if (n == 0 || nt == 0) return;
-    const bool rc = rec_flags & PHIENG_RECORD_CELLS, rd = rec_flags & PHIENG_RECORD_DENSITY,
-               ra = rec_flags & PHIENG_RECORD_ACCEL;
+    const bool rc = rec_flags & PHIENG_RECORD_CELLS;
     const int ns = 1 - cs, nt_ = 1 - ct;
+    // cell_key counts into zeroed buckets and scan_apply zeroes them again;
+    // a sort cut short in between leaves them to be cleared here.
+    if (counts_dirty)
+        run(KIND_FILL, [&] { return launch_fill_u32(stream, counts.p, (uint32_t)alloc_buckets, 0u); });
+    counts_dirty = true;
     run(KIND_CELL_KEY, [&] {
         return launch_cell_key(stream, nt, d_tiles.p, d_scenes.p, posm[cs].p, tag[ct].p, keys.p, slots.p, counts.p,
                                rc ? rec_cells.p : nullptr);
@@ -291,11 +302,19 @@ void Engine::substep() {
         return launch_scan_partials(stream, partial.p, nbuckets, start.p, d_bbox.p, (uint32_t)scenes.size());
     });
     run(KIND_SCAN_APPLY, [&] { return launch_scan_apply(stream, counts.p, nbuckets, partial.p, start.p); });
+    counts_dirty = false;
     run(KIND_SCATTER, [&] { return launch_scatter(stream, n, keys.p, slots.p, start.p, tmp.p); });
     run(KIND_RANK_GATHER, [&] {
         return launch_rank_gather(stream, nt, d_tiles.p, d_scenes.p, tmp.p, keys.p, start.p, posm[cs].p, vel[cs].p,
-                                  tag[ct].p, posm[ns].p, vel[ns].p, tag[nt_].p, rc ? rec_sorted.p : nullptr);
+                                  tag[ct].p, posm[ns].p, vel[ns].p, tag[nt_].p, rc ? rec_sorted.p : nullptr, perm_out);
     });
+}
+
+// K4 + K5 of the plain step on generation 1-cs, writing generation cs.
+void Engine::plain_phase(float4 *acc_sorted) {
+    const uint32_t nt = (uint32_t)tiles.size();
+    const bool rd = rec_flags & PHIENG_RECORD_DENSITY, ra = rec_flags & PHIENG_RECORD_ACCEL;
+    const int ns = 1 - cs, nt_ = 1 - ct;
     run(KIND_DENSITY, [&] {
         return launch_density(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p, masks.p,
                               flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr, rd ? rec_p.p : nullptr,
@@ -304,9 +323,15 @@ void Engine::substep() {
     run(KIND_FORCE, [&] {
         return launch_force_integrate(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p,
                                       vel[ns].p, rq.p, tag[nt_].p, masks.p, flags.p, (uint32_t)alloc_particles,
-                                      posm[cs].p, vel[cs].p, ra ? rec_acc.p : nullptr, d_bbox.p);
+                                      posm[cs].p, vel[cs].p, ra ? rec_acc.p : nullptr, d_bbox.p, acc_sorted);
     });
-    ct = nt_;
+    gen_open = true;
+}
+
+// Generation flip of the tags, records, domain monitor.
+void Engine::finish_substep() {
+    ct = 1 - ct;
+    gen_open = false;
     rec_valid = rec_flags != 0;
     ++substeps_done;
     if (!mon_pending && (substeps_done % monitor_every) == 0 && scenes.size() <= 4096) {
@@ -318,6 +343,15 @@ void Engine::substep() {
         PH_CK(hipEventRecord(mon_event, stream));
         mon_pending = true;
     }
+}
+
+void Engine::recover() {
+    (void)hipGetLastError();
+    if (gen_open) {
+        ct = 1 - ct;
+        gen_open = false;
+    }
+    aux_on = false;
 }
 
 bool Engine::poll_monitor(uint32_t) {

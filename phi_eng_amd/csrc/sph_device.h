@@ -297,3 +297,13 @@ __device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__r
 
 #define PH_ROW_BITS 32u
 #define PH_SLOW_PATH 0x80000000u
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8),
+// each with its own L2.  Give every XCD one contiguous run of tiles, so
+// the neighbour rows a tile reads were mostly fetched by tiles on the same
+// XCD.  A bijection on [0, nt).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nt) {
+    const uint32_t x = b & 7u, k = b >> 3, per = nt >> 3, rem = nt & 7u;
+    return x * per + min(x, rem) + k;
+}
+

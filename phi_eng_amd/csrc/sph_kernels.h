@@ -42,7 +42,8 @@ hipError_t launch_scatter(hipStream_t st, uint32_t n, const uint32_t *keys, cons
 hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                               const uint32_t *tmp, const uint32_t *keys, const uint32_t *start,
                               const float4 *posm_in, const float4 *vel_in, const uint32_t *tag_in,
-                              float4 *posm_out, float4 *vel_out, uint32_t *tag_out, uint32_t *rec_sorted);
+                              float4 *posm_out, float4 *vel_out, uint32_t *tag_out, uint32_t *rec_sorted,
+                              uint32_t *perm_out = nullptr);
 hipError_t launch_density(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                           const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
                           uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
@@ -51,7 +52,7 @@ hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev
                                   const BodyDev *bodies, const uint32_t *start, const float4 *posm,
                                   const float4 *vel, const float2 *rq, const uint32_t *tag, const uint32_t *masks,
                                   const uint32_t *flags, uint32_t mstride, float4 *posm_out, float4 *vel_out,
-                                  float *rec_acc, int32_t *bbox);
+                                  float *rec_acc, int32_t *bbox, float4 *acc_sorted = nullptr);
 hipError_t launch_vmax(hipStream_t st, uint32_t n, const float4 *vel, uint32_t *out_bits);
 hipError_t launch_export(hipStream_t st, uint32_t n, const float4 *posm, const float4 *vel, const uint32_t *tag,
                          uint32_t id_base, float *out6);

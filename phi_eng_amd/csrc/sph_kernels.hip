@@ -47,15 +47,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &total)
 
 }  // namespace
 
-// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8),
-// each with its own L2.  Give every XCD one contiguous run of tiles, so
-// the neighbour rows a tile reads were mostly fetched by tiles on the same
-// XCD.  A bijection on [0, nt).
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nt) {
-    const uint32_t x = b & 7u, k = b >> 3, per = nt >> 3, rem = nt & 7u;
-    return x * per + min(x, rem) + k;
-}
-
 // ---------------------------------------------------------------- K1
 // Bucket key per particle and its slot inside the bucket.  Particles come
 // in last substep's sorted order, so neighbouring lanes mostly share a
@@ -213,7 +204,8 @@ __global__ __launch_bounds__(PH_TILE) void k_scatter(uint32_t n, const uint32_t 
 __device__ __forceinline__ void dev_rank_gather(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const uint32_t *__restrict__ tmp,
     const uint32_t *__restrict__ keys, const uint32_t *__restrict__ start, const float4 *__restrict__ posm_in,
     const float4 *__restrict__ vel_in, const uint32_t *__restrict__ tag_in, float4 *__restrict__ posm_out,
-    float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted) {
+    float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted,
+    uint32_t *__restrict__ perm_out) {
     const TileDev t = tiles[xcd_tile(bid, nbl)];
     const SceneDev s = scenes[t.scene];
     const uint32_t q = t.first + threadIdx.x;
@@ -245,13 +237,14 @@ __device__ __forceinline__ void dev_rank_gather(uint32_t bid, uint32_t nbl, cons
     vel_out[d] = vel_in[src];
     tag_out[d] = mytag;
     if (rec_sorted) rec_sorted[d] = myid;
+    if (perm_out) perm_out[d] = src;
 }
 
 __global__ __launch_bounds__(PH_TILE) void k_rank_gather(const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const uint32_t *__restrict__ tmp,
     const uint32_t *__restrict__ keys, const uint32_t *__restrict__ start, const float4 *__restrict__ posm_in,
     const float4 *__restrict__ vel_in, const uint32_t *__restrict__ tag_in, float4 *__restrict__ posm_out,
-    float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted) {
-    dev_rank_gather(blockIdx.x, gridDim.x, tiles, scenes, tmp, keys, start, posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted);
+    float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted, uint32_t *__restrict__ perm_out) {
+    dev_rank_gather(blockIdx.x, gridDim.x, tiles, scenes, tmp, keys, start, posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted, perm_out);
 }
 
 
@@ -430,7 +423,8 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
     const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
     const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ masks,
     const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
-    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox) {
+    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox,
+    float4 *__restrict__ acc_sorted) {
     const uint32_t tile = xcd_tile(bid, nbl);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
@@ -524,6 +518,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         clamp_to_bodies(s, bodies, x, y, z, ux, uy, uz);
         posm_out[i] = make_float4(x, y, z, pi.w);
         vel_out[i] = make_float4(ux, uy, uz, 0.0f);
+        if (acc_sorted) acc_sorted[i] = make_float4(ax - s.gx, ay - s.gy, az - s.gz, 0.0f);
         if (rec_acc) {
             const uint32_t id = tag[i] & PH_ID_MASK;
             rec_acc[3 * id] = ax;
@@ -571,8 +566,8 @@ __global__ __launch_bounds__(PH_TILE) void k_force_integrate(const TileDev *__re
     const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
     const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ masks,
     const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
-    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox) {
-    dev_force_integrate(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox);
+    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, float4 *__restrict__ acc_sorted) {
+    dev_force_integrate(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted);
 }
 
 
@@ -640,7 +635,7 @@ __global__ __launch_bounds__(PH_TILE) void k_step_persistent(ph::PersistArgs a) 
     if (!grid_barrier(a.bar, G, a.spin_limit)) return;
     for (uint32_t k = blockIdx.x; k < a.ntiles; k += G)
         dev_rank_gather(k, a.ntiles, a.tiles, a.scenes, a.tmp, a.keys, a.start, a.posm_cur, a.vel_cur, a.tag_cur,
-                        a.posm_alt, a.vel_alt, a.tag_alt, a.rec_sorted);
+                        a.posm_alt, a.vel_alt, a.tag_alt, a.rec_sorted, nullptr);
     if (!grid_barrier(a.bar, G, a.spin_limit)) return;
     for (uint32_t k = blockIdx.x; k < a.ntiles; k += G)
         dev_density(k, a.ntiles, a.tiles, a.scenes, a.start, a.posm_alt, a.tag_alt, a.rq, a.masks, a.flags,
@@ -648,7 +643,8 @@ __global__ __launch_bounds__(PH_TILE) void k_step_persistent(ph::PersistArgs a) 
     if (!grid_barrier(a.bar, G, a.spin_limit)) return;
     for (uint32_t k = blockIdx.x; k < a.ntiles; k += G) {
         dev_force_integrate(k, a.ntiles, a.tiles, a.scenes, a.bodies, a.start, a.posm_alt, a.vel_alt, a.rq,
-                            a.tag_alt, a.masks, a.flags, a.mstride, a.posm_cur, a.vel_cur, a.rec_acc, a.tile_bbox);
+                            a.tag_alt, a.masks, a.flags, a.mstride, a.posm_cur, a.vel_cur, a.rec_acc, a.tile_bbox,
+                            nullptr);
         __syncthreads();
     }
 }
@@ -835,9 +831,10 @@ hipError_t launch_scatter(hipStream_t st, uint32_t n, const uint32_t *keys, cons
 hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                               const uint32_t *tmp, const uint32_t *keys, const uint32_t *start,
                               const float4 *posm_in, const float4 *vel_in, const uint32_t *tag_in,
-                              float4 *posm_out, float4 *vel_out, uint32_t *tag_out, uint32_t *rec_sorted) {
+                              float4 *posm_out, float4 *vel_out, uint32_t *tag_out, uint32_t *rec_sorted,
+                              uint32_t *perm_out) {
     hipLaunchKernelGGL(k_rank_gather, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, tmp, keys, start,
-                       posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted);
+                       posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted, perm_out);
     return hipGetLastError();
 }
 hipError_t launch_density(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
@@ -852,9 +849,9 @@ hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev
                                   const BodyDev *bodies, const uint32_t *start, const float4 *posm,
                                   const float4 *vel, const float2 *rq, const uint32_t *tag, const uint32_t *masks,
                                   const uint32_t *flags, uint32_t mstride, float4 *posm_out, float4 *vel_out,
-                                  float *rec_acc, int32_t *bbox) {
+                                  float *rec_acc, int32_t *bbox, float4 *acc_sorted) {
     hipLaunchKernelGGL(k_force_integrate, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm,
-                       vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox);
+                       vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted);
     return hipGetLastError();
 }
 hipError_t launch_vmax(hipStream_t st, uint32_t n, const float4 *vel, uint32_t *out_bits) {

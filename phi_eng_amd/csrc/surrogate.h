@@ -19,6 +19,8 @@
 #define SUR_AT (SUR_A0 + 3)
 #define SUR_S (SUR_AT + 3)
 #define SUR_SAMPLE 32
+// Most samples one Surrogate3 holds (2 GiB of fp32 samples).
+#define SUR_MAX_SAMPLES (1u << 24)
 
 struct Surrogate3 {
     int device = -1;
@@ -36,6 +38,9 @@ struct Surrogate3 {
     void download_params();
     void zero(float *p, uint32_t n);
     void reserve_samples(uint32_t cap);
+    // Room for `need` samples in total: geometric growth, at most
+    // SUR_MAX_SAMPLES (throws beyond it).
+    void grow_samples(uint64_t need);
     void run_epoch(uint32_t batch, float lr, std::mt19937 &rng, bool train, double &lsum, double &l0sum);
     MlpDev dev() const;
 };

@@ -198,6 +198,13 @@ void Surrogate3::reserve_samples(uint32_t cap) {
     sample_cap = cap;
 }
 
+void Surrogate3::grow_samples(uint64_t need) {
+    if (need > SUR_MAX_SAMPLES) throw std::runtime_error("surrogate sample store full (at most 2^24 samples)");
+    if (need <= sample_cap && data.p) return;
+    const uint64_t cap = std::min<uint64_t>(std::max<uint64_t>(need, 2ull * sample_cap), SUR_MAX_SAMPLES);
+    reserve_samples((uint32_t)cap);
+}
+
 // One pass over the dataset in shuffled batches (train) or in order (eval).
 void Surrogate3::run_epoch(uint32_t batch, float lr, std::mt19937 &rng, bool train, double &lsum, double &l0sum) {
     std::vector<uint32_t> perm(nsamples);
@@ -324,7 +331,7 @@ int32_t phieng_surrogate3_mlp_backward(Surrogate3 *s, const float *x, const floa
 int32_t phieng_surrogate3_add_samples(Surrogate3 *s, const float *samples, uint32_t count) {
     if (!s || !samples) return 0;
     try {
-        s->reserve_samples(std::max(s->nsamples + count, 2 * s->sample_cap));
+        s->grow_samples((uint64_t)s->nsamples + count);
         PH_CK(hipMemcpyAsync(s->data.p + (size_t)s->nsamples * SUR_SAMPLE, samples, sizeof(float) * SUR_SAMPLE * count,
                              hipMemcpyHostToDevice, s->stream));
         PH_CK(hipStreamSynchronize(s->stream));

@@ -15,6 +15,8 @@
 #include "../../include/phieng_fluid3.h"
 #include "engine.h"
 #include "sph_kernels.h"
+#include "surrogate.h"
+#include "surrogate_step.h"
 #include "world3_internal.h"
 
 static_assert(sizeof(FluidConfig3Ffi) == 112, "FluidConfig3Ffi must match ffi.rs:743-767");
@@ -104,6 +106,22 @@ struct World3 {
     uint32_t n = 0;
     bool domain_dirty = true;
     ph::Engine eng;
+    // learned pressure term in the loop
+    Surrogate3 *sur = nullptr;
+    uint32_t check_every = 10;
+    float max_err = 0.01f;
+    int use_net = 1;
+    uint32_t fallback_left = 0;
+    uint64_t step_index = 0;
+    uint64_t n_sur = 0, n_plain = 0, n_fallback = 0;
+    float last_err = -1.0f;
+    // training-sample collection from plain substeps
+    Surrogate3 *col = nullptr;
+    uint32_t col_every = 10, col_stride = 64, col_phase = 0;
+    float err_host[2] = {0.0f, 0.0f};
+    uint64_t n_col_skipped = 0;
+    // first error of a step since the last sync (steps return nothing)
+    std::string step_error;
 
     ph::SceneHost scene_host(float dt) const {
         ph::SceneHost h;
@@ -183,14 +201,138 @@ struct World3 {
     void step(float dt) {
         if (cfg.enabled && n > 0 && dt > 0.0f) {
             eng.init_device();
+            (void)hipGetLastError();  // a failed call before this step has already reported
             if (!domain_dirty) eng.poll_monitor(n);
             sync_scene(dt);
             const uint32_t sub = cfl_substeps(dt);
             const float sdt = dt / (float)sub;
             if (sub != 1) sync_scene(sdt);
-            for (uint32_t k = 0; k < sub; ++k) eng.substep();
+            for (uint32_t k = 0; k < sub; ++k) {
+                if (!sur && !col) {
+                    eng.substep();
+                    continue;
+                }
+                ensure_aux();
+                const bool collect = col && (step_index % col_every) == 0;
+                const bool judge = sur && (step_index % check_every) == 0;
+                if (sur && !judge && !collect && fallback_left == 0) {
+                    substep_surrogate();
+                    ++n_sur;
+                } else {
+                    substep_plain_aux(judge, collect);
+                    ++n_plain;
+                    if (fallback_left) --fallback_left;
+                    if (judge && last_err > max_err) {
+                        fallback_left = check_every - 1;  // plain until the next judge
+                        ++n_fallback;
+                    }
+                }
+                ++step_index;
+            }
         }
         time += dt;
+    }
+
+    float qs() const {
+        const float g = cfg.gamma > 1.0f ? cfg.gamma : 1.0f;
+        return std::max(g * cfg.stiffness * 1e-3f / std::max(cfg.rest_density, 1e-6f), 1e-12f);
+    }
+
+    // Per-slot history for the surrogate, reset to rest density.
+    void ensure_aux() {
+        ph::Engine &e = eng;
+        if (e.aux_on && e.aux[0].n >= e.alloc_particles) return;
+        const size_t m = e.alloc_particles;
+        e.aux[0].ensure(m);
+        e.aux[1].ensure(m);
+        e.perm.ensure(m);
+        e.acc_sorted.ensure(m);
+        e.tile_err.ensure(std::max<size_t>(e.tiles.size(), 1));
+        e.err_out.ensure(2);
+        const float4 r0 = make_float4(cfg.phase_rest_density[0], cfg.phase_rest_density[1],
+                                      cfg.phase_rest_density[2], cfg.phase_rest_density[3]);
+        e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_aux_reset(e.stream, n, e.tag[e.ct].p, r0, e.aux[e.cs].p); });
+        e.aux_on = true;
+    }
+
+    SurSweepArgs sweep_args(int mode) const {
+        const ph::Engine &e = eng;
+        const int ns = 1 - e.cs, nt = 1 - e.ct;
+        SurSweepArgs a{};
+        a.mode = mode;
+        a.use_net = use_net;
+        a.tiles = e.d_tiles.p;
+        a.scenes = e.d_scenes.p;
+        a.bodies = e.d_bodies.p;
+        a.start = e.start.p;
+        a.posm = e.posm[ns].p;
+        a.vel = e.vel[ns].p;
+        a.aux = e.aux[ns].p;
+        a.tag = e.tag[nt].p;
+        const Surrogate3 *net = mode == SUR_MODE_STEP ? sur : col;
+        if (net) a.mlp = net->dev();
+        a.qs = qs();
+        a.posm_out = e.posm[e.cs].p;
+        a.vel_out = e.vel[e.cs].p;
+        a.aux_out = e.aux[e.cs].p;
+        return a;
+    }
+
+    // Plain substep that also refreshes the history; optionally judged
+    // (bulk density deviation, read back) and/or collecting samples.
+    void substep_plain_aux(bool judge, bool collect) {
+        ph::Engine &e = eng;
+        const uint32_t nt = (uint32_t)e.tiles.size();
+        const int ns = 1 - e.cs, ntg = 1 - e.ct;
+        // Room for this collection, made before any launch of the substep
+        // so an allocation failure cannot cut the substep short; a full
+        // store stops the collection.
+        const uint32_t cnt = collect && n > col_phase ? (n - 1 - col_phase) / col_stride + 1 : 0;
+        if (collect && (uint64_t)col->nsamples + cnt > SUR_MAX_SAMPLES) {
+            collect = false;
+            ++n_col_skipped;
+        }
+        if (collect) col->grow_samples((uint64_t)col->nsamples + cnt);
+        e.sort_phase(e.perm.p);
+        e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_gather_aux(e.stream, n, e.perm.p, e.aux[e.cs].p, e.aux[ns].p); });
+        e.plain_phase(collect ? e.acc_sorted.p : nullptr);
+        if (collect) {
+            SurSweepArgs a = sweep_args(SUR_MODE_COLLECT);
+            a.acc_target = e.acc_sorted.p;
+            a.samples = col->data.p;
+            a.sample_base = col->nsamples;
+            a.stride = col_stride;
+            a.phase = col_phase;
+            a.max_samples = col->nsamples + cnt;
+            e.run(ph::KIND_SUR_SWEEP, [&] { return ph::launch_sur_sweep(e.stream, nt, a); });
+            PH_CK(hipStreamSynchronize(e.stream));
+            col->nsamples += cnt;
+            col_phase = (col_phase + 1) % col_stride;
+        }
+        e.run(ph::KIND_SUR_AUX, [&] {
+            return ph::launch_aux_update(e.stream, nt, e.d_tiles.p, e.d_scenes.p, e.rq.p, e.aux[ns].p, e.tag[ntg].p,
+                                         e.aux[e.cs].p, e.tile_err.p);
+        });
+        if (judge) {
+            e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_err_reduce(e.stream, e.tile_err.p, nt, e.err_out.p); });
+            PH_CK(hipMemcpyAsync(err_host, e.err_out.p, sizeof(err_host), hipMemcpyDeviceToHost, e.stream));
+            PH_CK(hipStreamSynchronize(e.stream));
+            last_err = err_host[0];
+        }
+        e.finish_substep();
+    }
+
+    void substep_surrogate() {
+        ph::Engine &e = eng;
+        const uint32_t nt = (uint32_t)e.tiles.size();
+        const int ns = 1 - e.cs;
+        e.sort_phase(e.perm.p);
+        e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_gather_aux(e.stream, n, e.perm.p, e.aux[e.cs].p, e.aux[ns].p); });
+        const SurSweepArgs a = sweep_args(SUR_MODE_STEP);
+        e.run(ph::KIND_SUR_SWEEP, [&] { return ph::launch_sur_sweep(e.stream, nt, a); });
+        e.gen_open = true;
+        e.finish_substep();
+        e.rec_valid = false;  // the density and force passes did not run
     }
 
     uint32_t add(const float *xyz, const uint8_t *phases, uint32_t count) {
@@ -223,6 +365,7 @@ struct World3 {
         eng.n_total = n;
         eng.build_tiles();
         eng.rec_valid = false;
+        eng.aux_on = false;  // history restarts from rest
         domain_dirty = true;
         return k;
     }
@@ -289,6 +432,8 @@ void phieng_world3_step(World3 *world, float dt) {
     try {
         world->step(std::min(dt, 1.0f / 15.0f));
     } catch (const std::exception &e) {
+        world->eng.recover();
+        if (world->step_error.empty()) world->step_error = e.what();
         set_error("phieng_world3_step", e.what());
     }
 }
@@ -508,8 +653,47 @@ int32_t phieng_world3_get_timing(const World3 *world, double *total_ms, uint64_t
 
 const char *phieng_kernel_name(int32_t kind) { return ph::kind_name(kind); }
 
+int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check_every, float max_density_error,
+                                    int32_t use_network) {
+    if (!world) return 0;
+    if (s && (check_every < 1 || !std::isfinite(max_density_error) || max_density_error <= 0.0f)) return 0;
+    world->sur = s;
+    world->check_every = check_every ? check_every : 1;
+    world->max_err = max_density_error;
+    world->use_net = use_network ? 1 : 0;
+    world->fallback_left = 0;
+    world->step_index = 0;
+    return 1;
+}
+
+int32_t phieng_world3_collect_samples(World3 *world, Surrogate3 *s, uint32_t every, uint32_t stride) {
+    if (!world || (s && (every < 1 || stride < 1))) return 0;
+    world->col = s;
+    world->col_every = every ? every : 1;
+    world->col_stride = stride ? stride : 1;
+    world->col_phase = 0;
+    world->step_index = 0;
+    return 1;
+}
+
+int32_t phieng_world3_surrogate_stats(const World3 *world, uint64_t *sur_steps, uint64_t *plain_steps,
+                                      uint64_t *fallbacks, float *last_density_error) {
+    if (!world) return 0;
+    if (sur_steps) *sur_steps = world->n_sur;
+    if (plain_steps) *plain_steps = world->n_plain;
+    if (fallbacks) *fallbacks = world->n_fallback;
+    if (last_density_error) *last_density_error = world->last_err;
+    return 1;
+}
+
 int32_t phieng_world3_sync(World3 *world) {
     if (!world) return 0;
+    if (!world->step_error.empty()) {
+        const std::string m = "an earlier step failed: " + world->step_error;
+        world->step_error.clear();
+        set_error("phieng_world3_step", m.c_str());
+        return 0;
+    }
     if (world->eng.device < 0) return 1;
     try {
         PH_CK(hipSetDevice(world->eng.device));

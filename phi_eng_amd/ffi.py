@@ -135,6 +135,9 @@ SIGNATURES = {
     "phieng_surrogate3_train": (_I32, [_P, _U32, _U32, _F, _U32, _fp, _fp, _fp]),
     "phieng_surrogate3_eval": (_I32, [_P, _fp, _fp]),
     "phieng_surrogate3_last_error": (ctypes.c_char_p, []),
+    "phieng_world3_set_surrogate": (_I32, [_P, _P, _U32, _F, _I32]),
+    "phieng_world3_collect_samples": (_I32, [_P, _P, _U32, _U32]),
+    "phieng_world3_surrogate_stats": (_I32, [_P, _u64p, _u64p, _u64p, _fp]),
 }
 
 _lib = None

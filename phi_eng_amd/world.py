@@ -136,6 +136,26 @@ class World3:
             raise RuntimeError(f"get_record returned {got}: {last_error()}")
         return dict(cells=cells, order=order, density=rho, pressure=p, accel=acc, count=cnt)
 
+    # -- learned pressure term ---------------------------------------------
+    def set_surrogate(self, sur, check_every=10, max_density_error=0.01, use_network=True):
+        ok = self._l.phieng_world3_set_surrogate(self.ptr, sur.ptr if sur is not None else None,
+                                                 int(check_every), float(max_density_error),
+                                                 1 if use_network else 0)
+        if not ok:
+            raise RuntimeError("set_surrogate rejected its arguments")
+
+    def collect_samples(self, sur, every=10, stride=64):
+        if not self._l.phieng_world3_collect_samples(self.ptr, sur.ptr if sur is not None else None,
+                                                     int(every), int(stride)):
+            raise RuntimeError("collect_samples rejected its arguments")
+
+    def surrogate_stats(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        e = ctypes.c_float()
+        self._l.phieng_world3_surrogate_stats(self.ptr, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                              ctypes.byref(e))
+        return dict(surrogate_steps=a.value, plain_steps=b.value, fallbacks=c.value, last_density_error=e.value)
+
     # -- timing -----------------------------------------------------------
     def set_timing(self, on):
         self._l.phieng_world3_set_timing(self.ptr, 1 if on else 0)

@@ -1,0 +1,144 @@
+"""The learned pressure term inside the fluid step: training-sample
+collection from plain substeps, the judge / fallback schedule, and the
+network-free limit of the surrogate substep.
+
+Collection and judging run on plain substeps, so a world that collects or
+is judged on every step must follow the plain trajectory bit for bit; a
+network whose weights are all zero must reproduce the network-free
+surrogate bit for bit (its coefficients are exactly 0)."""
+import numpy as np
+import pytest
+
+from phi_eng_amd import scenes
+from phi_eng_amd.world import RECORD_ACCEL, RECORD_DENSITY, Surrogate3
+
+pytestmark = pytest.mark.gpu
+
+SUR_X, SUR_AT = 0, 28
+
+
+def _pair(n_side=12):
+    sc = scenes.small_dam(n_side)
+    return sc, scenes.make_world(sc), scenes.make_world(sc)
+
+
+def _same_state(a, b, label):
+    pa, va = a.get_state()
+    pb, vb = b.get_state()
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb), label
+
+
+def test_collection_keeps_plain_trajectory_and_targets(gpu):
+    sc, plain, col = _pair()
+    sur = Surrogate3(seed=1)
+    stride, every = 4, 2
+    col.collect_samples(sur, every=every, stride=stride)
+    for w in (plain, col):
+        w.set_record(RECORD_DENSITY | RECORD_ACCEL)
+    n = sc.n
+    g = np.asarray(sc.gravity, dtype=np.float32)
+    phase = 0
+    for k in range(6):
+        before = sur.num_samples()
+        plain.step(sc.dt)
+        col.step(sc.dt)
+        col.sync()
+        _same_state(plain, col, f"step {k}: collection changed the trajectory")
+        if k % every:
+            assert sur.num_samples() == before
+            continue
+        cnt = (n - 1 - phase) // stride + 1
+        assert sur.num_samples() == before + cnt
+        smp = sur.samples(before, cnt)
+        ids = phase + stride * np.arange(cnt)
+        rec = plain.get_record()
+        target = rec["accel"][ids] - g  # pressure + viscous part, f32 like the kernel
+        assert np.array_equal(smp[:, SUR_AT:SUR_AT + 3], target), f"step {k}: targets differ"
+        rho0 = np.float32(sc.config.rest_density)
+        x0 = (rec["density"][ids] / rho0 - np.float32(1.0)) * np.float32(100.0)
+        assert np.allclose(smp[:, SUR_X], x0, rtol=1e-6, atol=1e-4), f"step {k}: density feature differs"
+        assert np.isfinite(smp).all()
+        print(f"step {k}: {cnt} samples, |a_t| mean {np.linalg.norm(target, axis=1).mean():.3f}", flush=True)
+        phase = (phase + 1) % stride
+    for w in (plain, col):
+        w.close()
+    sur.close()
+
+
+@pytest.mark.parametrize("check_every,max_err", [(1, 0.01), (4, 1e-12)])
+def test_judged_steps_follow_plain(gpu, check_every, max_err):
+    """Judged every step, or always failing the judge (fallback until the
+    next judge): every substep is plain, so the trajectory is the plain one."""
+    sc, plain, sw = _pair()
+    sur = Surrogate3(seed=2)
+    sw.set_surrogate(sur, check_every=check_every, max_density_error=max_err, use_network=True)
+    plain.set_record(RECORD_DENSITY)
+    steps = 9
+    for k in range(steps):
+        plain.step(sc.dt)
+        sw.step(sc.dt)
+    sw.sync()
+    _same_state(plain, sw, "judged run left the plain trajectory")
+    st = sw.surrogate_stats()
+    print(st, flush=True)
+    assert st["surrogate_steps"] == 0 and st["plain_steps"] == steps
+    if check_every == 4:
+        assert st["fallbacks"] == len(range(0, steps, check_every))
+    # the judge's measure: mean |rho/rho0 - 1| over bulk particles (rho >= 0.9 rho0)
+    r = plain.get_record()["density"] / np.float32(sc.config.rest_density)  # f32 like the kernel
+    bulk = r >= np.float32(0.9)
+    want = np.abs(r[bulk] - np.float32(1.0)).astype(np.float64).mean()
+    assert abs(st["last_density_error"] - want) <= 1e-5 * max(want, 1e-3)
+    for w in (plain, sw):
+        w.close()
+    sur.close()
+
+
+def test_zero_network_is_network_free(gpu):
+    sc, a, b = _pair()
+    net = Surrogate3(seed=3)
+    net.set_params(np.zeros(net.nparam, dtype=np.float32))
+    a.set_surrogate(net, check_every=1000, use_network=True)
+    b.set_surrogate(net, check_every=1000, use_network=False)
+    for k in range(8):
+        a.step(sc.dt)
+        b.step(sc.dt)
+    a.sync()
+    b.sync()
+    _same_state(a, b, "zero network differs from the network-free step")
+    st = a.surrogate_stats()
+    assert st["surrogate_steps"] == 7 and st["plain_steps"] == 1
+    for w in (a, b):
+        w.close()
+    net.close()
+
+
+def test_network_free_surrogate_tracks_plain(gpu):
+    """Short horizon: the surrogate step (lagged pressure, judged every 5
+    steps) stays close to the plain solver."""
+    sc, plain, sw = _pair(16)
+    sur = Surrogate3(seed=4)
+    sw.set_surrogate(sur, check_every=5, max_density_error=0.05, use_network=False)
+    for k in range(20):
+        plain.step(sc.dt)
+        sw.step(sc.dt)
+    sw.sync()
+    st = sw.surrogate_stats()
+    pa, va = plain.get_state()
+    pb, vb = sw.get_state()
+    assert np.isfinite(pb).all() and np.isfinite(vb).all()
+    dx = np.abs(pa - pb).max()
+    for w in (plain, sw):
+        w.set_surrogate(None)
+        w.set_record(RECORD_DENSITY)
+        w.step(sc.dt)
+    ra = plain.get_record()["density"].astype(np.float64)
+    rb = sw.get_record()["density"].astype(np.float64)
+    err = (np.abs(rb - ra) / ra).mean()
+    print(f"{st}: max |dx| {dx:.2e}, mean density err {err:.2e}", flush=True)
+    assert st["surrogate_steps"] > 0
+    assert dx < 1e-3
+    assert err < 0.01
+    for w in (plain, sw):
+        w.close()
+    sur.close()
