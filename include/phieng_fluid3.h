@@ -224,6 +224,10 @@ int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check
 int32_t phieng_world3_collect_samples(World3 *world, Surrogate3 *s, uint32_t every, uint32_t stride);
 int32_t phieng_world3_surrogate_stats(const World3 *world, uint64_t *sur_steps, uint64_t *plain_steps,
                                       uint64_t *fallbacks, float *last_density_error);
+/* Skin of the surrogate's neighbour list: surrogate substeps between two
+ * plain substeps see the candidates that were within (1 + skin) h at the
+ * plain one (default 0.05; 0 <= skin <= 1).  1 / 0. */
+int32_t phieng_world3_set_surrogate_skin(World3 *world, float skin);
 
 #ifdef __cplusplus
 }

@@ -31,8 +31,9 @@ enum Kind : int {
     KIND_FILL,
     KIND_SETVEL,
     KIND_PERSISTENT,
-    KIND_SUR_SWEEP,
+    KIND_SUR_STEP,
     KIND_SUR_AUX,
+    KIND_SUR_COLLECT,
     KIND_COUNT
 };
 const char *kind_name(int k);
@@ -85,6 +86,8 @@ void fill_scene_physics(SceneDev &s, const SceneHost &h);
 // its bodies.  The domain only affects speed: particles outside it land in
 // the scene's outlier bucket and are still handled exactly.
 void choose_domain(SceneDev &s, const int32_t *pbox, uint32_t nparticles, const std::vector<BodyDev> &bodies);
+
+struct ListOut;  // sph_kernels.h
 
 struct Engine {
     int device = -1;
@@ -157,13 +160,17 @@ struct Engine {
     // sort_phase + plain_phase + finish_substep.
     void substep();
     void sort_phase(uint32_t *perm_out);
-    void plain_phase(float4 *acc_sorted);
+    // With lo, the density pass also writes the surrogate's skin list.
+    void plain_phase(float4 *acc_sorted, const ListOut *lo = nullptr);
     void finish_substep();
     // After an exception inside a substep: drop a stale launch error and
     // make the generation selectors match what the device holds.  Counts
     // left non-zero by an interrupted sort are cleared by the next sort.
     void recover();
     bool counts_dirty = false;  // a sort started and did not clear counts
+    // Sorts run, and changes of the bucket / tile / buffer layout: a skin
+    // list is usable while both are as they were when it was built.
+    uint64_t sorts_done = 0, layout_epoch = 0;
     bool gen_open = false;      // generation cs rewritten, ct not flipped yet
     // surrogate history (previous rho, P = p/rho^2, its change) per slot
     DBuf<float4> aux[2];

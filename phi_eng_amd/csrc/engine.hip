@@ -34,7 +34,8 @@ const char *kind_name(int k) {
     static const char *names[KIND_COUNT] = {
         "cell_key", "scan_reduce", "scan_partials", "scan_apply", "scatter", "rank_gather",
         "density",  "force_integrate", "vmax", "export", "monitor", "append", "cell_bbox",
-        "fill", "set_velocity", "batch_persistent", "surrogate_sweep", "surrogate_history"};
+        "fill", "set_velocity", "batch_persistent", "surrogate_step", "surrogate_history",
+        "surrogate_collect"};
     return (k >= 0 && k < KIND_COUNT) ? names[k] : "";
 }
 
@@ -169,6 +170,7 @@ void Engine::init_device() {
 
 void Engine::ensure_particles(uint32_t cap) {
     if (cap <= alloc_particles && posm[0].p) return;
+    ++layout_epoch;
     size_t want = std::max<size_t>(cap, std::min<size_t>(2 * alloc_particles, (size_t)1 << 24));
     want = std::max<size_t>(want, 1024);
     DBuf<float4> np0, nv0;
@@ -219,6 +221,7 @@ void Engine::ensure_record() {
 }
 
 void Engine::build_tiles() {
+    ++layout_epoch;
     tiles.clear();
     for (uint32_t k = 0; k < scenes.size(); ++k) {
         scenes[k].tile_begin = (uint32_t)tiles.size();
@@ -230,6 +233,7 @@ void Engine::build_tiles() {
 }
 
 void Engine::assign_buckets() {
+    ++layout_epoch;
     uint32_t base = 0;
     for (auto &s : scenes) {
         s.bucket_base = base;
@@ -248,6 +252,7 @@ void Engine::assign_buckets() {
 }
 
 void Engine::upload_tables() {
+    const bool any = scenes_dirty || tiles_dirty || bodies_dirty;
     if (scenes_dirty) {
         d_scenes.ensure(scenes.size());
         PH_CK(hipMemcpyAsync(d_scenes.p, scenes.data(), sizeof(SceneDev) * scenes.size(), hipMemcpyHostToDevice,
@@ -272,7 +277,7 @@ void Engine::upload_tables() {
     }
     // Host vectors may be rewritten by the caller right after this call:
     // make the staged copies complete first.
-    PH_CK(hipStreamSynchronize(stream));
+    if (any) PH_CK(hipStreamSynchronize(stream));
 }
 
 void Engine::substep() {
@@ -293,6 +298,7 @@ void Engine::sort_phase(uint32_t *perm_out) {
     if (counts_dirty)
         run(KIND_FILL, [&] { return launch_fill_u32(stream, counts.p, (uint32_t)alloc_buckets, 0u); });
     counts_dirty = true;
+    ++sorts_done;
     run(KIND_CELL_KEY, [&] {
         return launch_cell_key(stream, nt, d_tiles.p, d_scenes.p, posm[cs].p, tag[ct].p, keys.p, slots.p, counts.p,
                                rc ? rec_cells.p : nullptr);
@@ -311,11 +317,15 @@ void Engine::sort_phase(uint32_t *perm_out) {
 }
 
 // K4 + K5 of the plain step on generation 1-cs, writing generation cs.
-void Engine::plain_phase(float4 *acc_sorted) {
+void Engine::plain_phase(float4 *acc_sorted, const ListOut *lo) {
     const uint32_t nt = (uint32_t)tiles.size();
     const bool rd = rec_flags & PHIENG_RECORD_DENSITY, ra = rec_flags & PHIENG_RECORD_ACCEL;
     const int ns = 1 - cs, nt_ = 1 - ct;
     run(KIND_DENSITY, [&] {
+        if (lo)
+            return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p,
+                                       masks.p, flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
+                                       rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr, *lo);
         return launch_density(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p, masks.p,
                               flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr, rd ? rec_p.p : nullptr,
                               rd ? rec_count.p : nullptr);

@@ -38,6 +38,8 @@ __host__ __device__ __forceinline__ uint16_t mlp_bf16(float f) {
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
+// The same rounding by the hardware converter (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint16_t mlp_bf16_hw(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __host__ __device__ __forceinline__ float mlp_f32(uint16_t h) {
     const uint32_t u = (uint32_t)h << 16;
     float f;
@@ -45,13 +47,17 @@ __host__ __device__ __forceinline__ float mlp_f32(uint16_t h) {
     return f;
 }
 
-// tanh-form GELU and its derivative.
+// tanh-form GELU, 0.5 z (1 + tanh(u)) with u = k0 (z + k1 z^3), written as
+// z * sigmoid(2u) with the hardware exp2 and reciprocal (about 1e-6
+// relative, far below the bf16 rounding of the hidden layer), and its
+// derivative sigmoid + z sigmoid (1 - sigmoid) d(2u)/dz.
 __device__ __forceinline__ float mlp_gelu(float z, float *dgelu) {
-    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-    const float u = k0 * (z + k1 * z * z * z);
-    const float th = 1.0f - 2.0f / (1.0f + expf(2.0f * u));
-    if (dgelu) *dgelu = 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * k0 * (1.0f + 3.0f * k1 * z * z);
-    return 0.5f * z * (1.0f + th);
+    const float c0 = 2.0f * 0.7978845608028654f, c1 = 2.0f * 0.7978845608028654f * 0.044715f;
+    const float z2 = z * z;
+    const float v = z * fmaf(c1, z2, c0);
+    const float sg = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(v * -1.4426950408889634f));
+    if (dgelu) *dgelu = fmaf(z * sg * (1.0f - sg), fmaf(3.0f * c1, z2, c0), sg);
+    return z * sg;
 }
 
 __device__ __forceinline__ mlp_f32x4 mlp_mfma(mlp_bf16x4 a, mlp_bf16x4 b, mlp_f32x4 c) {

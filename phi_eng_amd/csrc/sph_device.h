@@ -298,6 +298,13 @@ __device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__r
 #define PH_ROW_BITS 32u
 #define PH_SLOW_PATH 0x80000000u
 
+// Skin list of the surrogate substeps: candidates per particle at most, and
+// the count word's flags.
+#define SL_CAP 96u
+#define SLF_COUNT 0x00FFFFFFu
+#define SLF_SLOW 0x01000000u  // sweep the cells (outliers near the domain, or > SL_CAP candidates)
+#define SLF_NONE 0x02000000u  // non-finite position: no neighbours
+
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8),
 // each with its own L2.  Give every XCD one contiguous run of tiles, so
 // the neighbour rows a tile reads were mostly fetched by tiles on the same

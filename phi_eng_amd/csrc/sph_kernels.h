@@ -48,6 +48,17 @@ hipError_t launch_density(hipStream_t st, uint32_t ntiles, const TileDev *tiles,
                           const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
                           uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
                           uint32_t *rec_count);
+// Skin list written by the density pass for surrogate substeps.
+struct ListOut {
+    int4 *cell = nullptr;
+    uint32_t *cnt = nullptr;
+    uint32_t *idx = nullptr;
+    float skin_factor2 = 1.0f;
+};
+hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                               const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
+                               uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
+                               uint32_t *rec_count, const ListOut &lo);
 hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                                   const BodyDev *bodies, const uint32_t *start, const float4 *posm,
                                   const float4 *vel, const float2 *rq, const uint32_t *tag, const uint32_t *masks,

@@ -255,6 +255,11 @@ __global__ __launch_bounds__(PH_TILE) void k_rank_gather(const TileDev *__restri
 // neighbour order.  Rows with more than 32 candidates finish their tail
 // inline and flag it.  rq[i] = (rho_i, p_i / max(rho_i, EPS)^2), the
 // pressure quotient of the force pass (fluid3.rs:1046-1051).
+// With LIST, the same sweep also writes the skin list of the sorted state
+// for surrogate substeps (candidates within (1 + skin) h, skin_factor2 =
+// (1 + skin)^2; the layout k_sur_list documents); the density results are
+// the same either way.
+template <bool LIST>
 __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles,
                                                      const SceneDev *__restrict__ scenes,
                                                      const uint32_t *__restrict__ start,
@@ -262,7 +267,8 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                                                      const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
                                                      uint32_t *__restrict__ masks, uint32_t *__restrict__ flags,
                                                      uint32_t mstride, float *__restrict__ rec_rho,
-                                                     float *__restrict__ rec_p, uint32_t *__restrict__ rec_count) {
+                                                     float *__restrict__ rec_p, uint32_t *__restrict__ rec_count,
+                                                     const ph::ListOut &lo = ph::ListOut{}) {
     const TileDev t = tiles[xcd_tile(bid, nbl)];
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
@@ -270,9 +276,13 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
     const float4 pi = posm[i];
     float rho = 0.0f;
     uint32_t cnt = 0, fl = 0;
+    uint32_t lk = 0;                          // skin list length
+    uint32_t *lcol = LIST ? lo.idx + i : nullptr;
+    const float hs2 = LIST ? s.h_grid2 * lo.skin_factor2 : 0.0f;
     if (finite3(pi.x, pi.y, pi.z)) {
         int32_t cx, cy, cz;
         cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
+        if (LIST) lo.cell[i] = make_int4(cx, cy, cz, 0);
         const float hg2 = s.h_grid2;
         if (!needs_cell_path(s, start, cx, cy, cz)) {
 #pragma unroll 1
@@ -280,23 +290,36 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                 uint32_t br, er;
                 row_range(s, start, cx, cy, cz, r, br, er);
                 const uint32_t lim = min(er, br + PH_ROW_BITS);
-                uint32_t m = 0;
+                uint32_t m = 0, ml = 0;
                 uint32_t j = br;
                 // Four independent loads in flight per lane.
                 for (; j + 4 <= lim; j += 4) {
                     const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
                     const uint32_t o = j - br;
-                    m |= (dot3(q0.x - pi.x, q0.y - pi.y, q0.z - pi.z) < hg2 ? 1u : 0u) << o;
-                    m |= (dot3(q1.x - pi.x, q1.y - pi.y, q1.z - pi.z) < hg2 ? 2u : 0u) << o;
-                    m |= (dot3(q2.x - pi.x, q2.y - pi.y, q2.z - pi.z) < hg2 ? 4u : 0u) << o;
-                    m |= (dot3(q3.x - pi.x, q3.y - pi.y, q3.z - pi.z) < hg2 ? 8u : 0u) << o;
+                    const float d0 = dot3(q0.x - pi.x, q0.y - pi.y, q0.z - pi.z);
+                    const float d1 = dot3(q1.x - pi.x, q1.y - pi.y, q1.z - pi.z);
+                    const float d2 = dot3(q2.x - pi.x, q2.y - pi.y, q2.z - pi.z);
+                    const float d3 = dot3(q3.x - pi.x, q3.y - pi.y, q3.z - pi.z);
+                    m |= ((d0 < hg2 ? 1u : 0u) | (d1 < hg2 ? 2u : 0u) | (d2 < hg2 ? 4u : 0u) | (d3 < hg2 ? 8u : 0u)) << o;
+                    if (LIST)
+                        ml |= ((d0 < hs2 ? 1u : 0u) | (d1 < hs2 ? 2u : 0u) | (d2 < hs2 ? 4u : 0u) |
+                               (d3 < hs2 ? 8u : 0u)) << o;
                 }
                 for (; j < lim; ++j) {
                     const float4 pj = posm[j];
                     const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
                     m |= (r2 < hg2 ? 1u : 0u) << (j - br);
+                    if (LIST) ml |= (r2 < hs2 ? 1u : 0u) << (j - br);
                 }
                 masks[(size_t)r * mstride + i] = m;
+                if (LIST) {
+                    while (ml) {
+                        const uint32_t k = __ffs(ml) - 1;
+                        ml &= ml - 1;
+                        if (lk < SL_CAP) lcol[(size_t)lk * mstride] = br + k;
+                        ++lk;
+                    }
+                }
                 while (m) {
                     const uint32_t k = __ffs(m) - 1;
                     m &= m - 1;
@@ -314,11 +337,17 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                             ++cnt;
                             rho += pj.w * poly6(r2, s);
                         }
+                        if (LIST && r2 < hs2) {
+                            if (lk < SL_CAP) lcol[(size_t)lk * mstride] = j;
+                            ++lk;
+                        }
                     }
                 }
             }
+            if (LIST) lo.cnt[i] = lk <= SL_CAP ? lk : SLF_SLOW;
         } else {
             fl = PH_SLOW_PATH;
+            if (LIST) lo.cnt[i] = SLF_SLOW;
             for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
                 const float4 pj = posm[j];
                 const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
@@ -330,6 +359,7 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
         }
     } else {
         fl = PH_SLOW_PATH;  // no neighbours: the force pass skips the sweep
+        if (LIST) lo.cnt[i] = SLF_NONE;
     }
     flags[i] = fl;
     rho = rclampf(rho, 0.0f, s.rho_max);
@@ -356,7 +386,21 @@ __global__ __launch_bounds__(PH_TILE) void k_density(const TileDev *__restrict__
                                                      uint32_t *__restrict__ masks, uint32_t *__restrict__ flags,
                                                      uint32_t mstride, float *__restrict__ rec_rho,
                                                      float *__restrict__ rec_p, uint32_t *__restrict__ rec_count) {
-    dev_density(blockIdx.x, gridDim.x, tiles, scenes, start, posm, tag, rq, masks, flags, mstride, rec_rho, rec_p, rec_count);
+    dev_density<false>(blockIdx.x, gridDim.x, tiles, scenes, start, posm, tag, rq, masks, flags, mstride, rec_rho,
+                       rec_p, rec_count);
+}
+
+__global__ __launch_bounds__(PH_TILE) void k_density_list(const TileDev *__restrict__ tiles,
+                                                          const SceneDev *__restrict__ scenes,
+                                                          const uint32_t *__restrict__ start,
+                                                          const float4 *__restrict__ posm,
+                                                          const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
+                                                          uint32_t *__restrict__ masks, uint32_t *__restrict__ flags,
+                                                          uint32_t mstride, float *__restrict__ rec_rho,
+                                                          float *__restrict__ rec_p, uint32_t *__restrict__ rec_count,
+                                                          ph::ListOut lo) {
+    dev_density<true>(blockIdx.x, gridDim.x, tiles, scenes, start, posm, tag, rq, masks, flags, mstride, rec_rho,
+                      rec_p, rec_count, lo);
 }
 
 // ---------------------------------------------------------------- K5
@@ -638,7 +682,7 @@ __global__ __launch_bounds__(PH_TILE) void k_step_persistent(ph::PersistArgs a) 
                         a.posm_alt, a.vel_alt, a.tag_alt, a.rec_sorted, nullptr);
     if (!grid_barrier(a.bar, G, a.spin_limit)) return;
     for (uint32_t k = blockIdx.x; k < a.ntiles; k += G)
-        dev_density(k, a.ntiles, a.tiles, a.scenes, a.start, a.posm_alt, a.tag_alt, a.rq, a.masks, a.flags,
+        dev_density<false>(k, a.ntiles, a.tiles, a.scenes, a.start, a.posm_alt, a.tag_alt, a.rq, a.masks, a.flags,
                     a.mstride, a.rec_rho, a.rec_p, a.rec_count);
     if (!grid_barrier(a.bar, G, a.spin_limit)) return;
     for (uint32_t k = blockIdx.x; k < a.ntiles; k += G) {
@@ -843,6 +887,14 @@ hipError_t launch_density(hipStream_t st, uint32_t ntiles, const TileDev *tiles,
                           uint32_t *rec_count) {
     hipLaunchKernelGGL(k_density, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, rq, masks,
                        flags, mstride, rec_rho, rec_p, rec_count);
+    return hipGetLastError();
+}
+hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                               const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
+                               uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
+                               uint32_t *rec_count, const ListOut &lo) {
+    hipLaunchKernelGGL(k_density_list, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, rq, masks,
+                       flags, mstride, rec_rho, rec_p, rec_count, lo);
     return hipGetLastError();
 }
 hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,

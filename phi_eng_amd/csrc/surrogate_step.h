@@ -5,33 +5,53 @@
 #include "mlp.h"
 #include "sph_device.h"
 
-#define SUR_MODE_STEP 0
-#define SUR_MODE_COLLECT 1
-
-struct SurSweepArgs {
-    int mode;
-    int use_net;  // 0: the network-free method (c = 0)
+// Training samples from a plain substep: the surrogate's inputs on the
+// gathered state and the plain pressure acceleration as the target.
+struct SurCollectArgs {
     const TileDev *tiles;
     const SceneDev *scenes;
-    const BodyDev *bodies;
     const uint32_t *start;
     const float4 *posm, *vel, *aux;  // gathered state + previous-substep history
     const uint32_t *tag;
-    MlpDev mlp;
+    float4 mass;  // particle mass of phases 0..3
     float qs;  // pressure-quotient scale of the inputs
-    float4 *posm_out, *vel_out, *aux_out;
-    // collect mode
     const float4 *acc_target;
     float *samples;
     uint32_t sample_base, stride, phase, max_samples;
 };
 
+// One surrogate substep (sweep, then update) over the order and cell
+// table of the last sort.
+struct SurStepArgs {
+    int use_net;  // 0: the network-free method (c = 0)
+    const TileDev *tiles;
+    const SceneDev *scenes;
+    const BodyDev *bodies;
+    const uint32_t *start;
+    const float4 *posm, *vel, *aux;
+    const uint32_t *tag;
+    // skin list written by the density pass of the last plain substep
+    const int4 *cell;      // skin list: cell at list time
+    const uint32_t *cnt;   // skin list: candidate count | SLF_ flags
+    const uint32_t *idx;   // skin list: candidate k of particle i at idx[k * mstride + i]
+    uint32_t mstride;
+    // neighbour records of the current state, 2 float4 per slot:
+    // (x, y, z, P with the phase in its 2 low bits), (v, dP)
+    const float4 *rec;
+    float4 mass;  // particle mass of phases 0..3
+    MlpDev mlp;
+    float qs;
+    float4 *posm_out, *vel_out, *aux_out, *rec_out;
+};
+
 namespace ph {
-hipError_t launch_sur_sweep(hipStream_t st, uint32_t ntiles, const SurSweepArgs &a);
+hipError_t launch_sur_collect(hipStream_t st, uint32_t ntiles, const SurCollectArgs &a);
+hipError_t launch_sur_step(hipStream_t st, uint32_t ntiles, const SurStepArgs &a);
 hipError_t launch_gather_aux(hipStream_t st, uint32_t n, const uint32_t *perm, const float4 *in, float4 *out);
 hipError_t launch_aux_update(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                              const float2 *rq, const float4 *aux_prev, const uint32_t *tag, float4 *aux_out,
-                             float2 *tile_err);
+                             float2 *tile_err, const float4 *posm_new = nullptr, const float4 *vel_new = nullptr,
+                             float4 *rec_out = nullptr);
 hipError_t launch_err_reduce(hipStream_t st, const float2 *tile_err, uint32_t ntiles, float *out);
 hipError_t launch_aux_reset(hipStream_t st, uint32_t n, const uint32_t *tag, float4 rho0, float4 *aux);
 }  // namespace ph

@@ -1,9 +1,10 @@
 // surrogate_step.hip — the learned pressure term in the fluid step.
 //
-// A surrogate substep replaces the plain step's second neighbour sweep
-// (pressure force + XSPH, fluid3.rs:1029-1124) with one sweep that
-//   * sums the exact density (same candidates, order and arithmetic as
-//     the density pass, so rho is the plain value), and
+// A surrogate substep replaces the plain substep (cell sort, density pass,
+// pressure force + XSPH pass; fluid3.rs:990-1124) by ONE neighbour sweep
+// over the particle order and cell table of the last plain substep, using
+// a skin list built there (candidates within (1 + skin) h), which
+//   * sums the density rho_i over the current positions, and
 //   * aggregates pairwise messages m_j grad W_ij times (1, P_j, dP_j), where
 //     P_j = p_j / rho_j^2 and dP_j its last change are the neighbour's
 //     values from the previous substep, plus XSPH with previous densities;
@@ -12,7 +13,8 @@
 //   a0 = -(P_i sum m grad W + sum m P_j grad W) / rho_i,
 //   a  = a0 + c0 t + c1 a0 + c2 10 G^,   t = -(sum m dP_j grad W) / rho_i.
 // With c = 0 this is the network-free method.  Plain substeps every few
-// steps judge the result (bulk density deviation) and refresh history.
+// steps judge the result (bulk density deviation), refresh the history and
+// rebuild the skin list.
 #include <hip/hip_runtime.h>
 
 #include "mlp.h"
@@ -28,95 +30,69 @@ struct SurAcc {
     float gx = 0.0f, gy = 0.0f, gz = 0.0f;  // sum m_j grad W_ij
     float hx = 0.0f, hy = 0.0f, hz = 0.0f;  // sum m_j P_j grad W_ij
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;  // sum m_j dP_j grad W_ij
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;  // XSPH with previous densities
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;  // sum m_j W_ij (v_j - v_i)
 };
 
-// One neighbour j (r2 < h^2, j may be i) of particle i.  The density term
-// is the density pass's expression; the message terms use fast math.
-__device__ __forceinline__ void sur_pair(const SceneDev &s, uint32_t i, uint32_t j, const float4 &pi, const float4 &vi,
-                                         float rho_lag_i, const float4 &pj, const float4 *__restrict__ vel,
-                                         const float4 *__restrict__ aux, float r2, SurAcc &a) {
-    const float w = poly6(r2, s);
-    ++a.cnt;
-    a.rho += pj.w * w;
-    if (j == i || r2 <= PH_EPS_SQ) return;
-    const float4 aj = aux[j];
-    const float4 vj = vel[j];
-    const float inv_r = __frsqrt_rn(r2);
-    const float r = r2 * inv_r;
-    if (r < s.h) {
-        const float hr = s.h - r;
-        const float mg = pj.w * s.spiky_c * hr * hr * inv_r;  // m_j grad W = mg (x_i - x_j)
-        const float ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-        a.gx += mg * ex;
-        a.gy += mg * ey;
-        a.gz += mg * ez;
-        const float mh = mg * aj.y, md = mg * aj.z;
-        a.hx += mh * ex;
-        a.hy += mh * ey;
-        a.hz += mh * ez;
-        a.dx += md * ex;
-        a.dy += md * ey;
-        a.dz += md * ez;
-    }
-    if (s.viscosity > 0.0f) {
-        const float rb = 0.5f * (rmaxf(rho_lag_i, PH_EPS) + rmaxf(aj.x, PH_EPS));
-        const float q = pj.w * w / rb;
-        a.sx += (vj.x - vi.x) * q;
-        a.sy += (vj.y - vi.y) * q;
-        a.sz += (vj.z - vi.z) * q;
-    }
+// P with the phase (0..3) in its two lowest mantissa bits (a relative
+// change below 2^-21 of P, which the sweep reads as is).
+__device__ __forceinline__ float pack_pq(float q, uint32_t phase) {
+    return __uint_as_float((__float_as_uint(q) & ~3u) | min(phase, 3u));
 }
 
-// Sweep over particle i's neighbours (the density pass's traversal).
+// One candidate of particle i, branch-free: contributes when ok and
+// r2 < h^2; self = (j == i).  ra = (x, y, z, P|phase), rb = (v, dP) of the
+// candidate; its mass is the phase's entry of mt.  Candidates come from
+// grid cells, which hold finite positions only, so offsets and data are
+// finite and a zero weight zeroes a term.  Fast math throughout (hardware
+// rsqrt, fused multiply-adds): the surrogate is judged on its density
+// error, not on bit parity.
+__device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, const float4 &pi, const float4 &vi,
+                                         const float4 &mt, const float4 &ra, const float4 &rb, SurAcc &a) {
+    const float dx = ra.x - pi.x, dy = ra.y - pi.y, dz = ra.z - pi.z;
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const bool in = ok && r2 < s.h_grid2;
+    const float d = s.h2 - r2;
+    const uint32_t ph = __float_as_uint(ra.w) & 3u;
+    const float m = (ph & 2u) ? ((ph & 1u) ? mt.w : mt.z) : ((ph & 1u) ? mt.y : mt.x);
+    const float mw = in ? m * (s.poly6_c * d * d * d) : 0.0f;
+    a.cnt += in ? 1u : 0u;
+    a.rho += mw;
+    const bool nb = in && !self && r2 > PH_EPS_SQ;
+    const float inv_r = __builtin_amdgcn_rsqf(r2);
+    const float hr = s.h - r2 * inv_r;
+    const float mg = nb ? -(m * s.spiky_c) * (hr * hr * inv_r) : 0.0f;  // m_j grad W = mg (x_j - x_i)
+    a.gx = fmaf(mg, dx, a.gx);
+    a.gy = fmaf(mg, dy, a.gy);
+    a.gz = fmaf(mg, dz, a.gz);
+    const float mh = mg * ra.w, md = mg * rb.w;
+    a.hx = fmaf(mh, dx, a.hx);
+    a.hy = fmaf(mh, dy, a.hy);
+    a.hz = fmaf(mh, dz, a.hz);
+    a.dx = fmaf(md, dx, a.dx);
+    a.dy = fmaf(md, dy, a.dy);
+    a.dz = fmaf(md, dz, a.dz);
+    // XSPH numerator (mw = 0 off the neighbourhood; the self term has v_j - v_i = 0)
+    a.sx = fmaf(rb.x - vi.x, mw, a.sx);
+    a.sy = fmaf(rb.y - vi.y, mw, a.sy);
+    a.sz = fmaf(rb.z - vi.z, mw, a.sz);
+}
+
+// Sweep over particle i's grid candidates (collection on the gathered
+// state, records assembled from the state arrays).
 __device__ __forceinline__ void sur_sweep(const SceneDev &s, const uint32_t *__restrict__ start,
                                           const float4 *__restrict__ posm, const float4 *__restrict__ vel,
-                                          const float4 *__restrict__ aux, uint32_t i, const float4 &pi, const float4 &vi,
-                                          float rho_lag_i, SurAcc &a) {
+                                          const float4 *__restrict__ aux, const uint32_t *__restrict__ tag,
+                                          const float4 &mt, uint32_t i, const float4 &pi, const float4 &vi,
+                                          SurAcc &a) {
     if (!finite3(pi.x, pi.y, pi.z)) return;
     int32_t cx, cy, cz;
     cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
-    const float hg2 = s.h_grid2;
-    if (needs_cell_path(s, start, cx, cy, cz)) {
-        for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
-            const float4 pj = posm[j];
-            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-            if (r2 < hg2) sur_pair(s, i, j, pi, vi, rho_lag_i, pj, vel, aux, r2, a);
-        });
-        return;
-    }
-#pragma unroll 1
-    for (int r = 0; r < 9; ++r) {
-        uint32_t br, er;
-        row_range(s, start, cx, cy, cz, r, br, er);
-        const uint32_t lim = min(er, br + PH_ROW_BITS);
-        uint32_t m = 0;
-        uint32_t j = br;
-        for (; j + 4 <= lim; j += 4) {
-            const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
-            const uint32_t o = j - br;
-            m |= (dot3(q0.x - pi.x, q0.y - pi.y, q0.z - pi.z) < hg2 ? 1u : 0u) << o;
-            m |= (dot3(q1.x - pi.x, q1.y - pi.y, q1.z - pi.z) < hg2 ? 2u : 0u) << o;
-            m |= (dot3(q2.x - pi.x, q2.y - pi.y, q2.z - pi.z) < hg2 ? 4u : 0u) << o;
-            m |= (dot3(q3.x - pi.x, q3.y - pi.y, q3.z - pi.z) < hg2 ? 8u : 0u) << o;
-        }
-        for (; j < lim; ++j) {
-            const float4 pj = posm[j];
-            m |= (dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) < hg2 ? 1u : 0u) << (j - br);
-        }
-        while (m) {
-            const uint32_t k = __ffs(m) - 1;
-            m &= m - 1;
-            const float4 pj = posm[br + k];
-            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-            sur_pair(s, i, br + k, pi, vi, rho_lag_i, pj, vel, aux, r2, a);
-        }
-        for (j = lim; j < er; ++j) {
-            const float4 pj = posm[j];
-            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-            if (r2 < hg2) sur_pair(s, i, j, pi, vi, rho_lag_i, pj, vel, aux, r2, a);
-        }
-    }
+    for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
+        const float4 pj = posm[j], vj = vel[j], aj = aux[j];
+        const float4 ra = make_float4(pj.x, pj.y, pj.z, pack_pq(aj.y, tag[j] >> 24));
+        const float4 rb = make_float4(vj.x, vj.y, vj.z, aj.z);
+        sur_cand(s, true, j == i, pi, vi, mt, ra, rb, a);
+    });
 }
 
 __device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
@@ -129,23 +105,35 @@ struct SurFeat {
     float rho, p, q;
 };
 
+__device__ __forceinline__ float fast_log1p(float x) { return __logf(1.0f + x); }
+
+// Network inputs, basis vectors and the network-free acceleration of one
+// particle from its aggregates (fast math; the Tait power via exp2/log2).
 __device__ __forceinline__ void sur_features(const SceneDev &s, float rho0, float qs, const SurAcc &a,
                                              float rho_lag_i, float dq_lag_i, SurFeat &f) {
     const float rho = rclampf(a.rho, 0.0f, s.rho_max);
-    const float p = eos(rho, rho0, s.stiffness, s.gamma);
+    const float ir0 = __builtin_amdgcn_rcpf(rho0);
+    float p;
+    if (s.gamma == 1.0f) {
+        p = rmaxf(s.stiffness * (rho - rho0), 0.0f);
+    } else {
+        const float pw = __builtin_amdgcn_exp2f(s.gamma * __builtin_amdgcn_logf(rho * ir0));
+        p = rmaxf(s.stiffness * rho0 * (pw - 1.0f), 0.0f);
+    }
     const float rc = rmaxf(rho, PH_EPS);
-    const float q = p / (rc * rc);
+    const float inv = __builtin_amdgcn_rcpf(rc);
+    const float q = p * inv * inv;
     f.rho = rho;
     f.p = p;
     f.q = q;
-    const float inv = 1.0f / rc;
-    f.a0[0] = -(q * a.gx + a.hx) * inv;
-    f.a0[1] = -(q * a.gy + a.hy) * inv;
-    f.a0[2] = -(q * a.gz + a.hz) * inv;
+    f.a0[0] = -fmaf(q, a.gx, a.hx) * inv;
+    f.a0[1] = -fmaf(q, a.gy, a.hy) * inv;
+    f.a0[2] = -fmaf(q, a.gz, a.hz) * inv;
     const float t0 = -a.dx * inv, t1 = -a.dy * inv, t2 = -a.dz * inv;
-    const float gl = len3(a.gx, a.gy, a.gz);
-    const float gi = gl > 0.0f ? 1.0f / gl : 0.0f;
-    const float a0l = len3(f.a0[0], f.a0[1], f.a0[2]), tl = len3(t0, t1, t2);
+    const float gl = __builtin_amdgcn_sqrtf(fmaf(a.gz, a.gz, fmaf(a.gy, a.gy, a.gx * a.gx)));
+    const float gi = gl > 0.0f ? __builtin_amdgcn_rcpf(gl) : 0.0f;
+    const float a0l = __builtin_amdgcn_sqrtf(fmaf(f.a0[2], f.a0[2], fmaf(f.a0[1], f.a0[1], f.a0[0] * f.a0[0])));
+    const float tl = __builtin_amdgcn_sqrtf(fmaf(t2, t2, fmaf(t1, t1, t0 * t0)));
     f.b[0][0] = t0;
     f.b[0][1] = t1;
     f.b[0][2] = t2;
@@ -155,92 +143,32 @@ __device__ __forceinline__ void sur_features(const SceneDev &s, float rho0, floa
     f.b[2][0] = 10.0f * a.gx * gi;
     f.b[2][1] = 10.0f * a.gy * gi;
     f.b[2][2] = 10.0f * a.gz * gi;
-    const float tiny = 1e-20f;
-    f.x[0] = (rho / rho0 - 1.0f) * 100.0f;
-    f.x[1] = (rho_lag_i / rho0 - 1.0f) * 100.0f;
-    f.x[2] = log1pf(a0l * 0.1f);
-    f.x[3] = log1pf(tl * 0.1f);
-    f.x[4] = (f.a0[0] * t0 + f.a0[1] * t1 + f.a0[2] * t2) / (a0l * tl + tiny);
-    f.x[5] = gl * s.h / rho0;
-    f.x[6] = (f.a0[0] * a.gx + f.a0[1] * a.gy + f.a0[2] * a.gz) * gi / (a0l + tiny);
+    const float tiny = 1e-20f, iqs = __builtin_amdgcn_rcpf(qs);
+    f.x[0] = fmaf(rho, ir0, -1.0f) * 100.0f;
+    f.x[1] = fmaf(rho_lag_i, ir0, -1.0f) * 100.0f;
+    f.x[2] = fast_log1p(a0l * 0.1f);
+    f.x[3] = fast_log1p(tl * 0.1f);
+    f.x[4] = fmaf(f.a0[2], t2, fmaf(f.a0[1], t1, f.a0[0] * t0)) * __builtin_amdgcn_rcpf(fmaf(a0l, tl, tiny));
+    f.x[5] = gl * s.h * ir0;
+    f.x[6] = fmaf(f.a0[2], a.gz, fmaf(f.a0[1], a.gy, f.a0[0] * a.gx)) * gi * __builtin_amdgcn_rcpf(a0l + tiny);
     f.x[7] = (float)a.cnt * (1.0f / 30.0f) - 1.0f;
-    f.x[8] = log1pf(q / qs);
-    f.x[9] = copysignf(log1pf(fabsf(dq_lag_i) / qs), dq_lag_i);
+    f.x[8] = fast_log1p(q * iqs);
+    f.x[9] = copysignf(fast_log1p(fabsf(dq_lag_i) * iqs), dq_lag_i);
     f.x[10] = 1.0f;
     for (int c = 11; c < MLP_IN; ++c) f.x[c] = 0.0f;
 }
 
-}  // namespace
-
-// mode SUR_MODE_STEP: surrogate substep (writes the advanced state and the
-// new history).  mode SUR_MODE_COLLECT: after a plain substep on the same
-// gathered state, write training samples (target = the plain pressure
-// acceleration acc_target) for particles with id % stride == phase.
-__global__ __launch_bounds__(MLP_ROWS) void k_sur_sweep(SurSweepArgs a) {
-    __shared__ uint16_t xs[MLP_ROWS][MLP_IN];
-    __shared__ uint16_t hs[4][16][MLP_HPAD];
-    __shared__ float ys[MLP_ROWS][MLP_OUT];
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const TileDev t = a.tiles[tile];
-    const SceneDev s = a.scenes[t.scene];
-    const uint32_t i = t.first + threadIdx.x;
-    const bool active = i < s.part_end;
-    SurFeat f;
-    float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = pi, ai = pi;
-    float xs_x = 0.0f, xs_y = 0.0f, xs_z = 0.0f;  // XSPH sum
-    uint32_t tg = 0;
-    if (active) {
-        pi = a.posm[i];
-        vi = a.vel[i];
-        ai = a.aux[i];
-        tg = a.tag[i];
-        SurAcc acc;
-        sur_sweep(s, a.start, a.posm, a.vel, a.aux, i, pi, vi, ai.x, acc);
-        const float rho0 = a.scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
-        sur_features(s, rho0, a.qs, acc, ai.x, ai.z, f);
-        for (int c = 0; c < MLP_IN; ++c) xs[threadIdx.x][c] = mlp_bf16(f.x[c]);
-        xs_x = acc.sx;
-        xs_y = acc.sy;
-        xs_z = acc.sz;
-    } else {
-        for (int c = 0; c < MLP_IN; ++c) xs[threadIdx.x][c] = 0;
-    }
-    if (a.mode == SUR_MODE_COLLECT) {
-        if (active) {
-            const uint32_t id = tg & PH_ID_MASK;
-            if (id % a.stride == a.phase) {
-                const uint32_t k = a.sample_base + id / a.stride;
-                if (k < a.max_samples) {
-                    float *o = a.samples + (size_t)k * SUR_SAMPLE;
-                    for (int c = 0; c < MLP_IN; ++c) o[SUR_X + c] = f.x[c];
-                    for (int c = 0; c < SUR_NCOEF; ++c)
-                        for (int d = 0; d < 3; ++d) o[SUR_B + 3 * c + d] = f.b[c][d];
-                    const float4 at = a.acc_target[i];
-                    o[SUR_A0] = f.a0[0];
-                    o[SUR_A0 + 1] = f.a0[1];
-                    o[SUR_A0 + 2] = f.a0[2];
-                    o[SUR_AT] = at.x;
-                    o[SUR_AT + 1] = at.y;
-                    o[SUR_AT + 2] = at.z;
-                    o[SUR_S] = len3(at.x, at.y, at.z) + 10.0f;
-                }
-            }
-        }
-        return;
-    }
-    // SUR_MODE_STEP: network coefficients for the 256 particles of the tile
-    if (a.use_net) mlp_forward_block(a.mlp, xs, hs, ys);
-    if (!active) return;
-    float c[SUR_NCOEF];
-    for (int k = 0; k < SUR_NCOEF; ++k) c[k] = a.use_net ? ys[threadIdx.x][k] : 0.0f;
-    float ap[3];
-    for (int d = 0; d < 3; ++d) ap[d] = f.a0[d] + c[0] * f.b[0][d] + c[1] * f.b[1][d] + c[2] * f.b[2][d];
+// Integrate one particle with acceleration g + ap (the plain substep's
+// integrate, speed cap, finite guards and boundary clamp; fluid3.rs:1088-1179).
+__device__ __forceinline__ void sur_integrate(const SceneDev &s, const BodyDev *__restrict__ bodies, const float4 &pi,
+                                              const float4 &vi, const float ap[3], float xsx, float xsy, float xsz,
+                                              float4 &po, float4 &vo) {
     const float ax = s.gx + ap[0], ay = s.gy + ap[1], az = s.gz + ap[2];
     float ux = vi.x + ax * s.dt, uy = vi.y + ay * s.dt, uz = vi.z + az * s.dt;
     if (s.viscosity > 0.0f) {
-        ux += xs_x * s.viscosity;
-        uy += xs_y * s.viscosity;
-        uz += xs_z * s.viscosity;
+        ux += xsx * s.viscosity;
+        uy += xsy * s.viscosity;
+        uz += xsz * s.viscosity;
     }
     const float speed = sqrtf(dot3(ux, uy, uz));
     if (speed > s.max_velocity) {
@@ -256,10 +184,184 @@ __global__ __launch_bounds__(MLP_ROWS) void k_sur_sweep(SurSweepArgs a) {
         y = pi.y;
         z = pi.z;
     }
-    clamp_to_bodies(s, a.bodies, x, y, z, ux, uy, uz);
-    a.posm_out[i] = make_float4(x, y, z, pi.w);
-    a.vel_out[i] = make_float4(ux, uy, uz, 0.0f);
-    a.aux_out[i] = make_float4(f.rho, f.q, f.q - ai.y, 0.0f);
+    clamp_to_bodies(s, bodies, x, y, z, ux, uy, uz);
+    po = make_float4(x, y, z, pi.w);
+    vo = make_float4(ux, uy, uz, 0.0f);
+}
+
+// float4 loads through a buffer resource: 32-bit offsets (no 64-bit
+// address arithmetic per load), and reads past `count` elements return 0.
+struct BufF4 {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ BufF4(const float4 *p, uint32_t count)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(p), (short)0, (int)(count * 16u), 0x00020000)) {}
+    __device__ __forceinline__ float4 operator[](uint32_t j) const {
+        return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, j * 16u, 0, 0));
+    }
+};
+
+// Sweep over the skin list of particle i: its recorded candidates, two at a
+// time with their record loads issued together (the next pair's slots are
+// fetched meanwhile); the per-cell path when the list could not hold them.
+// rec holds the 32-byte record of slot j at float4s 2j, 2j + 1.
+__device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t *__restrict__ start,
+                                               const float4 *__restrict__ posm_p, const BufF4 &rec, const float4 &mt,
+                                               uint32_t cw, const int4 &cell, const uint32_t *__restrict__ idx,
+                                               uint32_t mstride, uint32_t i, const float4 &pi, const float4 &vi,
+                                               SurAcc &a) {
+    if (cw & SLF_NONE) return;
+    if (cw & SLF_SLOW) {
+        for_each_candidate(s, start, posm_p, cell.x, cell.y, cell.z,
+                           [&](uint32_t j) { sur_cand(s, true, j == i, pi, vi, mt, rec[2 * j], rec[2 * j + 1], a); });
+        return;
+    }
+    const uint32_t n = cw & SLF_COUNT;
+    // candidate k of this particle at byte 4 (k mstride + i) of idx
+    const __amdgpu_buffer_rsrc_t lr =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(idx), (short)0, (int)(SL_CAP * mstride * 4u), 0x00020000);
+    const uint32_t step = 4u * mstride;
+    uint32_t off = 4u * i;
+    uint32_t j0 = n > 0 ? __builtin_amdgcn_raw_buffer_load_b32(lr, off, 0, 0) : i;
+    uint32_t j1 = n > 1 ? __builtin_amdgcn_raw_buffer_load_b32(lr, off + step, 0, 0) : i;
+#pragma unroll 1
+    for (uint32_t k = 0; k < n; k += 2) {
+        off += 2u * step;
+        const uint32_t n0 = k + 2 < n ? __builtin_amdgcn_raw_buffer_load_b32(lr, off, 0, 0) : i;
+        const uint32_t n1 = k + 3 < n ? __builtin_amdgcn_raw_buffer_load_b32(lr, off + step, 0, 0) : i;
+        const float4 a0 = rec[2 * j0], b0 = rec[2 * j0 + 1];
+        const float4 a1 = rec[2 * j1], b1 = rec[2 * j1 + 1];
+        sur_cand(s, true, j0 == i, pi, vi, mt, a0, b0, a);
+        sur_cand(s, k + 1 < n, j1 == i, pi, vi, mt, a1, b1, a);
+        j0 = n0;
+        j1 = n1;
+    }
+}
+
+}  // namespace
+
+// After a plain substep, on the same gathered state: training samples
+// (target = the plain pressure + viscous acceleration acc_target) for the
+// particles with id % stride == phase, sample id / stride.
+__global__ __launch_bounds__(PH_TILE) void k_sur_collect(SurCollectArgs a) {
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const TileDev t = a.tiles[tile];
+    const SceneDev s = a.scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    if (i >= s.part_end) return;
+    const uint32_t tg = a.tag[i];
+    const uint32_t id = tg & PH_ID_MASK;
+    if (id % a.stride != a.phase) return;
+    const uint32_t k = a.sample_base + id / a.stride;
+    if (k >= a.max_samples) return;
+    const float4 pi = a.posm[i], vi = a.vel[i], ai = a.aux[i];
+    SurAcc acc;
+    sur_sweep(s, a.start, a.posm, a.vel, a.aux, a.tag, a.mass, i, pi, vi, acc);
+    const float rho0 = a.scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
+    SurFeat f;
+    sur_features(s, rho0, a.qs, acc, ai.x, ai.z, f);
+    float *o = a.samples + (size_t)k * SUR_SAMPLE;
+    for (int c = 0; c < MLP_IN; ++c) o[SUR_X + c] = f.x[c];
+    for (int c = 0; c < SUR_NCOEF; ++c)
+        for (int d = 0; d < 3; ++d) o[SUR_B + 3 * c + d] = f.b[c][d];
+    const float4 at = a.acc_target[i];
+    o[SUR_A0] = f.a0[0];
+    o[SUR_A0 + 1] = f.a0[1];
+    o[SUR_A0 + 2] = f.a0[2];
+    o[SUR_AT] = at.x;
+    o[SUR_AT + 1] = at.y;
+    o[SUR_AT + 2] = at.z;
+    o[SUR_S] = len3(at.x, at.y, at.z) + 10.0f;
+}
+
+// One surrogate substep: the skin-list sweep over the neighbour records,
+// the network on MFMA (computed transposed, H^T = W1^T X^T and
+// Y^T = W2^T H^T, so the hidden tile that one MFMA leaves in a lane is the
+// operand the next one needs there), and the integrate; writes the advanced
+// state, the new history and the records of the next substep.
+__global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
+    __shared__ uint16_t xs[PH_TILE][MLP_IN];
+    __shared__ float ys[PH_TILE][4];
+    __shared__ mlp_bf16x4 w1p[8][64], w2p[8][64];
+    __shared__ __align__(16) float b1s[MLP_HID];
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const TileDev t = a.tiles[tile];
+    const SceneDev s = a.scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    const bool active = i < s.part_end;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
+    if (a.use_net) {
+        // lane-ready weight tiles: w1p[nt][l] = W1[4q+i][16 nt + r], w2p[nt][l] = W2[16 nt + 4q + i][r]
+        for (int e = threadIdx.x; e < 8 * 64; e += PH_TILE) {
+            const int nt = e >> 6, l = e & 63, lr = l & 15, lq = l >> 4;
+            mlp_bf16x4 v1, v2;
+            for (int k = 0; k < 4; ++k) {
+                v1[k] = (short)a.mlp.w1[(4 * lq + k) * MLP_HID + nt * 16 + lr];
+                v2[k] = (short)a.mlp.w2[(nt * 16 + 4 * lq + k) * MLP_OUT + lr];
+            }
+            w1p[nt][l] = v1;
+            w2p[nt][l] = v2;
+        }
+        if (threadIdx.x < MLP_HID) b1s[threadIdx.x] = a.mlp.b1[threadIdx.x];
+    }
+    SurFeat f;
+    float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = pi, ai = pi;
+    float xsx = 0.0f, xsy = 0.0f, xsz = 0.0f;
+    uint32_t tg = 0;
+    if (active) {
+        pi = a.posm[i];
+        vi = a.vel[i];
+        ai = a.aux[i];
+        tg = a.tag[i];
+        const uint32_t cw = a.cnt[i];
+        const int4 cl = (cw & SLF_SLOW) ? a.cell[i] : make_int4(0, 0, 0, 0);
+        const BufF4 rec(a.rec, 2u * s.part_end);
+        SurAcc acc;
+        sur_list_sweep(s, a.start, a.posm, rec, a.mass, cw, cl, a.idx, a.mstride, i, pi, vi, acc);
+        const float rho0 = a.scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
+        sur_features(s, rho0, a.qs, acc, ai.x, ai.z, f);
+        const float irl = __builtin_amdgcn_rcpf(rmaxf(ai.x, PH_EPS));
+        xsx = acc.sx * irl;
+        xsy = acc.sy * irl;
+        xsz = acc.sz * irl;
+    }
+    float c[SUR_NCOEF] = {0.0f, 0.0f, 0.0f};
+    if (a.use_net) {
+        for (int k = 0; k < MLP_IN; ++k) xs[threadIdx.x][k] = active ? mlp_bf16_hw(f.x[k]) : (uint16_t)0;
+        __syncthreads();
+#pragma unroll 1
+        for (int mt = 0; mt < 4; ++mt) {
+            const int row = w * 64 + mt * 16 + r;
+            mlp_bf16x4 xb;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) xb[k] = (short)xs[row][4 * q + k];
+            mlp_f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+            for (int nt = 0; nt < 8; ++nt) {
+                const float4 bv = *reinterpret_cast<const float4 *>(&b1s[nt * 16 + 4 * q]);
+                const mlp_f32x4 z = mlp_mfma(w1p[nt][lane], xb, mlp_f32x4{bv.x, bv.y, bv.z, bv.w});  // + b1
+                mlp_bf16x4 hb;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) hb[k] = (short)mlp_bf16_hw(mlp_gelu(z[k], nullptr));
+                y = mlp_mfma(w2p[nt][lane], hb, y);
+            }
+            if (q == 0)
+                *reinterpret_cast<float4 *>(ys[row]) =
+                    make_float4(y[0] + a.mlp.b2[0], y[1] + a.mlp.b2[1], y[2] + a.mlp.b2[2], 0.0f);
+        }
+        __syncthreads();
+        for (int k = 0; k < SUR_NCOEF; ++k) c[k] = ys[threadIdx.x][k];
+    }
+    if (!active) return;
+    float ap[3];
+    for (int d = 0; d < 3; ++d) ap[d] = f.a0[d] + c[0] * f.b[0][d] + c[1] * f.b[1][d] + c[2] * f.b[2][d];
+    float4 po, vo;
+    sur_integrate(s, a.bodies, pi, vi, ap, xsx, xsy, xsz, po, vo);
+    a.posm_out[i] = po;
+    a.vel_out[i] = vo;
+    const float dq = f.q - ai.y;
+    a.aux_out[i] = make_float4(f.rho, f.q, dq, 0.0f);
+    a.rec_out[2 * i] = make_float4(po.x, po.y, po.z, pack_pq(f.q, tg >> 24));
+    a.rec_out[2 * i + 1] = make_float4(vo.x, vo.y, vo.z, dq);
 }
 
 // aux_out[d] = aux_in[perm[d]]: history follows its particle through the sort.
@@ -275,7 +377,8 @@ __global__ __launch_bounds__(256) void k_gather_aux(uint32_t n, const uint32_t *
 __global__ __launch_bounds__(256) void k_aux_update(const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes,
                                                     const float2 *__restrict__ rq, const float4 *__restrict__ aux_prev,
                                                     const uint32_t *__restrict__ tag, float4 *__restrict__ aux_out,
-                                                    float2 *__restrict__ tile_err) {
+                                                    float2 *__restrict__ tile_err, const float4 *__restrict__ posm_new,
+                                                    const float4 *__restrict__ vel_new, float4 *__restrict__ rec_out) {
     const uint32_t tile = blockIdx.x;
     const TileDev t = tiles[tile];
     const SceneDev &s = scenes[t.scene];
@@ -284,8 +387,15 @@ __global__ __launch_bounds__(256) void k_aux_update(const TileDev *__restrict__ 
     if (i < s.part_end) {
         const float2 v = rq[i];
         const float4 ap = aux_prev[i];
+        const uint32_t tg = tag[i];
         aux_out[i] = make_float4(v.x, v.y, v.y - ap.y, 0.0f);
-        const float rho0 = s.phase_rho0[min(tag[i] >> 24, 3u)];
+        if (rec_out) {
+            // the surrogate's neighbour records of the new state
+            const float4 p = posm_new[i], u = vel_new[i];
+            rec_out[2 * i] = make_float4(p.x, p.y, p.z, pack_pq(v.y, tg >> 24));
+            rec_out[2 * i + 1] = make_float4(u.x, u.y, u.z, v.y - ap.y);
+        }
+        const float rho0 = s.phase_rho0[min(tg >> 24, 3u)];
         const float r = v.x / rho0;
         if (r >= 0.9f) {
             e = fabsf(r - 1.0f);
@@ -342,8 +452,12 @@ __global__ __launch_bounds__(256) void k_aux_reset(uint32_t n, const uint32_t *_
 }
 
 namespace ph {
-hipError_t launch_sur_sweep(hipStream_t st, uint32_t ntiles, const SurSweepArgs &a) {
-    hipLaunchKernelGGL(k_sur_sweep, dim3(ntiles), dim3(MLP_ROWS), 0, st, a);
+hipError_t launch_sur_collect(hipStream_t st, uint32_t ntiles, const SurCollectArgs &a) {
+    hipLaunchKernelGGL(k_sur_collect, dim3(ntiles), dim3(PH_TILE), 0, st, a);
+    return hipGetLastError();
+}
+hipError_t launch_sur_step(hipStream_t st, uint32_t ntiles, const SurStepArgs &a) {
+    hipLaunchKernelGGL(k_sur_step, dim3(ntiles), dim3(PH_TILE), 0, st, a);
     return hipGetLastError();
 }
 hipError_t launch_gather_aux(hipStream_t st, uint32_t n, const uint32_t *perm, const float4 *in, float4 *out) {
@@ -352,9 +466,9 @@ hipError_t launch_gather_aux(hipStream_t st, uint32_t n, const uint32_t *perm, c
 }
 hipError_t launch_aux_update(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                              const float2 *rq, const float4 *aux_prev, const uint32_t *tag, float4 *aux_out,
-                             float2 *tile_err) {
+                             float2 *tile_err, const float4 *posm_new, const float4 *vel_new, float4 *rec_out) {
     hipLaunchKernelGGL(k_aux_update, dim3(ntiles), dim3(256), 0, st, tiles, scenes, rq, aux_prev, tag, aux_out,
-                       tile_err);
+                       tile_err, posm_new, vel_new, rec_out);
     return hipGetLastError();
 }
 hipError_t launch_err_reduce(hipStream_t st, const float2 *tile_err, uint32_t ntiles, float *out) {

@@ -119,6 +119,13 @@ struct World3 {
     Surrogate3 *col = nullptr;
     uint32_t col_every = 10, col_stride = 64, col_phase = 0;
     float err_host[2] = {0.0f, 0.0f};
+    // skin list of the last sort (surrogate substeps read it)
+    float skin = 0.05f;  // list radius (1 + skin) h
+    ph::DBuf<int4> sl_cell;
+    ph::DBuf<uint32_t> sl_cnt, sl_idx;
+    ph::DBuf<float4> sl_rec[2];  // neighbour records of generation k (2 float4 per slot)
+    uint64_t list_sorts = ~0ull, list_epoch = ~0ull;
+    uint32_t list_stride = 0;
     uint64_t n_col_skipped = 0;
     // first error of a step since the last sync (steps return nothing)
     std::string step_error;
@@ -215,11 +222,12 @@ struct World3 {
                 ensure_aux();
                 const bool collect = col && (step_index % col_every) == 0;
                 const bool judge = sur && (step_index % check_every) == 0;
-                if (sur && !judge && !collect && fallback_left == 0) {
+                if (sur && !judge && !collect && fallback_left == 0 && list_usable()) {
                     substep_surrogate();
                     ++n_sur;
                 } else {
-                    substep_plain_aux(judge, collect);
+                    // the next substep may be a surrogate one: leave it a list
+                    substep_plain_aux(judge, collect, sur && fallback_left <= 1);
                     ++n_plain;
                     if (fallback_left) --fallback_left;
                     if (judge && last_err > max_err) {
@@ -231,6 +239,11 @@ struct World3 {
             }
         }
         time += dt;
+    }
+
+    float4 mass4() const {
+        auto pm = [&](int p) { return std::max(cfg.phase_mass[p], 0.0f); };
+        return make_float4(pm(0), pm(1), pm(2), pm(3));
     }
 
     float qs() const {
@@ -255,32 +268,15 @@ struct World3 {
         e.aux_on = true;
     }
 
-    SurSweepArgs sweep_args(int mode) const {
-        const ph::Engine &e = eng;
-        const int ns = 1 - e.cs, nt = 1 - e.ct;
-        SurSweepArgs a{};
-        a.mode = mode;
-        a.use_net = use_net;
-        a.tiles = e.d_tiles.p;
-        a.scenes = e.d_scenes.p;
-        a.bodies = e.d_bodies.p;
-        a.start = e.start.p;
-        a.posm = e.posm[ns].p;
-        a.vel = e.vel[ns].p;
-        a.aux = e.aux[ns].p;
-        a.tag = e.tag[nt].p;
-        const Surrogate3 *net = mode == SUR_MODE_STEP ? sur : col;
-        if (net) a.mlp = net->dev();
-        a.qs = qs();
-        a.posm_out = e.posm[e.cs].p;
-        a.vel_out = e.vel[e.cs].p;
-        a.aux_out = e.aux[e.cs].p;
-        return a;
+    bool list_usable() const {
+        return sl_cell.p && sl_rec[0].p && list_sorts == eng.sorts_done && list_epoch == eng.layout_epoch &&
+               list_stride == eng.alloc_particles;
     }
 
     // Plain substep that also refreshes the history; optionally judged
-    // (bulk density deviation, read back) and/or collecting samples.
-    void substep_plain_aux(bool judge, bool collect) {
+    // (bulk density deviation, read back), collecting samples, and/or
+    // building the skin list for surrogate substeps.
+    void substep_plain_aux(bool judge, bool collect, bool list) {
         ph::Engine &e = eng;
         const uint32_t nt = (uint32_t)e.tiles.size();
         const int ns = 1 - e.cs, ntg = 1 - e.ct;
@@ -293,25 +289,55 @@ struct World3 {
             ++n_col_skipped;
         }
         if (collect) col->grow_samples((uint64_t)col->nsamples + cnt);
+        ph::ListOut lo;
+        if (list) {
+            sl_cell.ensure(e.alloc_particles);
+            sl_cnt.ensure(e.alloc_particles);
+            sl_idx.ensure((size_t)SL_CAP * e.alloc_particles);
+            lo.cell = sl_cell.p;
+            lo.cnt = sl_cnt.p;
+            lo.idx = sl_idx.p;
+            lo.skin_factor2 = (1.0f + skin) * (1.0f + skin);
+        }
         e.sort_phase(e.perm.p);
         e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_gather_aux(e.stream, n, e.perm.p, e.aux[e.cs].p, e.aux[ns].p); });
-        e.plain_phase(collect ? e.acc_sorted.p : nullptr);
+        e.plain_phase(collect ? e.acc_sorted.p : nullptr, list ? &lo : nullptr);
+        if (list) {
+            // the skin list of the order this substep sorted into
+            list_sorts = e.sorts_done;
+            list_epoch = e.layout_epoch;
+            list_stride = (uint32_t)e.alloc_particles;
+        }
         if (collect) {
-            SurSweepArgs a = sweep_args(SUR_MODE_COLLECT);
+            SurCollectArgs a{};
+            a.tiles = e.d_tiles.p;
+            a.scenes = e.d_scenes.p;
+            a.start = e.start.p;
+            a.posm = e.posm[ns].p;
+            a.vel = e.vel[ns].p;
+            a.aux = e.aux[ns].p;
+            a.tag = e.tag[ntg].p;
+            a.mass = mass4();
+            a.qs = qs();
             a.acc_target = e.acc_sorted.p;
             a.samples = col->data.p;
             a.sample_base = col->nsamples;
             a.stride = col_stride;
             a.phase = col_phase;
             a.max_samples = col->nsamples + cnt;
-            e.run(ph::KIND_SUR_SWEEP, [&] { return ph::launch_sur_sweep(e.stream, nt, a); });
+            e.run(ph::KIND_SUR_COLLECT, [&] { return ph::launch_sur_collect(e.stream, nt, a); });
             PH_CK(hipStreamSynchronize(e.stream));
             col->nsamples += cnt;
             col_phase = (col_phase + 1) % col_stride;
         }
+        if (sur) {
+            sl_rec[0].ensure(2 * e.alloc_particles);
+            sl_rec[1].ensure(2 * e.alloc_particles);
+        }
         e.run(ph::KIND_SUR_AUX, [&] {
             return ph::launch_aux_update(e.stream, nt, e.d_tiles.p, e.d_scenes.p, e.rq.p, e.aux[ns].p, e.tag[ntg].p,
-                                         e.aux[e.cs].p, e.tile_err.p);
+                                         e.aux[e.cs].p, e.tile_err.p, e.posm[e.cs].p, e.vel[e.cs].p,
+                                         sur ? sl_rec[e.cs].p : nullptr);
         });
         if (judge) {
             e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_err_reduce(e.stream, e.tile_err.p, nt, e.err_out.p); });
@@ -322,17 +348,36 @@ struct World3 {
         e.finish_substep();
     }
 
+    // Surrogate substep on the order of the last sort: generation cs is
+    // read, 1 - cs written, then becomes current (tags keep their order).
     void substep_surrogate() {
         ph::Engine &e = eng;
-        const uint32_t nt = (uint32_t)e.tiles.size();
-        const int ns = 1 - e.cs;
-        e.sort_phase(e.perm.p);
-        e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_gather_aux(e.stream, n, e.perm.p, e.aux[e.cs].p, e.aux[ns].p); });
-        const SurSweepArgs a = sweep_args(SUR_MODE_STEP);
-        e.run(ph::KIND_SUR_SWEEP, [&] { return ph::launch_sur_sweep(e.stream, nt, a); });
-        e.gen_open = true;
-        e.finish_substep();
+        SurStepArgs a{};
+        a.use_net = use_net;
+        a.tiles = e.d_tiles.p;
+        a.scenes = e.d_scenes.p;
+        a.bodies = e.d_bodies.p;
+        a.start = e.start.p;
+        a.posm = e.posm[e.cs].p;
+        a.vel = e.vel[e.cs].p;
+        a.aux = e.aux[e.cs].p;
+        a.tag = e.tag[e.ct].p;
+        a.cell = sl_cell.p;
+        a.cnt = sl_cnt.p;
+        a.idx = sl_idx.p;
+        a.mstride = list_stride;
+        a.rec = sl_rec[e.cs].p;
+        a.rec_out = sl_rec[1 - e.cs].p;
+        a.mass = mass4();
+        a.mlp = sur->dev();
+        a.qs = qs();
+        a.posm_out = e.posm[1 - e.cs].p;
+        a.vel_out = e.vel[1 - e.cs].p;
+        a.aux_out = e.aux[1 - e.cs].p;
+        e.run(ph::KIND_SUR_STEP, [&] { return ph::launch_sur_step(e.stream, (uint32_t)e.tiles.size(), a); });
+        e.cs = 1 - e.cs;
         e.rec_valid = false;  // the density and force passes did not run
+        ++e.substeps_done;
     }
 
     uint32_t add(const float *xyz, const uint8_t *phases, uint32_t count) {
@@ -573,6 +618,7 @@ uint32_t phieng_world3_fluid_set_velocity(World3 *world, const float *vel, uint3
         e.run(ph::KIND_SETVEL, [&] {
             return ph::launch_set_velocity(e.stream, world->n, e.tag[e.ct].p, e.staging_f.p, e.vel[e.cs].p);
         });
+        ++e.layout_epoch;  // surrogate records hold the old velocities: next substep plain
         PH_CK(hipStreamSynchronize(e.stream));
         return m;
     } catch (const std::exception &ex) {
@@ -673,6 +719,13 @@ int32_t phieng_world3_collect_samples(World3 *world, Surrogate3 *s, uint32_t eve
     world->col_stride = stride ? stride : 1;
     world->col_phase = 0;
     world->step_index = 0;
+    return 1;
+}
+
+int32_t phieng_world3_set_surrogate_skin(World3 *world, float skin) {
+    if (!world || !(skin >= 0.0f && skin <= 1.0f)) return 0;
+    world->skin = skin;
+    world->list_sorts = ~0ull;  // rebuild at the next plain substep
     return 1;
 }
 

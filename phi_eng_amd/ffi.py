@@ -138,6 +138,7 @@ SIGNATURES = {
     "phieng_world3_set_surrogate": (_I32, [_P, _P, _U32, _F, _I32]),
     "phieng_world3_collect_samples": (_I32, [_P, _P, _U32, _U32]),
     "phieng_world3_surrogate_stats": (_I32, [_P, _u64p, _u64p, _u64p, _fp]),
+    "phieng_world3_set_surrogate_skin": (_I32, [_P, _F]),
 }
 
 _lib = None

@@ -149,6 +149,10 @@ class World3:
                                                      int(every), int(stride)):
             raise RuntimeError("collect_samples rejected its arguments")
 
+    def set_surrogate_skin(self, skin):
+        if not self._l.phieng_world3_set_surrogate_skin(self.ptr, float(skin)):
+            raise RuntimeError("set_surrogate_skin rejected its argument")
+
     def surrogate_stats(self):
         a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         e = ctypes.c_float()
