@@ -53,9 +53,9 @@ __host__ __device__ __forceinline__ float mlp_f32(uint16_t h) {
 // derivative sigmoid + z sigmoid (1 - sigmoid) d(2u)/dz.
 __device__ __forceinline__ float mlp_gelu(float z, float *dgelu) {
     const float c0 = 2.0f * 0.7978845608028654f, c1 = 2.0f * 0.7978845608028654f * 0.044715f;
+    const float l2e = 1.4426950408889634f;  // exp(-v) = exp2(-v log2 e), folded into the constants
     const float z2 = z * z;
-    const float v = z * fmaf(c1, z2, c0);
-    const float sg = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(v * -1.4426950408889634f));
+    const float sg = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(-l2e * c1, z2, -l2e * c0)));
     if (dgelu) *dgelu = fmaf(z * sg * (1.0f - sg), fmaf(3.0f * c1, z2, c0), sg);
     return z * sg;
 }

@@ -47,13 +47,15 @@ __device__ __forceinline__ float pack_pq(float q, uint32_t phase) {
 // rsqrt, fused multiply-adds): the surrogate is judged on its density
 // error, not on bit parity.
 __device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, const float4 &pi, const float4 &vi,
-                                         const float4 &mt, const float4 &ra, const float4 &rb, SurAcc &a) {
+                                         float4 mt, const float4 &ra, const float4 &rb, SurAcc &a) {
     const float dx = ra.x - pi.x, dy = ra.y - pi.y, dz = ra.z - pi.z;
     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const bool in = ok && r2 < s.h_grid2;
     const float d = s.h2 - r2;
     const uint32_t ph = __float_as_uint(ra.w) & 3u;
-    const float m = (ph & 2u) ? ((ph & 1u) ? mt.w : mt.z) : ((ph & 1u) ? mt.y : mt.x);
+    const bool p1 = (ph & 1u) != 0u, p2 = (ph & 2u) != 0u;
+    const float m01 = p1 ? mt.y : mt.x, m23 = p1 ? mt.w : mt.z;
+    const float m = p2 ? m23 : m01;
     const float mw = in ? m * (s.poly6_c * d * d * d) : 0.0f;
     a.cnt += in ? 1u : 0u;
     a.rho += mw;
@@ -82,7 +84,7 @@ __device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, 
 __device__ __forceinline__ void sur_sweep(const SceneDev &s, const uint32_t *__restrict__ start,
                                           const float4 *__restrict__ posm, const float4 *__restrict__ vel,
                                           const float4 *__restrict__ aux, const uint32_t *__restrict__ tag,
-                                          const float4 &mt, uint32_t i, const float4 &pi, const float4 &vi,
+                                          float4 mt, uint32_t i, const float4 &pi, const float4 &vi,
                                           SurAcc &a) {
     if (!finite3(pi.x, pi.y, pi.z)) return;
     int32_t cx, cy, cz;
@@ -205,7 +207,7 @@ struct BufF4 {
 // fetched meanwhile); the per-cell path when the list could not hold them.
 // rec holds the 32-byte record of slot j at float4s 2j, 2j + 1.
 __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t *__restrict__ start,
-                                               const float4 *__restrict__ posm_p, const BufF4 &rec, const float4 &mt,
+                                               const float4 *__restrict__ posm_p, const BufF4 &rec, float4 mt,
                                                uint32_t cw, const int4 &cell, const uint32_t *__restrict__ idx,
                                                uint32_t mstride, uint32_t i, const float4 &pi, const float4 &vi,
                                                SurAcc &a) {
@@ -315,8 +317,9 @@ __global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
         const uint32_t cw = a.cnt[i];
         const int4 cl = (cw & SLF_SLOW) ? a.cell[i] : make_int4(0, 0, 0, 0);
         const BufF4 rec(a.rec, 2u * s.part_end);
+        const float4 mt = make_float4(a.mass.x, a.mass.y, a.mass.z, a.mass.w);
         SurAcc acc;
-        sur_list_sweep(s, a.start, a.posm, rec, a.mass, cw, cl, a.idx, a.mstride, i, pi, vi, acc);
+        sur_list_sweep(s, a.start, a.posm, rec, mt, cw, cl, a.idx, a.mstride, i, pi, vi, acc);
         const float rho0 = a.scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
         sur_features(s, rho0, a.qs, acc, ai.x, ai.z, f);
         const float irl = __builtin_amdgcn_rcpf(rmaxf(ai.x, PH_EPS));
