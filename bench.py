@@ -6,7 +6,11 @@ state, synthetic seeded input), with the HBM roofline of the dominant kernel
 from HIP-event kernel timings and the single-core CPU oracle timed on a
 bounded sample of the same scene.
 
-    python bench.py [--gpus 1] [--steps 500] [--warmup 20]
+    python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--sur-steps 200]
+
+The "surrogate" object holds the learned-pressure form on the same scene:
+its throughput, speed-up over the plain path over the same steps, and the
+mean density error against the plain run, with the network-free ablation.
 """
 import argparse
 import json
@@ -41,7 +45,86 @@ def _args():
                     help="plain: the 1M dam break (the headline); batch: 256 uneven scenes in one launch per step")
     ap.add_argument("--calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (rocprof counter scale)")
+    ap.add_argument("--sur-steps", type=int, default=200,
+                    help="steps of the surrogate vs plain comparison on the 1M scene (0 skips it)")
+    ap.add_argument("--sur-train-steps", type=int, default=2000,
+                    help="steps of the plain training trajectory the network is trained on")
     return ap.parse_args()
+
+
+def bench_surrogate(a, plain_value):
+    """The learned pressure term on the 1M scene: train the network on GPU
+    from a plain GPU trajectory of a half-scale dam (samples every 10 steps
+    from 1/16 of the particles), then run the 1M scene from the same start
+    with plain SPH, with the surrogate (plain judge every 10 steps, fallback
+    above 1% bulk density deviation), and with the surrogate's network
+    switched off; density error = mean |rho - rho_plain| / rho_plain after
+    the run."""
+    from phi_eng_amd import scenes
+    from phi_eng_amd.world import RECORD_DENSITY, Surrogate3
+
+    t0 = time.perf_counter()
+    tr = scenes.dam_break(32, 64, 64, box=(6.4, 3.2, 3.2), seed=4, name="train")
+    tr.config.stiffness = 1.8e6
+    wt = scenes.make_world(tr)
+    net = Surrogate3(seed=4)
+    wt.collect_samples(net, every=10, stride=16)
+    for _ in range(a.sur_train_steps):
+        wt.step(tr.dt)
+    wt.sync()
+    wt.close()
+    t1 = time.perf_counter()
+    loss0, base0 = net.evaluate()
+    first, last, base = net.train(epochs=8, batch=4096, lr=2e-3, seed=1)
+    t2 = time.perf_counter()
+
+    sc = scenes.c2_dam_break_1m()
+    n = sc.n
+    S = a.sur_steps
+    runs = {}
+    for name, use in (("plain", None), ("surrogate", True), ("network_free", False)):
+        w = scenes.make_world(sc)
+        if use is not None:
+            w.set_surrogate(net, check_every=10, max_density_error=0.01, use_network=use)
+        w.step(sc.dt)  # first step (set-up, judge) outside the timing
+        w.sync()
+        ta = time.perf_counter()
+        for _ in range(S):
+            w.step(sc.dt)
+        w.sync()
+        tb = time.perf_counter()
+        st = w.surrogate_stats()
+        w.set_surrogate(None)
+        w.set_record(RECORD_DENSITY)
+        w.step(sc.dt)
+        rho = w.get_record()["density"].astype(np.float64)
+        runs[name] = dict(wall=tb - ta, stats=st, rho=rho)
+        w.close()
+    ref = runs["plain"]["rho"]
+    out = {}
+    for name in ("surrogate", "network_free"):
+        r = runs[name]
+        e = np.abs(r["rho"] - ref) / np.maximum(ref, 1e-6)
+        out[name] = {"value": round(n * S / r["wall"], 1), "ms_per_step": round(r["wall"] * 1e3 / S, 4),
+                     "speedup_vs_plain": round(runs["plain"]["wall"] / r["wall"], 4),
+                     "density_err_mean": float(e.mean()), "density_err_p99": float(np.percentile(e, 99)),
+                     "surrogate_steps": r["stats"]["surrogate_steps"], "plain_steps": r["stats"]["plain_steps"],
+                     "fallbacks": r["stats"]["fallbacks"]}
+    net.close()
+    res = out["surrogate"]
+    res.update({
+        "plain_ms_per_step_same_steps": round(runs["plain"]["wall"] * 1e3 / S, 4),
+        "speedup_vs_plain_headline": round(res["value"] / plain_value, 4),
+        "steps": S, "check_every": 10, "max_density_error": 0.01, "skin": 0.05,
+        "network_free": out["network_free"],
+        "train": {"scene": "dam 32x64x64 (131072 particles), 6.4x3.2x3.2 tank, k 1.8e6",
+                  "trajectory_steps": a.sur_train_steps, "samples": int(a.sur_train_steps // 10 * 8192),
+                  "epochs": 8, "loss_untrained": round(loss0, 6), "loss_network_free": round(base, 6),
+                  "loss_first_epoch": round(first, 6), "loss_last_epoch": round(last, 6),
+                  "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)},
+        "data": "synthetic: seeded jittered lattice dam breaks; network trained in this run",
+    })
+    return res
 
 
 def calibrate(mib):
@@ -151,6 +234,9 @@ def main():
         "timed_with_events_ms_per_step": round((t3 - t2) * 1e3 / a.steps, 4),
         "cpu_baseline": cpu,
     }
+    w.close()
+    if a.sur_steps > 0:
+        out["surrogate"] = bench_surrogate(a, value)
     print(json.dumps(out))
 
 
