@@ -41,8 +41,9 @@ def _args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="HBM bytes per launch per kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE (optional)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "batch"],
-                    help="plain: the 1M dam break (the headline); batch: 256 uneven scenes in one launch per step")
+    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "scale8m"],
+                    help="plain: the 1M dam break (the headline); batch: 256 uneven scenes in one launch per step; "
+                         "scale8m: 8,388,608 particles, two colliding columns, plain vs surrogate")
     ap.add_argument("--calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (rocprof counter scale)")
     ap.add_argument("--sur-steps", type=int, default=200,
@@ -61,45 +62,12 @@ def bench_surrogate(a, plain_value):
     switched off; density error = mean |rho - rho_plain| / rho_plain after
     the run."""
     from phi_eng_amd import scenes
-    from phi_eng_amd.world import RECORD_DENSITY, Surrogate3
 
-    t0 = time.perf_counter()
-    tr = scenes.dam_break(32, 64, 64, box=(6.4, 3.2, 3.2), seed=4, name="train")
-    tr.config.stiffness = 1.8e6
-    wt = scenes.make_world(tr)
-    net = Surrogate3(seed=4)
-    wt.collect_samples(net, every=10, stride=16)
-    for _ in range(a.sur_train_steps):
-        wt.step(tr.dt)
-    wt.sync()
-    wt.close()
-    t1 = time.perf_counter()
-    loss0, base0 = net.evaluate()
-    first, last, base = net.train(epochs=8, batch=4096, lr=2e-3, seed=1)
-    t2 = time.perf_counter()
-
+    net, train = train_network(a.sur_train_steps)
     sc = scenes.c2_dam_break_1m()
     n = sc.n
     S = a.sur_steps
-    runs = {}
-    for name, use in (("plain", None), ("surrogate", True), ("network_free", False)):
-        w = scenes.make_world(sc)
-        if use is not None:
-            w.set_surrogate(net, check_every=10, max_density_error=0.01, use_network=use)
-        w.step(sc.dt)  # first step (set-up, judge) outside the timing
-        w.sync()
-        ta = time.perf_counter()
-        for _ in range(S):
-            w.step(sc.dt)
-        w.sync()
-        tb = time.perf_counter()
-        st = w.surrogate_stats()
-        w.set_surrogate(None)
-        w.set_record(RECORD_DENSITY)
-        w.step(sc.dt)
-        rho = w.get_record()["density"].astype(np.float64)
-        runs[name] = dict(wall=tb - ta, stats=st, rho=rho)
-        w.close()
+    runs = compare_runs(sc, net, S, ("plain", "surrogate", "network_free"))
     ref = runs["plain"]["rho"]
     out = {}
     for name in ("surrogate", "network_free"):
@@ -117,14 +85,112 @@ def bench_surrogate(a, plain_value):
         "speedup_vs_plain_headline": round(res["value"] / plain_value, 4),
         "steps": S, "check_every": 10, "max_density_error": 0.01, "skin": 0.05,
         "network_free": out["network_free"],
-        "train": {"scene": "dam 32x64x64 (131072 particles), 6.4x3.2x3.2 tank, k 1.8e6",
-                  "trajectory_steps": a.sur_train_steps, "samples": int(a.sur_train_steps // 10 * 8192),
-                  "epochs": 8, "loss_untrained": round(loss0, 6), "loss_network_free": round(base, 6),
-                  "loss_first_epoch": round(first, 6), "loss_last_epoch": round(last, 6),
-                  "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)},
+        "train": train,
         "data": "synthetic: seeded jittered lattice dam breaks; network trained in this run",
     })
     return res
+
+
+def train_network(steps):
+    """Collect samples from a plain GPU trajectory of a half-scale dam (every
+    10 steps, 1/16 of the particles) and train the network on GPU."""
+    from phi_eng_amd import scenes
+    from phi_eng_amd.world import Surrogate3
+
+    t0 = time.perf_counter()
+    tr = scenes.dam_break(32, 64, 64, box=(6.4, 3.2, 3.2), seed=4, name="train")
+    tr.config.stiffness = 1.8e6
+    wt = scenes.make_world(tr)
+    net = Surrogate3(seed=4)
+    wt.collect_samples(net, every=10, stride=16)
+    for _ in range(steps):
+        wt.step(tr.dt)
+    wt.sync()
+    wt.close()
+    t1 = time.perf_counter()
+    loss0, _ = net.evaluate()
+    first, last, base = net.train(epochs=8, batch=4096, lr=2e-3, seed=1)
+    t2 = time.perf_counter()
+    return net, {"scene": "dam 32x64x64 (131072 particles), 6.4x3.2x3.2 tank, k 1.8e6",
+                 "trajectory_steps": steps, "samples": int(net.num_samples()),
+                 "epochs": 8, "loss_untrained": round(loss0, 6), "loss_network_free": round(base, 6),
+                 "loss_first_epoch": round(first, 6), "loss_last_epoch": round(last, 6),
+                 "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)}
+
+
+def compare_runs(sc, net, steps, names):
+    """Run the scene from its start with each form (plain, surrogate,
+    network_free) for `steps` timed steps after one untimed set-up step;
+    return wall time, surrogate statistics and the density after the run."""
+    from phi_eng_amd import scenes
+    from phi_eng_amd.world import RECORD_DENSITY
+
+    runs = {}
+    for name in names:
+        w = scenes.make_world(sc)
+        if name != "plain":
+            w.set_surrogate(net, check_every=10, max_density_error=0.01, use_network=(name == "surrogate"))
+        w.step(sc.dt)  # first step (set-up, judge) outside the timing
+        w.sync()
+        ta = time.perf_counter()
+        for _ in range(steps):
+            w.step(sc.dt)
+        w.sync()
+        tb = time.perf_counter()
+        st = w.surrogate_stats()
+        w.set_surrogate(None)
+        w.set_record(RECORD_DENSITY)
+        w.step(sc.dt)
+        rho = w.get_record()["density"].astype(np.float64)
+        runs[name] = dict(wall=tb - ta, stats=st, rho=rho)
+        if name == "plain":
+            runs[name]["state"] = w.get_state()
+        w.close()
+        print(f"{name}: {steps} steps in {tb - ta:.3f} s", file=sys.stderr, flush=True)
+    return runs
+
+
+def bench_scale(a):
+    """8,388,608 particles (two 64 x 256 x 256 columns at the x-ends of a
+    12.8^3 tank, k 3.6e6, dt 1e-4): plain GPU, surrogate in the loop (plain
+    judge every 10 steps), and the CPU oracle on a bounded sample."""
+    from phi_eng_amd import scenes
+
+    steps = a.steps if a.steps != 500 else 300
+    net, train = train_network(a.sur_train_steps)
+    sc = scenes.c5_two_columns_8m()
+    n = sc.n
+    runs = compare_runs(sc, net, steps, ("plain", "surrogate"))
+    ref = runs["plain"]["rho"]
+    e = np.abs(runs["surrogate"]["rho"] - ref) / np.maximum(ref, 1e-6)
+    pv = n * steps / runs["plain"]["wall"]
+    sv = n * steps / runs["surrogate"]["wall"]
+    cpu = None
+    if not a.no_cpu and a.cpu_steps > 0:
+        from oracle import oracle as O
+        pos, vel = runs["plain"]["state"]
+        bodies = O.bodies_from_boxes(sc.boxes)
+        c0 = time.perf_counter()
+        O.run_f32(pos, vel, sc.config, sc.gravity, bodies, sc.dt, 1, jacobi=False)
+        c1 = time.perf_counter()
+        cpu = {"value": round(n / (c1 - c0), 1), "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"1 step of the same {n}-particle scene (state after the plain GPU run), "
+                         "f32 hash-grid Gauss-Seidel restatement of fluid3.rs, 1 thread"}
+    st = runs["surrogate"]["stats"]
+    out = {"metric": "particle-steps/sec at 8.4M-particle 3D SPH; surrogate vs plain", "value": round(sv, 1),
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+           "ms_per_step": round(runs["surrogate"]["wall"] * 1e3 / steps, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic: two seeded jittered lattice columns; network trained in this run",
+           "config": {"workload": "two_columns_3d_8m", "particles": n, "lattice": [[64, 256, 256], [64, 256, 256]],
+                      "tank": [12.8, 12.8, 12.8], "dt": sc.dt, "stiffness": sc.config.stiffness,
+                      "path": "surrogate", "check_every": 10, "max_density_error": 0.01, "skin": 0.05},
+           "plain": {"value": round(pv, 1), "ms_per_step": round(runs["plain"]["wall"] * 1e3 / steps, 4)},
+           "speedup_vs_plain": round(sv / pv, 4), "density_err_mean": float(e.mean()),
+           "density_err_p99": float(np.percentile(e, 99)),
+           "surrogate_steps": st["surrogate_steps"], "plain_steps": st["plain_steps"], "fallbacks": st["fallbacks"],
+           "train": train, "cpu_baseline": cpu}
+    print(json.dumps(out))
 
 
 def calibrate(mib):
@@ -154,6 +220,8 @@ def main():
         sys.exit(2)
     if a.workload == "batch":
         return bench_batch(a)
+    if a.workload == "scale8m":
+        return bench_scale(a)
     from phi_eng_amd import scenes
     from phi_eng_amd.ffi import last_error
 
