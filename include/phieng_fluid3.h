@@ -136,7 +136,9 @@ int32_t phieng_world3_get_timing(const World3 *world, double *total_ms, uint64_t
 const char *phieng_kernel_name(int32_t kind);
 
 /* Block until all queued GPU work of the world has finished.  Returns 1,
- * or 0 on null / a HIP error (then phieng_last_error names it). */
+ * or 0 on null, a HIP error, or a step that failed since the last sync
+ * (phieng_world3_step returns nothing, as in the reference); then
+ * phieng_last_error names the cause. */
 int32_t phieng_world3_sync(World3 *world);
 /* Text of the last error seen by any entry point of this thread ("" if none). */
 const char *phieng_last_error(void);
