@@ -84,11 +84,28 @@ __device__ __forceinline__ float poly6(float r2, const SceneDev &s) {
     return s.poly6_c * t * t * t;
 }
 
+// (float)x^gamma from the f64 power.  For gamma = 7 (the Tait exponent of
+// every configuration here) x^7 is four f64 products (relative error below
+// 2^-50), which round to the same f32 as the f64 pow unless x^7 lies within
+// 2^-48 (relative) of an f32 rounding midpoint; there the f64 pow decides.
+__device__ __forceinline__ float tait_pow(float x, float gamma) {
+    const double xd = (double)x;
+    if (gamma == 7.0f) {
+        const double x2 = xd * xd, x3 = x2 * xd, x6 = x3 * x3, x7 = x6 * xd;
+        const float f = (float)x7;
+        const double fd = (double)f;
+        const float g = nextafterf(f, x7 > fd ? INFINITY : -INFINITY);
+        const double mid = 0.5 * (fd + (double)g);  // exact: two f32 values
+        if (isfinite(x7) && fabs(x7 - mid) > fabs(x7) * 0x1p-48) return f;
+    }
+    return (float)pow(xd, (double)gamma);
+}
+
 // fluid3.rs:244-251; the Tait power is taken in f64 and rounded.
 __device__ __forceinline__ float eos(float density, float rest, float k, float gamma) {
     if (gamma == 1.0f) return rmaxf(k * (density - rest), 0.0f);
     float rho0 = rmaxf(rest, PH_EPS);
-    float pw = (float)pow((double)(density / rho0), (double)gamma);
+    float pw = tait_pow(density / rho0, gamma);
     return rmaxf(k * rho0 * (pw - 1.0f), 0.0f);
 }
 

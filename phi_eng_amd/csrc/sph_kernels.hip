@@ -294,7 +294,8 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                 uint32_t j = br;
                 // Four independent loads in flight per lane.
                 for (; j + 4 <= lim; j += 4) {
-                    const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
+                    const float4 *q = posm + j;  // one address, immediate offsets
+                    const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
                     const uint32_t o = j - br;
                     const float d0 = dot3(q0.x - pi.x, q0.y - pi.y, q0.z - pi.z);
                     const float d1 = dot3(q1.x - pi.x, q1.y - pi.y, q1.z - pi.z);
