@@ -91,6 +91,28 @@ def bench_surrogate(a, plain_value):
     return res
 
 
+def bench_cpu_2d():
+    """The 2D dam break (4,096 particles, 1,000 steps at dt 1/600): the
+    reference's 2D step is CPU code, timed here as its restatement on one
+    host thread."""
+    from oracle import oracle as O
+    from phi_eng_amd import scenes
+
+    sc = scenes.dam2d_config1()
+    cfg, mass, bodies = O.scene2_inputs(sc)
+    t0 = time.perf_counter()
+    out = O.run2_f32(sc.positions, np.zeros_like(sc.positions), mass, cfg, sc.gravity, bodies, sc.dt, sc.steps)
+    wall = time.perf_counter() - t0
+    d = out["density"]
+    return {"workload": sc.name, "particles": sc.n, "steps": sc.steps, "dt": sc.dt,
+            "value": round(sc.n * sc.steps / wall, 1), "unit": "particle-steps/s", "wall_s": round(wall, 3),
+            "cores": 1, "kind": "port",
+            "sample": "the whole workload: f32 restatement of fluid.rs (in-place integrate), 1 thread",
+            "final_density_median": float(np.median(d)),
+            "contained": bool(np.isfinite(out["pos"]).all() and (out["pos"][:, 0] > 0).all()
+                              and (out["pos"][:, 0] < 12.8).all() and (out["pos"][:, 1] > 0).all())}
+
+
 def train_network(steps):
     """Collect samples from a plain GPU trajectory of a half-scale dam (every
     10 steps, 1/16 of the particles) and train the network on GPU."""
@@ -280,6 +302,10 @@ def main():
                "sample": f"{a.cpu_steps} steps of the same {n}-particle scene (state after the GPU run), "
                          "f32 hash-grid Gauss-Seidel restatement of fluid3.rs, 1 thread"}
 
+    cpu2d = None
+    if not a.no_cpu:
+        cpu2d = bench_cpu_2d()
+
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -301,6 +327,7 @@ def main():
         "kernels": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
         "timed_with_events_ms_per_step": round((t3 - t2) * 1e3 / a.steps, 4),
         "cpu_baseline": cpu,
+        "cpu_dam_break_2d": cpu2d,
     }
     w.close()
     if a.sur_steps > 0:

@@ -9,13 +9,13 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle_sph.so")
-SRC = os.path.join(HERE, "sph_oracle.cpp")
+SRCS = [os.path.join(HERE, "sph_oracle.cpp"), os.path.join(HERE, "sph2d_oracle.cpp")]
 
 
 def build(verbose=False):
     tmp = LIB + ".tmp"
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-           "-fopenmp", SRC, "-o", tmp]
+           "-fopenmp", *SRCS, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")

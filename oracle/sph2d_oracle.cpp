@@ -1,0 +1,437 @@
+// sph2d_oracle.cpp — CPU restatement of the PhiEng 2D SPH fluid step.
+//
+// TEST INFRASTRUCTURE ONLY.  Like sph_oracle.cpp it is a checker and a CPU
+// baseline: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// legs load it, and the product library never links it.
+//
+// It follows crates/dynamics/src/fluid.rs (FluidState2) in f32 with the
+// reference's operation order (built with -ffp-contract=off): the grid of
+// rebuild_grid (:330-363), neighbors (:368-388), poly6 / spiky_grad /
+// pressure / artificial_pressure / cohesion (:265-327), cfl_substeps
+// (:562-589), compute_density_pressure (:592-614) and substep (:898-1136)
+// with the in-place Gauss-Seidel integrate of the reference (a Jacobi mode
+// reads pre-step state only) and the static circle / box boundary of
+// collide_particle_body (:463-525, identity rotation).  Buoyancy (:1076-1135)
+// only acts on dynamic bodies and is not restated.
+//
+// What pins it (tests/test_oracle2d.py): the reference's kernel sanity and
+// Tait-EOS known answers (fluid.rs:1209-1223, 1311-1331) including the
+// bit-level linear-EOS result, the 2D kernel normalisation, O(N^2)
+// brute-force neighbour counts and densities, the lattice closed form, the
+// two-particle force closed form and momentum antisymmetry, and the
+// reference's behaviour tests restated (dam break contained :1225-1256,
+// determinism :1292-1309, hydrostatic band :1676-1719).
+// Parity unpinned: glam 0.29.3 Vec2 association order ((x*x) + (y*y)) and
+// the last ulp of libm powf (taken in f64 and rounded, as in the 3D oracle).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+constexpr float kEps = 1e-6f;     // crates/math/src/math.rs:3
+constexpr float kEpsSq = 1e-12f;  // crates/math/src/math.rs:4
+constexpr float kPiF = 3.14159265358979323846f;
+
+inline float rmax(float a, float b) {  // Rust f32::max
+    if (a != a) return b;
+    if (b != b) return a;
+    return a > b ? a : b;
+}
+inline float rclamp(float v, float lo, float hi) {  // Rust f32::clamp
+    if (v < lo) v = lo;
+    if (v > hi) v = hi;
+    return v;
+}
+inline int32_t sat_i32(float v) {  // Rust `f32 as i32`
+    if (v != v) return 0;
+    if (v >= 2147483648.0f) return INT32_MAX;
+    if (v < -2147483648.0f) return INT32_MIN;
+    return (int32_t)v;
+}
+inline bool finite2(float x, float y) { return std::isfinite(x) && std::isfinite(y); }
+inline float len_sq(float x, float y) { return x * x + y * y; }  // glam Vec2::length_squared
+
+}  // namespace
+
+extern "C" {
+
+// Mirror of FluidConfig2 (fluid.rs:55-140) fields the step reads.
+struct Oracle2Cfg {
+    float rest_density, smoothing_length, stiffness, gamma, artificial_viscosity, surface_tension, cfl;
+    uint32_t max_substeps;
+    float viscosity, wall_friction;
+    float phase_rest_density[4];
+    float max_velocity, particle_radius;
+};
+
+// Static body, identity rotation: shape 0 circle (a = radius), 1 box
+// (a, b = half extents).
+struct Oracle2Body {
+    uint32_t shape;
+    float px, py, a, b;
+};
+
+}  // extern "C"
+
+namespace {
+
+struct P2 {
+    float x, y, vx, vy, rho, p, m;
+    uint8_t phase;
+};
+
+struct Fluid2 {
+    Oracle2Cfg cfg;
+    std::vector<P2> ps;
+    std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+    std::vector<float> fx, fy;
+
+    static uint64_t key(int32_t cx, int32_t cy) { return ((uint64_t)(uint32_t)cx << 32) | (uint32_t)cy; }
+
+    // fluid.rs:265-273
+    static float poly6(float r2, float h) {
+        if (r2 >= h * h || r2 < 0.0f) return 0.0f;
+        const float h2 = h * h;
+        const float t = h2 - r2;
+        return 4.0f / (kPiF * h2 * h2 * h2 * h2) * t * t * t;
+    }
+    // fluid.rs:278-285
+    static void spiky_grad(float rx, float ry, float r, float h, float &gx, float &gy) {
+        if (r >= h || r < kEps) {
+            gx = gy = 0.0f;
+            return;
+        }
+        const float s = -30.0f / (kPiF * h * h * h * h * h) * (h - r) * (h - r);
+        gx = rx / r * s;
+        gy = ry / r * s;
+    }
+    // fluid.rs:290-297; powf taken in f64 and rounded
+    static float pressure(float density, float rest, float k, float gamma) {
+        if (gamma == 1.0f) return rmax(k * (density - rest), 0.0f);
+        const float rho0 = rmax(rest, kEps);
+        const float pw = (float)std::pow((double)(density / rho0), (double)gamma);
+        return rmax(k * rho0 * (pw - 1.0f), 0.0f);
+    }
+    // fluid.rs:302-317
+    static float artificial_pressure(float dx, float dy, float vx, float vy, float r_sq, float h, float rho_bar, float c,
+                                     float alpha) {
+        const float vd = vx * dx + vy * dy;
+        if (vd >= 0.0f) return 0.0f;
+        const float mu = h * vd / (r_sq + 0.01f * h * h);
+        return -alpha * c * mu / rmax(rho_bar, kEps);
+    }
+    // fluid.rs:320-327
+    static float cohesion(float dist, float h) { return dist >= h ? 0.0f : 1.0f - dist / h; }
+
+    // fluid.rs:330-363 (cells hold indices in index order)
+    void rebuild_grid() {
+        grid.clear();
+        const float h = cfg.smoothing_length;
+        for (uint32_t i = 0; i < ps.size(); ++i) {
+            const P2 &p = ps[i];
+            if (!finite2(p.x, p.y)) continue;
+            grid[key(sat_i32(std::floor(p.x / h)), sat_i32(std::floor(p.y / h)))].push_back(i);
+        }
+    }
+    // fluid.rs:368-388: cells oy, ox in -1..=1, members in index order,
+    // d2 < h*h with the raw smoothing length; includes i.
+    void neighbors(uint32_t i, std::vector<uint32_t> &out) const { neighbors_in(ps, i, out); }
+    // the same against another copy of the particles (the Jacobi mode's pre-step state)
+    void neighbors_in(const std::vector<P2> &a, uint32_t i, std::vector<uint32_t> &out) const {
+        out.clear();
+        const float h = cfg.smoothing_length;
+        const float px = a[i].x, py = a[i].y;
+        const int32_t cx = sat_i32(std::floor(px / h)), cy = sat_i32(std::floor(py / h));
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int64_t x = (int64_t)cx + ox, y = (int64_t)cy + oy;
+                if (x < INT32_MIN || x > INT32_MAX || y < INT32_MIN || y > INT32_MAX) continue;
+                auto it = grid.find(key((int32_t)x, (int32_t)y));
+                if (it == grid.end()) continue;
+                for (uint32_t j : it->second) {
+                    const float d2 = len_sq(a[j].x - px, a[j].y - py);
+                    if (d2 < h * h) out.push_back(j);
+                }
+            }
+    }
+    // fluid.rs:562-589
+    uint32_t cfl_substeps(float dt) const {
+        const uint32_t mx = cfg.max_substeps > 1 ? cfg.max_substeps : 1;
+        if (cfg.cfl <= 0.0f || mx <= 1) return 1;
+        const float h = rmax(cfg.smoothing_length, kEps);
+        const float rho0 = rmax(cfg.rest_density, kEps);
+        const float c0 = cfg.gamma == 1.0f ? std::sqrt(rmax(cfg.stiffness, 0.0f))
+                                           : std::sqrt(cfg.gamma * rmax(cfg.stiffness, 0.0f) * rho0);
+        float vmax = 0.0f;
+        for (const P2 &p : ps) vmax = rmax(vmax, std::sqrt(len_sq(p.vx, p.vy)));
+        const float speed = c0 + vmax;
+        if (!std::isfinite(speed) || speed <= kEps) return 1;
+        const float dt_max = cfg.cfl * h / speed;
+        if (!std::isfinite(dt_max) || dt_max <= 0.0f) return 1;
+        const float q = std::ceil(dt / dt_max);
+        int64_t s;
+        if (q != q) s = 0;
+        else if (q >= 9.2233720368547758e18f) s = INT64_MAX;
+        else if (q <= -9.2233720368547758e18f) s = INT64_MIN;
+        else s = (int64_t)q;
+        if (s < 1) s = 1;
+        if (s > (int64_t)mx) s = mx;
+        return (uint32_t)s;
+    }
+    // fluid.rs:592-614
+    void density_pressure(float h, std::vector<uint32_t> &nb, uint32_t *count) {
+        for (uint32_t i = 0; i < ps.size(); ++i) {
+            neighbors(i, nb);
+            const float px = ps[i].x, py = ps[i].y;
+            float rho = 0.0f;
+            for (uint32_t j : nb) rho += ps[j].m * poly6(len_sq(ps[j].x - px, ps[j].y - py), h);
+            ps[i].rho = rclamp(rho, 0.0f, 3.0f * rmax(cfg.rest_density, kEps));
+            if (count) count[i] = (uint32_t)nb.size();
+        }
+        for (P2 &p : ps) {
+            const float rho0 = cfg.phase_rest_density[p.phase < 3 ? p.phase : 3];
+            p.p = pressure(p.rho, rho0, cfg.stiffness, cfg.gamma);
+        }
+    }
+    // fluid.rs:463-525 for identity-rotation circle and box; false when separated.
+    static bool collide(float px, float py, float pr, const Oracle2Body &b, float &nx, float &ny, float &depth) {
+        if (b.shape == 0u) {
+            const float dx = px - b.px, dy = py - b.py;
+            const float dist = std::sqrt(len_sq(dx, dy));
+            if (dist >= b.a + pr) return false;
+            if (dist > kEps) {
+                nx = dx / dist;
+                ny = dy / dist;
+            } else {
+                nx = 1.0f;
+                ny = 0.0f;
+            }
+            depth = b.a + pr - dist;
+            return true;
+        }
+        const float rx = px - b.px, ry = py - b.py;
+        // local = R^T rel with R = identity (x_axis (1, 0), y_axis (0, 1))
+        const float lx = 1.0f * rx + 0.0f * ry, ly = 0.0f * rx + 1.0f * ry;
+        const float cx = rclamp(lx, -b.a, b.a), cy = rclamp(ly, -b.b, b.b);
+        const float ddx = lx - cx, ddy = ly - cy;
+        float lnx, lny;
+        if (len_sq(ddx, ddy) > kEpsSq) {
+            const float dist = std::sqrt(len_sq(ddx, ddy));
+            if (dist >= pr) return false;
+            lnx = ddx / dist;
+            lny = ddy / dist;
+            depth = pr - dist;
+        } else {
+            // least-penetration axis; signum().copysign(1.0) is +1 (fluid.rs:507-512)
+            const float dx = b.a - std::fabs(lx), dy = b.b - std::fabs(ly);
+            if (dx < dy) {
+                lnx = 1.0f;
+                lny = 0.0f;
+                depth = pr + dx;
+            } else {
+                lnx = 0.0f;
+                lny = 1.0f;
+                depth = pr + dy;
+            }
+        }
+        // normal = R local_n (glam Mat2 * Vec2: x_axis * n.x + y_axis * n.y)
+        nx = 1.0f * lnx + 0.0f * lny;
+        ny = 0.0f * lnx + 1.0f * lny;
+        return true;
+    }
+
+    // fluid.rs:898-1136 (no adaptive resolution, boundary particles,
+    // turbulence or dynamic bodies).
+    void substep(float gx, float gy, const Oracle2Body *bodies, uint32_t nbodies, float dt, bool jacobi,
+                 uint32_t *count, float *acc_out) {
+        const float h = rmax(cfg.smoothing_length, kEps);
+        rebuild_grid();
+        std::vector<uint32_t> nb;
+        density_pressure(h, nb, count);
+        const float c_s = std::sqrt(rmax(cfg.stiffness, 0.0f));
+        const size_t n = ps.size();
+        fx.assign(n, 0.0f);
+        fy.assign(n, 0.0f);
+        for (uint32_t i = 0; i < n; ++i) {
+            neighbors(i, nb);
+            const P2 pi = ps[i];
+            const float rho_i = rmax(pi.rho, kEps);
+            const float p_i = pi.p;
+            float f_x = 0.0f, f_y = 0.0f;
+            for (uint32_t j : nb) {
+                if (j == i) continue;
+                const P2 &pj = ps[j];
+                const float rho_j = rmax(pj.rho, kEps);
+                const float dx = pi.x - pj.x, dy = pi.y - pj.y;
+                const float dist = std::sqrt(len_sq(dx, dy));
+                float gxx, gyy;
+                spiky_grad(dx, dy, dist, h, gxx, gyy);
+                float term = p_i / (rho_i * rho_i) + pj.p / (rho_j * rho_j);
+                if (cfg.artificial_viscosity > 0.0f && dist > kEps) {
+                    const float rho_bar = 0.5f * (rho_i + rho_j);
+                    term += artificial_pressure(dx, dy, pi.vx - pj.vx, pi.vy - pj.vy, dist * dist, h, rho_bar, c_s,
+                                                cfg.artificial_viscosity);
+                }
+                const float s = pj.m * term;
+                f_x = f_x - gxx * s;
+                f_y = f_y - gyy * s;
+                if (cfg.surface_tension > 0.0f && dist > kEps) {
+                    const float coh = cohesion(dist, h);
+                    if (coh > 0.0f) {
+                        const float c = cfg.surface_tension * pi.m * pj.m * coh;
+                        f_x = f_x - dx / dist * c;
+                        f_y = f_y - dy / dist * c;
+                    }
+                }
+            }
+            fx[i] = f_x;
+            fy[i] = f_y;
+        }
+        // Integrate: in place (reference) or from the pre-step state (Jacobi).
+        const std::vector<P2> pre = jacobi ? ps : std::vector<P2>();
+        const std::vector<P2> &src = jacobi ? pre : ps;
+        for (uint32_t i = 0; i < n; ++i) {
+            // neighbour list against the positions the reference reads here
+            // (in place: already-moved j < i; Jacobi: the pre-step state)
+            neighbors_in(src, i, nb);
+            const P2 &me = src[i];
+            const float rho_i = rmax(me.rho, kEps);
+            const float ax = gx + fx[i] / rho_i, ay = gy + fy[i] / rho_i;
+            if (acc_out) {
+                acc_out[2 * i] = ax;
+                acc_out[2 * i + 1] = ay;
+            }
+            float vx = me.vx + ax * dt, vy = me.vy + ay * dt;
+            if (cfg.viscosity > 0.0f) {
+                float sx = 0.0f, sy = 0.0f;
+                for (uint32_t j : nb) {
+                    if (j == i) continue;
+                    const P2 &pj = src[j];
+                    const float rho_bar = 0.5f * (rho_i + rmax(pj.rho, kEps));
+                    const float q = pj.m / rho_bar;
+                    const float w = poly6(len_sq(pj.x - me.x, pj.y - me.y), h);
+                    sx = sx + (pj.vx - me.vx) * q * w;
+                    sy = sy + (pj.vy - me.vy) * q * w;
+                }
+                vx = vx + sx * cfg.viscosity;
+                vy = vy + sy * cfg.viscosity;
+            }
+            const float speed = std::sqrt(len_sq(vx, vy));
+            if (speed > cfg.max_velocity) {
+                const float r = cfg.max_velocity / speed;
+                vx = vx * r;
+                vy = vy * r;
+            }
+            if (!finite2(vx, vy)) vx = vy = 0.0f;
+            float x = me.x + vx * dt, y = me.y + vy * dt;
+            if (!finite2(x, y)) {
+                x = me.x;
+                y = me.y;
+            }
+            ps[i].vx = vx;
+            ps[i].vy = vy;
+            ps[i].x = x;
+            ps[i].y = y;
+        }
+        // Boundaries (fluid.rs:1028-1071): push-out accumulates over bodies.
+        const float pr = rmax(cfg.particle_radius, 0.0f);
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t b = 0; b < nbodies; ++b) {
+                float nx, ny, depth;
+                if (!collide(ps[i].x, ps[i].y, pr, bodies[b], nx, ny, depth)) continue;
+                if (depth <= 0.0f) continue;
+                ps[i].x = ps[i].x + nx * depth;
+                ps[i].y = ps[i].y + ny * depth;
+                const float vn = ps[i].vx * nx + ps[i].vy * ny;
+                if (vn < 0.0f) {
+                    ps[i].vx = ps[i].vx - nx * vn;
+                    ps[i].vy = ps[i].vy - ny * vn;
+                    if (cfg.wall_friction > 0.0f) {
+                        const float mu = rclamp(cfg.wall_friction, 0.0f, 1.0f);
+                        // static wall: velocity 0, so wall_vt = 0 - n * 0
+                        const float wvx = 0.0f - nx * 0.0f, wvy = 0.0f - ny * 0.0f;
+                        const float dvx = (wvx - ps[i].vx) * mu, dvy = (wvy - ps[i].vy) * mu;
+                        ps[i].vx = ps[i].vx + dvx;
+                        ps[i].vy = ps[i].vy + dvy;
+                    }
+                }
+            }
+    }
+};
+
+Fluid2 make(uint32_t n, const float *pos, const float *vel, const float *mass, const uint8_t *phase,
+            const Oracle2Cfg *cfg) {
+    Fluid2 f;
+    f.cfg = *cfg;
+    f.ps.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        P2 &p = f.ps[i];
+        p.x = pos[2 * i];
+        p.y = pos[2 * i + 1];
+        p.vx = vel ? vel[2 * i] : 0.0f;
+        p.vy = vel ? vel[2 * i + 1] : 0.0f;
+        p.phase = phase ? phase[i] : 0;
+        p.m = mass[i];
+        p.rho = cfg->phase_rest_density[p.phase < 3 ? p.phase : 3];
+        p.p = 0.0f;
+    }
+    return f;
+}
+
+void store(const Fluid2 &f, float *pos, float *vel, float *density, float *pressure) {
+    for (size_t i = 0; i < f.ps.size(); ++i) {
+        pos[2 * i] = f.ps[i].x;
+        pos[2 * i + 1] = f.ps[i].y;
+        if (vel) {
+            vel[2 * i] = f.ps[i].vx;
+            vel[2 * i + 1] = f.ps[i].vy;
+        }
+        if (density) density[i] = f.ps[i].rho;
+        if (pressure) pressure[i] = f.ps[i].p;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// One substep; pos/vel (2n, updated in place), density/pressure/count/acc
+// (2n) of that substep out (each may be null).
+int oracle2_substep_f32(uint32_t n, float *pos, float *vel, const float *mass, const uint8_t *phase,
+                        const Oracle2Cfg *cfg, float gx, float gy, const Oracle2Body *bodies, uint32_t nbodies,
+                        float dt, int jacobi, float *density, float *pressure, uint32_t *count, float *acc) {
+    if (!n || !pos || !mass || !cfg) return 0;
+    Fluid2 f = make(n, pos, vel, mass, phase, cfg);
+    f.substep(gx, gy, bodies, nbodies, dt, jacobi != 0, count, acc);
+    store(f, pos, vel, density, pressure);
+    return 1;
+}
+
+// `steps` World2 steps of dt (fluid.rs:537-556: CFL substeps of dt / n).
+int oracle2_run_f32(uint32_t n, float *pos, float *vel, const float *mass, const uint8_t *phase,
+                    const Oracle2Cfg *cfg, float gx, float gy, const Oracle2Body *bodies, uint32_t nbodies, float dt,
+                    uint32_t steps, int jacobi, float *density, float *pressure) {
+    if (!n || !pos || !mass || !cfg) return 0;
+    Fluid2 f = make(n, pos, vel, mass, phase, cfg);
+    for (uint32_t s = 0; s < steps; ++s) {
+        if (dt <= 0.0f) break;
+        const uint32_t sub = f.cfl_substeps(dt);
+        const float sdt = dt / (float)sub;
+        for (uint32_t k = 0; k < sub; ++k) f.substep(gx, gy, bodies, nbodies, sdt, jacobi != 0, nullptr, nullptr);
+    }
+    store(f, pos, vel, density, pressure);
+    return 1;
+}
+
+float oracle2_poly6(float r2, float h) { return Fluid2::poly6(r2, h); }
+void oracle2_spiky_grad(float rx, float ry, float r, float h, float *out2) {
+    Fluid2::spiky_grad(rx, ry, r, h, out2[0], out2[1]);
+}
+float oracle2_pressure(float density, float rest, float k, float gamma) {
+    return Fluid2::pressure(density, rest, k, gamma);
+}
+
+}  // extern "C"

@@ -146,3 +146,48 @@ def make_world(scene, world_cls=None):
     if got != scene.n:
         raise RuntimeError(f"added {got} of {scene.n} particles")
     return w
+
+
+@dataclass
+class Scene2:
+    """A 2D scene for the CPU 2D restatement (oracle) only: the product has
+    no 2D GPU path."""
+    name: str
+    positions: np.ndarray  # (N, 2) float32
+    config: dict           # FluidConfig2 fields
+    boxes: list            # [(center (x, y), half_extents (hx, hy))]
+    gravity: tuple = (0.0, -9.81)
+    dt: float = 1.0 / 600.0
+    steps: int = 1000
+
+    @property
+    def n(self):
+        return len(self.positions)
+
+
+def lattice_sum_2d(h=H, s=SPACING):
+    """sum_j W2(|x_j|) over the spacing-s square lattice (the 2D poly6 kernel
+    4 / (pi h^8) (h^2 - r^2)^3, fluid.rs:265-273), in f64."""
+    k = int(math.ceil(h / s))
+    return sum(4.0 / (math.pi * h ** 8) * max(h * h - (a * a + b * b) * s * s, 0.0) ** 3
+               for a in range(-k, k + 1) for b in range(-k, k + 1))
+
+
+def dam2d_config1(seed=1):
+    """Config 1, the 2D dam break on the CPU: 4,096 particles on a 64 x 64
+    lattice at s = h/2 = 0.05 (+-1% seeded jitter), a 3.2 x 3.2 column against
+    the left wall of a 12.8-wide tank (walls 4.8 high), Tait gamma 7, rest
+    density 1000, dt = 1/600, 1,000 steps.  The step divides the pressure
+    force by rho a second time (fluid.rs:1006, acc = g + F / rho), so the
+    stiffness carries a factor rho0: k = rho0 c^2 / gamma with c = 20 m/s
+    (Courant number c dt / h = 0.33 without substeps)."""
+    rng = np.random.default_rng(seed)
+    ix, iy = np.meshgrid(np.arange(64), np.arange(64))
+    pos = np.stack([ix.ravel(), iy.ravel()], axis=1).astype(np.float64)
+    pos = (pos + 0.5) * SPACING + rng.uniform(-0.01, 0.01, size=pos.shape) * SPACING
+    mass = RHO0 / lattice_sum_2d()
+    cfg = dict(rest_density=RHO0, smoothing_length=H, stiffness=RHO0 * 20.0 ** 2 / 7.0, gamma=7.0,
+               viscosity=0.3, max_velocity=30.0, particle_radius=SPACING / 2, mass=mass)
+    t = WALL / 2
+    boxes = [((6.4, -t), (6.4 + WALL, t)), ((-t, 2.4), (t, 2.4)), ((12.8 + t, 2.4), (t, 2.4))]
+    return Scene2("dam_break_2d_4096", pos.astype(np.float32), cfg, boxes)
