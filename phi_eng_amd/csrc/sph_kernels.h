@@ -52,7 +52,7 @@ hipError_t launch_density(hipStream_t st, uint32_t ntiles, const TileDev *tiles,
 struct ListOut {
     int4 *cell = nullptr;
     uint32_t *cnt = nullptr;
-    uint32_t *idx = nullptr;
+    uint32_t *idx = nullptr;  // SL_CAP / 4 groups of 4 slots, group g of particle i at uint4 g * mstride + i
     float skin_factor2 = 1.0f;
 };
 hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,

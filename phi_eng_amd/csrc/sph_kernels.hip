@@ -277,7 +277,19 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
     float rho = 0.0f;
     uint32_t cnt = 0, fl = 0;
     uint32_t lk = 0;                          // skin list length
-    uint32_t *lcol = LIST ? lo.idx + i : nullptr;
+    // skin list entries are written four at a time: group g of this particle
+    // (entries 4g .. 4g + 3) is one 16-byte store at lgrp[g * mstride]
+    uint4 *lgrp = LIST ? reinterpret_cast<uint4 *>(lo.idx) + i : nullptr;
+    uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+    auto push = [&](uint32_t j) {
+        const uint32_t p = lk & 3u;
+        q0 = p == 0u ? j : q0;
+        q1 = p == 1u ? j : q1;
+        q2 = p == 2u ? j : q2;
+        q3 = p == 3u ? j : q3;
+        if (p == 3u && lk < SL_CAP) lgrp[(size_t)(lk >> 2) * mstride] = make_uint4(q0, q1, q2, q3);
+        ++lk;
+    };
     const float hs2 = LIST ? s.h_grid2 * lo.skin_factor2 : 0.0f;
     if (finite3(pi.x, pi.y, pi.z)) {
         int32_t cx, cy, cz;
@@ -317,8 +329,7 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                     while (ml) {
                         const uint32_t k = __ffs(ml) - 1;
                         ml &= ml - 1;
-                        if (lk < SL_CAP) lcol[(size_t)lk * mstride] = br + k;
-                        ++lk;
+                        push(br + k);
                     }
                 }
                 while (m) {
@@ -339,13 +350,16 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                             rho += pj.w * poly6(r2, s);
                         }
                         if (LIST && r2 < hs2) {
-                            if (lk < SL_CAP) lcol[(size_t)lk * mstride] = j;
-                            ++lk;
+                            push(j);
                         }
                     }
                 }
             }
-            if (LIST) lo.cnt[i] = lk <= SL_CAP ? lk : SLF_SLOW;
+            if (LIST) {
+                // the last, partly filled group
+                if ((lk & 3u) != 0u && lk < SL_CAP) lgrp[(size_t)(lk >> 2) * mstride] = make_uint4(q0, q1, q2, q3);
+                lo.cnt[i] = lk <= SL_CAP ? lk : SLF_SLOW;
+            }
         } else {
             fl = PH_SLOW_PATH;
             if (LIST) lo.cnt[i] = SLF_SLOW;

@@ -33,7 +33,7 @@ struct SurStepArgs {
     // skin list written by the density pass of the last plain substep
     const int4 *cell;      // skin list: cell at list time
     const uint32_t *cnt;   // skin list: candidate count | SLF_ flags
-    const uint32_t *idx;   // skin list: candidate k of particle i at idx[k * mstride + i]
+    const uint32_t *idx;   // skin list: candidates 4g..4g+3 of particle i as uint4 g * mstride + i
     uint32_t mstride;
     // neighbour records of the current state, 2 float4 per slot:
     // (x, y, z, P with the phase in its 2 low bits), (v, dP)

@@ -218,24 +218,35 @@ __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t
         return;
     }
     const uint32_t n = cw & SLF_COUNT;
-    // candidate k of this particle at byte 4 (k mstride + i) of idx
+    // group g (candidates 4g .. 4g + 3) of this particle: 16 bytes at byte
+    // 16 (g mstride + i) of idx; the next group is fetched while this one
+    // is processed, two candidates at a time
     const __amdgpu_buffer_rsrc_t lr =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(idx), (short)0, (int)(SL_CAP * mstride * 4u), 0x00020000);
-    const uint32_t step = 4u * mstride;
-    uint32_t off = 4u * i;
-    uint32_t j0 = n > 0 ? __builtin_amdgcn_raw_buffer_load_b32(lr, off, 0, 0) : i;
-    uint32_t j1 = n > 1 ? __builtin_amdgcn_raw_buffer_load_b32(lr, off + step, 0, 0) : i;
+    const uint32_t step = 16u * mstride;
+    uint32_t off = 16u * i;
+    uint4 grp = n > 0 ? __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lr, off, 0, 0))
+                      : make_uint4(i, i, i, i);
 #pragma unroll 1
-    for (uint32_t k = 0; k < n; k += 2) {
-        off += 2u * step;
-        const uint32_t n0 = k + 2 < n ? __builtin_amdgcn_raw_buffer_load_b32(lr, off, 0, 0) : i;
-        const uint32_t n1 = k + 3 < n ? __builtin_amdgcn_raw_buffer_load_b32(lr, off + step, 0, 0) : i;
-        const float4 a0 = rec[2 * j0], b0 = rec[2 * j0 + 1];
-        const float4 a1 = rec[2 * j1], b1 = rec[2 * j1 + 1];
-        sur_cand(s, true, j0 == i, pi, vi, mt, a0, b0, a);
-        sur_cand(s, k + 1 < n, j1 == i, pi, vi, mt, a1, b1, a);
-        j0 = n0;
-        j1 = n1;
+    for (uint32_t k = 0; k < n; k += 4) {
+        off += step;
+        const uint4 nxt = k + 4 < n ? __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lr, off, 0, 0))
+                                    : make_uint4(i, i, i, i);
+        {
+            const uint32_t j0 = grp.x, j1 = k + 1 < n ? grp.y : i;
+            const float4 a0 = rec[2 * j0], b0 = rec[2 * j0 + 1];
+            const float4 a1 = rec[2 * j1], b1 = rec[2 * j1 + 1];
+            sur_cand(s, true, j0 == i, pi, vi, mt, a0, b0, a);
+            sur_cand(s, k + 1 < n, j1 == i, pi, vi, mt, a1, b1, a);
+        }
+        if (k + 2 < n) {
+            const uint32_t j2 = grp.z, j3 = k + 3 < n ? grp.w : i;
+            const float4 a2 = rec[2 * j2], b2 = rec[2 * j2 + 1];
+            const float4 a3 = rec[2 * j3], b3 = rec[2 * j3 + 1];
+            sur_cand(s, true, j2 == i, pi, vi, mt, a2, b2, a);
+            sur_cand(s, k + 3 < n, j3 == i, pi, vi, mt, a3, b3, a);
+        }
+        grp = nxt;
     }
 }
 
