@@ -46,16 +46,20 @@ __device__ __forceinline__ float pack_pq(float q, uint32_t phase) {
 // finite and a zero weight zeroes a term.  Fast math throughout (hardware
 // rsqrt, fused multiply-adds): the surrogate is judged on its density
 // error, not on bit parity.
+template <bool UMASS = false>
 __device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, const float4 &pi, const float4 &vi,
                                          float4 mt, const float4 &ra, const float4 &rb, SurAcc &a) {
     const float dx = ra.x - pi.x, dy = ra.y - pi.y, dz = ra.z - pi.z;
     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const bool in = ok && r2 < s.h_grid2;
     const float d = s.h2 - r2;
-    const uint32_t ph = __float_as_uint(ra.w) & 3u;
-    const bool p1 = (ph & 1u) != 0u, p2 = (ph & 2u) != 0u;
-    const float m01 = p1 ? mt.y : mt.x, m23 = p1 ? mt.w : mt.z;
-    const float m = p2 ? m23 : m01;
+    float m = mt.x;  // one mass for every phase (UMASS), else the phase's
+    if (!UMASS) {
+        const uint32_t ph = __float_as_uint(ra.w) & 3u;
+        const bool p1 = (ph & 1u) != 0u, p2 = (ph & 2u) != 0u;
+        const float m01 = p1 ? mt.y : mt.x, m23 = p1 ? mt.w : mt.z;
+        m = p2 ? m23 : m01;
+    }
     const float mw = in ? m * (s.poly6_c * d * d * d) : 0.0f;
     a.cnt += in ? 1u : 0u;
     a.rho += mw;
@@ -206,6 +210,7 @@ struct BufF4 {
 // time with their record loads issued together (the next pair's slots are
 // fetched meanwhile); the per-cell path when the list could not hold them.
 // rec holds the 32-byte record of slot j at float4s 2j, 2j + 1.
+template <bool UMASS>
 __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t *__restrict__ start,
                                                const float4 *__restrict__ posm_p, const BufF4 &rec, float4 mt,
                                                uint32_t cw, const int4 &cell, const uint32_t *__restrict__ idx,
@@ -214,7 +219,7 @@ __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t
     if (cw & SLF_NONE) return;
     if (cw & SLF_SLOW) {
         for_each_candidate(s, start, posm_p, cell.x, cell.y, cell.z,
-                           [&](uint32_t j) { sur_cand(s, true, j == i, pi, vi, mt, rec[2 * j], rec[2 * j + 1], a); });
+                           [&](uint32_t j) { sur_cand<UMASS>(s, true, j == i, pi, vi, mt, rec[2 * j], rec[2 * j + 1], a); });
         return;
     }
     const uint32_t n = cw & SLF_COUNT;
@@ -236,15 +241,15 @@ __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t
             const uint32_t j0 = grp.x, j1 = k + 1 < n ? grp.y : i;
             const float4 a0 = rec[2 * j0], b0 = rec[2 * j0 + 1];
             const float4 a1 = rec[2 * j1], b1 = rec[2 * j1 + 1];
-            sur_cand(s, true, j0 == i, pi, vi, mt, a0, b0, a);
-            sur_cand(s, k + 1 < n, j1 == i, pi, vi, mt, a1, b1, a);
+            sur_cand<UMASS>(s, true, j0 == i, pi, vi, mt, a0, b0, a);
+            sur_cand<UMASS>(s, k + 1 < n, j1 == i, pi, vi, mt, a1, b1, a);
         }
         if (k + 2 < n) {
             const uint32_t j2 = grp.z, j3 = k + 3 < n ? grp.w : i;
             const float4 a2 = rec[2 * j2], b2 = rec[2 * j2 + 1];
             const float4 a3 = rec[2 * j3], b3 = rec[2 * j3 + 1];
-            sur_cand(s, true, j2 == i, pi, vi, mt, a2, b2, a);
-            sur_cand(s, k + 3 < n, j3 == i, pi, vi, mt, a3, b3, a);
+            sur_cand<UMASS>(s, true, j2 == i, pi, vi, mt, a2, b2, a);
+            sur_cand<UMASS>(s, k + 3 < n, j3 == i, pi, vi, mt, a3, b3, a);
         }
         grp = nxt;
     }
@@ -291,6 +296,7 @@ __global__ __launch_bounds__(PH_TILE) void k_sur_collect(SurCollectArgs a) {
 // Y^T = W2^T H^T, so the hidden tile that one MFMA leaves in a lane is the
 // operand the next one needs there), and the integrate; writes the advanced
 // state, the new history and the records of the next substep.
+template <bool UMASS>
 __global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
     __shared__ uint16_t xs[PH_TILE][MLP_IN];
     __shared__ float ys[PH_TILE][4];
@@ -330,7 +336,7 @@ __global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
         const BufF4 rec(a.rec, 2u * s.part_end);
         const float4 mt = make_float4(a.mass.x, a.mass.y, a.mass.z, a.mass.w);
         SurAcc acc;
-        sur_list_sweep(s, a.start, a.posm, rec, mt, cw, cl, a.idx, a.mstride, i, pi, vi, acc);
+        sur_list_sweep<UMASS>(s, a.start, a.posm, rec, mt, cw, cl, a.idx, a.mstride, i, pi, vi, acc);
         const float rho0 = a.scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
         sur_features(s, rho0, a.qs, acc, ai.x, ai.z, f);
         const float irl = __builtin_amdgcn_rcpf(rmaxf(ai.x, PH_EPS));
@@ -471,7 +477,13 @@ hipError_t launch_sur_collect(hipStream_t st, uint32_t ntiles, const SurCollectA
     return hipGetLastError();
 }
 hipError_t launch_sur_step(hipStream_t st, uint32_t ntiles, const SurStepArgs &a) {
-    hipLaunchKernelGGL(k_sur_step, dim3(ntiles), dim3(PH_TILE), 0, st, a);
+    // one mass for every phase (the common single-phase fluid): the sweep
+    // skips the per-candidate phase lookup
+    const bool umass = a.mass.x == a.mass.y && a.mass.x == a.mass.z && a.mass.x == a.mass.w;
+    if (umass)
+        hipLaunchKernelGGL(k_sur_step<true>, dim3(ntiles), dim3(PH_TILE), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_sur_step<false>, dim3(ntiles), dim3(PH_TILE), 0, st, a);
     return hipGetLastError();
 }
 hipError_t launch_gather_aux(hipStream_t st, uint32_t n, const uint32_t *perm, const float4 *in, float4 *out) {
