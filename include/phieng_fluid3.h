@@ -213,9 +213,11 @@ const char *phieng_surrogate3_last_error(void);
 
 /* Step `world` with the surrogate in the loop (null disables it): every
  * check_every-th substep is a plain substep that also judges the state —
- * the mean |rho/rho0 - 1| over particles with rho >= 0.9 rho0 — and if it
- * exceeds max_density_error the following check_every - 1 substeps stay
- * plain (a fallback).  use_network = 0 runs the same substep with the
+ * the mean |rho/rho0 - 1| over particles with rho >= 0.9 rho0.  The verdict
+ * is read back without stalling the stream and applied two substeps after
+ * the judging one: if it exceeds max_density_error, the check_every
+ * substeps from there on stay plain (a fallback), so while judges keep
+ * failing only the one substep after each judge can be a surrogate one.  use_network = 0 runs the same substep with the
  * network's coefficients fixed at zero (the method without the network).
  * The world must not be in a batch; 1 / 0. */
 int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check_every,
@@ -224,6 +226,9 @@ int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check
  * appends training samples to s from the particles whose original index
  * is congruent to a rotating phase modulo `stride`.  1 / 0. */
 int32_t phieng_world3_collect_samples(World3 *world, Surrogate3 *s, uint32_t every, uint32_t stride);
+/* Counts of surrogate, plain and fallback decisions and the newest judged
+ * density error (-1 before any), including a verdict not yet applied.
+ * 1 / 0. */
 int32_t phieng_world3_surrogate_stats(const World3 *world, uint64_t *sur_steps, uint64_t *plain_steps,
                                       uint64_t *fallbacks, float *last_density_error);
 /* Skin of the surrogate's neighbour list: surrogate substeps between two

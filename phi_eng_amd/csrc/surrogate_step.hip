@@ -206,14 +206,18 @@ struct BufF4 {
     }
 };
 
-// Sweep over the skin list of particle i: its recorded candidates, two at a
-// time with their record loads issued together (the next pair's slots are
-// fetched meanwhile); the per-cell path when the list could not hold them.
-// rec holds the 32-byte record of slot j at float4s 2j, 2j + 1.
+// Sweep over the skin list of particle i: its recorded candidates, a group
+// of four at a time with their eight record loads issued together while the
+// next group's slots are already in flight; the per-cell path when the list
+// could not hold them.  rec holds the 32-byte record of slot j at float4s
+// 2j, 2j + 1.  Group g (candidates 4g .. 4g + 3) of particle i is the
+// 16 bytes at grp[g * mstride + i]; the list holds one spare group row, so
+// the look-ahead load after the last group stays inside it, and every load
+// is unconditional (slots past the count are replaced by i and masked).
 template <bool UMASS>
 __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t *__restrict__ start,
                                                const float4 *__restrict__ posm_p, const BufF4 &rec, float4 mt,
-                                               uint32_t cw, const int4 &cell, const uint32_t *__restrict__ idx,
+                                               uint32_t cw, const int4 &cell, const uint4 *__restrict__ grp_p,
                                                uint32_t mstride, uint32_t i, const float4 &pi, const float4 &vi,
                                                SurAcc &a) {
     if (cw & SLF_NONE) return;
@@ -223,34 +227,33 @@ __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t
         return;
     }
     const uint32_t n = cw & SLF_COUNT;
-    // group g (candidates 4g .. 4g + 3) of this particle: 16 bytes at byte
-    // 16 (g mstride + i) of idx; the next group is fetched while this one
-    // is processed, two candidates at a time
-    const __amdgpu_buffer_rsrc_t lr =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(idx), (short)0, (int)(SL_CAP * mstride * 4u), 0x00020000);
-    const uint32_t step = 16u * mstride;
-    uint32_t off = 16u * i;
-    uint4 grp = n > 0 ? __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lr, off, 0, 0))
-                      : make_uint4(i, i, i, i);
+    // one group row (16 mstride bytes) per buffer resource, its base moved a
+    // row per group with scalar arithmetic: lane offsets stay 32-bit at any
+    // particle count
+    const uint32_t off = 16u * i;
+    const uint4 *row = grp_p;
+    auto load_row = [&](const uint4 *r) {
+        const __amdgpu_buffer_rsrc_t lr =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(r), (short)0, (int)(16u * mstride), 0x00020000);
+        return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lr, off, 0, 0));
+    };
+    uint4 grp = load_row(row);
 #pragma unroll 1
     for (uint32_t k = 0; k < n; k += 4) {
-        off += step;
-        const uint4 nxt = k + 4 < n ? __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lr, off, 0, 0))
-                                    : make_uint4(i, i, i, i);
-        {
-            const uint32_t j0 = grp.x, j1 = k + 1 < n ? grp.y : i;
-            const float4 a0 = rec[2 * j0], b0 = rec[2 * j0 + 1];
-            const float4 a1 = rec[2 * j1], b1 = rec[2 * j1 + 1];
-            sur_cand<UMASS>(s, true, j0 == i, pi, vi, mt, a0, b0, a);
-            sur_cand<UMASS>(s, k + 1 < n, j1 == i, pi, vi, mt, a1, b1, a);
-        }
-        if (k + 2 < n) {
-            const uint32_t j2 = grp.z, j3 = k + 3 < n ? grp.w : i;
-            const float4 a2 = rec[2 * j2], b2 = rec[2 * j2 + 1];
-            const float4 a3 = rec[2 * j3], b3 = rec[2 * j3 + 1];
-            sur_cand<UMASS>(s, true, j2 == i, pi, vi, mt, a2, b2, a);
-            sur_cand<UMASS>(s, k + 3 < n, j3 == i, pi, vi, mt, a3, b3, a);
-        }
+        const bool o1 = k + 1 < n, o2 = k + 2 < n, o3 = k + 3 < n;
+        const uint32_t j0 = grp.x, j1 = o1 ? grp.y : i, j2 = o2 ? grp.z : i, j3 = o3 ? grp.w : i;
+        const float4 a0 = rec[2 * j0], b0 = rec[2 * j0 + 1];
+        const float4 a1 = rec[2 * j1], b1 = rec[2 * j1 + 1];
+        const float4 a2 = rec[2 * j2], b2 = rec[2 * j2 + 1];
+        const float4 a3 = rec[2 * j3], b3 = rec[2 * j3 + 1];
+        // issued after the records: loads complete in order, so the records
+        // can be consumed while this one is still in flight
+        row += mstride;
+        const uint4 nxt = load_row(row);
+        sur_cand<UMASS>(s, true, j0 == i, pi, vi, mt, a0, b0, a);
+        sur_cand<UMASS>(s, o1, j1 == i, pi, vi, mt, a1, b1, a);
+        sur_cand<UMASS>(s, o2, j2 == i, pi, vi, mt, a2, b2, a);
+        sur_cand<UMASS>(s, o3, j3 == i, pi, vi, mt, a3, b3, a);
         grp = nxt;
     }
 }
@@ -336,7 +339,8 @@ __global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
         const BufF4 rec(a.rec, 2u * s.part_end);
         const float4 mt = make_float4(a.mass.x, a.mass.y, a.mass.z, a.mass.w);
         SurAcc acc;
-        sur_list_sweep<UMASS>(s, a.start, a.posm, rec, mt, cw, cl, a.idx, a.mstride, i, pi, vi, acc);
+        sur_list_sweep<UMASS>(s, a.start, a.posm, rec, mt, cw, cl, reinterpret_cast<const uint4 *>(a.idx), a.mstride, i,
+                              pi, vi, acc);
         const float rho0 = a.scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
         sur_features(s, rho0, a.qs, acc, ai.x, ai.z, f);
         const float irl = __builtin_amdgcn_rcpf(rmaxf(ai.x, PH_EPS));

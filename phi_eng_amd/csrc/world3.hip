@@ -118,7 +118,13 @@ struct World3 {
     // training-sample collection from plain substeps
     Surrogate3 *col = nullptr;
     uint32_t col_every = 10, col_stride = 64, col_phase = 0;
-    float err_host[2] = {0.0f, 0.0f};
+    // The judge's verdict is read back asynchronously (pinned memory +
+    // event) and applied exactly two substeps after its plain substep, so
+    // the host never drains the stream for it.
+    float *judge_host = nullptr;
+    hipEvent_t judge_ev = nullptr;
+    bool judge_pending = false;
+    uint64_t judge_at = 0;
     // skin list of the last sort (surrogate substeps read it)
     float skin = 0.05f;  // list radius (1 + skin) h
     ph::DBuf<int4> sl_cell;
@@ -220,6 +226,7 @@ struct World3 {
                     continue;
                 }
                 ensure_aux();
+                if (judge_pending && step_index >= judge_at + 2) apply_judge();
                 const bool collect = col && (step_index % col_every) == 0;
                 const bool judge = sur && (step_index % check_every) == 0;
                 if (sur && !judge && !collect && fallback_left == 0 && list_usable()) {
@@ -230,15 +237,33 @@ struct World3 {
                     substep_plain_aux(judge, collect, sur && fallback_left <= 1);
                     ++n_plain;
                     if (fallback_left) --fallback_left;
-                    if (judge && last_err > max_err) {
-                        fallback_left = check_every - 1;  // plain until the next judge
-                        ++n_fallback;
-                    }
                 }
                 ++step_index;
             }
         }
         time += dt;
+    }
+
+    ~World3() {
+        if (judge_ev) (void)hipEventDestroy(judge_ev);
+        if (judge_host) (void)hipHostFree(judge_host);
+    }
+
+    // Read the pending verdict (waiting for its plain substep if needed) and
+    // start the fallback if it failed: the next check_every substeps are
+    // plain, which covers the next judge and the substeps until its own
+    // verdict arrives, so a run of failing judges stays plain throughout.
+    // Called at fixed substep indices only, so the decisions never depend
+    // on timing.
+    void apply_judge() {
+        if (!judge_pending) return;
+        PH_CK(hipEventSynchronize(judge_ev));
+        judge_pending = false;
+        last_err = judge_host[0];
+        if (last_err > max_err) {
+            fallback_left = check_every;
+            ++n_fallback;
+        }
     }
 
     float4 mass4() const {
@@ -293,7 +318,7 @@ struct World3 {
         if (list) {
             sl_cell.ensure(e.alloc_particles);
             sl_cnt.ensure(e.alloc_particles);
-            sl_idx.ensure((size_t)SL_CAP * e.alloc_particles);
+            sl_idx.ensure((size_t)(SL_CAP + 4) * e.alloc_particles);  // + a spare group row for the sweep's look-ahead
             lo.cell = sl_cell.p;
             lo.cnt = sl_cnt.p;
             lo.idx = sl_idx.p;
@@ -340,10 +365,16 @@ struct World3 {
                                          sur ? sl_rec[e.cs].p : nullptr);
         });
         if (judge) {
+            apply_judge();  // an older verdict still pending (check_every < 2)
+            if (!judge_ev) {
+                PH_CK(hipEventCreateWithFlags(&judge_ev, hipEventDisableTiming));
+                PH_CK(hipHostMalloc((void **)&judge_host, 2 * sizeof(float), hipHostMallocDefault));
+            }
             e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_err_reduce(e.stream, e.tile_err.p, nt, e.err_out.p); });
-            PH_CK(hipMemcpyAsync(err_host, e.err_out.p, sizeof(err_host), hipMemcpyDeviceToHost, e.stream));
-            PH_CK(hipStreamSynchronize(e.stream));
-            last_err = err_host[0];
+            PH_CK(hipMemcpyAsync(judge_host, e.err_out.p, 2 * sizeof(float), hipMemcpyDeviceToHost, e.stream));
+            PH_CK(hipEventRecord(judge_ev, e.stream));
+            judge_pending = true;
+            judge_at = step_index;
         }
         e.finish_substep();
     }
@@ -709,6 +740,7 @@ int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check
     world->use_net = use_network ? 1 : 0;
     world->fallback_left = 0;
     world->step_index = 0;
+    world->judge_pending = false;
     return 1;
 }
 
@@ -732,10 +764,22 @@ int32_t phieng_world3_set_surrogate_skin(World3 *world, float skin) {
 int32_t phieng_world3_surrogate_stats(const World3 *world, uint64_t *sur_steps, uint64_t *plain_steps,
                                       uint64_t *fallbacks, float *last_density_error) {
     if (!world) return 0;
+    // Report a pending verdict without applying it: applying moves the
+    // fallback window, which only the step loop may do.
+    float err = world->last_err;
+    uint64_t fb = world->n_fallback;
+    if (world->judge_pending) {
+        if (hipEventSynchronize(world->judge_ev) != hipSuccess) {
+            set_error("phieng_world3_surrogate_stats", "waiting for the judge's readback failed");
+            return 0;
+        }
+        err = world->judge_host[0];
+        if (err > world->max_err) ++fb;
+    }
     if (sur_steps) *sur_steps = world->n_sur;
     if (plain_steps) *plain_steps = world->n_plain;
-    if (fallbacks) *fallbacks = world->n_fallback;
-    if (last_density_error) *last_density_error = world->last_err;
+    if (fallbacks) *fallbacks = fb;
+    if (last_density_error) *last_density_error = err;
     return 1;
 }
 
