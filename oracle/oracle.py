@@ -213,7 +213,7 @@ def lib2():
         l.oracle2_substep_f32.restype = ctypes.c_int
         l.oracle2_substep_f32.argtypes = [ctypes.c_uint32, fp, fp, fp, u8p, ctypes.c_void_p, ctypes.c_float,
                                           ctypes.c_float, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
-                                          ctypes.c_int, fp, fp, up, fp]
+                                          ctypes.c_int, fp, fp, up, fp, ctypes.POINTER(ctypes.c_double)]
         l.oracle2_run_f32.restype = ctypes.c_int
         l.oracle2_run_f32.argtypes = [ctypes.c_uint32, fp, fp, fp, u8p, ctypes.c_void_p, ctypes.c_float,
                                       ctypes.c_float, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
@@ -247,12 +247,13 @@ def substep2_f32(pos, vel, mass, cfg, gravity, bodies, dt, jacobi=False, phases=
     p = np.zeros(n, np.float32)
     cnt = np.zeros(n, np.uint32)
     acc = np.zeros((n, 2), np.float32)
+    scale = np.zeros(n, np.float64)
     arr, nb = bodies
     ok = lib2().oracle2_substep_f32(n, _p(pos), _p(vel), _p(mass), None if ph is None else _u8(ph),
                                     ctypes.byref(cfg), gravity[0], gravity[1], arr, nb, dt, 1 if jacobi else 0,
-                                    _p(rho), _p(p), _p(cnt, ctypes.c_uint32), _p(acc))
+                                    _p(rho), _p(p), _p(cnt, ctypes.c_uint32), _p(acc), _p(scale, ctypes.c_double))
     assert ok
-    return dict(pos=pos, vel=vel, density=rho, pressure=p, count=cnt, accel=acc)
+    return dict(pos=pos, vel=vel, density=rho, pressure=p, count=cnt, accel=acc, scale=scale)
 
 
 def run2_f32(pos, vel, mass, cfg, gravity, bodies, dt, steps, jacobi=False, phases=None):

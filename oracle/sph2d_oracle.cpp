@@ -245,8 +245,10 @@ struct Fluid2 {
 
     // fluid.rs:898-1136 (no adaptive resolution, boundary particles,
     // turbulence or dynamic bodies).
+    // scale_out: the cancellation-aware magnitude sum_j |m_j term grad W| / rho_i
+    // (plus the cohesion terms) that bounds the acceleration's rounding.
     void substep(float gx, float gy, const Oracle2Body *bodies, uint32_t nbodies, float dt, bool jacobi,
-                 uint32_t *count, float *acc_out) {
+                 uint32_t *count, float *acc_out, double *scale_out = nullptr) {
         const float h = rmax(cfg.smoothing_length, kEps);
         rebuild_grid();
         std::vector<uint32_t> nb;
@@ -261,6 +263,7 @@ struct Fluid2 {
             const float rho_i = rmax(pi.rho, kEps);
             const float p_i = pi.p;
             float f_x = 0.0f, f_y = 0.0f;
+            double mag = 0.0;
             for (uint32_t j : nb) {
                 if (j == i) continue;
                 const P2 &pj = ps[j];
@@ -278,17 +281,20 @@ struct Fluid2 {
                 const float s = pj.m * term;
                 f_x = f_x - gxx * s;
                 f_y = f_y - gyy * s;
+                mag += std::sqrt((double)gxx * gxx + (double)gyy * gyy) * std::fabs((double)s);
                 if (cfg.surface_tension > 0.0f && dist > kEps) {
                     const float coh = cohesion(dist, h);
                     if (coh > 0.0f) {
                         const float c = cfg.surface_tension * pi.m * pj.m * coh;
                         f_x = f_x - dx / dist * c;
                         f_y = f_y - dy / dist * c;
+                        mag += std::fabs((double)c);
                     }
                 }
             }
             fx[i] = f_x;
             fy[i] = f_y;
+            if (scale_out) scale_out[i] = mag / (double)rho_i;
         }
         // Integrate: in place (reference) or from the pre-step state (Jacobi).
         const std::vector<P2> pre = jacobi ? ps : std::vector<P2>();
@@ -399,13 +405,15 @@ void store(const Fluid2 &f, float *pos, float *vel, float *density, float *press
 extern "C" {
 
 // One substep; pos/vel (2n, updated in place), density/pressure/count/acc
-// (2n) of that substep out (each may be null).
+// (2n) and the acceleration's cancellation-aware scale of that substep out
+// (each may be null).
 int oracle2_substep_f32(uint32_t n, float *pos, float *vel, const float *mass, const uint8_t *phase,
                         const Oracle2Cfg *cfg, float gx, float gy, const Oracle2Body *bodies, uint32_t nbodies,
-                        float dt, int jacobi, float *density, float *pressure, uint32_t *count, float *acc) {
+                        float dt, int jacobi, float *density, float *pressure, uint32_t *count, float *acc,
+                        double *scale) {
     if (!n || !pos || !mass || !cfg) return 0;
     Fluid2 f = make(n, pos, vel, mass, phase, cfg);
-    f.substep(gx, gy, bodies, nbodies, dt, jacobi != 0, count, acc);
+    f.substep(gx, gy, bodies, nbodies, dt, jacobi != 0, count, acc, scale);
     store(f, pos, vel, density, pressure);
     return 1;
 }

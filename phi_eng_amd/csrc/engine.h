@@ -77,10 +77,12 @@ struct SceneHost {
     float g[3];
     float dt;
     std::vector<BodyDev> bodies;
+    int dims = 3;  // 2: a World2 fluid, particles and gravity in the z = 0 plane
 };
 
 // f32 constants of one scene, computed with the reference's operation
-// order (fluid3.rs:220-251, 649, 1028) so device results match it.
+// order (fluid3.rs:220-251, 649, 1028; the 2D kernels fluid.rs:265-285) so
+// device results match it.
 void fill_scene_physics(SceneDev &s, const SceneHost &h);
 // Pick a grid domain (cell box) for a scene from the particle cell box and
 // its bodies.  The domain only affects speed: particles outside it land in

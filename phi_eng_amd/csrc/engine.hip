@@ -47,8 +47,13 @@ void fill_scene_physics(SceneDev &s, const SceneHost &h) {
     const float hk = s.h;
     s.h2 = hk * hk;
     const float h2 = s.h2;
-    s.poly6_c = 315.0f / (64.0f * kPiF * h2 * h2 * h2 * h2 * hk);
-    s.spiky_c = -45.0f / (kPiF * hk * hk * hk * hk * hk * hk);
+    if (h.dims == 2) {
+        s.poly6_c = 4.0f / (kPiF * h2 * h2 * h2 * h2);              // fluid.rs:272
+        s.spiky_c = -30.0f / (kPiF * hk * hk * hk * hk * hk);       // fluid.rs:283
+    } else {
+        s.poly6_c = 315.0f / (64.0f * kPiF * h2 * h2 * h2 * h2 * hk);  // fluid3.rs:227
+        s.spiky_c = -45.0f / (kPiF * hk * hk * hk * hk * hk * hk);     // fluid3.rs:237
+    }
     s.rho_max = 3.0f * rmax_h(c.rest_density, kEps);
     s.stiffness = c.stiffness;
     s.gamma = c.gamma;
@@ -97,6 +102,7 @@ void choose_domain(SceneDev &s, const int32_t *pb, uint32_t np, const std::vecto
     for (const BodyDev &b : bodies) {
         float ex = b.a, ey = b.b, ez = b.c;
         if (b.shape == 0u) ey = ez = b.a;
+        if (b.shape == 3u) ez = 0.0f;
         if (b.shape == 2u) {
             ey = b.b + b.a;
             ez = b.a;

@@ -17,7 +17,7 @@
 
 // Static collision body (identity rotation; the C ABI cannot rotate one).
 struct BodyDev {
-    uint32_t shape;  // 0 sphere, 1 box, 2 capsule (ffi.rs:3193-3195)
+    uint32_t shape;  // 0 sphere, 1 box, 2 capsule (ffi.rs:3193-3195); 3 box of a 2D world
     float px, py, pz;
     float a, b, c;
 };
@@ -194,6 +194,32 @@ __device__ __forceinline__ bool collide(float px, float py, float pz, float pr, 
             nz = 0.0f;
         }
         depth = b.a + pr - dist;
+        return true;
+    }
+    if (b.shape == 3u) {  // 2D box in the z = 0 plane (fluid.rs:481-517), identity rotation
+        const float lx = px - b.px, ly = py - b.py;
+        const float cxl = rclampf(lx, -b.a, b.a), cyl = rclampf(ly, -b.b, b.b);
+        const float dx = lx - cxl, dy = ly - cyl;
+        nz = 0.0f;
+        if (dx * dx + dy * dy > PH_EPS_SQ) {
+            const float dist = sqrtf(dx * dx + dy * dy);
+            if (dist >= pr) return false;
+            nx = dx / dist;
+            ny = dy / dist;
+            depth = pr - dist;
+            return true;
+        }
+        // Centre inside: least-penetration axis, ties to y, always +axis (fluid.rs:505-512).
+        const float ex = b.a - fabsf(lx), ey = b.b - fabsf(ly);
+        if (ex < ey) {
+            nx = 1.0f;
+            ny = 0.0f;
+            depth = pr + ex;
+        } else {
+            nx = 0.0f;
+            ny = 1.0f;
+            depth = pr + ey;
+        }
         return true;
     }
     if (b.shape == 1u) {  // box

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/phieng_fluid2.h"
 #include "../../include/phieng_fluid3.h"
 #include "engine.h"
 #include "sph_kernels.h"
@@ -97,6 +98,7 @@ bool config_valid(const FluidConfig3Ffi &c) {
 }  // namespace
 
 struct World3 {
+    int dims = 3;  // 2 for a World2: particles, gravity and bodies in the z = 0 plane
     FluidConfig3Ffi cfg = default_config();
     float g[3] = {0.0f, -9.81f, 0.0f};  // World3::new (world.rs)
     std::vector<BodyDev> bodies;
@@ -144,6 +146,7 @@ struct World3 {
         h.g[2] = g[2];
         h.dt = dt;
         h.bodies = bodies;
+        h.dims = dims;
         return h;
     }
 
@@ -801,5 +804,224 @@ int32_t phieng_world3_sync(World3 *world) {
         return 0;
     }
 }
+
+// ---------------------------------------------------------------------
+// World2: the reference's 2D fluid exports (ffi.rs:51-94, 999-1026,
+// 1159-1203, 2486-2594) over the same engine, with the particles in the
+// z = 0 plane.  With z, v_z and g_z exactly zero every z term of the 3D
+// kernels adds an exact +0, the z row offsets hold no particles, and the
+// remaining arithmetic is the 2D step's (fluid.rs:541-1136) operation for
+// operation once the scene carries the 2D kernel constants and the 2D box.
+// ---------------------------------------------------------------------
+
+}  // extern "C"
+
+struct World2 : World3 {
+    World2() { dims = 2; }
+};
+
+namespace {
+World3 *as3(World2 *w) { return static_cast<World3 *>(w); }
+const World3 *as3(const World2 *w) { return static_cast<const World3 *>(w); }
+}  // namespace
+
+extern "C" {
+
+World2 *phieng_world_new(void) {
+    try {
+        return new World2();
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void phieng_world_free(World2 *world) {
+    if (!world) return;
+    try {
+        delete world;
+    } catch (...) {
+    }
+}
+
+void phieng_world_step(World2 *world, float dt) {
+    if (!world || !std::isfinite(dt) || dt <= 0.0f) return;
+    try {
+        world->step(std::min(dt, 1.0f / 15.0f));
+    } catch (const std::exception &e) {
+        world->eng.recover();
+        if (world->step_error.empty()) world->step_error = e.what();
+        set_error("phieng_world_step", e.what());
+    }
+}
+
+void phieng_world_set_gravity(World2 *world, float gx, float gy) {
+    if (!world || !std::isfinite(gx) || !std::isfinite(gy)) return;
+    world->g[0] = gx;
+    world->g[1] = gy;
+    world->g[2] = 0.0f;
+}
+
+int32_t phieng_world_add_body(World2 *world, float px, float py, uint32_t shape_tag, float a, float b, float density,
+                              uint8_t is_static, float friction, float restitution) {
+    // friction / restitution are sanitised, never rejected, by set_material
+    // (body.rs:627-637), and static walls take wall_friction from the config
+    (void)friction;
+    (void)restitution;
+    if (!world || !std::isfinite(px) || !std::isfinite(py)) return -1;
+    BodyDev bd{0u, px, py, 0.0f, a, 0.0f, 0.0f};
+    switch (shape_tag) {
+        case PHIENG_SHAPE2_CIRCLE:  // a disc in the plane: the sphere test restricted to z = 0
+            if (!(a > 0.0f && std::isfinite(a))) return -1;
+            bd.shape = 0u;
+            break;
+        case PHIENG_SHAPE2_BOX:
+            if (!(a > 0.0f && b > 0.0f && std::isfinite(a) && std::isfinite(b))) return -1;
+            bd.shape = 3u;
+            bd.b = b;
+            break;
+        default:
+            return -1;
+    }
+    if (!std::isfinite(density) || density <= 0.0f) return -1;
+    if (!is_static) {
+        set_error("phieng_world_add_body", "dynamic bodies are not supported by the GPU fluid path");
+        return -1;
+    }
+    if (world->bodies.size() >= PH_MAX_BODIES) {
+        set_error("phieng_world_add_body", "too many bodies");
+        return -1;
+    }
+    world->bodies.push_back(bd);
+    return world->next_body_id++;
+}
+
+int32_t phieng_world_set_fluid_config(World2 *world, const FluidConfig2Ffi *cfg) {
+    // fluid2_from_ffi (ffi.rs:773-840) applies the same rules as the 3D one
+    return world ? phieng_world3_set_fluid_config(as3(world), cfg) : 0;
+}
+
+int32_t phieng_world_get_fluid_config(const World2 *world, FluidConfig2Ffi *out) {
+    return world ? phieng_world3_get_fluid_config(as3(world), out) : 0;
+}
+
+namespace {
+// xy pairs -> xyz triples in the z = 0 plane (after the finite check of the
+// caller's values, so a rejected call stays atomic)
+std::vector<float> lift_xy(const float *xy, uint32_t m) {
+    std::vector<float> xyz(3 * (size_t)m);
+    for (uint32_t i = 0; i < m; ++i) {
+        xyz[3 * i] = xy[2 * i];
+        xyz[3 * i + 1] = xy[2 * i + 1];
+        xyz[3 * i + 2] = 0.0f;
+    }
+    return xyz;
+}
+bool all_finite(const float *v, size_t k) {
+    for (size_t i = 0; i < k; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+}  // namespace
+
+uint32_t phieng_world_fluid_add(World2 *world, const float *positions, uint32_t count) {
+    if (!world || !positions) return 0;
+    try {
+        const uint32_t m = std::min(count, kMaxPerCall);
+        if (!all_finite(positions, 2 * (size_t)m)) return 0;
+        const std::vector<float> xyz = lift_xy(positions, m);
+        return world->add(xyz.data(), nullptr, m);
+    } catch (const std::exception &e) {
+        set_error("phieng_world_fluid_add", e.what());
+        return 0;
+    }
+}
+
+uint32_t phieng_world_fluid_add_phase(World2 *world, const float *positions, const uint8_t *phases,
+                                      uint32_t count) {
+    if (!world || !positions || !phases) return 0;
+    try {
+        const uint32_t m = std::min(count, kMaxPerCall);
+        if (!all_finite(positions, 2 * (size_t)m)) return 0;
+        const std::vector<float> xyz = lift_xy(positions, m);
+        return world->add(xyz.data(), phases, m);
+    } catch (const std::exception &e) {
+        set_error("phieng_world_fluid_add_phase", e.what());
+        return 0;
+    }
+}
+
+uint32_t phieng_world_fluid_count(const World2 *world) { return world ? world->n : 0; }
+
+int32_t phieng_world_fluid_clear(World2 *world) { return world ? phieng_world3_fluid_clear(as3(world)) : 0; }
+
+uint32_t phieng_world_fluid_get(const World2 *world, float *out, uint32_t max) {
+    if (!world || !out) return 0;
+    try {
+        const uint32_t m = std::min(max, world->n);
+        if (m == 0) return 0;
+        std::vector<float> six(6 * (size_t)m);
+        const uint32_t got = const_cast<World2 *>(world)->get(six.data(), m);
+        for (uint32_t i = 0; i < got; ++i) {
+            out[4 * i] = six[6 * i];
+            out[4 * i + 1] = six[6 * i + 1];
+            out[4 * i + 2] = six[6 * i + 3];
+            out[4 * i + 3] = six[6 * i + 4];
+        }
+        return got;
+    } catch (const std::exception &e) {
+        set_error("phieng_world_fluid_get", e.what());
+        return 0;
+    }
+}
+
+uint32_t phieng_world_fluid_reserve(World2 *world, uint32_t capacity) {
+    return world ? phieng_world3_fluid_reserve(as3(world), capacity) : 0;
+}
+
+uint32_t phieng_world_fluid_set_velocity(World2 *world, const float *vel, uint32_t count) {
+    if (!world || !vel) return 0;
+    const uint32_t m = std::min(count, world->n);
+    if (!all_finite(vel, 2 * (size_t)m)) return 0;
+    std::vector<float> v3(3 * (size_t)m);
+    for (uint32_t i = 0; i < m; ++i) {
+        v3[3 * i] = vel[2 * i];
+        v3[3 * i + 1] = vel[2 * i + 1];
+        v3[3 * i + 2] = 0.0f;
+    }
+    return phieng_world3_fluid_set_velocity(as3(world), v3.data(), m);
+}
+
+int32_t phieng_world_fluid_set_record(World2 *world, uint32_t flags) {
+    return world ? phieng_world3_fluid_set_record(as3(world), flags) : 0;
+}
+
+uint32_t phieng_world_fluid_get_record(const World2 *world, int32_t *cell2, uint32_t *sorted_ids, float *density,
+                                       float *pressure, float *accel2, uint32_t *nbr_count, uint32_t max) {
+    if (!world) return 0;
+    try {
+        const uint32_t m = std::min(max, world->n);
+        std::vector<int32_t> c3(cell2 ? 3 * (size_t)m : 0);
+        std::vector<float> a3(accel2 ? 3 * (size_t)m : 0);
+        const uint32_t got = phieng_world3_fluid_get_record(as3(world), cell2 ? c3.data() : nullptr, sorted_ids,
+                                                            density, pressure, accel2 ? a3.data() : nullptr,
+                                                            nbr_count, m);
+        for (uint32_t i = 0; i < got; ++i) {
+            if (cell2) {
+                cell2[2 * i] = c3[3 * i];
+                cell2[2 * i + 1] = c3[3 * i + 1];
+            }
+            if (accel2) {
+                accel2[2 * i] = a3[3 * i];
+                accel2[2 * i + 1] = a3[3 * i + 1];
+            }
+        }
+        return got;
+    } catch (const std::exception &e) {
+        set_error("phieng_world_fluid_get_record", e.what());
+        return 0;
+    }
+}
+
+int32_t phieng_world_sync(World2 *world) { return world ? phieng_world3_sync(as3(world)) : 0; }
 
 }  // extern "C"

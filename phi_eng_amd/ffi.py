@@ -1,4 +1,4 @@
-"""ctypes binding of ``libphieng_hip.so`` (include/phieng_fluid3.h).
+"""ctypes binding of ``libphieng_hip.so`` (include/phieng_fluid3.h, phieng_fluid2.h).
 
 Loading fails loudly when the library has not been built: there is no
 fallback implementation of the fluid step in Python.
@@ -87,7 +87,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _cfgp = ctypes.POINTER(FluidConfig3Ffi)
 
-# name -> (restype, argtypes); every symbol include/phieng_fluid3.h declares.
+# name -> (restype, argtypes); every symbol include/phieng_fluid3.h and
+# include/phieng_fluid2.h declare.
 SIGNATURES = {
     "phieng_world3_new": (_P, []),
     "phieng_world3_free": (None, [_P]),
@@ -139,6 +140,24 @@ SIGNATURES = {
     "phieng_world3_collect_samples": (_I32, [_P, _P, _U32, _U32]),
     "phieng_world3_surrogate_stats": (_I32, [_P, _u64p, _u64p, _u64p, _fp]),
     "phieng_world3_set_surrogate_skin": (_I32, [_P, _F]),
+    # include/phieng_fluid2.h (World2; FluidConfig2Ffi has FluidConfig3Ffi's layout)
+    "phieng_world_new": (_P, []),
+    "phieng_world_free": (None, [_P]),
+    "phieng_world_step": (None, [_P, _F]),
+    "phieng_world_set_gravity": (None, [_P, _F, _F]),
+    "phieng_world_add_body": (_I32, [_P, _F, _F, _U32, _F, _F, _F, ctypes.c_uint8, _F, _F]),
+    "phieng_world_set_fluid_config": (_I32, [_P, _cfgp]),
+    "phieng_world_get_fluid_config": (_I32, [_P, _cfgp]),
+    "phieng_world_fluid_add": (_U32, [_P, _fp, _U32]),
+    "phieng_world_fluid_add_phase": (_U32, [_P, _fp, _u8p, _U32]),
+    "phieng_world_fluid_count": (_U32, [_P]),
+    "phieng_world_fluid_clear": (_I32, [_P]),
+    "phieng_world_fluid_get": (_U32, [_P, _fp, _U32]),
+    "phieng_world_fluid_reserve": (_U32, [_P, _U32]),
+    "phieng_world_fluid_set_velocity": (_U32, [_P, _fp, _U32]),
+    "phieng_world_fluid_set_record": (_I32, [_P, _U32]),
+    "phieng_world_fluid_get_record": (_U32, [_P, _i32p, _u32p, _fp, _fp, _fp, _u32p, _U32]),
+    "phieng_world_sync": (_I32, [_P]),
 }
 
 _lib = None

@@ -150,8 +150,8 @@ def make_world(scene, world_cls=None):
 
 @dataclass
 class Scene2:
-    """A 2D scene for the CPU 2D restatement (oracle) only: the product has
-    no 2D GPU path."""
+    """A 2D scene: stepped on the GPU through the World2 exports
+    (make_world2) and on the CPU by the 2D restatement."""
     name: str
     positions: np.ndarray  # (N, 2) float32
     config: dict           # FluidConfig2 fields
@@ -191,3 +191,40 @@ def dam2d_config1(seed=1):
     t = WALL / 2
     boxes = [((6.4, -t), (6.4 + WALL, t)), ((-t, 2.4), (t, 2.4)), ((12.8 + t, 2.4), (t, 2.4))]
     return Scene2("dam_break_2d_4096", pos.astype(np.float32), cfg, boxes)
+
+
+def scene2_config(sc):
+    """FluidConfig2Ffi (FluidConfig3Ffi's layout) of a Scene2: the defaults
+    with the scene's fields, one phase of mass c["mass"]."""
+    from .ffi import FluidConfig3Ffi
+    c = sc.config
+    cfg = FluidConfig3Ffi.default()
+    cfg.particle_mass = c["mass"]
+    cfg.rest_density = c["rest_density"]
+    cfg.smoothing_length = c["smoothing_length"]
+    cfg.stiffness = c["stiffness"]
+    cfg.gamma = c["gamma"]
+    cfg.viscosity = c["viscosity"]
+    cfg.max_velocity = c["max_velocity"]
+    cfg.particle_radius = c["particle_radius"]
+    for k in range(4):
+        cfg.phase_rest_density[k] = c["rest_density"]
+        cfg.phase_mass[k] = c["mass"]
+    return cfg
+
+
+def make_world2(sc, capacity=None):
+    """A World2 holding the Scene2 (static boxes in the scene's order)."""
+    from .world import World2
+    w = World2(capacity=capacity or max(sc.n, 4096))
+    if not w.set_fluid_config(scene2_config(sc)):
+        raise ValueError("scene config rejected")
+    w.set_gravity(sc.gravity)
+    for c, he in sc.boxes:
+        if w.add_static_box(c, he) < 0:
+            raise ValueError("body rejected")
+    got = w.add_particles(sc.positions)
+    if got != sc.n:
+        raise RuntimeError(f"added {got} of {sc.n} particles")
+    return w
+

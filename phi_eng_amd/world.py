@@ -1,4 +1,4 @@
-"""Python mirror of the reference World3 fluid surface over the C ABI.
+"""Python mirror of the reference World3 / World2 fluid surfaces over the C ABI.
 
 Method names follow the reference (``World3::step``, ``fluid.add_particles``,
 ``phieng_world3_*`` in crates/engine/src/ffi.rs); arrays are numpy.
@@ -174,6 +174,115 @@ class World3:
             if cnt[i]:
                 out[self._l.phieng_kernel_name(i).decode()] = (ms[i], cnt[i])
         return out
+
+
+SHAPE2_CIRCLE, SHAPE2_BOX = 0, 1
+
+
+class World2:
+    """Owns one ``World2*`` from ``phieng_world_new`` (ffi.rs:51-53): the 2D
+    fluid surface (``phieng_world_*``), arrays of xy pairs."""
+
+    def __init__(self, capacity=None):
+        self._l = lib()
+        self.ptr = self._l.phieng_world_new()
+        if not self.ptr:
+            raise MemoryError("phieng_world_new returned null")
+        if capacity is not None:
+            self.reserve(capacity)
+
+    def close(self):
+        if self.ptr:
+            self._l.phieng_world_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_gravity(self, g):
+        self._l.phieng_world_set_gravity(self.ptr, float(g[0]), float(g[1]))
+
+    def set_fluid_config(self, cfg: FluidConfig3Ffi) -> bool:
+        return bool(self._l.phieng_world_set_fluid_config(self.ptr, ctypes.byref(cfg)))
+
+    def get_fluid_config(self) -> FluidConfig3Ffi:
+        c = FluidConfig3Ffi()
+        if not self._l.phieng_world_get_fluid_config(self.ptr, ctypes.byref(c)):
+            raise RuntimeError("get_fluid_config failed")
+        return c
+
+    def add_body(self, center, shape, a, b=0.0, density=1.0, is_static=1):
+        return self._l.phieng_world_add_body(self.ptr, float(center[0]), float(center[1]), int(shape), float(a),
+                                             float(b), float(density), int(is_static), 0.5, 0.0)
+
+    def add_static_box(self, center, half_extents):
+        return self.add_body(center, SHAPE2_BOX, half_extents[0], half_extents[1])
+
+    def reserve(self, capacity):
+        return self._l.phieng_world_fluid_reserve(self.ptr, int(capacity))
+
+    def add_particles(self, positions, phases=None):
+        p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 2)
+        if phases is None:
+            return self._l.phieng_world_fluid_add(self.ptr, _fptr(p), len(p))
+        ph = np.ascontiguousarray(phases, dtype=np.uint8)
+        return self._l.phieng_world_fluid_add_phase(self.ptr, _fptr(p),
+                                                    ph.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), len(p))
+
+    def count(self):
+        return self._l.phieng_world_fluid_count(self.ptr)
+
+    def clear(self):
+        return self._l.phieng_world_fluid_clear(self.ptr)
+
+    def get_state(self):
+        """(positions, velocities) in insertion order, each (N, 2) float32."""
+        n = self.count()
+        out = np.empty((n, 4), dtype=np.float32)
+        got = self._l.phieng_world_fluid_get(self.ptr, _fptr(out), n)
+        if got != n:
+            raise RuntimeError(f"fluid_get returned {got} of {n}: {last_error()}")
+        return out[:, :2].copy(), out[:, 2:].copy()
+
+    def set_velocity(self, vel):
+        v = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 2)
+        return self._l.phieng_world_fluid_set_velocity(self.ptr, _fptr(v), len(v))
+
+    def step(self, dt):
+        self._l.phieng_world_step(self.ptr, float(dt))
+
+    def sync(self):
+        if not self._l.phieng_world_sync(self.ptr):
+            raise RuntimeError(last_error())
+
+    def set_record(self, flags):
+        if not self._l.phieng_world_fluid_set_record(self.ptr, int(flags)):
+            raise RuntimeError(last_error())
+
+    def get_record(self):
+        n = self.count()
+        cells = np.zeros((n, 2), dtype=np.int32)
+        order = np.zeros(n, dtype=np.uint32)
+        rho = np.zeros(n, dtype=np.float32)
+        p = np.zeros(n, dtype=np.float32)
+        acc = np.zeros((n, 2), dtype=np.float32)
+        cnt = np.zeros(n, dtype=np.uint32)
+        got = self._l.phieng_world_fluid_get_record(
+            self.ptr, cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            order.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _fptr(rho), _fptr(p), _fptr(acc),
+            cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n)
+        if got != n:
+            raise RuntimeError(f"get_record returned {got}: {last_error()}")
+        return dict(cells=cells, order=order, density=rho, pressure=p, accel=acc, count=cnt)
 
 
 BATCH_KERNELS, BATCH_PERSISTENT, BATCH_PERSISTENT_GRAPH = 0, 1, 2
