@@ -57,7 +57,7 @@ int32_t phieng_world_add_body(World2 *world, float pos_x, float pos_y, uint32_t 
                               float density, uint8_t is_static, float friction, float restitution);
 /* ffi.rs:999-1011 — 1 on success, 0 on null or invalid input (world
  * unchanged); validation ffi.rs:773-840.  Also 0 for adaptive_resolution
- * or boundary_particles set, which the GPU path does not implement. */
+ * set, which the GPU path does not implement. */
 int32_t phieng_world_set_fluid_config(World2 *world, const FluidConfig2Ffi *cfg);
 /* ffi.rs:1017-1026 — 1 on success, 0 on null. */
 int32_t phieng_world_get_fluid_config(const World2 *world, FluidConfig2Ffi *out);
@@ -95,6 +95,10 @@ uint32_t phieng_world_fluid_get_record(const World2 *world, int32_t *cell2, uint
                                        float *pressure, float *accel2, uint32_t *nbr_count, uint32_t max);
 /* Block until the world's GPU work is done; 1, or 0 with phieng_last_error set. */
 int32_t phieng_world_sync(World2 *world);
+/* Boundary ghosts (fluid.rs:661-819), as phieng_world3_boundary_info / _get
+ * with xy pairs. */
+int32_t phieng_world_boundary_info(const World2 *world, uint32_t *count, uint64_t *regens);
+uint32_t phieng_world_boundary_get(const World2 *world, float *xy, float *density, float *pressure, uint32_t max);
 
 #ifdef __cplusplus
 }

@@ -78,8 +78,8 @@ int32_t phieng_world3_add_body(World3 *world, float pos_x, float pos_y, float po
                                uint8_t is_static, float friction, float restitution);
 /* ffi.rs:1032-1044 — 1 on success, 0 on null or invalid input (world
  * unchanged); validation rules ffi.rs:857-914.  Also returns 0 for
- * adaptive_resolution or boundary_particles set, which the GPU path
- * does not implement. */
+ * adaptive_resolution set, which the GPU path does not implement, and for
+ * boundary_particles set while a surrogate is attached. */
 int32_t phieng_world3_set_fluid_config(World3 *world, const FluidConfig3Ffi *cfg);
 /* ffi.rs:1050-1059 — 1 on success, 0 on null. */
 int32_t phieng_world3_get_fluid_config(const World3 *world, FluidConfig3Ffi *out);
@@ -135,6 +135,17 @@ int32_t phieng_world3_get_timing(const World3 *world, double *total_ms, uint64_t
                                  int32_t max);
 const char *phieng_kernel_name(int32_t kind);
 
+/* Boundary ghosts (config boundary_particles, fluid3.rs:716-920): the
+ * number of ghosts and how many times they were generated (the reference's
+ * boundary_regens: once per change of the static bodies or the kernel
+ * config, fingerprinted as fluid3.rs:660-714).  1, or 0 on null. */
+int32_t phieng_world3_boundary_info(const World3 *world, uint32_t *count, uint64_t *regens);
+/* The ghosts in generation order: positions (3 floats), and the density
+ * and pressure the last substep gave them.  Null outputs are skipped;
+ * returns the number written. */
+uint32_t phieng_world3_boundary_get(const World3 *world, float *xyz, float *density, float *pressure,
+                                    uint32_t max);
+
 /* Block until all queued GPU work of the world has finished.  Returns 1,
  * or 0 on null, a HIP error, or a step that failed since the last sync
  * (phieng_world3_step returns nothing, as in the reference); then
@@ -156,7 +167,7 @@ typedef struct Batch3 Batch3;
 #define PHIENG_BATCH_PERSISTENT 1        /* one persistent launch per substep      */
 #define PHIENG_BATCH_PERSISTENT_GRAPH 2  /* the persistent launch from a hipGraph  */
 /* Null (with phieng_batch3_last_error) if a world is null, disabled, uses
- * CFL substepping, or the total exceeds 2^24 particles. */
+ * CFL substepping or boundary ghosts, or the total exceeds 2^24 particles. */
 Batch3 *phieng_batch3_new(World3 *const *worlds, uint32_t count);
 void phieng_batch3_free(Batch3 *batch);
 /* Launch configuration; blocks_per_cu = 0 uses the occupancy limit. */

@@ -109,6 +109,24 @@ def substep_f64(pos, vel, cfg, gravity, bodies, dt, phases=None, mass=None):
     return dict(pos=opos, vel=ovel, density=rho, pressure=p, accel=acc)
 
 
+def boundary_f32(pos, mass, cfg, bodies):
+    """The boundary ghosts of `bodies` for a fluid state (fluid3.rs:1018-1027):
+    dict(pos (G, 3), density, pressure) in the reference's generation order."""
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    mass = np.ascontiguousarray(mass, dtype=np.float32)
+    arr, nb = bodies
+    l = lib()
+    l.oracle3_boundary_f32.restype = ctypes.c_uint32
+    g = l.oracle3_boundary_f32(ctypes.c_uint32(len(pos)), _p(pos), _p(mass), ctypes.byref(cfg), arr,
+                               ctypes.c_uint32(nb), None, None, None, ctypes.c_uint32(0))
+    xyz = np.zeros((g, 3), np.float32)
+    rho = np.zeros(g, np.float32)
+    p = np.zeros(g, np.float32)
+    l.oracle3_boundary_f32(ctypes.c_uint32(len(pos)), _p(pos), _p(mass), ctypes.byref(cfg), arr, ctypes.c_uint32(nb),
+                           _p(xyz), _p(rho), _p(p), ctypes.c_uint32(g))
+    return dict(pos=xyz, density=rho, pressure=p)
+
+
 def run_f32(pos, vel, cfg, gravity, bodies, dt, steps, jacobi=False, phases=None):
     """`steps` world steps of dt (step_fluid incl. CFL substeps); returns (pos, vel, substeps)."""
     n = len(pos)
@@ -166,7 +184,8 @@ class Oracle2Cfg(ctypes.Structure):
                 ("artificial_viscosity", ctypes.c_float), ("surface_tension", ctypes.c_float),
                 ("cfl", ctypes.c_float), ("max_substeps", ctypes.c_uint32), ("viscosity", ctypes.c_float),
                 ("wall_friction", ctypes.c_float), ("phase_rest_density", ctypes.c_float * 4),
-                ("max_velocity", ctypes.c_float), ("particle_radius", ctypes.c_float)]
+                ("max_velocity", ctypes.c_float), ("particle_radius", ctypes.c_float),
+                ("boundary_particles", ctypes.c_uint32), ("particle_mass", ctypes.c_float)]
 
     @classmethod
     def default(cls):
@@ -186,6 +205,8 @@ class Oracle2Cfg(ctypes.Structure):
             c.phase_rest_density[k] = 1.0
         c.max_velocity = 30.0
         c.particle_radius = 0.125
+        c.boundary_particles = 0
+        c.particle_mass = 0.0625
         return c
 
 
@@ -301,4 +322,21 @@ def scene2_inputs(sc):
     cfg.particle_radius = c["particle_radius"]
     for k in range(4):
         cfg.phase_rest_density[k] = c["rest_density"]
+    cfg.particle_mass = c["mass"]
     return cfg, np.full(sc.n, c["mass"], np.float32), bodies2(boxes=sc.boxes)
+
+
+def boundary2_f32(pos, mass, cfg, bodies):
+    """The 2D boundary ghosts of `bodies` for a fluid state (fluid.rs:929-938):
+    dict(pos (G, 2), density, pressure) in generation order."""
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    mass = np.ascontiguousarray(mass, dtype=np.float32)
+    arr, nb = bodies
+    l = lib2()
+    l.oracle2_boundary_f32.restype = ctypes.c_uint32
+    g = l.oracle2_boundary_f32(len(pos), _p(pos), _p(mass), ctypes.byref(cfg), arr, nb, None, None, None, 0)
+    xy = np.zeros((g, 2), np.float32)
+    rho = np.zeros(g, np.float32)
+    p = np.zeros(g, np.float32)
+    l.oracle2_boundary_f32(len(pos), _p(pos), _p(mass), ctypes.byref(cfg), arr, nb, _p(xy), _p(rho), _p(p), g)
+    return dict(pos=xy, density=rho, pressure=p)

@@ -23,10 +23,12 @@
 // determinism :1292-1309, hydrostatic band :1676-1719).
 // Parity unpinned: glam 0.29.3 Vec2 association order ((x*x) + (y*y)) and
 // the last ulp of libm powf (taken in f64 and rounded, as in the 3D oracle).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -65,6 +67,8 @@ struct Oracle2Cfg {
     float viscosity, wall_friction;
     float phase_rest_density[4];
     float max_velocity, particle_radius;
+    uint32_t boundary_particles;  // FluidConfig2::boundary_particles
+    float particle_mass;          // FluidConfig2::particle_mass (ghost mass)
 };
 
 // Static body, identity rotation: shape 0 circle (a = radius), 1 box
@@ -83,11 +87,17 @@ struct P2 {
     uint8_t phase;
 };
 
+struct Ghost2 {
+    float x, y, rho, p, m;
+};
+
 struct Fluid2 {
     Oracle2Cfg cfg;
     std::vector<P2> ps;
     std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
     std::vector<float> fx, fy;
+    std::vector<Ghost2> ghosts;
+    std::unordered_map<uint64_t, std::vector<uint32_t>> ghost_grid;
 
     static uint64_t key(int32_t cx, int32_t cy) { return ((uint64_t)(uint32_t)cx << 32) | (uint32_t)cy; }
 
@@ -157,6 +167,78 @@ struct Fluid2 {
                 }
             }
     }
+    // fluid.rs:666-743 for the bodies the C ABI creates (static circle and
+    // box at angle 0: transform_point(p) = p + position up to zero signs).
+    void refresh_boundaries(const Oracle2Body *bodies, uint32_t nb, float spacing_in) {
+        ghosts.clear();
+        const float rest = cfg.rest_density, gm = rmax(cfg.particle_mass, kEps);
+        const float spacing = rmax(spacing_in, kEps);
+        auto runs = [](float v) {  // (v).ceil() as usize, then .max(1)
+            const float c = std::ceil(v);
+            return c >= 1.0f ? (uint32_t)c : 1u;
+        };
+        for (uint32_t bi = 0; bi < nb; ++bi) {
+            const Oracle2Body &b = bodies[bi];
+            if (b.shape == 0u) {  // ring
+                const uint32_t n = runs((2.0f * kPiF * b.a) / spacing);
+                for (uint32_t k = 0; k < n; ++k) {
+                    const float a = 2.0f * kPiF * (float)k / (float)n;
+                    ghosts.push_back({std::cos(a) * b.a + b.px, std::sin(a) * b.a + b.py, rest, 0.0f, gm});
+                }
+            } else {  // edges: (origin, tangent) as listed in the reference
+                const float org[4][2] = {{-b.a, -b.b}, {-b.a, b.b}, {-b.a, -b.b}, {b.a, -b.b}};
+                const float tan[4][2] = {{0.0f, 1.0f}, {0.0f, 1.0f}, {1.0f, 0.0f}, {1.0f, 0.0f}};
+                for (int e = 0; e < 4; ++e) {
+                    const float extent = tan[e][0] == 1.0f ? b.a : b.b;
+                    const uint32_t n = runs((2.0f * extent) / spacing);
+                    for (uint32_t k = 0; k < n; ++k) {
+                        const float t = ((float)k + 0.5f) / (float)n;
+                        const float d = 2.0f * extent * t;
+                        ghosts.push_back({org[e][0] + tan[e][0] * d + b.px, org[e][1] + tan[e][1] * d + b.py, rest,
+                                          0.0f, gm});
+                    }
+                }
+            }
+        }
+    }
+    // fluid.rs:746-758, 762-780, 784-802, 804-818
+    void update_boundaries(const Oracle2Body *bodies, uint32_t nb, float h) {
+        refresh_boundaries(bodies, nb, h * 0.5f);
+        ghost_grid.clear();
+        for (uint32_t g = 0; g < ghosts.size(); ++g) {
+            if (!finite2(ghosts[g].x, ghosts[g].y)) continue;
+            ghost_grid[key(sat_i32(std::floor(ghosts[g].x / h)), sat_i32(std::floor(ghosts[g].y / h)))].push_back(g);
+        }
+        std::vector<uint32_t> nbr;
+        for (Ghost2 &gh : ghosts) {
+            near_cells(grid, [&](uint32_t j) { return std::make_pair(ps[j].x, ps[j].y); }, gh.x, gh.y, nbr);
+            float rho = 0.0f;
+            for (uint32_t j : nbr) rho += ps[j].m * poly6(len_sq(ps[j].x - gh.x, ps[j].y - gh.y), h);
+            gh.rho = rclamp(rho, 0.0f, 3.0f * rmax(cfg.rest_density, kEps));
+            gh.p = pressure(rho, cfg.rest_density, cfg.stiffness, cfg.gamma);
+        }
+    }
+    // The 9 cells around a point at the raw smoothing length, oy -> ox,
+    // members in index order, d2 < h * h (boundary_near / fluid_near).
+    template <class At>
+    void near_cells(const std::unordered_map<uint64_t, std::vector<uint32_t>> &g, At at, float px, float py,
+                    std::vector<uint32_t> &out) const {
+        out.clear();
+        const float h = cfg.smoothing_length;
+        const int32_t cx = sat_i32(std::floor(px / h)), cy = sat_i32(std::floor(py / h));
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int64_t x = (int64_t)cx + ox, y = (int64_t)cy + oy;
+                if (x < INT32_MIN || x > INT32_MAX || y < INT32_MIN || y > INT32_MAX) continue;
+                auto it = g.find(key((int32_t)x, (int32_t)y));
+                if (it == g.end()) continue;
+                for (uint32_t j : it->second) {
+                    const std::pair<float, float> q = at(j);
+                    if (len_sq(q.first - px, q.second - py) < h * h) out.push_back(j);
+                }
+            }
+    }
+
     // fluid.rs:562-589
     uint32_t cfl_substeps(float dt) const {
         const uint32_t mx = cfg.max_substeps > 1 ? cfg.max_substeps : 1;
@@ -253,6 +335,9 @@ struct Fluid2 {
         rebuild_grid();
         std::vector<uint32_t> nb;
         density_pressure(h, nb, count);
+        // fluid.rs:929-938: the ghosts are a pure function of the static
+        // bodies and the config, rebuilt here every substep
+        if (cfg.boundary_particles) update_boundaries(bodies, nbodies, h);
         const float c_s = std::sqrt(rmax(cfg.stiffness, 0.0f));
         const size_t n = ps.size();
         fx.assign(n, 0.0f);
@@ -290,6 +375,24 @@ struct Fluid2 {
                         f_y = f_y - dy / dist * c;
                         mag += std::fabs((double)c);
                     }
+                }
+            }
+            // fluid.rs:985-995: ghost pressure after the fluid neighbours
+            if (cfg.boundary_particles) {
+                std::vector<uint32_t> gb;
+                near_cells(ghost_grid, [&](uint32_t j) { return std::make_pair(ghosts[j].x, ghosts[j].y); }, pi.x,
+                           pi.y, gb);
+                for (uint32_t k : gb) {
+                    const Ghost2 &gh = ghosts[k];
+                    const float dx = pi.x - gh.x, dy = pi.y - gh.y;
+                    const float dist = std::sqrt(len_sq(dx, dy));
+                    float gxx, gyy;
+                    spiky_grad(dx, dy, dist, h, gxx, gyy);
+                    const float rho_b = rmax(gh.rho, kEps);
+                    const float s = gh.m * (p_i / (rho_i * rho_i) + gh.p / (rho_b * rho_b));
+                    f_x = f_x - gxx * s;
+                    f_y = f_y - gyy * s;
+                    mag += std::sqrt((double)gxx * gxx + (double)gyy * gyy) * std::fabs((double)s);
                 }
             }
             fx[i] = f_x;
@@ -432,6 +535,27 @@ int oracle2_run_f32(uint32_t n, float *pos, float *vel, const float *mass, const
     }
     store(f, pos, vel, density, pressure);
     return 1;
+}
+
+// The boundary ghosts of `bodies` for a fluid state (fluid.rs:929-938):
+// positions (2 per ghost), density and pressure in generation order, up to
+// max.  Returns the ghost count.
+uint32_t oracle2_boundary_f32(uint32_t n, const float *pos, const float *mass, const Oracle2Cfg *cfg,
+                              const Oracle2Body *bodies, uint32_t nbodies, float *xy, float *density,
+                              float *pressure, uint32_t max) {
+    Fluid2 f = make(n, pos, nullptr, mass, nullptr, cfg);
+    f.rebuild_grid();
+    f.update_boundaries(bodies, nbodies, rmax(cfg->smoothing_length, kEps));
+    const uint32_t m = std::min<uint32_t>(max, (uint32_t)f.ghosts.size());
+    for (uint32_t g = 0; g < m; ++g) {
+        if (xy) {
+            xy[2 * g] = f.ghosts[g].x;
+            xy[2 * g + 1] = f.ghosts[g].y;
+        }
+        if (density) density[g] = f.ghosts[g].rho;
+        if (pressure) pressure[g] = f.ghosts[g].p;
+    }
+    return (uint32_t)f.ghosts.size();
 }
 
 float oracle2_poly6(float r2, float h) { return Fluid2::poly6(r2, h); }

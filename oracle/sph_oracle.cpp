@@ -22,6 +22,11 @@
 //   integrate + boundary clamp     — containment, count preserved,
 //       run-twice determinism, hydrostatic band (fluid3.rs:1313-1347,
 //       1398-1416, 1560-1605), single-particle closed forms.
+//   boundary ghosts                — counts and positions against the
+//       closed forms of the generation rules (Fibonacci sphere, box face
+//       sweep, capsule rings), ghost density against brute force, the
+//       reference's contained / persisted tests (fluid3.rs:1490-1555),
+//       ghost pressure pushing a particle away from a wall.
 //   Parity unpinned: glam 0.29.3 Vec3 association order and libm powf
 //   (the Tait power is taken in f64 and rounded to f32 here and on the GPU).
 #include <cmath>
@@ -112,6 +117,14 @@ struct Consts<double> {
     static double pi() { return 3.14159265358979323846; }
 };
 
+// A boundary ghost (fluid3.rs:716-841): a static particle on a body's
+// surface with its own density and pressure from the fluid around it.
+template <class T>
+struct Ghost {
+    V3<T> p;
+    T density, pressure, mass;
+};
+
 template <class T>
 struct Fluid {
     // FluidConfig3 (fluid3.rs:51-135) as the kernels read it.
@@ -122,6 +135,8 @@ struct Fluid {
     std::vector<uint8_t> phase;
     std::unordered_map<Key, std::vector<uint32_t>, KeyHash> grid;
     std::vector<V3<T>> forces;
+    std::vector<Ghost<T>> ghosts;
+    std::unordered_map<Key, std::vector<uint32_t>, KeyHash> ghost_grid;
 
     explicit Fluid(const FluidConfig3Ffi &c) : cfg(c), n(0) {}
 
@@ -194,6 +209,133 @@ struct Fluid {
                     }
                 }
     }
+    // fluid3.rs:716-841 for the shapes the C ABI creates (static, identity
+    // rotation: transform_point(p) = p + position up to zero signs).  cos,
+    // sin and sqrt are the platform libm's in f32, as Rust's f32 methods
+    // (parity unpinned across libms).
+    void refresh_boundaries(const Body *bodies, uint32_t nb, float spacing_in) {
+        ghosts.clear();
+        const float rest = cfg.rest_density;
+        const float gm = rmax<float>(cfg.particle_mass, kEps);
+        const float spacing = rmax<float>(spacing_in, kEps);
+        const float s2 = spacing * spacing;  // spacing.powi(2)
+        auto push = [&](const Body &b, float lx, float ly, float lz) {
+            ghosts.push_back({{(T)(lx + b.px), (T)(ly + b.py), (T)(lz + b.pz)}, (T)rest, (T)0, (T)gm});
+        };
+        auto count = [](float v, float cap) {  // v.ceil().clamp(1.0, cap) as usize
+            float c = std::ceil(v);
+            if (c < 1.0f) c = 1.0f;
+            if (c > cap) c = cap;
+            return (uint32_t)c;
+        };
+        for (uint32_t bi = 0; bi < nb; ++bi) {
+            const Body &b = bodies[bi];
+            if (b.shape == PHIENG_SHAPE3_SPHERE) {  // Fibonacci sphere, :725-748
+                const float radius = b.a;
+                const uint32_t m = count((4.0f * kPiF * radius * radius) / s2, 512.0f);
+                const float golden = kPiF * (3.0f - std::sqrt(5.0f));
+                for (uint32_t k = 0; k < m; ++k) {
+                    const float z = 1.0f - 2.0f * ((float)k + 0.5f) / (float)m;
+                    const float r = std::sqrt(1.0f - z * z);
+                    const float theta = golden * (float)k;
+                    push(b, std::cos(theta) * r * radius, z * radius, std::sin(theta) * r * radius);
+                }
+            } else if (b.shape == PHIENG_SHAPE3_BOX) {  // :749-791 (the reference's face layout kept as is)
+                const float he[3] = {b.a, b.b, b.c};
+                const int axes[6] = {0, 0, 1, 1, 2, 2};
+                const float fixed[6] = {-he[1], -he[1], -he[0], -he[0], -he[2], -he[2]};
+                for (int f = 0; f < 6; ++f) {
+                    const int axis = axes[f];
+                    const float area = axis == 0 ? 4.0f * he[1] * he[2] : (axis == 1 ? 4.0f * he[0] * he[2]
+                                                                                     : 4.0f * he[0] * he[1]);
+                    const uint32_t m = count(area / s2, 512.0f);
+                    for (uint32_t k = 0; k < m; ++k) {
+                        const float t = ((float)k + 0.5f) / (float)m;
+                        const float u = 2.0f * t - 1.0f, w = 2.0f * std::fmod(t * 7.0f, 1.0f) - 1.0f;
+                        float l[3] = {0.0f, 0.0f, fixed[f]};
+                        if (axis == 0) {
+                            l[1] = u * he[1];
+                            l[2] = w * he[2];
+                        } else if (axis == 1) {
+                            l[0] = u * he[0];
+                            l[2] = w * he[2];
+                        } else {
+                            l[0] = u * he[0];
+                            l[1] = w * he[1];
+                        }
+                        push(b, l[0], l[1], l[2]);
+                    }
+                }
+            } else if (b.shape == PHIENG_SHAPE3_CAPSULE) {  // :792-826
+                const float r = b.a, hl = b.b;
+                const float side = 4.0f * kPiF * r * hl;
+                const uint32_t m = count(side / s2, 256.0f);
+                const float sys[2] = {-hl * 0.5f, hl * 0.5f};
+                for (float sy : sys)
+                    for (uint32_t k = 0; k < m; ++k) {
+                        const float a = 2.0f * kPiF * (float)k / (float)m;
+                        push(b, std::cos(a) * r, sy, std::sin(a) * r);
+                    }
+                for (float sg : {-1.0f, 1.0f}) push(b, 0.0f * sg * (hl + r), sg * (hl + r), 0.0f * sg * (hl + r));
+            }
+        }
+    }
+    // fluid3.rs:843-857: ghost cells at the kernel h, non-finite x skipped.
+    void rebuild_boundary_grid(T h) {
+        ghost_grid.clear();
+        for (uint32_t g = 0; g < ghosts.size(); ++g) {
+            const V3<T> p = ghosts[g].p;
+            if (!std::isfinite(p.x)) continue;
+            ghost_grid[Key{sat_i32((double)std::floor(p.x / h)), sat_i32((double)std::floor(p.y / h)),
+                           sat_i32((double)std::floor(p.z / h))}]
+                .push_back(g);
+        }
+    }
+    // fluid3.rs:860-880 (boundary_near) and :883-903 (fluid_near): the 27
+    // cells around `point` at the raw smoothing length, oz -> oy -> ox,
+    // members in index order, d2 < h * h.
+    template <class Pos>
+    void near_in(const std::unordered_map<Key, std::vector<uint32_t>, KeyHash> &g, const Pos &at, V3<T> point,
+                 std::vector<uint32_t> &out) const {
+        out.clear();
+        const T h = (T)cfg.smoothing_length;
+        const Key c = cell_of(point);
+        for (int oz = -1; oz <= 1; ++oz)
+            for (int oy = -1; oy <= 1; ++oy)
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const int64_t kx = (int64_t)c.x + ox, ky = (int64_t)c.y + oy, kz = (int64_t)c.z + oz;
+                    if (kx > INT32_MAX || kx < INT32_MIN || ky > INT32_MAX || ky < INT32_MIN || kz > INT32_MAX ||
+                        kz < INT32_MIN)
+                        continue;
+                    auto it = g.find(Key{(int32_t)kx, (int32_t)ky, (int32_t)kz});
+                    if (it == g.end()) continue;
+                    for (uint32_t j : it->second) {
+                        const V3<T> d = sub(at(j), point);
+                        if (dot(d, d) < h * h) out.push_back(j);
+                    }
+                }
+    }
+    // fluid3.rs:906-920: the ghost's density from the fluid (clamped) and
+    // its pressure from the unclamped sum with the base rest density.
+    void compute_boundary_density_pressure(T h) {
+        std::vector<uint32_t> nb;
+        for (Ghost<T> &gh : ghosts) {
+            near_in(grid, [&](uint32_t j) { return pos[j]; }, gh.p, nb);
+            T rho = 0;
+            for (uint32_t j : nb) rho += mass[j] * poly6(dot(sub(pos[j], gh.p), sub(pos[j], gh.p)), h);
+            gh.density = rclamp<T>(rho, (T)0, (T)3.0 * rmax<T>((T)cfg.rest_density, (T)kEps));
+            gh.pressure = eos(rho, (T)cfg.rest_density, (T)cfg.stiffness, (T)cfg.gamma);
+        }
+    }
+    // fluid3.rs:1018-1027: the ghosts are a pure function of the static
+    // bodies and the config (the fingerprint only caches them), so they are
+    // rebuilt here on every substep.
+    void update_boundaries(const Body *bodies, uint32_t nb, T h) {
+        refresh_boundaries(bodies, nb, (float)h * 0.5f);
+        rebuild_boundary_grid(h);
+        compute_boundary_density_pressure(h);
+    }
+
     // fluid3.rs:639-655
     void compute_density_pressure(T h, uint32_t *out_count) {
         std::vector<uint32_t> nb;
@@ -248,6 +390,21 @@ struct Fluid {
                         f = sub(f, muls(divs(delta, dist), cs));
                         mag += std::fabs((double)cs);
                     }
+                }
+            }
+            // fluid3.rs:1074-1084: ghost pressure after the fluid neighbours
+            if (cfg.boundary_particles) {
+                std::vector<uint32_t> gb;
+                near_in(ghost_grid, [&](uint32_t j) { return ghosts[j].p; }, pi, gb);
+                for (uint32_t k : gb) {
+                    const Ghost<T> &gh = ghosts[k];
+                    V3<T> delta = sub(pi, gh.p);
+                    T dist = len(delta);
+                    V3<T> grad = spiky_grad(delta, dist, h);
+                    T rho_b = rmax<T>(gh.density, (T)kEps);
+                    T s = gh.mass * (p_i / (rho_i * rho_i) + gh.pressure / (rho_b * rho_b));
+                    f = sub(f, muls(grad, s));
+                    mag += std::sqrt((double)dot(grad, grad)) * std::fabs((double)s);
                 }
             }
             forces[i] = f;
@@ -388,6 +545,7 @@ struct Fluid {
         T h = h_kernel();
         rebuild_grid();
         compute_density_pressure(h, count_out);
+        if (cfg.boundary_particles) update_boundaries(bodies, nbodies, h);
         compute_forces(h, scale_out);
         std::vector<uint32_t> nb;
         if (jacobi) {
@@ -498,6 +656,29 @@ int oracle3_substep_f32(uint32_t n, float *pos, float *vel, float *density, floa
         if (pressure) pressure[i] = f.pressure[i];
     }
     return 1;
+}
+
+// The boundary ghosts of the bodies for a fluid state (fluid3.rs:1018-1027
+// on that state): positions (3 per ghost), density and pressure, in the
+// reference's generation order, up to max.  Returns the ghost count.
+uint32_t oracle3_boundary_f32(uint32_t n, const float *pos, const float *mass, const FluidConfig3Ffi *cfg,
+                              const OracleBody3 *bodies, uint32_t nbodies, float *xyz, float *density,
+                              float *pressure, uint32_t max) {
+    Fluid<float> f(*cfg);
+    f.load(n, pos, nullptr, mass, nullptr, nullptr);
+    f.rebuild_grid();
+    f.update_boundaries(reinterpret_cast<const Body *>(bodies), nbodies, f.h_kernel());
+    const uint32_t m = std::min<uint32_t>(max, (uint32_t)f.ghosts.size());
+    for (uint32_t g = 0; g < m; ++g) {
+        if (xyz) {
+            xyz[3 * g] = f.ghosts[g].p.x;
+            xyz[3 * g + 1] = f.ghosts[g].p.y;
+            xyz[3 * g + 2] = f.ghosts[g].p.z;
+        }
+        if (density) density[g] = f.ghosts[g].density;
+        if (pressure) pressure[g] = f.ghosts[g].pressure;
+    }
+    return (uint32_t)f.ghosts.size();
 }
 
 // FluidState3::step_fluid (fluid3.rs:591-607) for `steps` world steps of dt

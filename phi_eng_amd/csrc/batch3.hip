@@ -263,6 +263,8 @@ Batch3 *phieng_batch3_new(World3 *const *worlds, uint32_t count) {
             if (c.cfl > 0.0f && c.max_substeps > 1)
                 throw std::runtime_error("CFL substepping differs per scene: not supported in a batch");
             if (!c.enabled) throw std::runtime_error("a disabled fluid cannot join a batch");
+            if (c.boundary_particles)
+                throw std::runtime_error("boundary ghosts are per world: a world with boundary_particles cannot join a batch");
             b->worlds.push_back(worlds[k]);
         }
         build_batch(*b);

@@ -34,6 +34,7 @@ enum Kind : int {
     KIND_SUR_STEP,
     KIND_SUR_AUX,
     KIND_SUR_COLLECT,
+    KIND_GHOST,
     KIND_COUNT
 };
 const char *kind_name(int k);
@@ -84,6 +85,10 @@ struct SceneHost {
 // order (fluid3.rs:220-251, 649, 1028; the 2D kernels fluid.rs:265-285) so
 // device results match it.
 void fill_scene_physics(SceneDev &s, const SceneHost &h);
+// Boundary ghosts of the scene's static bodies in the reference's
+// generation order (xyz, mass in w): fluid3.rs:716-841 for a 3D scene,
+// fluid.rs:666-743 for a 2D one, with spacing = h_kernel / 2.
+void make_ghosts(const SceneHost &h, std::vector<float4> &out);
 // Pick a grid domain (cell box) for a scene from the particle cell box and
 // its bodies.  The domain only affects speed: particles outside it land in
 // the scene's outlier bucket and are still handled exactly.
@@ -181,6 +186,17 @@ struct Engine {
     DBuf<float4> acc_sorted;
     DBuf<float2> tile_err;
     DBuf<float> err_out;
+    // Boundary ghosts of scene 0 (single-world engines; fluid3.rs:716-920):
+    // cell-sorted positions + mass, per-substep state, cell table.
+    DBuf<float4> g_pos, g_state;
+    DBuf<int4> g_key;
+    DBuf<uint2> g_range;
+    uint32_t g_count = 0, g_mask = 0;
+    std::vector<uint32_t> g_order;  // cell-sorted slot -> generation index
+    // Replace the ghosts (generation order, xyz + mass) keyed by their cells
+    // at the kernel smoothing length h_kernel (rebuild_boundary_grid).
+    void set_ghosts(const std::vector<float4> &gen, float h_kernel);
+    GhostDev ghost_dev();
     // Asynchronous domain monitor: returns true if some scene changed domain.
     bool poll_monitor(uint32_t nparticles_hint);
     // Cell box of the current particles of scene k (synchronous).

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <stdexcept>
 
 #include "sph_kernels.h"
 
@@ -35,7 +37,7 @@ const char *kind_name(int k) {
         "cell_key", "scan_reduce", "scan_partials", "scan_apply", "scatter", "rank_gather",
         "density",  "force_integrate", "vmax", "export", "monitor", "append", "cell_bbox",
         "fill", "set_velocity", "batch_persistent", "surrogate_step", "surrogate_history",
-        "surrogate_collect"};
+        "surrogate_collect", "boundary_ghosts"};
     return (k >= 0 && k < KIND_COUNT) ? names[k] : "";
 }
 
@@ -55,6 +57,7 @@ void fill_scene_physics(SceneDev &s, const SceneHost &h) {
         s.spiky_c = -45.0f / (kPiF * hk * hk * hk * hk * hk * hk);     // fluid3.rs:237
     }
     s.rho_max = 3.0f * rmax_h(c.rest_density, kEps);
+    s.rest_density = c.rest_density;
     s.stiffness = c.stiffness;
     s.gamma = c.gamma;
     for (int k = 0; k < 4; ++k) s.phase_rho0[k] = c.phase_rest_density[k];
@@ -70,6 +73,95 @@ void fill_scene_physics(SceneDev &s, const SceneHost &h) {
     s.gy = h.g[1];
     s.gz = h.g[2];
     s.dt = h.dt;
+}
+
+void make_ghosts(const SceneHost &h, std::vector<float4> &out) {
+    constexpr size_t kMaxGhosts = size_t(1) << 22;
+    out.clear();
+    const FluidConfig3Ffi &c = h.cfg;
+    const float hk = rmax_h(c.smoothing_length, kEps);
+    const float spacing = rmax_h(hk * 0.5f, kEps);
+    const float gm = rmax_h(c.particle_mass, kEps);
+    const float sp2 = spacing * spacing;  // spacing.powi(2)
+    // ceil(v).clamp(1, cap) as usize; cap 0: no upper clamp (n.max(1))
+    auto runs = [](float v, float cap) -> uint32_t {
+        float k = std::ceil(v);
+        if (!(k >= 1.0f)) k = 1.0f;
+        if (cap > 0.0f && k > cap) k = cap;
+        return k >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)k;
+    };
+    // transform_point with the identity rotation: local + position
+    auto put = [&](const BodyDev &b, float x, float y, float z) {
+        if (out.size() >= kMaxGhosts) throw std::runtime_error("boundary ghosts exceed 4,194,304");
+        out.push_back(make_float4(x + b.px, y + b.py, z + b.pz, gm));
+    };
+    for (const BodyDev &b : h.bodies) {  // the C ABI only holds static bodies
+        if (h.dims == 2) {
+            if (b.shape == 0u) {  // circle ring, fluid.rs:673-688
+                const uint32_t m = runs((2.0f * kPiF * b.a) / spacing, 0.0f);
+                for (uint32_t k = 0; k < m; ++k) {
+                    const float a = 2.0f * kPiF * (float)k / (float)m;
+                    put(b, std::cos(a) * b.a, std::sin(a) * b.a, 0.0f);
+                }
+            } else if (b.shape == 3u) {  // four edges, fluid.rs:689-714
+                // origin and tangent per edge in the reference's order (the
+                // first two run along y, the last two along x)
+                const float ox[4] = {-b.a, -b.a, -b.a, b.a}, oy[4] = {-b.b, b.b, -b.b, -b.b};
+                for (int e = 0; e < 4; ++e) {
+                    const float tanx = (e >= 2) ? 1.0f : 0.0f, tany = (e >= 2) ? 0.0f : 1.0f;
+                    const float extent = tanx == 1.0f ? b.a : b.b;
+                    const uint32_t m = runs((2.0f * extent) / spacing, 0.0f);
+                    for (uint32_t k = 0; k < m; ++k) {
+                        const float t = ((float)k + 0.5f) / (float)m;
+                        const float d = 2.0f * extent * t;
+                        put(b, ox[e] + tanx * d, oy[e] + tany * d, 0.0f);
+                    }
+                }
+            }
+            continue;
+        }
+        if (b.shape == 0u) {  // Fibonacci sphere, fluid3.rs:725-748
+            const uint32_t m = runs((4.0f * kPiF * b.a * b.a) / sp2, 512.0f);
+            const float golden = kPiF * (3.0f - std::sqrt(5.0f));
+            for (uint32_t k = 0; k < m; ++k) {
+                const float z = 1.0f - 2.0f * ((float)k + 0.5f) / (float)m;
+                const float r = std::sqrt(1.0f - z * z);
+                const float th = golden * (float)k;
+                put(b, std::cos(th) * r * b.a, z * b.a, std::sin(th) * r * b.a);
+            }
+        } else if (b.shape == 1u) {  // box, fluid3.rs:749-791: six sweeps, the
+            // first coordinate of each axis pair left at 0 and z at the
+            // "fixed" value, as the reference lays them out
+            const float he[3] = {b.a, b.b, b.c};
+            for (int f = 0; f < 6; ++f) {
+                const int axis = f / 2;
+                const float fixed = axis == 0 ? -he[1] : (axis == 1 ? -he[0] : -he[2]);
+                const float area = axis == 0 ? 4.0f * he[1] * he[2]
+                                             : (axis == 1 ? 4.0f * he[0] * he[2] : 4.0f * he[0] * he[1]);
+                const uint32_t m = runs(area / sp2, 512.0f);
+                const int u_ax = axis == 0 ? 1 : 0, w_ax = axis == 2 ? 1 : 2;
+                for (uint32_t k = 0; k < m; ++k) {
+                    const float t = ((float)k + 0.5f) / (float)m;
+                    float l[3] = {0.0f, 0.0f, fixed};
+                    l[u_ax] = (2.0f * t - 1.0f) * he[u_ax];
+                    l[w_ax] = (2.0f * std::fmod(t * 7.0f, 1.0f) - 1.0f) * he[w_ax];
+                    put(b, l[0], l[1], l[2]);
+                }
+            }
+        } else if (b.shape == 2u) {  // capsule rings and end points, fluid3.rs:792-826
+            const float r = b.a, hl = b.b;
+            const uint32_t m = runs((4.0f * kPiF * r * hl) / sp2, 256.0f);
+            for (float sy : {-hl * 0.5f, hl * 0.5f})
+                for (uint32_t k = 0; k < m; ++k) {
+                    const float a = 2.0f * kPiF * (float)k / (float)m;
+                    put(b, std::cos(a) * r, sy, std::sin(a) * r);
+                }
+            for (float sg : {-1.0f, 1.0f}) {
+                const float e = hl + r;
+                put(b, 0.0f * sg * e, 1.0f * sg * e, 0.0f * sg * e);  // Vec3::Y * s * (hl + r)
+            }
+        }
+    }
 }
 
 void choose_domain(SceneDev &s, const int32_t *pb, uint32_t np, const std::vector<BodyDev> &bodies) {
@@ -216,6 +308,73 @@ void Engine::ensure_particles(uint32_t cap) {
     if (rec_flags) ensure_record();
 }
 
+void Engine::set_ghosts(const std::vector<float4> &gen, float hk) {
+    init_device();
+    const uint32_t g = (uint32_t)gen.size();
+    struct Cell {
+        int32_t x, y, z;
+    };
+    // rebuild_boundary_grid (fluid3.rs:843-857): cells at the kernel h; a
+    // ghost with a non-finite x is in no cell (the 2D rule also checks y,
+    // fluid.rs:749; ghosts of finite bodies are finite either way)
+    std::vector<Cell> cell(g);
+    std::vector<uint32_t> order;
+    order.reserve(g);
+    for (uint32_t k = 0; k < g; ++k) {
+        const float4 p = gen[k];
+        if (!std::isfinite(p.x) || !std::isfinite(p.y)) continue;
+        cell[k] = {sat_i32_h(std::floor(p.x / hk)), sat_i32_h(std::floor(p.y / hk)), sat_i32_h(std::floor(p.z / hk))};
+        order.push_back(k);
+    }
+    // cell by cell, ghost index ascending inside a cell
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const Cell &p = cell[a], &q = cell[b];
+        if (p.z != q.z) return p.z < q.z;
+        if (p.y != q.y) return p.y < q.y;
+        return p.x < q.x;
+    });
+    const uint32_t m = (uint32_t)order.size();
+    std::vector<float4> pos(std::max(m, 1u));
+    for (uint32_t k = 0; k < m; ++k) pos[k] = gen[order[k]];
+    uint32_t ncell = 0;
+    for (uint32_t k = 0; k < m; ++k)
+        if (k == 0 || std::memcmp(&cell[order[k]], &cell[order[k - 1]], sizeof(Cell)) != 0) ++ncell;
+    uint32_t cap = 16;
+    while (cap < 2 * ncell) cap <<= 1;
+    std::vector<int4> key(cap, make_int4(0, 0, 0, 0));
+    std::vector<uint2> range(cap, make_uint2(0, 0));
+    for (uint32_t k = 0; k < m;) {
+        uint32_t e = k + 1;
+        while (e < m && std::memcmp(&cell[order[e]], &cell[order[k]], sizeof(Cell)) == 0) ++e;
+        const Cell c = cell[order[k]];
+        uint32_t slot = ghost_hash(c.x, c.y, c.z) & (cap - 1);
+        while (range[slot].y != 0) slot = (slot + 1) & (cap - 1);
+        key[slot] = make_int4(c.x, c.y, c.z, 0);
+        range[slot] = make_uint2(k, e);
+        k = e;
+    }
+    g_pos.ensure(pos.size());
+    g_state.ensure(pos.size());
+    g_key.ensure(cap);
+    g_range.ensure(cap);
+    PH_CK(hipMemcpyAsync(g_pos.p, pos.data(), sizeof(float4) * pos.size(), hipMemcpyHostToDevice, stream));
+    PH_CK(hipMemcpyAsync(g_key.p, key.data(), sizeof(int4) * cap, hipMemcpyHostToDevice, stream));
+    PH_CK(hipMemcpyAsync(g_range.p, range.data(), sizeof(uint2) * cap, hipMemcpyHostToDevice, stream));
+    PH_CK(hipStreamSynchronize(stream));  // the host vectors go out of scope
+    g_count = m;
+    g_mask = cap - 1;
+    g_order = std::move(order);
+}
+
+GhostDev Engine::ghost_dev() {
+    GhostDev d;
+    d.pos = g_pos.p;
+    d.state = g_state.p;
+    d.key = g_key.p;
+    d.range = g_range.p;
+    return d;
+}
+
 void Engine::ensure_record() {
     const size_t m = std::max<size_t>(alloc_particles, 1);
     rec_cells.ensure(3 * m);
@@ -336,10 +495,18 @@ void Engine::plain_phase(float4 *acc_sorted, const ListOut *lo) {
                               flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr, rd ? rec_p.p : nullptr,
                               rd ? rec_count.p : nullptr);
     });
+    // boundary ghosts (fluid3.rs:1018-1027): their density and pressure from
+    // the fluid of this substep, before the force pass reads them
+    const bool ghosts = scenes.size() == 1 && scenes[0].ghost_count;
+    if (ghosts)
+        run(KIND_GHOST, [&] {
+            return launch_ghost_density(stream, d_scenes.p, 0u, start.p, posm[ns].p, ghost_dev(), g_count);
+        });
     run(KIND_FORCE, [&] {
         return launch_force_integrate(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p,
                                       vel[ns].p, rq.p, tag[nt_].p, masks.p, flags.p, (uint32_t)alloc_particles,
-                                      posm[cs].p, vel[cs].p, ra ? rec_acc.p : nullptr, d_bbox.p, acc_sorted);
+                                      posm[cs].p, vel[cs].p, ra ? rec_acc.p : nullptr, d_bbox.p, acc_sorted,
+                                      ghosts ? ghost_dev() : GhostDev{});
     });
     gen_open = true;
 }

@@ -37,6 +37,9 @@ struct SceneDev {
     float poly6_c;  // 315 / (64 pi h2 h2 h2 h2 h)      (fluid3.rs:227)
     float spiky_c;  // -45 / (pi h h h h h h)           (fluid3.rs:237)
     float rho_max;  // 3 * rest_density.max(EPS)        (fluid3.rs:649)
+    float rest_density;  // the config's base rest density (ghost pressure, fluid3.rs:917-918)
+    uint32_t ghost_count;  // boundary ghosts of this scene (0: none)
+    uint32_t ghost_mask;   // ghost cell table size - 1 (a power of two minus one)
     float stiffness, gamma;
     float phase_rho0[4];
     float av_alpha, c_s, kappa;
@@ -355,5 +358,85 @@ __device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__r
 __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nt) {
     const uint32_t x = b & 7u, k = b >> 3, per = nt >> 3, rem = nt & 7u;
     return x * per + min(x, rem) + k;
+}
+
+// ---------------------------------------------------------------- ghosts
+// Boundary ghosts of one scene (fluid3.rs:716-920): positions sorted by
+// ghost cell with the ghost index ascending inside a cell (xyz, mass), the
+// per-substep state (clamped density, pressure, q = p / max(rho, EPS)^2),
+// and an open-addressing table from a cell to its run of ghosts (an entry
+// with range.y == 0 is empty; the table is at most half full).
+struct GhostDev {
+    const float4 *pos = nullptr;
+    float4 *state = nullptr;
+    const int4 *key = nullptr;
+    const uint2 *range = nullptr;
+};
+
+__host__ __device__ inline uint32_t ghost_hash(int32_t x, int32_t y, int32_t z) {
+    uint32_t h = (uint32_t)x * 0x9E3779B1u;
+    h ^= (uint32_t)y * 0x85EBCA77u;
+    h ^= (uint32_t)z * 0xC2B2AE3Du;
+    return h ^ (h >> 15);
+}
+
+// Run [b, e) of the ghosts in cell (x, y, z); false when the cell has none.
+// The probe visits at most mask + 1 slots and the table always has an
+// empty one, so it ends.
+__device__ __forceinline__ bool ghost_cell(const GhostDev &g, uint32_t mask, int32_t x, int32_t y, int32_t z,
+                                           uint32_t &b, uint32_t &e) {
+    uint32_t slot = ghost_hash(x, y, z) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const uint2 r = g.range[slot];
+        if (r.y == 0u) return false;
+        const int4 k = g.key[slot];
+        if (k.x == x && k.y == y && k.z == z) {
+            b = r.x;
+            e = r.y;
+            return true;
+        }
+        slot = (slot + 1u) & mask;
+    }
+    return false;
+}
+
+// Ghost pressure on a fluid particle after its fluid neighbours
+// (fluid3.rs:1074-1084): the ghosts of the 27 cells around it at the raw
+// smoothing length (boundary_near, :860-880: oz -> oy -> ox, ascending ghost
+// index, d2 < h * h), each f -= grad W * m_b (q_i + q_b).
+__device__ __forceinline__ void ghost_force(const SceneDev &s, const GhostDev &g, const float4 &pi, float q_i,
+                                            float &fx, float &fy, float &fz) {
+    int32_t cx, cy, cz;
+    cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
+    const float h = s.h;
+    for (int oz = -1; oz <= 1; ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int64_t x = (int64_t)cx + ox, y = (int64_t)cy + oy, z = (int64_t)cz + oz;
+                if (x < INT32_MIN || x > INT32_MAX || y < INT32_MIN || y > INT32_MAX || z < INT32_MIN ||
+                    z > INT32_MAX)
+                    continue;
+                uint32_t b, e;
+                if (!ghost_cell(g, s.ghost_mask, (int32_t)x, (int32_t)y, (int32_t)z, b, e)) continue;
+                for (uint32_t k = b; k < e; ++k) {
+                    const float4 gp = g.pos[k];
+                    const float d2 = dot3(gp.x - pi.x, gp.y - pi.y, gp.z - pi.z);
+                    if (!(d2 < s.h_grid2)) continue;
+                    // delta = x_i - x_b; |delta|^2 is d2 exactly
+                    const float ex = pi.x - gp.x, ey = pi.y - gp.y, ez = pi.z - gp.z;
+                    const float dist = sqrtf(d2);
+                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                    if (!(dist >= h || dist < PH_EPS)) {
+                        const float sc = s.spiky_c * (h - dist) * (h - dist);
+                        gx = ex / dist * sc;
+                        gy = ey / dist * sc;
+                        gz = ez / dist * sc;
+                    }
+                    const float sm = gp.w * (q_i + g.state[k].z);
+                    fx = fx - gx * sm;
+                    fy = fy - gy * sm;
+                    fz = fz - gz * sm;
+                }
+            }
 }
 

@@ -63,7 +63,11 @@ hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev
                                   const BodyDev *bodies, const uint32_t *start, const float4 *posm,
                                   const float4 *vel, const float2 *rq, const uint32_t *tag, const uint32_t *masks,
                                   const uint32_t *flags, uint32_t mstride, float4 *posm_out, float4 *vel_out,
-                                  float *rec_acc, int32_t *bbox, float4 *acc_sorted = nullptr);
+                                  float *rec_acc, int32_t *bbox, float4 *acc_sorted = nullptr,
+                                  const GhostDev &gh = GhostDev{});
+// Boundary ghosts' density, pressure and q from the gathered fluid state.
+hipError_t launch_ghost_density(hipStream_t st, const SceneDev *scenes, uint32_t scene, const uint32_t *start,
+                                const float4 *posm, const GhostDev &gh, uint32_t count);
 hipError_t launch_vmax(hipStream_t st, uint32_t n, const float4 *vel, uint32_t *out_bits);
 hipError_t launch_export(hipStream_t st, uint32_t n, const float4 *posm, const float4 *vel, const uint32_t *tag,
                          uint32_t id_base, float *out6);

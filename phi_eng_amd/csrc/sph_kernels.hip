@@ -483,7 +483,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
     const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ masks,
     const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox,
-    float4 *__restrict__ acc_sorted) {
+    float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}) {
     const uint32_t tile = xcd_tile(bid, nbl);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
@@ -552,6 +552,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
                 });
             }
         }
+        if (s.ghost_count) ghost_force(s, gh, pi, q_i, acc.fx, acc.fy, acc.fz);
         // fluid3.rs:1095 acc = g + F / rho (the second division by rho).
         const float ax = s.gx + acc.fx / rho_i, ay = s.gy + acc.fy / rho_i, az = s.gz + acc.fz / rho_i;
         float ux = vi.x + ax * s.dt, uy = vi.y + ay * s.dt, uz = vi.z + az * s.dt;
@@ -625,8 +626,36 @@ __global__ __launch_bounds__(PH_TILE) void k_force_integrate(const TileDev *__re
     const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
     const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ masks,
     const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
-    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, float4 *__restrict__ acc_sorted) {
-    dev_force_integrate(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted);
+    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, float4 *__restrict__ acc_sorted,
+    GhostDev gh) {
+    dev_force_integrate(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh);
+}
+
+// Boundary ghost density and pressure from the fluid (fluid3.rs:906-920):
+// the fluid neighbours of each ghost (fluid_near, :883-903: the 27 cells at
+// the raw smoothing length, reference order, d2 < h * h), rho = sum m W,
+// stored clamped; the pressure takes the unclamped sum and the config's
+// base rest density.  One thread per ghost (cell-sorted slot).
+__global__ __launch_bounds__(256) void k_ghost_density(const SceneDev *__restrict__ scenes, uint32_t scene,
+                                                       const uint32_t *__restrict__ start,
+                                                       const float4 *__restrict__ posm, GhostDev gh,
+                                                       uint32_t count) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= count) return;
+    const SceneDev s = scenes[scene];
+    const float4 gp = gh.pos[k];
+    int32_t cx, cy, cz;
+    cell_of(gp.x, gp.y, gp.z, s.h_grid, cx, cy, cz);
+    float rho = 0.0f;
+    for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
+        const float4 pj = posm[j];
+        const float d2 = dot3(pj.x - gp.x, pj.y - gp.y, pj.z - gp.z);
+        if (d2 < s.h_grid2) rho = rho + pj.w * poly6(d2, s);
+    });
+    const float rc = rclampf(rho, 0.0f, s.rho_max);
+    const float p = eos(rho, s.rest_density, s.stiffness, s.gamma);
+    const float rb = rmaxf(rc, PH_EPS);
+    gh.state[k] = make_float4(rc, p, p / (rb * rb), 0.0f);
 }
 
 
@@ -916,9 +945,16 @@ hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev
                                   const BodyDev *bodies, const uint32_t *start, const float4 *posm,
                                   const float4 *vel, const float2 *rq, const uint32_t *tag, const uint32_t *masks,
                                   const uint32_t *flags, uint32_t mstride, float4 *posm_out, float4 *vel_out,
-                                  float *rec_acc, int32_t *bbox, float4 *acc_sorted) {
+                                  float *rec_acc, int32_t *bbox, float4 *acc_sorted, const GhostDev &gh) {
     hipLaunchKernelGGL(k_force_integrate, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm,
-                       vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted);
+                       vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh);
+    return hipGetLastError();
+}
+hipError_t launch_ghost_density(hipStream_t st, const SceneDev *scenes, uint32_t scene, const uint32_t *start,
+                                const float4 *posm, const GhostDev &gh, uint32_t count) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ghost_density, dim3((count + 255u) / 256u), dim3(256), 0, st, scenes, scene, start, posm, gh,
+                       count);
     return hipGetLastError();
 }
 hipError_t launch_vmax(hipStream_t st, uint32_t n, const float4 *vel, uint32_t *out_bits) {

@@ -137,6 +137,86 @@ struct World3 {
     uint64_t n_col_skipped = 0;
     // first error of a step since the last sync (steps return nothing)
     std::string step_error;
+    // boundary ghosts (fluid3.rs:1018-1027): generation-order copy, the
+    // fingerprint they were made for, and how often they were made
+    std::vector<float4> ghost_gen;
+    uint64_t ghost_fp = 0, ghost_regens = 0;
+
+    // boundary_fingerprint (fluid3.rs:660-714, fluid.rs:617-659): FNV-1a
+    // over the kernel config and every static body's transform and shape
+    // (identity rotation: quaternion (0, 0, 0, 1), or the 2D matrix columns).
+    uint64_t boundary_fingerprint() const {
+        uint64_t hsh = 1469598103934665603ull;
+        auto mix = [&](const void *p, size_t k) {
+            const uint8_t *b = static_cast<const uint8_t *>(p);
+            for (size_t i = 0; i < k; ++i) {
+                hsh ^= b[i];
+                hsh *= 1099511628211ull;
+            }
+        };
+        auto mixf = [&](float v) { mix(&v, 4); };  // to_le_bytes on a little-endian host
+        auto mixb = [&](uint8_t v) { mix(&v, 1); };
+        mixf(cfg.smoothing_length);
+        mixf(cfg.particle_mass);
+        mixf(cfg.rest_density);
+        for (const BodyDev &b : bodies) {
+            mixf(b.px);
+            mixf(b.py);
+            if (dims == 2) {
+                mixf(1.0f), mixf(0.0f), mixf(0.0f), mixf(1.0f);
+                if (b.shape == 0u) {
+                    mixb(1);
+                    mixf(b.a);
+                } else {
+                    mixb(2);
+                    mixf(b.a);
+                    mixf(b.b);
+                }
+                continue;
+            }
+            mixf(b.pz);
+            mixf(0.0f), mixf(0.0f), mixf(0.0f), mixf(1.0f);
+            if (b.shape == 0u) {
+                mixb(1);
+                mixf(b.a);
+            } else if (b.shape == 1u) {
+                mixb(2);
+                mixf(b.a);
+                mixf(b.b);
+                mixf(b.c);
+            } else {
+                mixb(3);
+                mixf(b.a);
+                mixf(b.b);
+            }
+        }
+        return hsh;
+    }
+
+    // Per substep, as the reference (fluid3.rs:1018-1025): regenerate the
+    // ghosts when the fingerprint changed or there are none; with the option
+    // off the ghosts are kept but unused.
+    void update_ghosts() {
+        SceneDev &s = eng.scenes[0];
+        uint32_t want_count = 0, want_mask = 0;
+        if (cfg.boundary_particles) {
+            const uint64_t fp = boundary_fingerprint();
+            if (fp != ghost_fp || ghost_gen.empty()) {
+                ph::make_ghosts(scene_host(s.dt), ghost_gen);
+                eng.set_ghosts(ghost_gen, std::max(cfg.smoothing_length, 1e-6f));
+                ghost_fp = fp;
+                ++ghost_regens;
+            }
+            want_count = eng.g_count;
+            want_mask = eng.g_mask;
+        }
+        if (s.ghost_count != want_count || s.ghost_mask != want_mask) {
+            s.ghost_count = want_count;
+            s.ghost_mask = want_mask;
+            eng.scenes_dirty = true;
+            eng.upload_tables();
+        }
+    }
 
     ph::SceneHost scene_host(float dt) const {
         ph::SceneHost h;
@@ -224,6 +304,7 @@ struct World3 {
             const float sdt = dt / (float)sub;
             if (sub != 1) sync_scene(sdt);
             for (uint32_t k = 0; k < sub; ++k) {
+                if (cfg.boundary_particles || eng.scenes[0].ghost_count) update_ghosts();
                 if (!sur && !col) {
                     eng.substep();
                     continue;
@@ -562,15 +643,19 @@ int32_t phieng_world3_add_body(World3 *world, float px, float py, float pz, uint
 int32_t phieng_world3_set_fluid_config(World3 *world, const FluidConfig3Ffi *cfg) {
     if (!world || !cfg) return 0;
     if (!config_valid(*cfg)) return 0;
-    if (cfg->adaptive_resolution || cfg->boundary_particles) {
+    if (cfg->adaptive_resolution) {
+        set_error("phieng_world3_set_fluid_config", "adaptive_resolution is not implemented on the GPU path");
+        return 0;
+    }
+    if (cfg->boundary_particles && world->sur) {
         set_error("phieng_world3_set_fluid_config",
-                  "adaptive_resolution and boundary_particles are not implemented on the GPU path");
+                  "boundary_particles with the learned surrogate attached: the surrogate does not model ghosts");
         return 0;
     }
     FluidConfig3Ffi c = *cfg;
     // Rust bools read back as 0/1.
     c.adaptive_resolution = 0;
-    c.boundary_particles = 0;
+    c.boundary_particles = cfg->boundary_particles ? 1 : 0;
     c.enabled = cfg->enabled ? 1 : 0;
     world->cfg = c;
     return 1;
@@ -737,6 +822,10 @@ int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check
                                     int32_t use_network) {
     if (!world) return 0;
     if (s && (check_every < 1 || !std::isfinite(max_density_error) || max_density_error <= 0.0f)) return 0;
+    if (s && world->cfg.boundary_particles) {
+        set_error("phieng_world3_set_surrogate", "the surrogate does not model boundary ghosts (boundary_particles)");
+        return 0;
+    }
     world->sur = s;
     world->check_every = check_every ? check_every : 1;
     world->max_err = max_density_error;
@@ -784,6 +873,47 @@ int32_t phieng_world3_surrogate_stats(const World3 *world, uint64_t *sur_steps, 
     if (fallbacks) *fallbacks = fb;
     if (last_density_error) *last_density_error = err;
     return 1;
+}
+
+int32_t phieng_world3_boundary_info(const World3 *world, uint32_t *count, uint64_t *regens) {
+    if (!world) return 0;
+    if (count) *count = (uint32_t)world->ghost_gen.size();
+    if (regens) *regens = world->ghost_regens;
+    return 1;
+}
+
+uint32_t phieng_world3_boundary_get(const World3 *world, float *xyz, float *density, float *pressure,
+                                    uint32_t max) {
+    if (!world) return 0;
+    try {
+        const uint32_t g = (uint32_t)world->ghost_gen.size();
+        const uint32_t m = std::min(max, g);
+        if (m == 0) return 0;
+        const ph::Engine &e = world->eng;
+        std::vector<float4> st(e.g_count);
+        if ((density || pressure) && e.g_count) {
+            PH_CK(hipMemcpyAsync(st.data(), e.g_state.p, sizeof(float4) * e.g_count, hipMemcpyDeviceToHost,
+                                 e.stream));
+            PH_CK(hipStreamSynchronize(e.stream));
+        }
+        // cell-sorted slot -> generation index; a ghost in no cell (a
+        // non-finite one) has no fluid neighbours: density 0, pressure 0
+        std::vector<float2> rp(g, make_float2(0.0f, 0.0f));
+        for (uint32_t k = 0; k < e.g_count; ++k) rp[e.g_order[k]] = make_float2(st[k].x, st[k].y);
+        for (uint32_t k = 0; k < m; ++k) {
+            if (xyz) {
+                xyz[3 * k] = world->ghost_gen[k].x;
+                xyz[3 * k + 1] = world->ghost_gen[k].y;
+                xyz[3 * k + 2] = world->ghost_gen[k].z;
+            }
+            if (density) density[k] = rp[k].x;
+            if (pressure) pressure[k] = rp[k].y;
+        }
+        return m;
+    } catch (const std::exception &ex) {
+        set_error("phieng_world3_boundary_get", ex.what());
+        return 0;
+    }
 }
 
 int32_t phieng_world3_sync(World3 *world) {
@@ -1023,5 +1153,26 @@ uint32_t phieng_world_fluid_get_record(const World2 *world, int32_t *cell2, uint
 }
 
 int32_t phieng_world_sync(World2 *world) { return world ? phieng_world3_sync(as3(world)) : 0; }
+
+int32_t phieng_world_boundary_info(const World2 *world, uint32_t *count, uint64_t *regens) {
+    return world ? phieng_world3_boundary_info(as3(world), count, regens) : 0;
+}
+
+uint32_t phieng_world_boundary_get(const World2 *world, float *xy, float *density, float *pressure, uint32_t max) {
+    if (!world) return 0;
+    try {
+        const uint32_t m = std::min<uint32_t>(max, (uint32_t)world->ghost_gen.size());
+        std::vector<float> xyz(xy ? 3 * (size_t)m : 0);
+        const uint32_t got = phieng_world3_boundary_get(as3(world), xy ? xyz.data() : nullptr, density, pressure, m);
+        for (uint32_t k = 0; xy && k < got; ++k) {
+            xy[2 * k] = xyz[3 * k];
+            xy[2 * k + 1] = xyz[3 * k + 1];
+        }
+        return got;
+    } catch (const std::exception &e) {
+        set_error("phieng_world_boundary_get", e.what());
+        return 0;
+    }
+}
 
 }  // extern "C"

@@ -136,6 +136,23 @@ class World3:
             raise RuntimeError(f"get_record returned {got}: {last_error()}")
         return dict(cells=cells, order=order, density=rho, pressure=p, accel=acc, count=cnt)
 
+    # -- boundary ghosts ---------------------------------------------------
+    def boundary_info(self):
+        c, r = ctypes.c_uint32(), ctypes.c_uint64()
+        self._l.phieng_world3_boundary_info(self.ptr, ctypes.byref(c), ctypes.byref(r))
+        return c.value, r.value
+
+    def boundary(self):
+        """Ghosts in generation order: dict(pos (G, 3), density, pressure)."""
+        g, _ = self.boundary_info()
+        xyz = np.zeros((g, 3), np.float32)
+        rho = np.zeros(g, np.float32)
+        p = np.zeros(g, np.float32)
+        got = self._l.phieng_world3_boundary_get(self.ptr, _fptr(xyz), _fptr(rho), _fptr(p), g)
+        if got != g:
+            raise RuntimeError(f"boundary_get returned {got} of {g}: {last_error()}")
+        return dict(pos=xyz, density=rho, pressure=p)
+
     # -- learned pressure term ---------------------------------------------
     def set_surrogate(self, sur, check_every=10, max_density_error=0.01, use_network=True):
         ok = self._l.phieng_world3_set_surrogate(self.ptr, sur.ptr if sur is not None else None,
@@ -283,6 +300,22 @@ class World2:
         if got != n:
             raise RuntimeError(f"get_record returned {got}: {last_error()}")
         return dict(cells=cells, order=order, density=rho, pressure=p, accel=acc, count=cnt)
+
+    def boundary_info(self):
+        c, r = ctypes.c_uint32(), ctypes.c_uint64()
+        self._l.phieng_world_boundary_info(self.ptr, ctypes.byref(c), ctypes.byref(r))
+        return c.value, r.value
+
+    def boundary(self):
+        """Ghosts in generation order: dict(pos (G, 2), density, pressure)."""
+        g, _ = self.boundary_info()
+        xy = np.zeros((g, 2), np.float32)
+        rho = np.zeros(g, np.float32)
+        p = np.zeros(g, np.float32)
+        got = self._l.phieng_world_boundary_get(self.ptr, _fptr(xy), _fptr(rho), _fptr(p), g)
+        if got != g:
+            raise RuntimeError(f"boundary_get returned {got} of {g}: {last_error()}")
+        return dict(pos=xy, density=rho, pressure=p)
 
 
 BATCH_KERNELS, BATCH_PERSISTENT, BATCH_PERSISTENT_GRAPH = 0, 1, 2

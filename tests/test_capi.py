@@ -86,12 +86,18 @@ def test_phase_tables_validated():
 
 
 def test_unimplemented_options_rejected():
+    """adaptive_resolution is refused (not built); boundary_particles is
+    accepted and reads back as a Rust bool."""
     with World3() as w:
-        for f in ("adaptive_resolution", "boundary_particles"):
-            c = FluidConfig3Ffi.default()
-            setattr(c, f, 1)
-            assert not w.set_fluid_config(c)
-            assert f in ffi.last_error()
+        c = FluidConfig3Ffi.default()
+        c.adaptive_resolution = 1
+        assert not w.set_fluid_config(c)
+        assert "adaptive_resolution" in ffi.last_error()
+        c = FluidConfig3Ffi.default()
+        c.boundary_particles = 7
+        assert w.set_fluid_config(c)
+        assert w.get_fluid_config().boundary_particles == 1
+        assert w.boundary_info() == (0, 0)  # no ghosts before a step
 
 
 def test_null_tolerance():

@@ -40,10 +40,13 @@ def test_invalid_config_rejected_world_unchanged(field, value):
 
 def test_unimplemented_options_rejected():
     with World2() as w:
-        for f in ("adaptive_resolution", "boundary_particles"):
-            c = FluidConfig3Ffi.default()
-            setattr(c, f, 1)
-            assert not w.set_fluid_config(c)
+        c = FluidConfig3Ffi.default()
+        c.adaptive_resolution = 1
+        assert not w.set_fluid_config(c)
+        c = FluidConfig3Ffi.default()
+        c.boundary_particles = 1
+        assert w.set_fluid_config(c)
+        assert w.boundary_info() == (0, 0)
 
 
 def test_add_body_rules():

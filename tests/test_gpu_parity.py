@@ -95,6 +95,31 @@ def test_outliers_and_negative_cells(gpu, oracle_lib):
     _compare_step(w, sc, oracle_lib, "outliers")
 
 
+def test_cfl_substeps_follow_oracle(gpu, oracle_lib):
+    """CFL substepping (fluid3.rs:591-636) on the GPU: the substep count comes
+    from the device speed maximum, each substep is dt / n; three steps of
+    the world against the oracle's Jacobi run of the same steps."""
+    O = oracle_lib
+    sc = scenes.small_dam(10, seed=16)
+    sc.config.stiffness = 0.02  # c0 = sqrt(7 k rho0) ~ 11.8 m/s: a few substeps at dt = 6e-3
+    sc.config.cfl = 0.3
+    sc.config.max_substeps = 8
+    dt = 6e-3
+    w = scenes.make_world(sc)
+    pos0, vel0 = w.get_state()
+    for _ in range(3):
+        w.step(dt)
+    pos, vel = w.get_state()
+    rp, rv, sub = O.run_f32(pos0, vel0, sc.config, sc.gravity, O.bodies_from_boxes(sc.boxes), dt, 3, jacobi=True)
+    dx = np.abs(pos.astype(np.float64) - rp).max()
+    dv = np.abs(vel.astype(np.float64) - rv).max() / max(np.abs(rv).max(), 1.0)
+    print(f"cfl: oracle substeps in the last step {sub}, max |dx| {dx:.3e}, vel rel {dv:.3e}, "
+          f"bit-equal {np.mean(pos.view(np.uint32) == rp.view(np.uint32)):.4f}", flush=True)
+    assert 2 <= sub <= 8
+    assert dx <= REL * max(1.0, np.abs(rp).max()) and dv <= REL
+    w.close()
+
+
 def test_run_twice_bit_identical(gpu):
     """fluid3.rs:1398-1416 determinism, on the GPU path."""
     sc = scenes.small_dam(10, seed=15)

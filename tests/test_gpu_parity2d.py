@@ -157,6 +157,34 @@ def test_outliers_and_negative_cells(gpu):
     w.close()
 
 
+def test_cfl_substeps_follow_oracle(gpu):
+    """CFL substepping (fluid.rs:541-589) on the 2D path against the oracle's
+    Jacobi run of the same three steps."""
+    sc, cfg, mass, bodies = _config1()
+    sc.config["stiffness"] = 0.02  # c0 ~ 11.8 m/s: a few substeps at dt = 6e-3
+    cfgg = scenes.scene2_config(sc)
+    cfgg.cfl = 0.3
+    cfgg.max_substeps = 8
+    cfg.stiffness = 0.02
+    cfg.cfl = 0.3
+    cfg.max_substeps = 8
+    w = World2()
+    assert w.set_fluid_config(cfgg)
+    for c, he in sc.boxes:
+        assert w.add_static_box(c, he) >= 0
+    assert w.add_particles(sc.positions) == sc.n
+    for _ in range(3):
+        w.step(6e-3)
+    pos, vel = w.get_state()
+    ref = O.run2_f32(sc.positions, np.zeros_like(sc.positions), mass, cfg, sc.gravity, bodies, 6e-3, 3,
+                     jacobi=True)
+    dx = np.abs(pos.astype(np.float64) - ref["pos"]).max()
+    print(f"2D cfl: max |dx| {dx:.3e}, bit-equal {np.mean(pos.view(np.uint32) == ref['pos'].view(np.uint32)):.4f}",
+          flush=True)
+    assert dx <= REL * max(1.0, np.abs(ref["pos"]).max())
+    w.close()
+
+
 def test_reference_dam_break_contained(gpu):
     """fluid.rs:1225-1256 on the GPU: an 8 x 12 column, default config,
     600 steps at 60 Hz stays finite, collapses and stays in the tank."""
