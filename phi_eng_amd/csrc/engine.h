@@ -107,7 +107,17 @@ struct Engine {
     size_t alloc_particles = 0;
     DBuf<uint32_t> keys, slots, tmp;
     DBuf<float2> rq;
-    DBuf<uint32_t> masks, flags;  // per-row neighbour bit masks (9 x N) + row flags
+    DBuf<uint32_t> masks, flags;  // per-row neighbour bit masks (9 x N) + row flags (persistent step)
+    // Neighbour list of the last density pass: the candidates within
+    // (1 + list_skin) h in reference order, four slots per 16-byte group
+    // (group g of slot i at uint4 g * alloc_particles + i, SL_CAP / 4 groups
+    // plus one spare row), the count word with its SLF_ flags, and (when
+    // list_cells) each slot's cell.  The force pass reads it; a surrogate
+    // substep reuses it with skin > 0.
+    DBuf<uint32_t> nl_cnt, nl_idx;
+    DBuf<int4> nl_cell;
+    float list_skin = 0.0f;
+    bool list_cells = false;
     // bucket tables
     DBuf<uint32_t> counts, start, partial;
     size_t alloc_buckets = 0;
@@ -167,8 +177,8 @@ struct Engine {
     // sort_phase + plain_phase + finish_substep.
     void substep();
     void sort_phase(uint32_t *perm_out);
-    // With lo, the density pass also writes the surrogate's skin list.
-    void plain_phase(float4 *acc_sorted, const ListOut *lo = nullptr);
+    // Density pass (writing the neighbour list), boundary ghosts, force pass.
+    void plain_phase(float4 *acc_sorted);
     void finish_substep();
     // After an exception inside a substep: drop a stale launch error and
     // make the generation selectors match what the device holds.  Counts

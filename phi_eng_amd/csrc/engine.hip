@@ -304,6 +304,9 @@ void Engine::ensure_particles(uint32_t cap) {
     rq.ensure(want);
     masks.ensure(9 * want);
     flags.ensure(want);
+    nl_cnt.ensure(want);
+    nl_idx.ensure((size_t)(SL_CAP + 4) * want);
+    if (list_cells) nl_cell.ensure(want);
     alloc_particles = want;
     if (rec_flags) ensure_record();
 }
@@ -482,18 +485,20 @@ void Engine::sort_phase(uint32_t *perm_out) {
 }
 
 // K4 + K5 of the plain step on generation 1-cs, writing generation cs.
-void Engine::plain_phase(float4 *acc_sorted, const ListOut *lo) {
+void Engine::plain_phase(float4 *acc_sorted) {
     const uint32_t nt = (uint32_t)tiles.size();
     const bool rd = rec_flags & PHIENG_RECORD_DENSITY, ra = rec_flags & PHIENG_RECORD_ACCEL;
     const int ns = 1 - cs, nt_ = 1 - ct;
+    if (list_cells) nl_cell.ensure(alloc_particles);
+    ListOut lo;
+    lo.cell = list_cells ? nl_cell.p : nullptr;
+    lo.cnt = nl_cnt.p;
+    lo.idx = nl_idx.p;
+    lo.skin_factor2 = (1.0f + list_skin) * (1.0f + list_skin);
     run(KIND_DENSITY, [&] {
-        if (lo)
-            return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p,
-                                       masks.p, flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
-                                       rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr, *lo);
-        return launch_density(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p, masks.p,
-                              flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr, rd ? rec_p.p : nullptr,
-                              rd ? rec_count.p : nullptr);
+        return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p, masks.p,
+                                   flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
+                                   rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr, lo);
     });
     // boundary ghosts (fluid3.rs:1018-1027): their density and pressure from
     // the fluid of this substep, before the force pass reads them
@@ -503,10 +508,9 @@ void Engine::plain_phase(float4 *acc_sorted, const ListOut *lo) {
             return launch_ghost_density(stream, d_scenes.p, 0u, start.p, posm[ns].p, ghost_dev(), g_count);
         });
     run(KIND_FORCE, [&] {
-        return launch_force_integrate(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p,
-                                      vel[ns].p, rq.p, tag[nt_].p, masks.p, flags.p, (uint32_t)alloc_particles,
-                                      posm[cs].p, vel[cs].p, ra ? rec_acc.p : nullptr, d_bbox.p, acc_sorted,
-                                      ghosts ? ghost_dev() : GhostDev{});
+        return launch_force_list(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p, vel[ns].p, rq.p,
+                                 tag[nt_].p, nl_cnt.p, nl_idx.p, (uint32_t)alloc_particles, posm[cs].p, vel[cs].p,
+                                 ra ? rec_acc.p : nullptr, d_bbox.p, acc_sorted, ghosts ? ghost_dev() : GhostDev{});
     });
     gen_open = true;
 }

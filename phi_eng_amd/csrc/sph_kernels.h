@@ -65,6 +65,12 @@ hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev
                                   const uint32_t *flags, uint32_t mstride, float4 *posm_out, float4 *vel_out,
                                   float *rec_acc, int32_t *bbox, float4 *acc_sorted = nullptr,
                                   const GhostDev &gh = GhostDev{});
+// The force pass over the density pass's neighbour list (ListOut cnt/idx).
+hipError_t launch_force_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                             const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
+                             const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
+                             uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
+                             float4 *acc_sorted, const GhostDev &gh);
 // Boundary ghosts' density, pressure and q from the gathered fluid state.
 hipError_t launch_ghost_density(hipStream_t st, const SceneDev *scenes, uint32_t scene, const uint32_t *start,
                                 const float4 *posm, const GhostDev &gh, uint32_t count);

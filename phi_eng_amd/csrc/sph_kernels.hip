@@ -259,7 +259,10 @@ __global__ __launch_bounds__(PH_TILE) void k_rank_gather(const TileDev *__restri
 // for surrogate substeps (candidates within (1 + skin) h, skin_factor2 =
 // (1 + skin)^2; the layout k_sur_list documents); the density results are
 // the same either way.
-template <bool LIST>
+// LIST: write the neighbour list instead of the row masks; SKIN: the list
+// radius is (1 + skin) h > h (else the list is exactly the neighbours and
+// reuses the cut-off test's masks).
+template <bool LIST, bool SKIN = LIST>
 __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles,
                                                      const SceneDev *__restrict__ scenes,
                                                      const uint32_t *__restrict__ start,
@@ -290,11 +293,11 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
         if (p == 3u && lk < SL_CAP) lgrp[(size_t)(lk >> 2) * mstride] = make_uint4(q0, q1, q2, q3);
         ++lk;
     };
-    const float hs2 = LIST ? s.h_grid2 * lo.skin_factor2 : 0.0f;
+    const float hs2 = SKIN ? s.h_grid2 * lo.skin_factor2 : 0.0f;
     if (finite3(pi.x, pi.y, pi.z)) {
         int32_t cx, cy, cz;
         cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
-        if (LIST) lo.cell[i] = make_int4(cx, cy, cz, 0);
+        if (LIST && lo.cell) lo.cell[i] = make_int4(cx, cy, cz, 0);
         const float hg2 = s.h_grid2;
         if (!needs_cell_path(s, start, cx, cy, cz)) {
 #pragma unroll 1
@@ -314,7 +317,7 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                     const float d2 = dot3(q2.x - pi.x, q2.y - pi.y, q2.z - pi.z);
                     const float d3 = dot3(q3.x - pi.x, q3.y - pi.y, q3.z - pi.z);
                     m |= ((d0 < hg2 ? 1u : 0u) | (d1 < hg2 ? 2u : 0u) | (d2 < hg2 ? 4u : 0u) | (d3 < hg2 ? 8u : 0u)) << o;
-                    if (LIST)
+                    if (SKIN)
                         ml |= ((d0 < hs2 ? 1u : 0u) | (d1 < hs2 ? 2u : 0u) | (d2 < hs2 ? 4u : 0u) |
                                (d3 < hs2 ? 8u : 0u)) << o;
                 }
@@ -322,9 +325,10 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                     const float4 pj = posm[j];
                     const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
                     m |= (r2 < hg2 ? 1u : 0u) << (j - br);
-                    if (LIST) ml |= (r2 < hs2 ? 1u : 0u) << (j - br);
+                    if (SKIN) ml |= (r2 < hs2 ? 1u : 0u) << (j - br);
                 }
-                masks[(size_t)r * mstride + i] = m;
+                if (LIST && !SKIN) ml = m;
+                if (!LIST) masks[(size_t)r * mstride + i] = m;
                 if (LIST) {
                     while (ml) {
                         const uint32_t k = __ffs(ml) - 1;
@@ -349,7 +353,7 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                             ++cnt;
                             rho += pj.w * poly6(r2, s);
                         }
-                        if (LIST && r2 < hs2) {
+                        if (LIST && (SKIN ? r2 < hs2 : r2 < hg2)) {
                             push(j);
                         }
                     }
@@ -376,7 +380,7 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
         fl = PH_SLOW_PATH;  // no neighbours: the force pass skips the sweep
         if (LIST) lo.cnt[i] = SLF_NONE;
     }
-    flags[i] = fl;
+    if (!LIST) flags[i] = fl;
     rho = rclampf(rho, 0.0f, s.rho_max);
     const uint32_t tg = tag[i];
     // Indexed through global memory: a lane-varying index into the scene
@@ -405,6 +409,7 @@ __global__ __launch_bounds__(PH_TILE) void k_density(const TileDev *__restrict__
                        rec_p, rec_count);
 }
 
+template <bool SKIN>
 __global__ __launch_bounds__(PH_TILE) void k_density_list(const TileDev *__restrict__ tiles,
                                                           const SceneDev *__restrict__ scenes,
                                                           const uint32_t *__restrict__ start,
@@ -414,8 +419,8 @@ __global__ __launch_bounds__(PH_TILE) void k_density_list(const TileDev *__restr
                                                           uint32_t mstride, float *__restrict__ rec_rho,
                                                           float *__restrict__ rec_p, uint32_t *__restrict__ rec_count,
                                                           ph::ListOut lo) {
-    dev_density<true>(blockIdx.x, gridDim.x, tiles, scenes, start, posm, tag, rq, masks, flags, mstride, rec_rho,
-                      rec_p, rec_count, lo);
+    dev_density<true, SKIN>(blockIdx.x, gridDim.x, tiles, scenes, start, posm, tag, rq, masks, flags, mstride, rec_rho,
+                            rec_p, rec_count, lo);
 }
 
 // ---------------------------------------------------------------- K5
@@ -475,15 +480,22 @@ __device__ __forceinline__ void pair_terms(const SceneDev &s, const float4 &pi, 
     }
 }
 
-// Force + XSPH over the neighbours the density pass marked (same state,
+// Force + XSPH over the neighbours the density pass found (same state,
 // same test), then integrate, speed cap, finite guards and the boundary
-// clamp of this particle (fluid3.rs:1088-1179, Jacobi).
+// clamp of this particle (fluid3.rs:1088-1179, Jacobi).  The neighbours come
+// from the density pass's per-row bit masks, or (LIST) from its neighbour
+// list: the candidates within (1 + skin) h in reference order, four per
+// 16-byte group (group g of slot i at lgrp[g * mstride + i]), re-tested
+// here against h.  The list takes one loop over a particle's own entries
+// instead of nine row loops whose trip counts differ between lanes.
+template <bool LIST>
 __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const BodyDev *__restrict__ bodies,
     const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
     const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ masks,
     const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox,
-    float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}) {
+    float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}, const uint32_t *__restrict__ lcnt = nullptr,
+    const uint4 *__restrict__ lgrp = nullptr) {
     const uint32_t tile = xcd_tile(bid, nbl);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
@@ -497,9 +509,42 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         const float rho_i = rmaxf(rqi.x, PH_EPS);
         const float q_i = rqi.y;
         PairAcc acc;
-        const uint32_t fl = flags[i];
         const float hg2 = s.h_grid2;
-        if (finite3(pi.x, pi.y, pi.z)) {
+        if (LIST) {
+            const uint32_t cw = lcnt[i];
+            // one list entry: the self entry and those outside h add nothing
+            auto entry = [&](uint32_t j, const float4 &pj, const float4 &vj, const float2 &qj) {
+                if (j == i) return;
+                const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                if (r2 < hg2) pair_terms(s, pi, vi, rho_i, q_i, pj, vj, qj, r2, acc);
+            };
+            if (cw & SLF_NONE) {
+                // non-finite position: no neighbours
+            } else if (cw & SLF_SLOW) {
+                int32_t cx, cy, cz;
+                cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
+                for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) { entry(j, posm[j], vel[j], rq[j]); });
+            } else {
+                const uint32_t n = cw & SLF_COUNT;
+                const uint4 *g = lgrp + i;
+#pragma unroll 1
+                for (uint32_t k = 0; k < n; k += 4) {
+                    const uint4 e = *g;
+                    g += mstride;
+                    // slots past the count are replaced by i (skipped)
+                    const uint32_t j0 = e.x, j1 = k + 1 < n ? e.y : i, j2 = k + 2 < n ? e.z : i,
+                                   j3 = k + 3 < n ? e.w : i;
+                    const float4 p0 = posm[j0], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
+                    const float4 v0 = vel[j0], v1 = vel[j1], v2 = vel[j2], v3 = vel[j3];
+                    const float2 q0 = rq[j0], q1 = rq[j1], q2 = rq[j2], q3 = rq[j3];
+                    entry(j0, p0, v0, q0);
+                    entry(j1, p1, v1, q1);
+                    entry(j2, p2, v2, q2);
+                    entry(j3, p3, v3, q3);
+                }
+            }
+        } else if (finite3(pi.x, pi.y, pi.z)) {
+            const uint32_t fl = flags[i];
             int32_t cx, cy, cz;
             cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
             if (!(fl & PH_SLOW_PATH)) {
@@ -628,7 +673,17 @@ __global__ __launch_bounds__(PH_TILE) void k_force_integrate(const TileDev *__re
     const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, float4 *__restrict__ acc_sorted,
     GhostDev gh) {
-    dev_force_integrate(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh);
+    dev_force_integrate<false>(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh);
+}
+
+__global__ __launch_bounds__(PH_TILE) void k_force_list(const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes,
+    const BodyDev *__restrict__ bodies, const uint32_t *__restrict__ start, const float4 *__restrict__ posm,
+    const float4 *__restrict__ vel, const float2 *__restrict__ rq, const uint32_t *__restrict__ tag,
+    const uint32_t *__restrict__ lcnt, const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
+    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, float4 *__restrict__ acc_sorted,
+    GhostDev gh) {
+    dev_force_integrate<true>(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, nullptr, nullptr,
+                              mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh, lcnt, lgrp);
 }
 
 // Boundary ghost density and pressure from the fluid (fluid3.rs:906-920):
@@ -730,7 +785,7 @@ __global__ __launch_bounds__(PH_TILE) void k_step_persistent(ph::PersistArgs a) 
                     a.mstride, a.rec_rho, a.rec_p, a.rec_count);
     if (!grid_barrier(a.bar, G, a.spin_limit)) return;
     for (uint32_t k = blockIdx.x; k < a.ntiles; k += G) {
-        dev_force_integrate(k, a.ntiles, a.tiles, a.scenes, a.bodies, a.start, a.posm_alt, a.vel_alt, a.rq,
+        dev_force_integrate<false>(k, a.ntiles, a.tiles, a.scenes, a.bodies, a.start, a.posm_alt, a.vel_alt, a.rq,
                             a.tag_alt, a.masks, a.flags, a.mstride, a.posm_cur, a.vel_cur, a.rec_acc, a.tile_bbox,
                             nullptr);
         __syncthreads();
@@ -937,8 +992,12 @@ hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *t
                                const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
                                uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
                                uint32_t *rec_count, const ListOut &lo) {
-    hipLaunchKernelGGL(k_density_list, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, rq, masks,
-                       flags, mstride, rec_rho, rec_p, rec_count, lo);
+    if (lo.skin_factor2 != 1.0f)
+        hipLaunchKernelGGL(k_density_list<true>, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag,
+                           rq, masks, flags, mstride, rec_rho, rec_p, rec_count, lo);
+    else
+        hipLaunchKernelGGL(k_density_list<false>, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag,
+                           rq, masks, flags, mstride, rec_rho, rec_p, rec_count, lo);
     return hipGetLastError();
 }
 hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
@@ -948,6 +1007,16 @@ hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev
                                   float *rec_acc, int32_t *bbox, float4 *acc_sorted, const GhostDev &gh) {
     hipLaunchKernelGGL(k_force_integrate, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm,
                        vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh);
+    return hipGetLastError();
+}
+hipError_t launch_force_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                             const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
+                             const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
+                             uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
+                             float4 *acc_sorted, const GhostDev &gh) {
+    hipLaunchKernelGGL(k_force_list, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm, vel, rq,
+                       tag, lcnt, reinterpret_cast<const uint4 *>(lidx), mstride, posm_out, vel_out, rec_acc, bbox,
+                       acc_sorted, gh);
     return hipGetLastError();
 }
 hipError_t launch_ghost_density(hipStream_t st, const SceneDev *scenes, uint32_t scene, const uint32_t *start,

@@ -127,10 +127,9 @@ struct World3 {
     hipEvent_t judge_ev = nullptr;
     bool judge_pending = false;
     uint64_t judge_at = 0;
-    // skin list of the last sort (surrogate substeps read it)
+    // skin list of the last sort (the engine's neighbour list, written with
+    // skin > 0 while a surrogate is attached; surrogate substeps read it)
     float skin = 0.05f;  // list radius (1 + skin) h
-    ph::DBuf<int4> sl_cell;
-    ph::DBuf<uint32_t> sl_cnt, sl_idx;
     ph::DBuf<float4> sl_rec[2];  // neighbour records of generation k (2 float4 per slot)
     uint64_t list_sorts = ~0ull, list_epoch = ~0ull;
     uint32_t list_stride = 0;
@@ -299,6 +298,9 @@ struct World3 {
             eng.init_device();
             (void)hipGetLastError();  // a failed call before this step has already reported
             if (!domain_dirty) eng.poll_monitor(n);
+            // plain substeps leave a skin list for surrogate substeps
+            eng.list_skin = sur ? skin : 0.0f;
+            eng.list_cells = sur != nullptr;
             sync_scene(dt);
             const uint32_t sub = cfl_substeps(dt);
             const float sdt = dt / (float)sub;
@@ -378,7 +380,7 @@ struct World3 {
     }
 
     bool list_usable() const {
-        return sl_cell.p && sl_rec[0].p && list_sorts == eng.sorts_done && list_epoch == eng.layout_epoch &&
+        return eng.nl_cell.p && sl_rec[0].p && list_sorts == eng.sorts_done && list_epoch == eng.layout_epoch &&
                list_stride == eng.alloc_particles;
     }
 
@@ -398,20 +400,10 @@ struct World3 {
             ++n_col_skipped;
         }
         if (collect) col->grow_samples((uint64_t)col->nsamples + cnt);
-        ph::ListOut lo;
-        if (list) {
-            sl_cell.ensure(e.alloc_particles);
-            sl_cnt.ensure(e.alloc_particles);
-            sl_idx.ensure((size_t)(SL_CAP + 4) * e.alloc_particles);  // + a spare group row for the sweep's look-ahead
-            lo.cell = sl_cell.p;
-            lo.cnt = sl_cnt.p;
-            lo.idx = sl_idx.p;
-            lo.skin_factor2 = (1.0f + skin) * (1.0f + skin);
-        }
         e.sort_phase(e.perm.p);
         e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_gather_aux(e.stream, n, e.perm.p, e.aux[e.cs].p, e.aux[ns].p); });
-        e.plain_phase(collect ? e.acc_sorted.p : nullptr, list ? &lo : nullptr);
-        if (list) {
+        e.plain_phase(collect ? e.acc_sorted.p : nullptr);
+        if (list && e.list_cells && e.list_skin == skin) {
             // the skin list of the order this substep sorted into
             list_sorts = e.sorts_done;
             list_epoch = e.layout_epoch;
@@ -477,9 +469,9 @@ struct World3 {
         a.vel = e.vel[e.cs].p;
         a.aux = e.aux[e.cs].p;
         a.tag = e.tag[e.ct].p;
-        a.cell = sl_cell.p;
-        a.cnt = sl_cnt.p;
-        a.idx = sl_idx.p;
+        a.cell = e.nl_cell.p;
+        a.cnt = e.nl_cnt.p;
+        a.idx = e.nl_idx.p;
         a.mstride = list_stride;
         a.rec = sl_rec[e.cs].p;
         a.rec_out = sl_rec[1 - e.cs].p;
