@@ -83,7 +83,7 @@ def bench_surrogate(a, plain_value):
     res.update({
         "plain_ms_per_step_same_steps": round(runs["plain"]["wall"] * 1e3 / S, 4),
         "speedup_vs_plain_headline": round(res["value"] / plain_value, 4),
-        "steps": S, "check_every": 10, "max_density_error": 0.01, "skin": 0.05,
+        "steps": S, "check_every": 10, "max_density_error": 0.01, "skin": "Verlet: max(0.01, 4 check_every dt vmax / h)",
         "network_free": out["network_free"],
         "train": train,
         "data": "synthetic: seeded jittered lattice dam breaks; network trained in this run",
@@ -234,7 +234,7 @@ def bench_scale(a):
            "data": "synthetic: two seeded jittered lattice columns; network trained in this run",
            "config": {"workload": "two_columns_3d_8m", "particles": n, "lattice": [[64, 256, 256], [64, 256, 256]],
                       "tank": [12.8, 12.8, 12.8], "dt": sc.dt, "stiffness": sc.config.stiffness,
-                      "path": "surrogate", "check_every": 10, "max_density_error": 0.01, "skin": 0.05},
+                      "path": "surrogate", "check_every": 10, "max_density_error": 0.01, "skin": "Verlet, min 0.01"},
            "plain": {"value": round(pv, 1), "ms_per_step": round(runs["plain"]["wall"] * 1e3 / steps, 4)},
            "speedup_vs_plain": round(sv / pv, 4), "density_err_mean": float(e.mean()),
            "density_err_p99": float(np.percentile(e, 99)),

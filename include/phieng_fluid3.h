@@ -242,9 +242,11 @@ int32_t phieng_world3_collect_samples(World3 *world, Surrogate3 *s, uint32_t eve
  * 1 / 0. */
 int32_t phieng_world3_surrogate_stats(const World3 *world, uint64_t *sur_steps, uint64_t *plain_steps,
                                       uint64_t *fallbacks, float *last_density_error);
-/* Skin of the surrogate's neighbour list: surrogate substeps between two
- * plain substeps see the candidates that were within (1 + skin) h at the
- * plain one (default 0.05; 0 <= skin <= 1).  1 / 0. */
+/* Minimum skin of the surrogate's neighbour list: surrogate substeps
+ * between two plain substeps see the candidates that were within
+ * (1 + s) h at the plain one, with the Verlet skin
+ * s = max(skin, 4 check_every dt vmax / h) from the largest particle speed
+ * at that plain substep (default skin 0.01; 0 <= skin <= 1).  1 / 0. */
 int32_t phieng_world3_set_surrogate_skin(World3 *world, float skin);
 
 #ifdef __cplusplus

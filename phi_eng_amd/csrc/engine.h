@@ -57,6 +57,14 @@ struct DBuf {
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
+    void swap(DBuf &o) {
+        T *q = p;
+        p = o.p;
+        o.p = q;
+        const size_t k = n;
+        n = o.n;
+        o.n = k;
+    }
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -118,6 +126,10 @@ struct Engine {
     DBuf<int4> nl_cell;
     float list_skin = 0.0f;
     bool list_cells = false;
+    // With list_coef > 0 the list radius is a Verlet skin from the current
+    // speeds, at least list_skin (launch_list_radius)
+    float list_coef = 0.0f;
+    DBuf<float> tile_vmax, list_f2;
     // bucket tables
     DBuf<uint32_t> counts, start, partial;
     size_t alloc_buckets = 0;
@@ -177,8 +189,9 @@ struct Engine {
     // sort_phase + plain_phase + finish_substep.
     void substep();
     void sort_phase(uint32_t *perm_out);
-    // Density pass (writing the neighbour list), boundary ghosts, force pass.
-    void plain_phase(float4 *acc_sorted);
+    // Density pass (writing the neighbour list), boundary ghosts, force pass
+    // (with af, also the surrogate's history, records and judge sums).
+    void plain_phase(float4 *acc_sorted, const AuxFuse *af = nullptr);
     void finish_substep();
     // After an exception inside a substep: drop a stale launch error and
     // make the generation selectors match what the device holds.  Counts

@@ -485,7 +485,7 @@ void Engine::sort_phase(uint32_t *perm_out) {
 }
 
 // K4 + K5 of the plain step on generation 1-cs, writing generation cs.
-void Engine::plain_phase(float4 *acc_sorted) {
+void Engine::plain_phase(float4 *acc_sorted, const AuxFuse *af) {
     const uint32_t nt = (uint32_t)tiles.size();
     const bool rd = rec_flags & PHIENG_RECORD_DENSITY, ra = rec_flags & PHIENG_RECORD_ACCEL;
     const int ns = 1 - cs, nt_ = 1 - ct;
@@ -495,6 +495,15 @@ void Engine::plain_phase(float4 *acc_sorted) {
     lo.cnt = nl_cnt.p;
     lo.idx = nl_idx.p;
     lo.skin_factor2 = (1.0f + list_skin) * (1.0f + list_skin);
+    if (list_cells && list_coef > 0.0f) {
+        tile_vmax.ensure(std::max<size_t>(nt, 1));
+        list_f2.ensure(1);
+        run(KIND_SUR_AUX, [&] {
+            return launch_list_radius(stream, nt, d_tiles.p, d_scenes.p, vel[ns].p, tile_vmax.p, list_coef, list_skin,
+                                      list_f2.p);
+        });
+        lo.skin_factor2_dev = list_f2.p;
+    }
     run(KIND_DENSITY, [&] {
         return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p, masks.p,
                                    flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
@@ -510,7 +519,8 @@ void Engine::plain_phase(float4 *acc_sorted) {
     run(KIND_FORCE, [&] {
         return launch_force_list(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p, vel[ns].p, rq.p,
                                  tag[nt_].p, nl_cnt.p, nl_idx.p, (uint32_t)alloc_particles, posm[cs].p, vel[cs].p,
-                                 ra ? rec_acc.p : nullptr, d_bbox.p, acc_sorted, ghosts ? ghost_dev() : GhostDev{});
+                                 ra ? rec_acc.p : nullptr, d_bbox.p, acc_sorted, ghosts ? ghost_dev() : GhostDev{},
+                                 af ? *af : AuxFuse{});
     });
     gen_open = true;
 }

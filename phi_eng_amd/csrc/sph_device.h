@@ -440,3 +440,24 @@ __device__ __forceinline__ void ghost_force(const SceneDev &s, const GhostDev &g
             }
 }
 
+// Surrogate neighbour record word: P = p / rho^2 with the phase (0..3) in
+// its two lowest mantissa bits (a relative change below 2^-21 of P, which
+// the surrogate sweep reads as is).
+__device__ __forceinline__ float pack_pq(float q, uint32_t phase) {
+    return __uint_as_float((__float_as_uint(q) & ~3u) | min(phase, 3u));
+}
+
+// What a judging plain substep of a surrogate world also writes from the
+// force pass (one pass instead of a history gather and an update kernel):
+// the new history (rho, P, P - P_prev) of each sorted slot, with P_prev read
+// through the sort's permutation from the previous order, the neighbour
+// records of the advanced state, and per tile the bulk density deviation
+// sum |rho/rho0 - 1| over rho >= 0.9 rho0 and its count.
+struct AuxFuse {
+    const uint32_t *perm = nullptr;   // sorted slot -> slot before the sort
+    const float4 *aux_in = nullptr;   // history in the order before the sort
+    float4 *aux_out = nullptr;        // null: nothing fused
+    float4 *rec_out = nullptr;        // may be null
+    float2 *tile_err = nullptr;
+};
+

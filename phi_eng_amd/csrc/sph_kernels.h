@@ -53,8 +53,16 @@ struct ListOut {
     int4 *cell = nullptr;
     uint32_t *cnt = nullptr;
     uint32_t *idx = nullptr;  // SL_CAP / 4 groups of 4 slots, group g of particle i at uint4 g * mstride + i
-    float skin_factor2 = 1.0f;
+    float skin_factor2 = 1.0f;              // ((1 + skin))^2, or
+    const float *skin_factor2_dev = nullptr;  // the same computed on the device (k_list_radius)
 };
+// List radius of a surrogate world from the current speeds (a Verlet skin):
+// out = (1 + max(skin_min, coef * vmax))^2 with vmax the largest speed
+// (per tile into tile_vmax, then over tiles) and coef = 4 x the substeps a
+// list serves x dt / h: two particles approach at most at twice the
+// largest speed, and speeds are allowed to double before the next list.
+hipError_t launch_list_radius(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                              const float4 *vel, float *tile_vmax, float coef, float skin_min, float *out);
 hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                                const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
                                uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
@@ -70,7 +78,7 @@ hipError_t launch_force_list(hipStream_t st, uint32_t ntiles, const TileDev *til
                              const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
                              const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
                              uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
-                             float4 *acc_sorted, const GhostDev &gh);
+                             float4 *acc_sorted, const GhostDev &gh, const AuxFuse &af = AuxFuse{});
 // Boundary ghosts' density, pressure and q from the gathered fluid state.
 hipError_t launch_ghost_density(hipStream_t st, const SceneDev *scenes, uint32_t scene, const uint32_t *start,
                                 const float4 *posm, const GhostDev &gh, uint32_t count);

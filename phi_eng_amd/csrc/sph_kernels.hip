@@ -293,7 +293,7 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
         if (p == 3u && lk < SL_CAP) lgrp[(size_t)(lk >> 2) * mstride] = make_uint4(q0, q1, q2, q3);
         ++lk;
     };
-    const float hs2 = SKIN ? s.h_grid2 * lo.skin_factor2 : 0.0f;
+    const float hs2 = SKIN ? s.h_grid2 * (lo.skin_factor2_dev ? *lo.skin_factor2_dev : lo.skin_factor2) : 0.0f;
     if (finite3(pi.x, pi.y, pi.z)) {
         int32_t cx, cy, cz;
         cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
@@ -495,13 +495,14 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
     const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox,
     float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}, const uint32_t *__restrict__ lcnt = nullptr,
-    const uint4 *__restrict__ lgrp = nullptr) {
+    const uint4 *__restrict__ lgrp = nullptr, const AuxFuse &af = AuxFuse{}) {
     const uint32_t tile = xcd_tile(bid, nbl);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     const bool active = i < s.part_end;
     int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bz0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN, bz1 = INT32_MIN;
+    float jerr = 0.0f, jcnt = 0.0f;  // judge sums (AuxFuse)
     if (active) {
         const float4 pi = posm[i];
         const float4 vi = vel[i];
@@ -630,6 +631,21 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             rec_acc[3 * id + 1] = ay;
             rec_acc[3 * id + 2] = az;
         }
+        if (LIST && af.aux_out) {
+            const float4 ap = af.aux_in[af.perm[i]];
+            const uint32_t tg = tag[i];
+            const float dq = rqi.y - ap.y;
+            af.aux_out[i] = make_float4(rqi.x, rqi.y, dq, 0.0f);
+            if (af.rec_out) {
+                af.rec_out[2 * i] = make_float4(x, y, z, pack_pq(rqi.y, tg >> 24));
+                af.rec_out[2 * i + 1] = make_float4(ux, uy, uz, dq);
+            }
+            const float r = rqi.x / scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
+            if (r >= 0.9f) {
+                jerr = fabsf(r - 1.0f);
+                jcnt = 1.0f;
+            }
+        }
         if (finite3(x, y, z)) {
             cell_of(x, y, z, s.h_grid, bx0, by0, bz0);
             bx1 = bx0;
@@ -648,7 +664,14 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         by1 = max(by1, __shfl_xor(by1, d, 64));
         bz1 = max(bz1, __shfl_xor(bz1, d, 64));
     }
+    const bool judge = LIST && af.tile_err;
+    if (judge)
+        for (int d = 32; d >= 1; d >>= 1) {
+            jerr += __shfl_xor(jerr, d, 64);
+            jcnt += __shfl_xor(jcnt, d, 64);
+        }
     __shared__ int32_t wb[4][6];
+    __shared__ float wj[4][2];
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         wb[w][0] = bx0;
@@ -657,6 +680,8 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         wb[w][3] = bx1;
         wb[w][4] = by1;
         wb[w][5] = bz1;
+        wj[w][0] = jerr;
+        wj[w][1] = jcnt;
     }
     __syncthreads();
     if (threadIdx.x < 6) {
@@ -664,6 +689,47 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         int32_t v = wb[0][c];
         for (int k = 1; k < PH_TILE / 64; ++k) v = c < 3 ? min(v, wb[k][c]) : max(v, wb[k][c]);
         bbox[6 * (size_t)tile + c] = v;
+    }
+    if (judge && threadIdx.x == 0)
+        af.tile_err[tile] = make_float2(wj[0][0] + wj[1][0] + wj[2][0] + wj[3][0], wj[0][1] + wj[1][1] + wj[2][1] + wj[3][1]);
+}
+
+// Largest speed of each tile's particles (the list radius's input).
+__global__ __launch_bounds__(PH_TILE) void k_tile_vmax(const TileDev *__restrict__ tiles,
+                                                       const SceneDev *__restrict__ scenes,
+                                                       const float4 *__restrict__ vel, float *__restrict__ out) {
+    const TileDev t = tiles[blockIdx.x];
+    const uint32_t i = t.first + threadIdx.x;
+    float v = 0.0f;
+    if (i < scenes[t.scene].part_end) {
+        const float4 u = vel[i];
+        const float sp = sqrtf(dot3(u.x, u.y, u.z));
+        v = sp == sp ? sp : 0.0f;
+    }
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    __shared__ float wv[PH_TILE / 64];
+    if ((threadIdx.x & 63) == 0) wv[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = wv[0];
+        for (int k = 1; k < PH_TILE / 64; ++k) m = fmaxf(m, wv[k]);
+        out[blockIdx.x] = m;
+    }
+}
+
+// One block: (1 + max(skin_min, coef * vmax))^2 from the per-tile speeds.
+__global__ __launch_bounds__(256) void k_list_radius(const float *__restrict__ tile_vmax, uint32_t ntiles, float coef,
+                                                     float skin_min, float *__restrict__ out) {
+    float v = 0.0f;
+    for (uint32_t k = threadIdx.x; k < ntiles; k += 256) v = fmaxf(v, tile_vmax[k]);
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    __shared__ float wv[4];
+    if ((threadIdx.x & 63) == 0) wv[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float vm = fmaxf(fmaxf(wv[0], wv[1]), fmaxf(wv[2], wv[3]));
+        const float f = 1.0f + fmaxf(skin_min, coef * vm);
+        out[0] = f * f;
     }
 }
 
@@ -681,9 +747,9 @@ __global__ __launch_bounds__(PH_TILE) void k_force_list(const TileDev *__restric
     const float4 *__restrict__ vel, const float2 *__restrict__ rq, const uint32_t *__restrict__ tag,
     const uint32_t *__restrict__ lcnt, const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, float4 *__restrict__ acc_sorted,
-    GhostDev gh) {
+    GhostDev gh, AuxFuse af) {
     dev_force_integrate<true>(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, nullptr, nullptr,
-                              mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh, lcnt, lgrp);
+                              mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh, lcnt, lgrp, af);
 }
 
 // Boundary ghost density and pressure from the fluid (fluid3.rs:906-920):
@@ -992,7 +1058,7 @@ hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *t
                                const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
                                uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
                                uint32_t *rec_count, const ListOut &lo) {
-    if (lo.skin_factor2 != 1.0f)
+    if (lo.skin_factor2 != 1.0f || lo.skin_factor2_dev)
         hipLaunchKernelGGL(k_density_list<true>, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag,
                            rq, masks, flags, mstride, rec_rho, rec_p, rec_count, lo);
     else
@@ -1013,10 +1079,18 @@ hipError_t launch_force_list(hipStream_t st, uint32_t ntiles, const TileDev *til
                              const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
                              const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
                              uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
-                             float4 *acc_sorted, const GhostDev &gh) {
+                             float4 *acc_sorted, const GhostDev &gh, const AuxFuse &af) {
     hipLaunchKernelGGL(k_force_list, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm, vel, rq,
                        tag, lcnt, reinterpret_cast<const uint4 *>(lidx), mstride, posm_out, vel_out, rec_acc, bbox,
-                       acc_sorted, gh);
+                       acc_sorted, gh, af);
+    return hipGetLastError();
+}
+hipError_t launch_list_radius(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                              const float4 *vel, float *tile_vmax, float coef, float skin_min, float *out) {
+    hipLaunchKernelGGL(k_tile_vmax, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, vel, tile_vmax);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_list_radius, dim3(1), dim3(256), 0, st, tile_vmax, ntiles, coef, skin_min, out);
     return hipGetLastError();
 }
 hipError_t launch_ghost_density(hipStream_t st, const SceneDev *scenes, uint32_t scene, const uint32_t *start,

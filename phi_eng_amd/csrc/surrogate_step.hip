@@ -30,14 +30,9 @@ struct SurAcc {
     float gx = 0.0f, gy = 0.0f, gz = 0.0f;  // sum m_j grad W_ij
     float hx = 0.0f, hy = 0.0f, hz = 0.0f;  // sum m_j P_j grad W_ij
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;  // sum m_j dP_j grad W_ij
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;  // sum m_j W_ij (v_j - v_i)
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;  // sum m_j W_ij v_j (the XSPH sum is this - v_i rho)
 };
 
-// P with the phase (0..3) in its two lowest mantissa bits (a relative
-// change below 2^-21 of P, which the sweep reads as is).
-__device__ __forceinline__ float pack_pq(float q, uint32_t phase) {
-    return __uint_as_float((__float_as_uint(q) & ~3u) | min(phase, 3u));
-}
 
 // One candidate of particle i, branch-free: contributes when ok and
 // r2 < h^2; self = (j == i).  ra = (x, y, z, P|phase), rb = (v, dP) of the
@@ -77,10 +72,12 @@ __device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, 
     a.dx = fmaf(md, dx, a.dx);
     a.dy = fmaf(md, dy, a.dy);
     a.dz = fmaf(md, dz, a.dz);
-    // XSPH numerator (mw = 0 off the neighbourhood; the self term has v_j - v_i = 0)
-    a.sx = fmaf(rb.x - vi.x, mw, a.sx);
-    a.sy = fmaf(rb.y - vi.y, mw, a.sy);
-    a.sz = fmaf(rb.z - vi.z, mw, a.sz);
+    // XSPH: sum m_j W_ij v_j (mw = 0 off the neighbourhood); the numerator
+    // sum m_j W_ij (v_j - v_i) is this minus v_i rho, formed once per particle
+    (void)vi;
+    a.sx = fmaf(rb.x, mw, a.sx);
+    a.sy = fmaf(rb.y, mw, a.sy);
+    a.sz = fmaf(rb.z, mw, a.sz);
 }
 
 // Sweep over particle i's grid candidates (collection on the gathered
@@ -258,6 +255,14 @@ __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t
     }
 }
 
+// LDS exchange among the lanes of one wave: its LDS operations complete in
+// issue order, so ordering the compiler's memory operations is enough.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 }  // namespace
 
 // After a plain substep, on the same gathered state: training samples
@@ -324,6 +329,7 @@ __global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
             w2p[nt][l] = v2;
         }
         if (threadIdx.x < MLP_HID) b1s[threadIdx.x] = a.mlp.b1[threadIdx.x];
+        __syncthreads();  // the only block-wide wait: each wave's MLP below uses its own LDS rows
     }
     SurFeat f;
     float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = pi, ai = pi;
@@ -344,14 +350,14 @@ __global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
         const float rho0 = a.scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
         sur_features(s, rho0, a.qs, acc, ai.x, ai.z, f);
         const float irl = __builtin_amdgcn_rcpf(rmaxf(ai.x, PH_EPS));
-        xsx = acc.sx * irl;
-        xsy = acc.sy * irl;
-        xsz = acc.sz * irl;
+        xsx = fmaf(-vi.x, acc.rho, acc.sx) * irl;
+        xsy = fmaf(-vi.y, acc.rho, acc.sy) * irl;
+        xsz = fmaf(-vi.z, acc.rho, acc.sz) * irl;
     }
     float c[SUR_NCOEF] = {0.0f, 0.0f, 0.0f};
     if (a.use_net) {
         for (int k = 0; k < MLP_IN; ++k) xs[threadIdx.x][k] = active ? mlp_bf16_hw(f.x[k]) : (uint16_t)0;
-        __syncthreads();
+        wave_sync();
 #pragma unroll 1
         for (int mt = 0; mt < 4; ++mt) {
             const int row = w * 64 + mt * 16 + r;
@@ -372,7 +378,7 @@ __global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
                 *reinterpret_cast<float4 *>(ys[row]) =
                     make_float4(y[0] + a.mlp.b2[0], y[1] + a.mlp.b2[1], y[2] + a.mlp.b2[2], 0.0f);
         }
-        __syncthreads();
+        wave_sync();
         for (int k = 0; k < SUR_NCOEF; ++k) c[k] = ys[threadIdx.x][k];
     }
     if (!active) return;

@@ -129,7 +129,9 @@ struct World3 {
     uint64_t judge_at = 0;
     // skin list of the last sort (the engine's neighbour list, written with
     // skin > 0 while a surrogate is attached; surrogate substeps read it)
-    float skin = 0.05f;  // list radius (1 + skin) h
+    // list radius (1 + skin') h with skin' = max(skin, 4 check_every dt vmax / h):
+    // a Verlet skin over the substeps one list serves, at least `skin`
+    float skin = 0.01f;
     ph::DBuf<float4> sl_rec[2];  // neighbour records of generation k (2 float4 per slot)
     uint64_t list_sorts = ~0ull, list_epoch = ~0ull;
     uint32_t list_stride = 0;
@@ -304,6 +306,7 @@ struct World3 {
             sync_scene(dt);
             const uint32_t sub = cfl_substeps(dt);
             const float sdt = dt / (float)sub;
+            eng.list_coef = sur ? 4.0f * (float)check_every * sdt / cfg.smoothing_length : 0.0f;
             if (sub != 1) sync_scene(sdt);
             for (uint32_t k = 0; k < sub; ++k) {
                 if (cfg.boundary_particles || eng.scenes[0].ghost_count) update_ghosts();
@@ -400,9 +403,30 @@ struct World3 {
             ++n_col_skipped;
         }
         if (collect) col->grow_samples((uint64_t)col->nsamples + cnt);
+        if (sur) {
+            sl_rec[0].ensure(2 * e.alloc_particles);
+            sl_rec[1].ensure(2 * e.alloc_particles);
+        }
         e.sort_phase(e.perm.p);
-        e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_gather_aux(e.stream, n, e.perm.p, e.aux[e.cs].p, e.aux[ns].p); });
-        e.plain_phase(collect ? e.acc_sorted.p : nullptr);
+        // Without a collection the force pass also writes the new history
+        // (into the other aux buffer, then swapped in), the records and the
+        // judge sums; a collection reads the gathered history, so it takes
+        // the separate gather and update kernels.
+        const bool fuse = !collect;
+        if (fuse) {
+            AuxFuse af;
+            af.perm = e.perm.p;
+            af.aux_in = e.aux[e.cs].p;
+            af.aux_out = e.aux[ns].p;
+            af.rec_out = sur ? sl_rec[e.cs].p : nullptr;
+            af.tile_err = e.tile_err.p;
+            e.plain_phase(nullptr, &af);
+            e.aux[0].swap(e.aux[1]);  // the new history is generation cs
+        } else {
+            e.run(ph::KIND_SUR_AUX,
+                  [&] { return ph::launch_gather_aux(e.stream, n, e.perm.p, e.aux[e.cs].p, e.aux[ns].p); });
+            e.plain_phase(e.acc_sorted.p);
+        }
         if (list && e.list_cells && e.list_skin == skin) {
             // the skin list of the order this substep sorted into
             list_sorts = e.sorts_done;
@@ -431,15 +455,12 @@ struct World3 {
             col->nsamples += cnt;
             col_phase = (col_phase + 1) % col_stride;
         }
-        if (sur) {
-            sl_rec[0].ensure(2 * e.alloc_particles);
-            sl_rec[1].ensure(2 * e.alloc_particles);
-        }
-        e.run(ph::KIND_SUR_AUX, [&] {
-            return ph::launch_aux_update(e.stream, nt, e.d_tiles.p, e.d_scenes.p, e.rq.p, e.aux[ns].p, e.tag[ntg].p,
-                                         e.aux[e.cs].p, e.tile_err.p, e.posm[e.cs].p, e.vel[e.cs].p,
-                                         sur ? sl_rec[e.cs].p : nullptr);
-        });
+        if (!fuse)
+            e.run(ph::KIND_SUR_AUX, [&] {
+                return ph::launch_aux_update(e.stream, nt, e.d_tiles.p, e.d_scenes.p, e.rq.p, e.aux[ns].p,
+                                             e.tag[ntg].p, e.aux[e.cs].p, e.tile_err.p, e.posm[e.cs].p, e.vel[e.cs].p,
+                                             sur ? sl_rec[e.cs].p : nullptr);
+            });
         if (judge) {
             apply_judge();  // an older verdict still pending (check_every < 2)
             if (!judge_ev) {
