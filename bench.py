@@ -30,6 +30,9 @@ HBM_PEAK_GBS = 8000.0
 ALG_BYTES = {"cell_key": 20, "scan": 0, "scatter": 16, "rank_gather": 68, "density": 20,
              "force_integrate": 56}
 B_ALG_STEP = 185
+# VALU issue peak of the MI355X: 256 CUs x 4 SIMDs, one wave instruction per
+# 4 cycles each, at the 2.4 GHz peak engine clock (wave instructions / s).
+VALU_PEAK_WIPS = 1024 * 2.4e9 / 4
 
 
 def _args():
@@ -41,6 +44,8 @@ def _args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="HBM bytes per launch per kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE (optional)")
+    ap.add_argument("--valu", default=os.path.join(ROOT, "profiles", "valu.json"),
+                    help="VALU wave instructions per launch per kernel from rocprofv3 SQ_INSTS_VALU (optional)")
     ap.add_argument("--workload", default="plain", choices=["plain", "batch", "scale8m"],
                     help="plain: the 1M dam break (the headline); batch: 256 uneven scenes in one launch per step; "
                          "scale8m: 8,388,608 particles, two colliding columns, plain vs surrogate")
@@ -53,12 +58,19 @@ def _args():
     return ap.parse_args()
 
 
+# Plain judge every CHECK_EVERY substeps in the surrogate runs.  With the
+# Verlet list radius the interval trades the judge's cost against the list's
+# size; 15 was the fastest of 10 / 15 / 20 over 200 and 1,000 steps of the
+# 1M dam break at unchanged density error (the bench re-measures 10 and 20).
+CHECK_EVERY = 15
+
+
 def bench_surrogate(a, plain_value):
     """The learned pressure term on the 1M scene: train the network on GPU
     from a plain GPU trajectory of a half-scale dam (samples every 10 steps
     from 1/16 of the particles), then run the 1M scene from the same start
-    with plain SPH, with the surrogate (plain judge every 10 steps, fallback
-    above 1% bulk density deviation), and with the surrogate's network
+    with plain SPH, with the surrogate (plain judge every CHECK_EVERY steps,
+    fallback above 1% bulk density deviation), and with the surrogate's network
     switched off; density error = mean |rho - rho_plain| / rho_plain after
     the run."""
     from phi_eng_amd import scenes
@@ -67,10 +79,11 @@ def bench_surrogate(a, plain_value):
     sc = scenes.c2_dam_break_1m()
     n = sc.n
     S = a.sur_steps
-    runs = compare_runs(sc, net, S, ("plain", "surrogate", "network_free"))
+    sweep = ("surrogate_ce10", "surrogate_ce20")
+    runs = compare_runs(sc, net, S, ("plain", "surrogate", "network_free") + sweep)
     ref = runs["plain"]["rho"]
     out = {}
-    for name in ("surrogate", "network_free"):
+    for name in ("surrogate", "network_free") + sweep:
         r = runs[name]
         e = np.abs(r["rho"] - ref) / np.maximum(ref, 1e-6)
         out[name] = {"value": round(n * S / r["wall"], 1), "ms_per_step": round(r["wall"] * 1e3 / S, 4),
@@ -83,8 +96,12 @@ def bench_surrogate(a, plain_value):
     res.update({
         "plain_ms_per_step_same_steps": round(runs["plain"]["wall"] * 1e3 / S, 4),
         "speedup_vs_plain_headline": round(res["value"] / plain_value, 4),
-        "steps": S, "check_every": 10, "max_density_error": 0.01, "skin": "Verlet: max(0.01, 4 check_every dt vmax / h)",
+        "steps": S, "check_every": CHECK_EVERY, "max_density_error": 0.01,
+        "skin": "Verlet: max(0.01, 4 check_every dt vmax / h)",
         "network_free": out["network_free"],
+        "check_every_sweep": {k.split("_ce")[1]: {"speedup_vs_plain": out[k]["speedup_vs_plain"],
+                                                   "density_err_mean": out[k]["density_err_mean"],
+                                                   "fallbacks": out[k]["fallbacks"]} for k in sweep},
         "train": train,
         "data": "synthetic: seeded jittered lattice dam breaks; network trained in this run",
     })
@@ -179,7 +196,8 @@ def compare_runs(sc, net, steps, names):
     for name in names:
         w = scenes.make_world(sc)
         if name != "plain":
-            w.set_surrogate(net, check_every=10, max_density_error=0.01, use_network=(name == "surrogate"))
+            ce = int(name.split("_ce")[1]) if "_ce" in name else CHECK_EVERY
+            w.set_surrogate(net, check_every=ce, max_density_error=0.01, use_network=not name.startswith("network_free"))
         w.step(sc.dt)  # first step (set-up, judge) outside the timing
         w.sync()
         ta = time.perf_counter()
@@ -203,7 +221,7 @@ def compare_runs(sc, net, steps, names):
 def bench_scale(a):
     """8,388,608 particles (two 64 x 256 x 256 columns at the x-ends of a
     12.8^3 tank, k 3.6e6, dt 1e-4): plain GPU, surrogate in the loop (plain
-    judge every 10 steps), and the CPU oracle on a bounded sample."""
+    judge every CHECK_EVERY steps), and the CPU oracle on a bounded sample."""
     from phi_eng_amd import scenes
 
     steps = a.steps if a.steps != 500 else 300
@@ -234,7 +252,7 @@ def bench_scale(a):
            "data": "synthetic: two seeded jittered lattice columns; network trained in this run",
            "config": {"workload": "two_columns_3d_8m", "particles": n, "lattice": [[64, 256, 256], [64, 256, 256]],
                       "tank": [12.8, 12.8, 12.8], "dt": sc.dt, "stiffness": sc.config.stiffness,
-                      "path": "surrogate", "check_every": 10, "max_density_error": 0.01, "skin": "Verlet, min 0.01"},
+                      "path": "surrogate", "check_every": CHECK_EVERY, "max_density_error": 0.01, "skin": "Verlet, min 0.01"},
            "plain": {"value": round(pv, 1), "ms_per_step": round(runs["plain"]["wall"] * 1e3 / steps, 4)},
            "speedup_vs_plain": round(sv / pv, 4), "density_err_mean": float(e.mean()),
            "density_err_p99": float(np.percentile(e, 99)),
@@ -315,6 +333,16 @@ def main():
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "alg_bytes_per_particle": ALG_BYTES[dom], "avg_launch_us": round(kernels[dom]["avg_us"], 2)}
+    # The stencil passes are VALU-issue-bound, not HBM-bound: the same kernel
+    # against the VALU issue peak, its instruction count from rocprofv3.
+    if os.path.exists(a.valu):
+        with open(a.valu) as f:
+            vi = json.load(f).get("per_launch", {}).get(dom)
+        if vi:
+            wips = vi["insts_valu"] / avg_s
+            roof["valu"] = {"insts_per_launch": vi["insts_valu"], "achieved": round(wips / 1e9, 1),
+                            "peak": round(VALU_PEAK_WIPS / 1e9, 1), "unit": "G wave-instructions/s",
+                            "frac": round(wips / VALU_PEAK_WIPS, 4)}
     step_frac = B_ALG_STEP * n / (wall / a.steps) / (HBM_PEAK_GBS * 1e9)
 
     cpu = None
