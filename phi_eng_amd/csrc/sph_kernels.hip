@@ -250,18 +250,20 @@ __global__ __launch_bounds__(PH_TILE) void k_rank_gather(const TileDev *__restri
 
 // ---------------------------------------------------------------- K4
 // Density (fluid3.rs:639-655) in two phases per row: the cut-off test over
-// the row's candidates builds a bit mask (stored for the force pass), then
-// the set bits are summed in ascending slot order, i.e. the reference's
-// neighbour order.  Rows with more than 32 candidates finish their tail
-// inline and flag it.  rq[i] = (rho_i, p_i / max(rho_i, EPS)^2), the
-// pressure quotient of the force pass (fluid3.rs:1046-1051).
-// With LIST, the same sweep also writes the skin list of the sorted state
-// for surrogate substeps (candidates within (1 + skin) h, skin_factor2 =
-// (1 + skin)^2; the layout k_sur_list documents); the density results are
-// the same either way.
-// LIST: write the neighbour list instead of the row masks; SKIN: the list
-// radius is (1 + skin) h > h (else the list is exactly the neighbours and
-// reuses the cut-off test's masks).
+// the row's candidates builds a bit mask, then the set bits are summed in
+// ascending slot order, i.e. the reference's neighbour order.  Rows with
+// more than 32 candidates finish their tail inline.  rq[i] = (rho_i,
+// p_i / max(rho_i, EPS)^2), the pressure quotient of the force pass
+// (fluid3.rs:1046-1051).  The density results are the same in every mode.
+// !LIST: the row masks and long-row flags are stored for the mask-based
+//   force pass (the batch's persistent kernel).
+// LIST: the neighbours are written to the particle's list instead, in
+//   order, four slots per 16-byte store (ListOut documents the layout);
+//   the count word carries SLF_SLOW / SLF_NONE where the force pass must
+//   sweep the cells or has nothing to sweep.
+// SKIN: the list radius is sqrt(skin_factor2) h > h (the surrogate's
+//   Verlet list, from skin_factor2_dev when set); else the list is exactly
+//   the neighbours and reuses the cut-off test's masks.
 template <bool LIST, bool SKIN = LIST>
 __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles,
                                                      const SceneDev *__restrict__ scenes,
