@@ -56,8 +56,9 @@ void phieng_world_set_gravity(World2 *world, float gx, float gy);
 int32_t phieng_world_add_body(World2 *world, float pos_x, float pos_y, uint32_t shape_tag, float a, float b,
                               float density, uint8_t is_static, float friction, float restitution);
 /* ffi.rs:999-1011 — 1 on success, 0 on null or invalid input (world
- * unchanged); validation ffi.rs:773-840.  Also 0 for adaptive_resolution
- * set, which the GPU path does not implement. */
+ * unchanged); validation ffi.rs:773-840.  adaptive_resolution merges and
+ * splits particles each substep as fluid.rs:826-891 does (split cap: the
+ * capacity, MAX_PARTICLES_2D = 4096 unless reserved). */
 int32_t phieng_world_set_fluid_config(World2 *world, const FluidConfig2Ffi *cfg);
 /* ffi.rs:1017-1026 — 1 on success, 0 on null. */
 int32_t phieng_world_get_fluid_config(const World2 *world, FluidConfig2Ffi *out);
@@ -83,6 +84,8 @@ uint32_t phieng_world_fluid_get(const World2 *world, float *out, uint32_t max);
 
 /* Lift the cap to `capacity` (at most 16,777,216); returns the cap in force. */
 uint32_t phieng_world_fluid_reserve(World2 *world, uint32_t capacity);
+/* Mass and phase per particle in insertion order, up to max. */
+uint32_t phieng_world_fluid_get_mass(const World2 *world, float *mass, uint8_t *phase, uint32_t max);
 /* Overwrite velocities (vx,vy per particle, insertion order); 0 on null
  * or any non-finite value, else the number set. */
 uint32_t phieng_world_fluid_set_velocity(World2 *world, const float *vel, uint32_t count);

@@ -78,8 +78,11 @@ int32_t phieng_world3_add_body(World3 *world, float pos_x, float pos_y, float po
                                uint8_t is_static, float friction, float restitution);
 /* ffi.rs:1032-1044 — 1 on success, 0 on null or invalid input (world
  * unchanged); validation rules ffi.rs:857-914.  Also returns 0 for
- * adaptive_resolution set, which the GPU path does not implement, and for
- * boundary_particles set while a surrogate is attached. */
+ * boundary_particles or adaptive_resolution set while a surrogate is
+ * attached (or collecting).  With adaptive_resolution each substep merges
+ * and splits particles as fluid3.rs:923-987 does, so the count and the
+ * insertion order change; the split cap is the fluid capacity (4096 as
+ * MAX_PARTICLES_3D unless reserved higher). */
 int32_t phieng_world3_set_fluid_config(World3 *world, const FluidConfig3Ffi *cfg);
 /* ffi.rs:1050-1059 — 1 on success, 0 on null. */
 int32_t phieng_world3_get_fluid_config(const World3 *world, FluidConfig3Ffi *out);
@@ -106,6 +109,11 @@ uint32_t phieng_world3_fluid_get(const World3 *world, float *out, uint32_t max);
 /* Lift the per-fluid particle cap (MAX_PARTICLES_3D = 4096, fluid3.rs:28)
  * to `capacity` (at most 16,777,216).  Returns the capacity in force. */
 uint32_t phieng_world3_fluid_reserve(World3 *world, uint32_t capacity);
+
+/* Mass and phase per particle in insertion order (as fluid_get; either
+ * pointer may be null, not both), up to max; returns the number written.
+ * Masses change under adaptive_resolution. */
+uint32_t phieng_world3_fluid_get_mass(const World3 *world, float *mass, uint8_t *phase, uint32_t max);
 
 /* Overwrite velocities (insertion order, vx,vy,vz per particle) — used to
  * start parity checks from an arbitrary state.  Returns the number set,

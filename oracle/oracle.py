@@ -142,6 +142,107 @@ def run_f32(pos, vel, cfg, gravity, bodies, dt, steps, jacobi=False, phases=None
     return pos, vel, sub
 
 
+def _cap_arrays(pos, vel, mass, phases, room, dim, density=None):
+    n = len(pos)
+    P = np.zeros((room, dim), np.float32)
+    V = np.zeros((room, dim), np.float32)
+    M = np.zeros(room, np.float32)
+    H = np.zeros(room, np.uint8)
+    R = np.zeros(room, np.float32)
+    P[:n] = pos
+    if vel is not None:
+        V[:n] = vel
+    M[:n] = mass
+    if phases is not None:
+        H[:n] = phases
+    if density is not None:
+        R[:n] = density
+    return P, V, M, H, R
+
+
+def _variable(call, pos, vel, mass, phases, dim, density=None):
+    """Run a variable-count oracle call, growing the output room until the
+    result fits (the split cap check lets the count pass the cap)."""
+    room = 2 * max(len(pos), 1)
+    while True:
+        P, V, M, H, R = _cap_arrays(pos, vel, mass, phases, room, dim, density)
+        n = ctypes.c_uint32(len(pos))
+        if call(n, room, P, V, M, H, R):
+            k = n.value
+            return dict(pos=P[:k], vel=V[:k], mass=M[:k], phase=H[:k], density=R[:k])
+        assert n.value > room
+        room = n.value
+
+
+def resample_f32(pos, vel, mass, density, cfg, h, cap=4096, phases=None):
+    """resample() alone (fluid3.rs:923-987) with the given densities;
+    returns dict(pos, vel, mass, phase, density) of the new particle set."""
+    def call(n, room, P, V, M, H, R):
+        return lib().oracle3_resample_f32(ctypes.byref(n), ctypes.c_uint32(cap), ctypes.c_uint32(room), _p(P),
+                                          _p(V), _p(M), _p(H, ctypes.c_uint8), _p(R), ctypes.byref(cfg),
+                                          ctypes.c_float(h))
+    return _variable(call, pos, vel, mass, phases, 3, density)
+
+
+def substep_cap_f32(pos, vel, mass, cfg, gravity, bodies, dt, cap, jacobi=True, phases=None):
+    """One substep whose particle count may change (adaptive resolution);
+    returns dict(pos, vel, mass, phase, density, pressure, accel, count)."""
+    g = np.asarray(gravity, dtype=np.float32)
+    arr, nb = bodies
+    extra = {}
+
+    def call(n, room, P, V, M, H, R):
+        extra["pressure"] = np.zeros(room, np.float32)
+        extra["accel"] = np.zeros((room, 3), np.float32)
+        extra["count"] = np.zeros(room, np.uint32)
+        extra["scale"] = np.zeros(room, np.float64)
+        return lib().oracle3_substep_cap_f32(
+            ctypes.byref(n), ctypes.c_uint32(cap), ctypes.c_uint32(room), _p(P), _p(V), _p(M),
+            _p(H, ctypes.c_uint8), _p(R), _p(extra["pressure"]), ctypes.byref(cfg), _p(g), arr, ctypes.c_uint32(nb),
+            ctypes.c_float(dt), ctypes.c_int(int(jacobi)), _p(extra["accel"]), _p(extra["count"], ctypes.c_uint32),
+            _p(extra["scale"], ctypes.c_double))
+    out = _variable(call, pos, vel, mass, phases, 3)
+    k = len(out["pos"])
+    out.update({key: v[:k] for key, v in extra.items()})
+    return out
+
+
+def run_cap_f32(pos, vel, mass, cfg, gravity, bodies, dt, steps, cap, jacobi=True, phases=None):
+    """`steps` world steps with a variable particle count; returns
+    dict(pos, vel, mass, phase)."""
+    g = np.asarray(gravity, dtype=np.float32)
+    arr, nb = bodies
+
+    def call(n, room, P, V, M, H, R):
+        return lib().oracle3_run_cap_f32(ctypes.byref(n), ctypes.c_uint32(cap), ctypes.c_uint32(room), _p(P),
+                                         _p(V), _p(M), _p(H, ctypes.c_uint8), ctypes.byref(cfg), _p(g), arr,
+                                         ctypes.c_uint32(nb), ctypes.c_float(dt), ctypes.c_uint32(steps),
+                                         ctypes.c_int(int(jacobi)))
+    return _variable(call, pos, vel, mass, phases, 3)
+
+
+def resample2_f32(pos, vel, mass, density, cfg, h, cap=4096, phases=None):
+    """The 2D resample() (fluid.rs:826-891) with the given densities."""
+    def call(n, room, P, V, M, H, R):
+        return lib2().oracle2_resample_f32(ctypes.byref(n), ctypes.c_uint32(cap), ctypes.c_uint32(room), _p(P),
+                                           _p(V), _p(M), _p(H, ctypes.c_uint8), _p(R), ctypes.byref(cfg),
+                                           ctypes.c_float(h))
+    return _variable(call, pos, vel, mass, phases, 2, density)
+
+
+def run2_cap_f32(pos, vel, mass, cfg, gravity, bodies, dt, steps, cap, jacobi=True, phases=None):
+    """`steps` 2D world steps with a variable particle count."""
+    arr, nb = bodies
+
+    def call(n, room, P, V, M, H, R):
+        return lib2().oracle2_run_cap_f32(ctypes.byref(n), ctypes.c_uint32(cap), ctypes.c_uint32(room), _p(P),
+                                          _p(V), _p(M), _p(H, ctypes.c_uint8), ctypes.byref(cfg),
+                                          ctypes.c_float(gravity[0]), ctypes.c_float(gravity[1]), arr,
+                                          ctypes.c_uint32(nb), ctypes.c_float(dt), ctypes.c_uint32(steps),
+                                          ctypes.c_int(int(jacobi)))
+    return _variable(call, pos, vel, mass, phases, 2)
+
+
 def cells(pos, h):
     pos = np.ascontiguousarray(pos, dtype=np.float32)
     out = np.zeros((len(pos), 3), dtype=np.int32)
@@ -185,7 +286,9 @@ class Oracle2Cfg(ctypes.Structure):
                 ("cfl", ctypes.c_float), ("max_substeps", ctypes.c_uint32), ("viscosity", ctypes.c_float),
                 ("wall_friction", ctypes.c_float), ("phase_rest_density", ctypes.c_float * 4),
                 ("max_velocity", ctypes.c_float), ("particle_radius", ctypes.c_float),
-                ("boundary_particles", ctypes.c_uint32), ("particle_mass", ctypes.c_float)]
+                ("boundary_particles", ctypes.c_uint32), ("particle_mass", ctypes.c_float),
+                ("adaptive_resolution", ctypes.c_uint32), ("split_density_ratio", ctypes.c_float),
+                ("merge_density_ratio", ctypes.c_float)]
 
     @classmethod
     def default(cls):
@@ -207,6 +310,9 @@ class Oracle2Cfg(ctypes.Structure):
         c.particle_radius = 0.125
         c.boundary_particles = 0
         c.particle_mass = 0.0625
+        c.adaptive_resolution = 0
+        c.split_density_ratio = 0.5
+        c.merge_density_ratio = 1.6
         return c
 
 

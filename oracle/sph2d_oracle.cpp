@@ -68,7 +68,9 @@ struct Oracle2Cfg {
     float phase_rest_density[4];
     float max_velocity, particle_radius;
     uint32_t boundary_particles;  // FluidConfig2::boundary_particles
-    float particle_mass;          // FluidConfig2::particle_mass (ghost mass)
+    float particle_mass;          // FluidConfig2::particle_mass (ghost mass, min split mass)
+    uint32_t adaptive_resolution; // FluidConfig2::adaptive_resolution
+    float split_density_ratio, merge_density_ratio;
 };
 
 // Static body, identity rotation: shape 0 circle (a = radius), 1 box
@@ -98,6 +100,7 @@ struct Fluid2 {
     std::vector<float> fx, fy;
     std::vector<Ghost2> ghosts;
     std::unordered_map<uint64_t, std::vector<uint32_t>> ghost_grid;
+    uint32_t cap = 4096;  // MAX_PARTICLES_2D (fluid.rs:26), the split cap of resample()
 
     static uint64_t key(int32_t cx, int32_t cy) { return ((uint64_t)(uint32_t)cx << 32) | (uint32_t)cy; }
 
@@ -325,8 +328,55 @@ struct Fluid2 {
         return true;
     }
 
-    // fluid.rs:898-1136 (no adaptive resolution, boundary particles,
-    // turbulence or dynamic bodies).
+    // fluid.rs:826-891, adaptive resolution (the 3D restatement documents
+    // the rules; the split offset is (h/4, 0)).
+    void resample(float h) {
+        const float rest = rmax(cfg.rest_density, kEps);
+        const float split_lo = cfg.split_density_ratio * rest;
+        const float merge_hi = cfg.merge_density_ratio * rest;
+        const float min_mass = rmax(cfg.particle_mass, kEps) * 0.125f;
+        const size_t n = ps.size();
+        std::vector<uint8_t> absorbed(n, 0);
+        const float merge_dist_sq = (0.25f * h) * (0.25f * h);
+        for (size_t i = 0; i < n; ++i) {
+            if (absorbed[i] || ps[i].rho <= merge_hi) continue;
+            for (size_t j = i; j-- > 0;) {
+                if (absorbed[j] || ps[j].rho <= merge_hi) continue;
+                if (len_sq(ps[i].x - ps[j].x, ps[i].y - ps[j].y) < merge_dist_sq) {
+                    const float mi = ps[i].m, mj = ps[j].m, mt = mi + mj;
+                    if (mt > kEps) {
+                        ps[i].vx = (ps[i].vx * mi + ps[j].vx * mj) / mt;
+                        ps[i].vy = (ps[i].vy * mi + ps[j].vy * mj) / mt;
+                        ps[i].m = mt;
+                    }
+                    absorbed[j] = 1;
+                    break;
+                }
+            }
+        }
+        std::vector<P2> out;
+        out.reserve(n);
+        for (size_t k = 0; k < n; ++k) {
+            if (absorbed[k]) continue;
+            const P2 p = ps[k];
+            if (p.rho < split_lo && p.m > min_mass && out.size() + 1 < cap) {
+                P2 a = p, b = p;
+                a.m = b.m = p.m * 0.5f;
+                const float off = 0.25f * h;
+                a.x = p.x + off;
+                a.y = p.y + 0.0f;
+                b.x = p.x - off;
+                b.y = p.y - 0.0f;
+                out.push_back(a);
+                out.push_back(b);
+            } else {
+                out.push_back(p);
+            }
+        }
+        ps.swap(out);
+    }
+
+    // fluid.rs:898-1136 (no turbulence or dynamic bodies).
     // scale_out: the cancellation-aware magnitude sum_j |m_j term grad W| / rho_i
     // (plus the cohesion terms) that bounds the acceleration's rounding.
     void substep(float gx, float gy, const Oracle2Body *bodies, uint32_t nbodies, float dt, bool jacobi,
@@ -334,7 +384,13 @@ struct Fluid2 {
         const float h = rmax(cfg.smoothing_length, kEps);
         rebuild_grid();
         std::vector<uint32_t> nb;
-        density_pressure(h, nb, count);
+        density_pressure(h, nb, cfg.adaptive_resolution ? nullptr : count);
+        // fluid.rs:918-923: resample, rebuild, recompute
+        if (cfg.adaptive_resolution) {
+            resample(h);
+            rebuild_grid();
+            density_pressure(h, nb, count);
+        }
         // fluid.rs:929-938: the ghosts are a pure function of the static
         // bodies and the config, rebuilt here every substep
         if (cfg.boundary_particles) update_boundaries(bodies, nbodies, h);
@@ -556,6 +612,48 @@ uint32_t oracle2_boundary_f32(uint32_t n, const float *pos, const float *mass, c
         if (pressure) pressure[g] = f.ghosts[g].p;
     }
     return (uint32_t)f.ghosts.size();
+}
+
+// The variable-count forms (adaptive resolution): *n in and out, every
+// array holds `room` particles (pos, vel 2 each), mass and phase updated;
+// a result larger than room is not written (0 returned, *n the count).
+static void store_all(const Fluid2 &f, float *pos, float *vel, float *mass, uint8_t *phase, float *density) {
+    store(f, pos, vel, density, nullptr);
+    for (size_t i = 0; i < f.ps.size(); ++i) {
+        if (mass) mass[i] = f.ps[i].m;
+        if (phase) phase[i] = f.ps[i].phase;
+    }
+}
+
+// resample() alone (fluid.rs:826-891) on a state whose densities are given.
+int oracle2_resample_f32(uint32_t *n, uint32_t cap, uint32_t room, float *pos, float *vel, float *mass, uint8_t *phase,
+                         float *density, const Oracle2Cfg *cfg, float h) {
+    Fluid2 f = make(*n, pos, vel, mass, phase, cfg);
+    f.cap = cap;
+    for (uint32_t i = 0; i < *n; ++i) f.ps[i].rho = density[i];
+    f.resample(h);
+    *n = (uint32_t)f.ps.size();
+    if (f.ps.size() > room) return 0;
+    store_all(f, pos, vel, mass, phase, density);
+    return 1;
+}
+
+// `steps` World2 steps (as oracle2_run_f32) with a variable count.
+int oracle2_run_cap_f32(uint32_t *n, uint32_t cap, uint32_t room, float *pos, float *vel, float *mass, uint8_t *phase,
+                        const Oracle2Cfg *cfg, float gx, float gy, const Oracle2Body *bodies, uint32_t nbodies,
+                        float dt, uint32_t steps, int jacobi) {
+    Fluid2 f = make(*n, pos, vel, mass, phase, cfg);
+    f.cap = cap;
+    for (uint32_t s = 0; s < steps; ++s) {
+        if (dt <= 0.0f || f.ps.empty()) break;
+        const uint32_t sub = f.cfl_substeps(dt);
+        const float sdt = dt / (float)sub;
+        for (uint32_t k = 0; k < sub; ++k) f.substep(gx, gy, bodies, nbodies, sdt, jacobi != 0, nullptr, nullptr);
+    }
+    *n = (uint32_t)f.ps.size();
+    if (f.ps.size() > room) return 0;
+    store_all(f, pos, vel, mass, phase, nullptr);
+    return 1;
 }
 
 float oracle2_poly6(float r2, float h) { return Fluid2::poly6(r2, h); }

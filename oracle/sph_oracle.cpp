@@ -137,6 +137,9 @@ struct Fluid {
     std::vector<V3<T>> forces;
     std::vector<Ghost<T>> ghosts;
     std::unordered_map<Key, std::vector<uint32_t>, KeyHash> ghost_grid;
+    // MAX_PARTICLES_3D (fluid3.rs:28), the split cap of resample(); a world
+    // whose capacity was lifted passes its capacity instead.
+    uint32_t cap = 4096;
 
     explicit Fluid(const FluidConfig3Ffi &c) : cfg(c), n(0) {}
 
@@ -537,6 +540,69 @@ struct Fluid {
         out_x = x;
     }
 
+    // fluid3.rs:923-987, adaptive resolution.  Merge: in index order, a
+    // particle denser than merge_hi absorbs the highest-index earlier dense,
+    // not yet absorbed particle within h/4 (mass summed, velocity
+    // mass-weighted), and the absorbed ones are removed in order.  Split: in
+    // the new order, a particle thinner than split_lo and heavier than
+    // min_mass becomes two halves at -+ h/4 along x while the output stays
+    // under the cap.  `density` is this substep's (clamped) density.
+    void resample(T h) {
+        const T rest = rmax<T>((T)cfg.rest_density, (T)kEps);
+        const T split_lo = (T)cfg.split_density_ratio * rest;
+        const T merge_hi = (T)cfg.merge_density_ratio * rest;
+        const T min_mass = rmax<T>((T)cfg.particle_mass, (T)kEps) * (T)0.125;
+        std::vector<uint8_t> absorbed(n, 0);
+        const T merge_dist_sq = ((T)0.25 * h) * ((T)0.25 * h);
+        bool any = false;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (absorbed[i] || density[i] <= merge_hi) continue;
+            for (uint32_t j = i; j-- > 0;) {
+                if (absorbed[j] || density[j] <= merge_hi) continue;
+                const V3<T> d = sub(pos[i], pos[j]);
+                if (dot(d, d) < merge_dist_sq) {
+                    const T mi = mass[i], mj = mass[j], mt = mi + mj;
+                    if (mt > (T)kEps) {
+                        vel[i] = divs(add(muls(vel[i], mi), muls(vel[j], mj)), mt);
+                        mass[i] = mt;
+                    }
+                    absorbed[j] = 1;
+                    any = true;
+                    break;
+                }
+            }
+        }
+        std::vector<V3<T>> np_, nv;
+        std::vector<T> nrho, npr, nm;
+        std::vector<uint8_t> nph;
+        auto push = [&](uint32_t k, V3<T> x, T m) {
+            np_.push_back(x);
+            nv.push_back(vel[k]);
+            nrho.push_back(density[k]);
+            npr.push_back(pressure[k]);
+            nm.push_back(m);
+            nph.push_back(phase[k]);
+        };
+        const V3<T> off{(T)0.25 * h, (T)0, (T)0};
+        for (uint32_t k = 0; k < n; ++k) {
+            if (any && absorbed[k]) continue;
+            if (density[k] < split_lo && mass[k] > min_mass && np_.size() + 1 < cap) {
+                const T half = mass[k] * (T)0.5;
+                push(k, add(pos[k], off), half);
+                push(k, sub(pos[k], off), half);
+            } else {
+                push(k, pos[k], mass[k]);
+            }
+        }
+        pos.swap(np_);
+        vel.swap(nv);
+        density.swap(nrho);
+        pressure.swap(npr);
+        mass.swap(nm);
+        phase.swap(nph);
+        n = (uint32_t)pos.size();
+    }
+
     // fluid3.rs:990-1184.  jacobi = false replays the reference's in-place
     // Gauss-Seidel integrate (neighbours j < i already advanced, stale grid);
     // jacobi = true reads only pre-step state (the GPU contract).
@@ -544,7 +610,13 @@ struct Fluid {
                  uint32_t *count_out, double *scale_out) {
         T h = h_kernel();
         rebuild_grid();
-        compute_density_pressure(h, count_out);
+        compute_density_pressure(h, cfg.adaptive_resolution ? nullptr : count_out);
+        // fluid3.rs:1007-1012: resample, rebuild, recompute
+        if (cfg.adaptive_resolution) {
+            resample(h);
+            rebuild_grid();
+            compute_density_pressure(h, count_out);
+        }
         if (cfg.boundary_particles) update_boundaries(bodies, nbodies, h);
         compute_forces(h, scale_out);
         std::vector<uint32_t> nb;
@@ -709,6 +781,81 @@ int oracle3_run_f32(uint32_t n, float *pos, float *vel, const float *mass, const
         vel[3 * i + 2] = f.vel[i].z;
     }
     return (int)sub;
+}
+
+// The variable-count forms (adaptive resolution changes the particle
+// count).  *n is the count in and out; every array holds `room` particles
+// (pos, vel 3 each); mass and phase are updated with the state; cap is the
+// split cap of fluid3.rs:972 (MAX_PARTICLES_3D = 4096 in the reference, the
+// world's capacity on the GPU).  The cap check lets the count pass the
+// cap, so a result larger than room is not written: the call returns 0
+// with *n set to the count it needs.
+static void store_state(const Fluid<float> &f, float *pos, float *vel, float *mass, uint8_t *phase,
+                        float *density, float *pressure) {
+    for (uint32_t i = 0; i < f.n; ++i) {
+        pos[3 * i] = f.pos[i].x;
+        pos[3 * i + 1] = f.pos[i].y;
+        pos[3 * i + 2] = f.pos[i].z;
+        if (vel) {
+            vel[3 * i] = f.vel[i].x;
+            vel[3 * i + 1] = f.vel[i].y;
+            vel[3 * i + 2] = f.vel[i].z;
+        }
+        if (mass) mass[i] = f.mass[i];
+        if (phase) phase[i] = f.phase[i];
+        if (density) density[i] = f.density[i];
+        if (pressure) pressure[i] = f.pressure[i];
+    }
+}
+
+// resample() alone (fluid3.rs:923-987) on a state whose densities are given.
+int oracle3_resample_f32(uint32_t *n, uint32_t cap, uint32_t room, float *pos, float *vel, float *mass, uint8_t *phase,
+                         float *density, const FluidConfig3Ffi *cfg, float h) {
+    Fluid<float> f(*cfg);
+    f.cap = cap;
+    f.load(*n, pos, vel, mass, phase, density);
+    f.resample(h);
+    *n = f.n;
+    if (f.n > room) return 0;
+    store_state(f, pos, vel, mass, phase, density, nullptr);
+    return 1;
+}
+
+// One substep (as oracle3_substep_f32) with a variable count.
+int oracle3_substep_cap_f32(uint32_t *n, uint32_t cap, uint32_t room, float *pos, float *vel, float *mass, uint8_t *phase,
+                            float *density, float *pressure, const FluidConfig3Ffi *cfg, const float *g3,
+                            const OracleBody3 *bodies, uint32_t nbodies, float dt, int jacobi, float *acc3,
+                            uint32_t *count, double *scale) {
+    Fluid<float> f(*cfg);
+    f.cap = cap;
+    f.load(*n, pos, vel, mass, phase, nullptr);
+    f.substep({g3[0], g3[1], g3[2]}, dt, jacobi != 0, reinterpret_cast<const Body *>(bodies), nbodies, acc3,
+              count, scale);
+    *n = f.n;
+    if (f.n > room) return 0;
+    store_state(f, pos, vel, mass, phase, density, pressure);
+    return 1;
+}
+
+// `steps` world steps (as oracle3_run_f32) with a variable count.
+int oracle3_run_cap_f32(uint32_t *n, uint32_t cap, uint32_t room, float *pos, float *vel, float *mass, uint8_t *phase,
+                        const FluidConfig3Ffi *cfg, const float *g3, const OracleBody3 *bodies, uint32_t nbodies,
+                        float dt, uint32_t steps, int jacobi) {
+    Fluid<float> f(*cfg);
+    f.cap = cap;
+    f.load(*n, pos, vel, mass, phase, nullptr);
+    for (uint32_t s = 0; s < steps; ++s) {
+        if (!cfg->enabled || f.n == 0 || !(dt > 0.0f)) break;
+        const uint32_t sub = f.cfl_substeps(dt);
+        const float sdt = dt / (float)sub;
+        for (uint32_t k = 0; k < sub; ++k)
+            f.substep({g3[0], g3[1], g3[2]}, sdt, jacobi != 0, reinterpret_cast<const Body *>(bodies), nbodies,
+                      nullptr, nullptr, nullptr);
+    }
+    *n = f.n;
+    if (f.n > room) return 0;
+    store_state(f, pos, vel, mass, phase, nullptr, nullptr);
+    return 1;
 }
 
 // FP64 twin of one Jacobi substep from an f32 state.  Outputs in f64.

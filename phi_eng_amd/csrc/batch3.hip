@@ -265,6 +265,8 @@ Batch3 *phieng_batch3_new(World3 *const *worlds, uint32_t count) {
             if (!c.enabled) throw std::runtime_error("a disabled fluid cannot join a batch");
             if (c.boundary_particles)
                 throw std::runtime_error("boundary ghosts are per world: a world with boundary_particles cannot join a batch");
+            if (c.adaptive_resolution)
+                throw std::runtime_error("adaptive resolution changes a world's particle count: it cannot join a batch");
             b->worlds.push_back(worlds[k]);
         }
         build_batch(*b);

@@ -35,6 +35,7 @@ enum Kind : int {
     KIND_SUR_AUX,
     KIND_SUR_COLLECT,
     KIND_GHOST,
+    KIND_RESAMPLE,
     KIND_COUNT
 };
 const char *kind_name(int k);
@@ -192,6 +193,14 @@ struct Engine {
     // Density pass (writing the neighbour list), boundary ghosts, force pass
     // (with af, also the surrogate's history, records and judge sums).
     void plain_phase(float4 *acc_sorted, const AuxFuse *af = nullptr);
+    // The density pass alone (the first of an adaptive substep).
+    void density_phase();
+    // Adaptive resolution (fluid3.rs:923-987) over the sorted state and
+    // densities of this substep.  Writes the new particle set, in its new
+    // index order, into the unsorted generation and returns its count;
+    // `changed` is false when nothing merged or split (the sorted state is
+    // then still the current one).  Synchronises the stream once.
+    uint32_t resample_phase(float merge_hi, float split_lo, float min_mass, float h, uint32_t cap, bool &changed);
     void finish_substep();
     // After an exception inside a substep: drop a stale launch error and
     // make the generation selectors match what the device holds.  Counts
@@ -202,6 +211,12 @@ struct Engine {
     // list is usable while both are as they were when it was built.
     uint64_t sorts_done = 0, layout_epoch = 0;
     bool gen_open = false;      // generation cs rewritten, ct not flipped yet
+    // adaptive resolution work arrays (resample.h documents them)
+    DBuf<unsigned long long> rs_ctr;
+    unsigned long long *rs_host = nullptr;  // pinned copy of rs_ctr
+    DBuf<uint32_t> rs_flag, rs_items, rs_minT, rs_pick, rs_state, rs_cK, rs_cE, rs_rK, rs_rE, rs_slot, rs_partial;
+    DBuf<uint4> rs_takers;
+    DBuf<float4> rs_mv;
     // surrogate history (previous rho, P = p/rho^2, its change) per slot
     DBuf<float4> aux[2];
     bool aux_on = false;

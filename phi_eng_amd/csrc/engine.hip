@@ -6,6 +6,7 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "resample.h"
 #include "sph_kernels.h"
 
 namespace ph {
@@ -37,7 +38,7 @@ const char *kind_name(int k) {
         "cell_key", "scan_reduce", "scan_partials", "scan_apply", "scatter", "rank_gather",
         "density",  "force_integrate", "vmax", "export", "monitor", "append", "cell_bbox",
         "fill", "set_velocity", "batch_persistent", "surrogate_step", "surrogate_history",
-        "surrogate_collect", "boundary_ghosts"};
+        "surrogate_collect", "boundary_ghosts", "resample"};
     return (k >= 0 && k < KIND_COUNT) ? names[k] : "";
 }
 
@@ -246,6 +247,7 @@ Engine::~Engine() {
         for (auto e : ev_pool) (void)hipEventDestroy(e);
         if (mon_event) (void)hipEventDestroy(mon_event);
         if (h_monitor) (void)hipHostFree(h_monitor);
+        if (rs_host) (void)hipHostFree(rs_host);
         if (stream) (void)hipStreamDestroy(stream);
     }
 }
@@ -523,6 +525,90 @@ void Engine::plain_phase(float4 *acc_sorted, const AuxFuse *af) {
                                  af ? *af : AuxFuse{});
     });
     gen_open = true;
+}
+
+void Engine::density_phase() {
+    const uint32_t nt = (uint32_t)tiles.size();
+    const int ns = 1 - cs, nt_ = 1 - ct;
+    ListOut lo;
+    lo.cnt = nl_cnt.p;
+    lo.idx = nl_idx.p;
+    lo.skin_factor2 = 1.0f;
+    run(KIND_DENSITY, [&] {
+        return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p, masks.p,
+                                   flags.p, (uint32_t)alloc_particles, nullptr, nullptr, nullptr, lo);
+    });
+}
+
+uint32_t Engine::resample_phase(float merge_hi, float split_lo, float min_mass, float h, uint32_t cap,
+                                bool &changed) {
+    const uint32_t n = n_total;
+    const int ns = 1 - cs, nt_ = 1 - ct;
+    // room for every split the cap allows: an accepted split has
+    // kept + eligible before it + 1 < cap, so at most cap / 2 are accepted
+    if ((size_t)n + std::min<size_t>(n, cap / 2 + 1) > alloc_particles)
+        throw std::runtime_error("resample: the particle arrays cannot hold the split particles");
+    const size_t m = alloc_particles;
+    rs_ctr.ensure(4);
+    if (!rs_host) PH_CK(hipHostMalloc((void **)&rs_host, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+    rs_flag.ensure(m);
+    rs_minT.ensure(m);
+    rs_pick.ensure(m);
+    rs_state.ensure(m);
+    rs_cK.ensure(m);
+    rs_cE.ensure(m);
+    rs_rK.ensure(m + 1);
+    rs_rE.ensure(m + 1);
+    rs_slot.ensure(m);
+    rs_partial.ensure(scan_partials_needed((uint32_t)m) + 1);
+    rs_takers.ensure(m);
+    rs_mv.ensure(m);
+    if (rs_items.n < m) rs_items.ensure(m);
+    ResampleArgs a;
+    a.tiles = d_tiles.p;
+    a.scenes = d_scenes.p;
+    a.start = start.p;
+    a.posm = posm[ns].p;
+    a.vel = vel[ns].p;
+    a.tag = tag[nt_].p;
+    a.rq = rq.p;
+    a.n = n;
+    a.merge_hi = merge_hi;
+    a.split_lo = split_lo;
+    a.min_mass = min_mass;
+    const float q = 0.25f * h;
+    a.md2 = q * q;
+    a.off = q;
+    a.cap = cap;
+    a.ctr = rs_ctr.p;
+    a.flag = rs_flag.p;
+    a.takers = rs_takers.p;
+    a.minT = rs_minT.p;
+    a.pick = rs_pick.p;
+    a.state = rs_state.p;
+    a.mv = rs_mv.p;
+    a.cK = rs_cK.p;
+    a.cE = rs_cE.p;
+    a.rK = rs_rK.p;
+    a.rE = rs_rE.p;
+    a.slot_of = rs_slot.p;
+    a.partial = rs_partial.p;
+    a.posm_out = posm[cs].p;
+    a.vel_out = vel[cs].p;
+    a.tag_out = tag[ct].p;
+    for (;;) {
+        a.items = rs_items.p;
+        a.pcap = (uint32_t)std::min<size_t>(rs_items.n, 0xFFFFFFFFu);
+        run(KIND_RESAMPLE, [&] { return launch_resample(stream, a); });
+        PH_CK(hipMemcpyAsync(rs_host, rs_ctr.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+        PH_CK(hipStreamSynchronize(stream));
+        if (rs_host[1] <= a.pcap) break;
+        if (rs_host[1] > 0xFFFFFFFFull) throw std::runtime_error("resample: more than 2^32 merge candidates");
+        rs_items.ensure((size_t)rs_host[1]);  // the candidate store was too small: grow, run again
+    }
+    const uint64_t kept = rs_host[3], splits = rs_host[2];
+    changed = kept != n || splits != 0;
+    return (uint32_t)(kept + splits);
 }
 
 // Generation flip of the tags, records, domain monitor.

@@ -15,6 +15,7 @@
 #include "../../include/phieng_fluid2.h"
 #include "../../include/phieng_fluid3.h"
 #include "engine.h"
+#include "resample.h"
 #include "sph_kernels.h"
 #include "surrogate.h"
 #include "surrogate_step.h"
@@ -310,6 +311,10 @@ struct World3 {
             if (sub != 1) sync_scene(sdt);
             for (uint32_t k = 0; k < sub; ++k) {
                 if (cfg.boundary_particles || eng.scenes[0].ghost_count) update_ghosts();
+                if (cfg.adaptive_resolution) {
+                    substep_adaptive(sdt);
+                    continue;
+                }
                 if (!sur && !col) {
                     eng.substep();
                     continue;
@@ -508,6 +513,53 @@ struct World3 {
         ++e.substeps_done;
     }
 
+    // A substep with adaptive resolution (fluid3.rs:1001-1012): density,
+    // resample (merge, split), then the cell sort and density of the new
+    // particle set before the ghosts and the force pass.  The split cap is
+    // the world's capacity (MAX_PARTICLES_3D, 4,096, unless reserved
+    // higher).  One host synchronisation per substep reads the new count.
+    void substep_adaptive(float sdt) {
+        ph::Engine &e = eng;
+        const uint64_t room = std::min<uint64_t>(n, capacity / 2 + 1);
+        if (n + room > kMaxCapacity)
+            throw std::runtime_error("adaptive resolution: the particle count plus its split room exceeds 2^24");
+        e.ensure_particles((uint32_t)(n + room));
+        e.sort_phase(nullptr);
+        e.density_phase();
+        const float rest = std::max(cfg.rest_density, 1e-6f);
+        bool changed = false;
+        const uint32_t n2 =
+            e.resample_phase(cfg.merge_density_ratio * rest, cfg.split_density_ratio * rest,
+                             std::max(cfg.particle_mass, 1e-6f) * 0.125f, std::max(cfg.smoothing_length, 1e-6f),
+                             capacity, changed);
+        if (changed) {
+            n = n2;
+            sync_scene(sdt);
+            e.sort_phase(nullptr);
+        }
+        e.plain_phase(nullptr);
+        e.finish_substep();
+    }
+
+    // Mass and phase per particle, in index order (an extension: the
+    // reference's C ABI does not read them back).
+    uint32_t get_mass(float *mass, uint8_t *phase, uint32_t max) {
+        const uint32_t m = std::min(max, n);
+        if (m == 0) return 0;
+        eng.init_device();
+        eng.export_buf.ensure(n);
+        eng.staging_u8.ensure(n);
+        eng.run(ph::KIND_EXPORT, [&] {
+            return ph::launch_export_mass(eng.stream, n, eng.posm[eng.cs].p, eng.tag[eng.ct].p, eng.export_buf.p,
+                                          eng.staging_u8.p);
+        });
+        if (mass)
+            PH_CK(hipMemcpyAsync(mass, eng.export_buf.p, sizeof(float) * m, hipMemcpyDeviceToHost, eng.stream));
+        if (phase) PH_CK(hipMemcpyAsync(phase, eng.staging_u8.p, m, hipMemcpyDeviceToHost, eng.stream));
+        PH_CK(hipStreamSynchronize(eng.stream));
+        return m;
+    }
+
     uint32_t add(const float *xyz, const uint8_t *phases, uint32_t count) {
         const uint32_t m = std::min(count, kMaxPerCall);
         for (size_t i = 0; i < 3 * (size_t)m; ++i)
@@ -656,8 +708,9 @@ int32_t phieng_world3_add_body(World3 *world, float px, float py, float pz, uint
 int32_t phieng_world3_set_fluid_config(World3 *world, const FluidConfig3Ffi *cfg) {
     if (!world || !cfg) return 0;
     if (!config_valid(*cfg)) return 0;
-    if (cfg->adaptive_resolution) {
-        set_error("phieng_world3_set_fluid_config", "adaptive_resolution is not implemented on the GPU path");
+    if (cfg->adaptive_resolution && (world->sur || world->col)) {
+        set_error("phieng_world3_set_fluid_config",
+                  "adaptive_resolution with the learned surrogate attached: the surrogate keeps a fixed particle set");
         return 0;
     }
     if (cfg->boundary_particles && world->sur) {
@@ -667,7 +720,7 @@ int32_t phieng_world3_set_fluid_config(World3 *world, const FluidConfig3Ffi *cfg
     }
     FluidConfig3Ffi c = *cfg;
     // Rust bools read back as 0/1.
-    c.adaptive_resolution = 0;
+    c.adaptive_resolution = cfg->adaptive_resolution ? 1 : 0;
     c.boundary_particles = cfg->boundary_particles ? 1 : 0;
     c.enabled = cfg->enabled ? 1 : 0;
     world->cfg = c;
@@ -720,6 +773,16 @@ uint32_t phieng_world3_fluid_get(const World3 *world, float *out, uint32_t max) 
         return const_cast<World3 *>(world)->get(out, max);
     } catch (const std::exception &e) {
         set_error("phieng_world3_fluid_get", e.what());
+        return 0;
+    }
+}
+
+uint32_t phieng_world3_fluid_get_mass(const World3 *world, float *mass, uint8_t *phase, uint32_t max) {
+    if (!world || (!mass && !phase)) return 0;
+    try {
+        return const_cast<World3 *>(world)->get_mass(mass, phase, max);
+    } catch (const std::exception &e) {
+        set_error("phieng_world3_fluid_get_mass", e.what());
         return 0;
     }
 }
@@ -839,6 +902,10 @@ int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check
         set_error("phieng_world3_set_surrogate", "the surrogate does not model boundary ghosts (boundary_particles)");
         return 0;
     }
+    if (s && world->cfg.adaptive_resolution) {
+        set_error("phieng_world3_set_surrogate", "the surrogate keeps a fixed particle set (adaptive_resolution)");
+        return 0;
+    }
     world->sur = s;
     world->check_every = check_every ? check_every : 1;
     world->max_err = max_density_error;
@@ -851,6 +918,10 @@ int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check
 
 int32_t phieng_world3_collect_samples(World3 *world, Surrogate3 *s, uint32_t every, uint32_t stride) {
     if (!world || (s && (every < 1 || stride < 1))) return 0;
+    if (s && world->cfg.adaptive_resolution) {
+        set_error("phieng_world3_collect_samples", "sample collection keeps a fixed particle set (adaptive_resolution)");
+        return 0;
+    }
     world->col = s;
     world->col_every = every ? every : 1;
     world->col_stride = stride ? stride : 1;
@@ -1096,6 +1167,9 @@ uint32_t phieng_world_fluid_add_phase(World2 *world, const float *positions, con
 uint32_t phieng_world_fluid_count(const World2 *world) { return world ? world->n : 0; }
 
 int32_t phieng_world_fluid_clear(World2 *world) { return world ? phieng_world3_fluid_clear(as3(world)) : 0; }
+uint32_t phieng_world_fluid_get_mass(const World2 *world, float *mass, uint8_t *phase, uint32_t max) {
+    return world ? phieng_world3_fluid_get_mass(as3(world), mass, phase, max) : 0;
+}
 
 uint32_t phieng_world_fluid_get(const World2 *world, float *out, uint32_t max) {
     if (!world || !out) return 0;

@@ -103,6 +103,7 @@ SIGNATURES = {
     "phieng_world3_fluid_clear": (_I32, [_P]),
     "phieng_world3_fluid_get": (_U32, [_P, _fp, _U32]),
     "phieng_world3_fluid_reserve": (_U32, [_P, _U32]),
+    "phieng_world3_fluid_get_mass": (_U32, [_P, _fp, _u8p, _U32]),
     "phieng_world3_fluid_set_velocity": (_U32, [_P, _fp, _U32]),
     "phieng_world3_fluid_set_record": (_I32, [_P, _U32]),
     "phieng_world3_fluid_get_record": (_U32, [_P, _i32p, _u32p, _fp, _fp, _fp, _u32p, _U32]),
@@ -156,6 +157,7 @@ SIGNATURES = {
     "phieng_world_fluid_clear": (_I32, [_P]),
     "phieng_world_fluid_get": (_U32, [_P, _fp, _U32]),
     "phieng_world_fluid_reserve": (_U32, [_P, _U32]),
+    "phieng_world_fluid_get_mass": (_U32, [_P, _fp, _u8p, _U32]),
     "phieng_world_fluid_set_velocity": (_U32, [_P, _fp, _U32]),
     "phieng_world_fluid_set_record": (_I32, [_P, _U32]),
     "phieng_world_fluid_get_record": (_U32, [_P, _i32p, _u32p, _fp, _fp, _fp, _u32p, _U32]),

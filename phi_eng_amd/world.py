@@ -103,6 +103,18 @@ class World3:
             raise RuntimeError(f"fluid_get returned {got} of {n}: {last_error()}")
         return out[:, :3].copy(), out[:, 3:].copy()
 
+    def get_mass(self):
+        """(masses float32, phases uint8) in insertion order."""
+        n = self.count()
+        m = np.empty(n, dtype=np.float32)
+        ph = np.empty(n, dtype=np.uint8)
+        if n == 0:
+            return m, ph
+        got = self._l.phieng_world3_fluid_get_mass(self.ptr, _fptr(m), ph.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n)
+        if got != n:
+            raise RuntimeError(f"fluid_get_mass returned {got} of {n}: {last_error()}")
+        return m, ph
+
     def set_velocity(self, vel):
         v = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 3)
         return self._l.phieng_world3_fluid_set_velocity(self.ptr, _fptr(v), len(v))
@@ -269,6 +281,18 @@ class World2:
         if got != n:
             raise RuntimeError(f"fluid_get returned {got} of {n}: {last_error()}")
         return out[:, :2].copy(), out[:, 2:].copy()
+
+    def get_mass(self):
+        """(masses float32, phases uint8) in insertion order."""
+        n = self.count()
+        m = np.empty(n, dtype=np.float32)
+        ph = np.empty(n, dtype=np.uint8)
+        if n == 0:
+            return m, ph
+        got = self._l.phieng_world_fluid_get_mass(self.ptr, _fptr(m), ph.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n)
+        if got != n:
+            raise RuntimeError(f"fluid_get_mass returned {got} of {n}: {last_error()}")
+        return m, ph
 
     def set_velocity(self, vel):
         v = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 2)

@@ -85,14 +85,15 @@ def test_phase_tables_validated():
         assert not w.set_fluid_config(c)
 
 
-def test_unimplemented_options_rejected():
-    """adaptive_resolution is refused (not built); boundary_particles is
-    accepted and reads back as a Rust bool."""
+def test_option_flags_read_back_as_bools():
+    """adaptive_resolution and boundary_particles are accepted and read back
+    as Rust bools; adaptive resolution is refused while a surrogate is
+    attached (the surrogate keeps a fixed particle set)."""
     with World3() as w:
         c = FluidConfig3Ffi.default()
-        c.adaptive_resolution = 1
-        assert not w.set_fluid_config(c)
-        assert "adaptive_resolution" in ffi.last_error()
+        c.adaptive_resolution = 5
+        assert w.set_fluid_config(c)
+        assert w.get_fluid_config().adaptive_resolution == 1
         c = FluidConfig3Ffi.default()
         c.boundary_particles = 7
         assert w.set_fluid_config(c)
@@ -117,10 +118,13 @@ def test_null_tolerance():
     assert lib.phieng_world3_fluid_clear(None) == 0
     assert lib.phieng_world3_fluid_get(None, fp, 1) == 0
     assert lib.phieng_world3_fluid_reserve(None, 10) == 0
+    assert lib.phieng_world3_fluid_get_mass(None, fp, None, 1) == 0
     assert lib.phieng_world3_sync(None) == 0
     with World3() as w:
         assert lib.phieng_world3_fluid_add(w.ptr, None, 1) == 0
         assert lib.phieng_world3_fluid_get(w.ptr, None, 1) == 0
+        assert lib.phieng_world3_fluid_get_mass(w.ptr, None, None, 1) == 0
+        assert lib.phieng_world3_fluid_get_mass(w.ptr, fp, None, 1) == 0  # empty fluid
         assert lib.phieng_world3_get_fluid_config(w.ptr, None) == 0
         lib.phieng_world3_step(w.ptr, float("nan"))
         lib.phieng_world3_step(w.ptr, -1.0)

@@ -38,11 +38,12 @@ def test_invalid_config_rejected_world_unchanged(field, value):
         assert bytes(w.get_fluid_config()) == before
 
 
-def test_unimplemented_options_rejected():
+def test_option_flags_accepted():
     with World2() as w:
         c = FluidConfig3Ffi.default()
         c.adaptive_resolution = 1
-        assert not w.set_fluid_config(c)
+        assert w.set_fluid_config(c)
+        assert w.get_fluid_config().adaptive_resolution == 1
         c = FluidConfig3Ffi.default()
         c.boundary_particles = 1
         assert w.set_fluid_config(c)
