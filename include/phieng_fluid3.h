@@ -16,6 +16,7 @@
 #ifndef PHIENG_FLUID3_H
 #define PHIENG_FLUID3_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -101,6 +102,28 @@ int32_t phieng_world3_fluid_clear(World3 *world);
 /* ffi.rs:3672-3697 — writes [x,y,z,vx,vy,vz] per particle in insertion
  * order, up to max; returns the number written. */
 uint32_t phieng_world3_fluid_get(const World3 *world, float *out, uint32_t max);
+
+/* Snapshots (ffi.rs:3097-3105, 4036-4117; formats snapshot.rs).  Format
+ * tags and load modes as the reference's. */
+#define PHIENG_SNAPSHOT_JSON 0u
+#define PHIENG_SNAPSHOT_BINARY 1u
+#define PHIENG_SNAPSHOT_EXACT 0u
+#define PHIENG_SNAPSHOT_FRESH 1u
+/* ffi.rs:4042-4082 — write the world to path_ptr[0 .. path_len) (UTF-8,
+ * at most 4096 bytes) as JSON or as the PHE1 binary envelope, in the
+ * reference's World3Dto layout at the lowest schema the world needs (3 to
+ * 7).  Returns 1 on success, 0 for invalid input (null world or path, bad
+ * format or mode), -1 on IO failure, -2 on encode failure. */
+int32_t phieng_world3_save(const World3 *world, const uint8_t *path_ptr, size_t path_len, uint32_t format,
+                           uint32_t mode);
+/* ffi.rs:4084-4111 — a new world from a file (format detected from its
+ * first non-blank byte), or null on any failure (phieng_last_error says
+ * why).  Besides the reference's checks it refuses what the GPU fluid path
+ * cannot hold: schema 8 or later, dynamic, rotated, moving or hull bodies,
+ * joints, contacts, ambient turbulence, configs the C ABI rejects; it
+ * accepts more than MAX_PARTICLES_3D particles.  Both modes load the same
+ * state (a fluid world holds no warm-start cache or joint impulse). */
+World3 *phieng_world3_load(const uint8_t *path_ptr, size_t path_len, uint32_t mode);
 
 /* ---------------------------------------------------------------------
  * Extensions (no reference counterpart).

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -96,6 +97,24 @@ bool config_valid(const FluidConfig3Ffi &c) {
     return true;
 }
 
+// The snapshot record of a static body made by add_body: the material is
+// sanitised as PhysicsMaterial::new and RigidBody3::set_material do
+// (material.rs:40-60, body.rs:1306-1317).
+SnapBody snap_body(const BodyDev &bd, float density, float friction, float restitution) {
+    SnapBody b;
+    b.shape = bd.shape;
+    b.pos[0] = bd.px;
+    b.pos[1] = bd.py;
+    b.pos[2] = bd.pz;
+    b.a = bd.a;
+    b.b = bd.b;
+    b.c = bd.c;
+    b.friction = std::isfinite(friction) && friction >= 0.0f ? friction : 0.4f;
+    b.restitution = std::isfinite(restitution) ? std::min(std::max(restitution, 0.0f), 1.0f) : 0.1f;
+    b.density = density;
+    return b;
+}
+
 }  // namespace
 
 struct World3 {
@@ -103,6 +122,8 @@ struct World3 {
     FluidConfig3Ffi cfg = default_config();
     float g[3] = {0.0f, -9.81f, 0.0f};  // World3::new (world.rs)
     std::vector<BodyDev> bodies;
+    std::vector<SnapBody> body_meta;  // material, damping, ... of each body, for snapshots
+    SnapMeta meta;                    // world fields the GPU step does not read, for snapshots
     int32_t next_body_id = 0;
     float time = 0.0f;
     uint32_t capacity = kDefaultCapacity;
@@ -560,6 +581,35 @@ struct World3 {
         return m;
     }
 
+    // Replace the fluid by n particles with the given state (insertion
+    // order), as a snapshot load does (snapshot.rs:4587-4612).
+    void load_particles(const float *xyz, const float *v3, const float *mass, const uint8_t *phases, uint32_t count) {
+        eng.init_device();
+        n = 0;
+        eng.n_total = 0;
+        eng.ensure_particles(count);
+        std::vector<float4> pm(count), vv(count);
+        std::vector<uint32_t> tg(count);
+        for (uint32_t i = 0; i < count; ++i) {
+            pm[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], mass[i]);
+            vv[i] = make_float4(v3[3 * i], v3[3 * i + 1], v3[3 * i + 2], 0.0f);
+            tg[i] = i | ((uint32_t)phases[i] << 24);
+        }
+        PH_CK(hipMemcpyAsync(eng.posm[eng.cs].p, pm.data(), sizeof(float4) * count, hipMemcpyHostToDevice,
+                             eng.stream));
+        PH_CK(hipMemcpyAsync(eng.vel[eng.cs].p, vv.data(), sizeof(float4) * count, hipMemcpyHostToDevice,
+                             eng.stream));
+        PH_CK(hipMemcpyAsync(eng.tag[eng.ct].p, tg.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice,
+                             eng.stream));
+        PH_CK(hipStreamSynchronize(eng.stream));
+        n = count;
+        eng.n_total = n;
+        eng.build_tiles();
+        eng.rec_valid = false;
+        eng.aux_on = false;
+        domain_dirty = true;
+    }
+
     uint32_t add(const float *xyz, const uint8_t *phases, uint32_t count) {
         const uint32_t m = std::min(count, kMaxPerCall);
         for (size_t i = 0; i < 3 * (size_t)m; ++i)
@@ -632,6 +682,69 @@ void world3_device_state_mut(World3 *w, float4 **posm, float4 **vel, uint32_t **
     e.rec_valid = false;
 }
 
+void snapshot_set_error(const char *where, const std::string &what) { set_error(where, what.c_str()); }
+
+void world3_snapshot(World3 *w, SnapWorld &out) {
+    if (w->dims != 3) throw std::runtime_error("a 2D world is not written in the 3D snapshot format");
+    out.gravity[0] = w->g[0];
+    out.gravity[1] = w->g[1];
+    out.gravity[2] = w->g[2];
+    out.time = w->time;
+    out.meta = w->meta;
+    out.bodies = w->body_meta;
+    out.cfg = w->cfg;
+    const uint32_t n = w->n;
+    out.pos.assign(3 * (size_t)n, 0.0f);
+    out.vel.assign(3 * (size_t)n, 0.0f);
+    out.mass.assign(n, 0.0f);
+    out.phase.assign(n, 0);
+    if (!n) return;
+    std::vector<float> pv(6 * (size_t)n);
+    if (w->get(pv.data(), n) != n || w->get_mass(out.mass.data(), out.phase.data(), n) != n)
+        throw std::runtime_error("reading the particles back failed");
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            out.pos[3 * i + k] = pv[6 * i + k];
+            out.vel[3 * i + k] = pv[6 * i + 3 + k];
+        }
+}
+
+World3 *world3_from_snapshot(const SnapWorld &s, std::string &err) {
+    std::unique_ptr<World3> w(new World3());
+    if (!config_valid(s.cfg)) {
+        err = "the fluid config fails the C ABI's validation (ffi.rs:857-914)";
+        return nullptr;
+    }
+    if (s.meta.turb_on) {
+        err = "ambient turbulence acts on the fluid and is not implemented on the GPU path";
+        return nullptr;
+    }
+    if (s.bodies.size() > PH_MAX_BODIES) {
+        err = "too many bodies";
+        return nullptr;
+    }
+    w->cfg = s.cfg;
+    w->cfg.adaptive_resolution = s.cfg.adaptive_resolution ? 1 : 0;
+    w->cfg.boundary_particles = s.cfg.boundary_particles ? 1 : 0;
+    w->cfg.enabled = s.cfg.enabled ? 1 : 0;
+    for (int k = 0; k < 3; ++k) w->g[k] = s.gravity[k];
+    w->time = s.time;
+    w->meta = s.meta;
+    for (const SnapBody &b : s.bodies) {
+        w->bodies.push_back(BodyDev{b.shape, b.pos[0], b.pos[1], b.pos[2], b.a, b.b, b.c});
+        w->body_meta.push_back(b);
+        ++w->next_body_id;
+    }
+    const uint32_t n = (uint32_t)s.mass.size();
+    if (n > kMaxCapacity) {
+        err = "more than 2^24 particles";
+        return nullptr;
+    }
+    w->capacity = std::max(kDefaultCapacity, n);
+    if (n) w->load_particles(s.pos.data(), s.vel.data(), s.mass.data(), s.phase.data(), n);
+    return w.release();
+}
+
 extern "C" {
 
 const char *phieng_last_error(void) { return g_last_error.c_str(); }
@@ -672,8 +785,6 @@ void phieng_world3_set_gravity(World3 *world, float gx, float gy, float gz) {
 
 int32_t phieng_world3_add_body(World3 *world, float px, float py, float pz, uint32_t shape_tag, float a, float b,
                                float c, float density, uint8_t is_static, float friction, float restitution) {
-    (void)friction;
-    (void)restitution;
     if (!world || !std::isfinite(px) || !std::isfinite(py) || !std::isfinite(pz)) return -1;
     BodyDev bd{shape_tag, px, py, pz, a, b, c};
     switch (shape_tag) {
@@ -702,6 +813,7 @@ int32_t phieng_world3_add_body(World3 *world, float px, float py, float pz, uint
         return -1;
     }
     world->bodies.push_back(bd);
+    world->body_meta.push_back(snap_body(bd, density, friction, restitution));
     return world->next_body_id++;
 }
 

@@ -4,6 +4,7 @@ Method names follow the reference (``World3::step``, ``fluid.add_particles``,
 ``phieng_world3_*`` in crates/engine/src/ffi.rs); arrays are numpy.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -20,13 +21,26 @@ def _fptr(a):
 class World3:
     """Owns one ``World3*`` from ``phieng_world3_new`` (ffi.rs:3199)."""
 
-    def __init__(self, capacity=None):
+    def __init__(self, capacity=None, _ptr=None):
         self._l = lib()
-        self.ptr = self._l.phieng_world3_new()
+        self.ptr = _ptr if _ptr else self._l.phieng_world3_new()
         if not self.ptr:
             raise MemoryError("phieng_world3_new returned null")
         if capacity is not None:
             self.reserve(capacity)
+
+    # -- snapshots (ffi.rs:4036-4117) -----------------------------------
+    def save(self, path, binary=False, fresh=False):
+        """phieng_world3_save; returns its code (1, 0, -1 or -2)."""
+        b = os.fsencode(path)
+        return self._l.phieng_world3_save(self.ptr, b, len(b), int(binary), int(fresh))
+
+    @classmethod
+    def load(cls, path, fresh=False):
+        """phieng_world3_load; a new World3, or None (last_error says why)."""
+        b = os.fsencode(path)
+        ptr = lib().phieng_world3_load(b, len(b), int(fresh))
+        return cls(_ptr=ptr) if ptr else None
 
     def close(self):
         if self.ptr:
