@@ -46,9 +46,10 @@ def _args():
                     help="HBM bytes per launch per kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE (optional)")
     ap.add_argument("--valu", default=os.path.join(ROOT, "profiles", "valu.json"),
                     help="VALU wave instructions per launch per kernel from rocprofv3 SQ_INSTS_VALU (optional)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "scale8m"],
+    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "scale8m", "extras"],
                     help="plain: the 1M dam break (the headline); batch: 256 uneven scenes in one launch per step; "
-                         "scale8m: 8,388,608 particles, two colliding columns, plain vs surrogate")
+                         "scale8m: 8,388,608 particles, two colliding columns, plain vs surrogate; extras: the "
+                         "1M dam break with adaptive resolution, and snapshot save / load of the 1M world")
     ap.add_argument("--calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (rocprof counter scale)")
     ap.add_argument("--sur-steps", type=int, default=200,
@@ -290,6 +291,8 @@ def main():
         return bench_batch(a)
     if a.workload == "scale8m":
         return bench_scale(a)
+    if a.workload == "extras":
+        return bench_extras(a)
     from phi_eng_amd import scenes
     from phi_eng_amd.ffi import last_error
 
@@ -390,6 +393,85 @@ def main():
     w.close()
     if a.sur_steps > 0:
         out["surrogate"] = bench_surrogate(a, value)
+    print(json.dumps(out))
+
+
+def bench_extras(a):
+    """The 1M dam break with adaptive resolution (capacity 4M, so splits are
+    not capped; split ratio 0.6, so the thinnest surface particles, whose
+    density starts at ~0.51 rest, split from the first substep) against the
+    same run without it; then JSON and binary snapshots of the 1M world:
+    write, read back, and check the loaded state."""
+    import tempfile
+
+    import numpy as np
+
+    from phi_eng_amd import scenes
+    from phi_eng_amd.ffi import last_error
+    from phi_eng_amd.world import World3
+
+    sc = scenes.c2_dam_break_1m()
+    steps = max(1, min(a.steps, 200))
+    runs = {}
+    for name, adaptive in (("plain", False), ("adaptive", True)):
+        w = scenes.make_world(sc)
+        w.reserve(4 * sc.n)
+        cfg = w.get_fluid_config()
+        cfg.adaptive_resolution = int(adaptive)
+        cfg.split_density_ratio = 0.6
+        cfg.particle_mass = cfg.phase_mass[0]
+        assert w.set_fluid_config(cfg)
+        for _ in range(5):
+            w.step(sc.dt)
+        w.sync()
+        counts = 0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            counts += w.count()
+            w.step(sc.dt)
+        w.sync()
+        wall = time.perf_counter() - t0
+        err = last_error()
+        if err:
+            raise RuntimeError(err)
+        m, _ = w.get_mass()
+        runs[name] = {"value": round(counts / wall, 1), "ms_per_step": round(wall * 1e3 / steps, 4),
+                      "particles_end": int(w.count()), "mass_total": float(m.astype(np.float64).sum())}
+        if name == "adaptive":
+            snap = {}
+            with tempfile.TemporaryDirectory() as d:
+                for fmt, binary in (("json", False), ("binary", True)):
+                    path = os.path.join(d, "w." + fmt)
+                    t0 = time.perf_counter()
+                    rc = w.save(path, binary=binary)
+                    t1 = time.perf_counter()
+                    if rc != 1:
+                        raise RuntimeError(f"save {fmt}: {rc} {last_error()}")
+                    r = World3.load(path)
+                    t2 = time.perf_counter()
+                    if r is None:
+                        raise RuntimeError(f"load {fmt}: {last_error()}")
+                    p0, v0 = w.get_state()
+                    p1, v1 = r.get_state()
+                    same = bool(np.array_equal(p0.view(np.uint32), p1.view(np.uint32))
+                                and np.array_equal(v0.view(np.uint32), v1.view(np.uint32)))
+                    r.close()
+                    mb = os.path.getsize(path) / 1e6
+                    snap[fmt] = {"bytes": os.path.getsize(path), "save_s": round(t1 - t0, 3),
+                                 "load_s": round(t2 - t1, 3), "save_MB_s": round(mb / (t1 - t0), 1),
+                                 "load_MB_s": round(mb / (t2 - t1), 1), "state_identical": same}
+            runs[name]["snapshot"] = snap
+        w.close()
+    ad, pl = runs["adaptive"], runs["plain"]
+    out = {"metric": "particle-steps/sec at 1M-particle 3D SPH with adaptive resolution", "value": ad["value"],
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": 5, "ms_per_step": ad["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic: seeded jittered lattice dam break",
+           "config": {"workload": "dam_break_3d_1m_adaptive", "particles": sc.n, "capacity": 4 * sc.n,
+                      "split_density_ratio": 0.6, "merge_density_ratio": 1.6, "dt": sc.dt,
+                      "path": "plain + resample"},
+           "plain": pl, "adaptive": ad,
+           "cost_vs_plain": round(ad["ms_per_step"] / pl["ms_per_step"], 3)}
     print(json.dumps(out))
 
 

@@ -192,8 +192,11 @@ struct Engine {
     void sort_phase(uint32_t *perm_out);
     // Density pass (writing the neighbour list), boundary ghosts, force pass
     // (with af, also the surrogate's history, records and judge sums).
-    void plain_phase(float4 *acc_sorted, const AuxFuse *af = nullptr);
-    // The density pass alone (the first of an adaptive substep).
+    // density_done: the density pass of this sorted state already ran
+    // (density_phase); only the ghosts and the force pass remain.
+    void plain_phase(float4 *acc_sorted, const AuxFuse *af = nullptr, bool density_done = false);
+    // The density pass alone (the first of an adaptive substep), with the
+    // same outputs as plain_phase's.
     void density_phase();
     // Adaptive resolution (fluid3.rs:923-987) over the sorted state and
     // densities of this substep.  Writes the new particle set, in its new

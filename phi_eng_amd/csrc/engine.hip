@@ -487,7 +487,7 @@ void Engine::sort_phase(uint32_t *perm_out) {
 }
 
 // K4 + K5 of the plain step on generation 1-cs, writing generation cs.
-void Engine::plain_phase(float4 *acc_sorted, const AuxFuse *af) {
+void Engine::plain_phase(float4 *acc_sorted, const AuxFuse *af, bool density_done) {
     const uint32_t nt = (uint32_t)tiles.size();
     const bool rd = rec_flags & PHIENG_RECORD_DENSITY, ra = rec_flags & PHIENG_RECORD_ACCEL;
     const int ns = 1 - cs, nt_ = 1 - ct;
@@ -506,11 +506,12 @@ void Engine::plain_phase(float4 *acc_sorted, const AuxFuse *af) {
         });
         lo.skin_factor2_dev = list_f2.p;
     }
-    run(KIND_DENSITY, [&] {
-        return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p, masks.p,
-                                   flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
-                                   rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr, lo);
-    });
+    if (!density_done)
+        run(KIND_DENSITY, [&] {
+            return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p,
+                                       masks.p, flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
+                                       rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr, lo);
+        });
     // boundary ghosts (fluid3.rs:1018-1027): their density and pressure from
     // the fluid of this substep, before the force pass reads them
     const bool ghosts = scenes.size() == 1 && scenes[0].ghost_count;
@@ -529,6 +530,7 @@ void Engine::plain_phase(float4 *acc_sorted, const AuxFuse *af) {
 
 void Engine::density_phase() {
     const uint32_t nt = (uint32_t)tiles.size();
+    const bool rd = rec_flags & PHIENG_RECORD_DENSITY;
     const int ns = 1 - cs, nt_ = 1 - ct;
     ListOut lo;
     lo.cnt = nl_cnt.p;
@@ -536,7 +538,8 @@ void Engine::density_phase() {
     lo.skin_factor2 = 1.0f;
     run(KIND_DENSITY, [&] {
         return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p, masks.p,
-                                   flags.p, (uint32_t)alloc_particles, nullptr, nullptr, nullptr, lo);
+                                   flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
+                                   rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr, lo);
     });
 }
 

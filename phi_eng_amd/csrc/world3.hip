@@ -558,7 +558,9 @@ struct World3 {
             sync_scene(sdt);
             e.sort_phase(nullptr);
         }
-        e.plain_phase(nullptr);
+        // nothing merged or split: the sorted state and its densities are
+        // those of the second pass of fluid3.rs:1010-1011 already
+        e.plain_phase(nullptr, nullptr, !changed);
         e.finish_substep();
     }
 
