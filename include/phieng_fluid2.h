@@ -78,6 +78,14 @@ int32_t phieng_world_fluid_clear(World2 *world);
  * up to max; returns the number written. */
 uint32_t phieng_world_fluid_get(const World2 *world, float *out, uint32_t max);
 
+/* ffi.rs:3107-3160 — as phieng_world3_save, for a World2 (World2Dto:
+ * 2-vectors, the broad-phase cell size, circle and box shapes, kind 2). */
+int32_t phieng_world_save(const World2 *world, const uint8_t *path_ptr, size_t path_len, uint32_t format,
+                          uint32_t mode);
+/* ffi.rs:3162-3190 — as phieng_world3_load, for a World2 (polygon bodies
+ * are refused with the rest of phieng_world3_load's list). */
+World2 *phieng_world_load(const uint8_t *path_ptr, size_t path_len, uint32_t mode);
+
 /* ---------------------------------------------------------------------
  * Extensions (no reference counterpart), as their World3 namesakes.
  * ------------------------------------------------------------------- */

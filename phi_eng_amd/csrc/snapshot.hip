@@ -32,6 +32,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/phieng_fluid2.h"
 #include "../../include/phieng_fluid3.h"
 #include "world3_internal.h"
 
@@ -41,7 +42,7 @@ namespace {
 
 constexpr uint32_t kSchemaMin = 3, kSchemaGravity = 4, kSchemaAmbient = 5, kSchemaCcd = 6, kSchemaSolverFlags = 7,
                    kSchemaMax = 17, kSchemaFirstUnsupported = 8;
-constexpr uint8_t kKindWorld3 = 3;
+constexpr uint8_t kKindWorld2 = 2, kKindWorld3 = 3;
 
 struct SnapError : std::runtime_error {
     int code;  // -1 IO, -2 encode / parse / corrupt
@@ -161,12 +162,13 @@ struct JsonOut {
         key(k);
         b(v);
     }
+    int dims = 3;  // components of every vector written
     void kv3(const char *k, const float *v) {
         key(k);
         open('{');
         kf("x", v[0]);
         kf("y", v[1]);
-        kf("z", v[2]);
+        if (dims == 3) kf("z", v[2]);
         close('}');
     }
     void item() {
@@ -180,6 +182,7 @@ std::string to_json(const SnapWorld &w) {
     const SnapMeta &m = w.meta;
     const FluidConfig3Ffi &c = w.cfg;
     JsonOut o;
+    o.dims = w.dims;
     o.open('{');
     o.ku("schema", schema_of(w));
     o.kv3("gravity", w.gravity);
@@ -263,6 +266,7 @@ std::string to_json(const SnapWorld &w) {
     o.close('}');
     o.ku("next_body_id", w.bodies.size());
     o.ku("next_joint_id", 0);
+    if (w.dims == 2) o.kf("cell_size", w.cell_size);
     o.key("bodies");
     o.open('[');
     const float zero3[3] = {0.0f, 0.0f, 0.0f};
@@ -287,7 +291,14 @@ std::string to_json(const SnapWorld &w) {
         o.key("shape");
         o.open('{');
         o.key("type");
-        if (b.shape == PHIENG_SHAPE3_SPHERE) {
+        if (w.dims == 2 && b.shape == 0) {  // Shape2Dto::Circle
+            o.s += "\"circle\"";
+            o.kf("radius", b.a);
+        } else if (w.dims == 2) {  // Shape2Dto::Box
+            o.s += "\"box\"";
+            const float he[3] = {b.a, b.b, 0.0f};
+            o.kv3("half_extents", he);
+        } else if (b.shape == PHIENG_SHAPE3_SPHERE) {
             o.s += "\"sphere\"";
             o.kf("radius", b.a);
         } else if (b.shape == PHIENG_SHAPE3_BOX) {
@@ -302,9 +313,13 @@ std::string to_json(const SnapWorld &w) {
         o.close('}');
         o.kv3("position", b.pos);
         o.key("rotation");
-        o.s += "[0.0, 0.0, 0.0, 1.0]";
+        // 2D: the columns of Mat2::IDENTITY (body2_dto); 3D: the identity quaternion
+        o.s += w.dims == 2 ? "[1.0, 0.0, 0.0, 1.0]" : "[0.0, 0.0, 0.0, 1.0]";
         o.kv3("velocity", zero3);
-        o.kv3("angular_velocity", zero3);
+        if (w.dims == 2)
+            o.kf("angular_velocity", 0.0f);
+        else
+            o.kv3("angular_velocity", zero3);
         if (b.gravity_scale != 1.0f) o.kf("gravity_scale", b.gravity_scale);
         if (b.ambient_scale != 1.0f) o.kf("ambient_scale", b.ambient_scale);
         o.close('}');
@@ -543,11 +558,12 @@ float of(const JVal &o, const char *k, float d) {
     const JVal *v = o.get(k);
     return v ? jf(*v, k) : d;
 }
+int g_dims = 3;  // dimension of the file being read (set by the loader)
 void rv3(const JVal &o, const char *k, float *out) {
     const JVal &v = req(o, k);
     out[0] = rf(v, "x");
     out[1] = rf(v, "y");
-    out[2] = rf(v, "z");
+    out[2] = g_dims == 3 ? rf(v, "z") : 0.0f;
 }
 
 void check_finite(float v, const char *what) {
@@ -565,6 +581,8 @@ struct BodyIn {
     float pos[3], rot[4], vel[3], angvel[3];
 };
 
+// BodyIn.shape_tag is the file's tag: 3D sphere 0, box 1, capsule 2, hull
+// 3, mesh 4; 2D circle 0, box 1, polygon 2 (encode_shape2/3).
 void add_body(SnapWorld &w, const BodyIn &in, SnapBody b) {
     check_finite(in.pos[0], "body.position");
     check_finite(in.pos[1], "body.position");
@@ -580,25 +598,31 @@ void add_body(SnapWorld &w, const BodyIn &in, SnapBody b) {
     if (in.id != w.bodies.size()) fail("body id out of order");
     if (!in.is_static) fail("a dynamic body: two-way coupling is not implemented on the GPU path");
     if (!in.has_shape) fail("a body without a shape");
-    if (in.shape_tag > 2) fail("a convex hull or mesh body: not a GPU boundary shape");
-    if (in.rot[0] != 0.0f || in.rot[1] != 0.0f || in.rot[2] != 0.0f || !(in.rot[3] > 0.0f))
-        fail("a rotated static body: the GPU boundary has identity rotations only");
+    if (in.shape_tag > (w.dims == 2 ? 1u : 2u)) fail("a hull, mesh or polygon body: not a GPU boundary shape");
+    const bool identity = w.dims == 2 ? in.rot[0] == 1.0f && in.rot[1] == 0.0f && in.rot[2] == 0.0f && in.rot[3] == 1.0f
+                                      : in.rot[0] == 0.0f && in.rot[1] == 0.0f && in.rot[2] == 0.0f && in.rot[3] > 0.0f;
+    if (!identity) fail("a rotated static body: the GPU boundary has identity rotations only");
     if (in.vel[0] != 0.0f || in.vel[1] != 0.0f || in.vel[2] != 0.0f || in.angvel[0] != 0.0f ||
         in.angvel[1] != 0.0f || in.angvel[2] != 0.0f)
         fail("a moving static body");
     for (int k = 0; k < 3; ++k) check_finite(in.dims[k], "shape");
     const bool ok = in.shape_tag == 0   ? in.dims[0] > 0.0f
-                    : in.shape_tag == 1 ? in.dims[0] > 0.0f && in.dims[1] > 0.0f && in.dims[2] > 0.0f
+                    : in.shape_tag == 1 ? in.dims[0] > 0.0f && in.dims[1] > 0.0f && (w.dims == 2 || in.dims[2] > 0.0f)
                                         : in.dims[0] > 0.0f && in.dims[1] > 0.0f;
     if (!ok || !(b.density > 0.0f)) fail("invalid shape or density");
     // set_material sanitises friction and restitution (body.rs:1306-1317)
     if (!(b.friction >= 0.0f)) b.friction = 0.4f;
     b.restitution = std::min(std::max(b.restitution, 0.0f), 1.0f);
-    b.shape = in.shape_tag == 0 ? PHIENG_SHAPE3_SPHERE : in.shape_tag == 1 ? PHIENG_SHAPE3_BOX : PHIENG_SHAPE3_CAPSULE;
+    if (w.dims == 2)  // a circle is a sphere in the z = 0 plane; the 2D box is its own shape
+        b.shape = in.shape_tag == 0 ? 0u : 3u;
+    else
+        b.shape = in.shape_tag == 0   ? PHIENG_SHAPE3_SPHERE
+                  : in.shape_tag == 1 ? PHIENG_SHAPE3_BOX
+                                      : PHIENG_SHAPE3_CAPSULE;
     for (int k = 0; k < 3; ++k) b.pos[k] = in.pos[k];
     b.a = in.dims[0];
     b.b = in.shape_tag == 0 ? 0.0f : in.dims[1];
-    b.c = in.shape_tag == 1 ? in.dims[2] : 0.0f;
+    b.c = w.dims == 3 && in.shape_tag == 1 ? in.dims[2] : 0.0f;
     b.sleeping = in.sleeping;
     w.bodies.push_back(b);
 }
@@ -674,12 +698,14 @@ void world_checks(const SnapWorld &w) {
     if (m.ccd_max_substeps < 1) fail("ccd.max_substeps must be >= 1");
 }
 
-SnapWorld from_json(const std::string &text) {
+SnapWorld from_json(const std::string &text, int dims) {
     JParser jp{text.data(), text.data() + text.size()};
     const JVal root = jp.val();
     jp.ws();
     if (jp.p != jp.e) fail("JSON: trailing characters");
+    g_dims = dims;
     SnapWorld w;
+    w.dims = dims;
     SnapMeta &m = w.meta;
     check_schema((uint32_t)ju(req(root, "schema"), "schema", 0xFFFFFFFFull));
     rv3(root, "gravity", w.gravity);
@@ -740,6 +766,11 @@ SnapWorld from_json(const std::string &text) {
     m.log_warnings = jb(req(va, "log_warnings"), "log_warnings");
     m.auto_correct = jb(req(va, "auto_correct"), "auto_correct");
     const uint64_t next_body = ju(req(root, "next_body_id"), "next_body_id", 0xFFFFFFFFull);
+    if (dims == 2) {
+        if (!root.get("next_joint_id")) fail("JSON: missing field 'next_joint_id'");
+        w.cell_size = rf(root, "cell_size");
+        check_finite(w.cell_size, "cell_size");
+    }
     const JVal &bodies = req(root, "bodies");
     if (bodies.t != JVal::ARR) fail("JSON: 'bodies' is not an array");
     for (const JVal &bj : bodies.a) {
@@ -762,7 +793,19 @@ SnapWorld from_json(const std::string &text) {
         if (in.has_shape) {
             const JVal &ty = req(*sh, "type");
             if (ty.t != JVal::STR) fail("JSON: shape type");
-            if (ty.text == "sphere") {
+            if (dims == 2) {
+                if (ty.text == "circle") {
+                    in.shape_tag = 0;
+                    in.dims[0] = rf(*sh, "radius");
+                } else if (ty.text == "box") {
+                    in.shape_tag = 1;
+                    rv3(*sh, "half_extents", in.dims);
+                } else if (ty.text == "polygon") {
+                    in.shape_tag = 2;
+                } else {
+                    fail("JSON: unknown shape type");
+                }
+            } else if (ty.text == "sphere") {
                 in.shape_tag = 0;
                 in.dims[0] = rf(*sh, "radius");
             } else if (ty.text == "box") {
@@ -783,14 +826,17 @@ SnapWorld from_json(const std::string &text) {
         if (ro.t != JVal::ARR || ro.a.size() != 4) fail("JSON: rotation");
         for (int k = 0; k < 4; ++k) in.rot[k] = jf(ro.a[k], "rotation");
         rv3(bj, "velocity", in.vel);
-        rv3(bj, "angular_velocity", in.angvel);
+        if (dims == 2)
+            in.angvel[0] = rf(bj, "angular_velocity");
+        else
+            rv3(bj, "angular_velocity", in.angvel);
         b.gravity_scale = of(bj, "gravity_scale", 1.0f);
         b.ambient_scale = of(bj, "ambient_scale", 1.0f);
         add_body(w, in, b);
     }
     if (next_body != w.bodies.size()) fail("next_body_id disagrees with body count");
-    if (const JVal *j = root.get("joints"))
-        if (j->t != JVal::ARR || !j->a.empty()) fail("joints are outside the GPU fluid path");
+    const JVal *joints = dims == 2 ? &req(root, "joints") : root.get("joints");
+    if (joints && (joints->t != JVal::ARR || !joints->a.empty())) fail("joints are outside the GPU fluid path");
     if (const JVal *nj = root.get("next_joint_id"))
         if (ju(*nj, "next_joint_id", 0xFFFFFFFFull) != 0) fail("next_joint_id disagrees with joint count");
     const JVal &co = req(root, "contacts");
@@ -872,10 +918,11 @@ struct BinOut {
         std::memcpy(&u, &v, 4);
         u32(u);
     }
+    int dims = 3;
     void v3(const float *v) {
         f32(v[0]);
         f32(v[1]);
-        f32(v[2]);
+        if (dims == 3) f32(v[2]);
     }
 };
 
@@ -899,19 +946,23 @@ struct BinIn {
         std::memcpy(&v, &u, 4);
         return v;
     }
+    int dims = 3;
     void v3(float *v) {
         v[0] = f32();
         v[1] = f32();
-        v[2] = f32();
+        v[2] = dims == 3 ? f32() : 0.0f;
     }
 };
 
-// encode_world3 (snapshot.rs:7920-8126) for schemas 3 .. 7.
+// encode_world3 (snapshot.rs:7920-8126) for schemas 3 .. 7; encode_world2
+// (:6033-6238) is the same payload with 2-vectors, the broad-phase cell
+// size after next_joint_id, 2D shapes and a scalar angular velocity.
 std::vector<uint8_t> to_binary(const SnapWorld &w) {
     const uint32_t schema = schema_of(w);
     const SnapMeta &m = w.meta;
     const FluidConfig3Ffi &c = w.cfg;
     BinOut o;
+    o.dims = w.dims;
     o.v3(w.gravity);
     o.f32(m.dt);
     o.f32(m.linear_sleep);
@@ -970,6 +1021,7 @@ std::vector<uint8_t> to_binary(const SnapWorld &w) {
     }
     o.u32((uint32_t)w.bodies.size());  // next_body_id
     o.u32(0);                          // next_joint_id
+    if (w.dims == 2) o.f32(w.cell_size);
     o.u32((uint32_t)w.bodies.size());
     for (size_t k = 0; k < w.bodies.size(); ++k) {
         const SnapBody &b = w.bodies[k];
@@ -980,7 +1032,11 @@ std::vector<uint8_t> to_binary(const SnapWorld &w) {
         o.f32(b.friction);
         o.f32(b.restitution);
         o.f32(b.density);
-        if (b.shape == PHIENG_SHAPE3_SPHERE) {  // encode_shape3 (:5911-5954)
+        if (w.dims == 2) {  // encode_shape2 (:5852-5872): circle, box
+            o.u8(b.shape == 0 ? 0 : 1);
+            o.f32(b.a);
+            if (b.shape != 0) o.f32(b.b);
+        } else if (b.shape == PHIENG_SHAPE3_SPHERE) {  // encode_shape3 (:5911-5954)
             o.u8(0);
             o.f32(b.a);
         } else if (b.shape == PHIENG_SHAPE3_BOX) {
@@ -994,13 +1050,17 @@ std::vector<uint8_t> to_binary(const SnapWorld &w) {
             o.f32(b.b);
         }
         o.v3(b.pos);
-        o.f32(0.0f);  // rotation x, y, z, w
+        // rotation: 3D the quaternion x, y, z, w; 2D the columns of the matrix
+        o.f32(w.dims == 2 ? 1.0f : 0.0f);
         o.f32(0.0f);
         o.f32(0.0f);
         o.f32(1.0f);
         const float z3[3] = {0.0f, 0.0f, 0.0f};
         o.v3(z3);  // velocity
-        o.v3(z3);  // angular velocity
+        if (w.dims == 2)
+            o.f32(0.0f);  // angular velocity
+        else
+            o.v3(z3);
         if (schema >= kSchemaGravity) o.f32(b.gravity_scale);
         if (schema >= kSchemaAmbient) o.f32(b.ambient_scale);
     }
@@ -1050,7 +1110,7 @@ std::vector<uint8_t> to_binary(const SnapWorld &w) {
     std::vector<uint8_t> out = {'P', 'H', 'E', '1'};
     BinOut h;
     h.u32(schema);
-    h.u8(kKindWorld3);
+    h.u8(w.dims == 2 ? kKindWorld2 : kKindWorld3);
     h.u32((uint32_t)o.b.size());
     out.insert(out.end(), h.b.begin(), h.b.end());
     out.insert(out.end(), o.b.begin(), o.b.end());
@@ -1060,8 +1120,9 @@ std::vector<uint8_t> to_binary(const SnapWorld &w) {
     return out;
 }
 
-// decode_envelope + world3_from_binary (snapshot.rs:5585-5616, 8128-8433).
-SnapWorld from_binary(const std::vector<uint8_t> &bytes) {
+// decode_envelope + world3_from_binary / world2_from_binary (snapshot.rs:
+// 5585-5616, 8128-8433, 7720-7910).
+SnapWorld from_binary(const std::vector<uint8_t> &bytes, int dims) {
     if (bytes.size() < 13) fail("binary snapshot too short");
     if (std::memcmp(bytes.data(), "PHE1", 4) != 0) fail("bad snapshot magic");
     BinIn hdr{bytes.data() + 4, 9};
@@ -1069,13 +1130,15 @@ SnapWorld from_binary(const std::vector<uint8_t> &bytes) {
     const uint8_t kind = hdr.u8();
     const uint32_t len = hdr.u32();
     if (version < kSchemaMin || version > kSchemaMax) fail("unsupported snapshot schema version " + std::to_string(version));
-    if (kind != kKindWorld3) fail("snapshot kind mismatch");
+    if (kind != (dims == 2 ? kKindWorld2 : kKindWorld3)) fail("snapshot kind mismatch");
     if (bytes.size() != 13 + (size_t)len + 4) fail("binary snapshot length mismatch");
     BinIn crc{bytes.data() + 13 + len, 4};
     if (crc32(bytes.data() + 13, len) != crc.u32()) fail("snapshot CRC mismatch");
     check_schema(version);
     BinIn r{bytes.data() + 13, len};
+    r.dims = dims;
     SnapWorld w;
+    w.dims = dims;
     SnapMeta &m = w.meta;
     r.v3(w.gravity);
     m.dt = r.f32();
@@ -1135,6 +1198,10 @@ SnapWorld from_binary(const std::vector<uint8_t> &bytes) {
     }
     const uint32_t next_body = r.u32();
     const uint32_t next_joint = r.u32();
+    if (dims == 2) {
+        w.cell_size = r.f32();
+        check_finite(w.cell_size, "cell_size");
+    }
     const uint32_t nb = r.u32();
     if (nb > 1000000u) fail("body count exceeds 1,000,000");
     for (uint32_t k = 0; k < nb; ++k) {
@@ -1153,7 +1220,18 @@ SnapWorld from_binary(const std::vector<uint8_t> &bytes) {
         const uint8_t tag = r.u8();
         in.has_shape = tag != 255;
         in.shape_tag = tag;
-        if (tag == 0) {
+        if (dims == 2) {
+            if (tag == 0) {
+                in.dims[0] = r.f32();
+            } else if (tag == 1) {
+                in.dims[0] = r.f32();
+                in.dims[1] = r.f32();
+            } else if (tag == 2) {
+                fail("a polygon body: not a GPU boundary shape");
+            } else if (tag != 255) {
+                fail("unknown 2D shape tag");
+            }
+        } else if (tag == 0) {
             in.dims[0] = r.f32();
         } else if (tag == 1) {
             r.v3(in.dims);
@@ -1168,7 +1246,10 @@ SnapWorld from_binary(const std::vector<uint8_t> &bytes) {
         r.v3(in.pos);
         for (int q = 0; q < 4; ++q) in.rot[q] = r.f32();
         r.v3(in.vel);
-        r.v3(in.angvel);
+        if (dims == 2)
+            in.angvel[0] = r.f32();
+        else
+            r.v3(in.angvel);
         if (version >= kSchemaGravity) b.gravity_scale = r.f32();
         if (version >= kSchemaAmbient) b.ambient_scale = r.f32();
         add_body(w, in, b);
@@ -1208,7 +1289,7 @@ SnapWorld from_binary(const std::vector<uint8_t> &bytes) {
     world_checks(w);
     const uint32_t np = r.u32();
     if (np > PH_ID_MASK + 1u) fail("more than 2^24 fluid particles");
-    const size_t per = version >= kSchemaAmbient ? 29 : 24;
+    const size_t per = (dims == 2 ? 16 : 24) + (version >= kSchemaAmbient ? 5 : 0);
     if ((size_t)np * per > r.n - r.pos) fail("binary payload truncated");
     w.pos.reserve(3 * (size_t)np);
     w.vel.reserve(3 * (size_t)np);
@@ -1268,19 +1349,17 @@ std::vector<uint8_t> read_file(const std::string &path) {
     return out;
 }
 
-}  // namespace
-
-extern "C" {
-
-int32_t phieng_world3_save(const World3 *world, const uint8_t *path_ptr, size_t path_len, uint32_t format,
-                           uint32_t mode) {
+// phieng_world3_save / phieng_world_save (ffi.rs:4042-4082, 3107-3160).
+int32_t save_world(const char *where, World3 *world, int dims, const uint8_t *path_ptr, size_t path_len,
+                   uint32_t format, uint32_t mode) {
     std::string path;
     if (!world || !path_arg(path_ptr, path_len, path)) return 0;
     if (mode != PHIENG_SNAPSHOT_EXACT && mode != PHIENG_SNAPSHOT_FRESH) return 0;
     if (format != PHIENG_SNAPSHOT_JSON && format != PHIENG_SNAPSHOT_BINARY) return 0;
     try {
         SnapWorld s;
-        world3_snapshot(const_cast<World3 *>(world), s);
+        world3_snapshot(world, s);
+        if (s.dims != dims) fail("the world's dimension does not match the entry point");
         if (format == PHIENG_SNAPSHOT_JSON) {
             const std::string text = to_json(s);
             write_file(path, reinterpret_cast<const uint8_t *>(text.data()), text.size());
@@ -1290,15 +1369,17 @@ int32_t phieng_world3_save(const World3 *world, const uint8_t *path_ptr, size_t 
         }
         return 1;
     } catch (const SnapError &e) {
-        snapshot_set_error("phieng_world3_save", e.what());
+        snapshot_set_error(where, e.what());
         return e.code;
     } catch (const std::exception &e) {
-        snapshot_set_error("phieng_world3_save", e.what());
+        snapshot_set_error(where, e.what());
         return -2;
     }
 }
 
-World3 *phieng_world3_load(const uint8_t *path_ptr, size_t path_len, uint32_t mode) {
+// phieng_world3_load / phieng_world_load (ffi.rs:4084-4111, 3162-3190):
+// JSON when the first non-blank byte is '{', else binary.
+World3 *load_world(const char *where, int dims, const uint8_t *path_ptr, size_t path_len, uint32_t mode) {
     std::string path;
     if (!path_arg(path_ptr, path_len, path)) return nullptr;
     if (mode != PHIENG_SNAPSHOT_EXACT && mode != PHIENG_SNAPSHOT_FRESH) return nullptr;
@@ -1309,16 +1390,40 @@ World3 *phieng_world3_load(const uint8_t *path_ptr, size_t path_len, uint32_t mo
                                     bytes[i] == '\f'))
             ++i;
         SnapWorld s = i < bytes.size() && bytes[i] == '{'
-                          ? from_json(std::string(bytes.begin(), bytes.end()))
-                          : from_binary(bytes);
+                          ? from_json(std::string(bytes.begin(), bytes.end()), dims)
+                          : from_binary(bytes, dims);
         std::string err;
         World3 *w = world3_from_snapshot(s, err);
-        if (!w) snapshot_set_error("phieng_world3_load", err);
+        if (!w) snapshot_set_error(where, err);
         return w;
     } catch (const std::exception &e) {
-        snapshot_set_error("phieng_world3_load", e.what());
+        snapshot_set_error(where, e.what());
         return nullptr;
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t phieng_world3_save(const World3 *world, const uint8_t *path_ptr, size_t path_len, uint32_t format,
+                           uint32_t mode) {
+    return save_world("phieng_world3_save", const_cast<World3 *>(world), 3, path_ptr, path_len, format, mode);
+}
+
+World3 *phieng_world3_load(const uint8_t *path_ptr, size_t path_len, uint32_t mode) {
+    return load_world("phieng_world3_load", 3, path_ptr, path_len, mode);
+}
+
+int32_t phieng_world_save(const World2 *world, const uint8_t *path_ptr, size_t path_len, uint32_t format,
+                          uint32_t mode) {
+    return save_world("phieng_world_save", world ? world2_base(const_cast<World2 *>(world)) : nullptr, 2, path_ptr,
+                      path_len, format, mode);
+}
+
+World2 *phieng_world_load(const uint8_t *path_ptr, size_t path_len, uint32_t mode) {
+    World3 *w = load_world("phieng_world_load", 2, path_ptr, path_len, mode);
+    return w ? world2_of(w) : nullptr;
 }
 
 }  // extern "C"

@@ -124,6 +124,7 @@ struct World3 {
     std::vector<BodyDev> bodies;
     std::vector<SnapBody> body_meta;  // material, damping, ... of each body, for snapshots
     SnapMeta meta;                    // world fields the GPU step does not read, for snapshots
+    float cell_size = 1.0f;           // a World2's broad-phase cell, kept for its snapshots
     int32_t next_body_id = 0;
     float time = 0.0f;
     uint32_t capacity = kDefaultCapacity;
@@ -662,6 +663,15 @@ struct World3 {
     }
 };
 
+// The 2D world: a World3 in the z = 0 plane (2D coefficients, ties of the
+// 2D box rule); the 2D exports below take it.
+struct World2 : World3 {
+    World2() { dims = 2; }
+};
+
+World3 *world2_base(World2 *w) { return w; }
+World2 *world2_of(World3 *w) { return static_cast<World2 *>(w); }
+
 uint32_t world3_count(const World3 *w) { return w->n; }
 const FluidConfig3Ffi &world3_config(const World3 *w) { return w->cfg; }
 const std::vector<BodyDev> &world3_bodies(const World3 *w) { return w->bodies; }
@@ -687,7 +697,8 @@ void world3_device_state_mut(World3 *w, float4 **posm, float4 **vel, uint32_t **
 void snapshot_set_error(const char *where, const std::string &what) { set_error(where, what.c_str()); }
 
 void world3_snapshot(World3 *w, SnapWorld &out) {
-    if (w->dims != 3) throw std::runtime_error("a 2D world is not written in the 3D snapshot format");
+    out.dims = w->dims;
+    out.cell_size = w->cell_size;
     out.gravity[0] = w->g[0];
     out.gravity[1] = w->g[1];
     out.gravity[2] = w->g[2];
@@ -712,7 +723,8 @@ void world3_snapshot(World3 *w, SnapWorld &out) {
 }
 
 World3 *world3_from_snapshot(const SnapWorld &s, std::string &err) {
-    std::unique_ptr<World3> w(new World3());
+    std::unique_ptr<World3> w(s.dims == 2 ? static_cast<World3 *>(new World2()) : new World3());
+    w->cell_size = s.cell_size;
     if (!config_valid(s.cfg)) {
         err = "the fluid config fails the C ABI's validation (ffi.rs:857-914)";
         return nullptr;
@@ -1144,9 +1156,6 @@ int32_t phieng_world3_sync(World3 *world) {
 
 }  // extern "C"
 
-struct World2 : World3 {
-    World2() { dims = 2; }
-};
 
 namespace {
 World3 *as3(World2 *w) { return static_cast<World3 *>(w); }
@@ -1192,9 +1201,8 @@ void phieng_world_set_gravity(World2 *world, float gx, float gy) {
 int32_t phieng_world_add_body(World2 *world, float px, float py, uint32_t shape_tag, float a, float b, float density,
                               uint8_t is_static, float friction, float restitution) {
     // friction / restitution are sanitised, never rejected, by set_material
-    // (body.rs:627-637), and static walls take wall_friction from the config
-    (void)friction;
-    (void)restitution;
+    // (body.rs:627-637), and static walls take wall_friction from the config;
+    // they are kept for snapshots only
     if (!world || !std::isfinite(px) || !std::isfinite(py)) return -1;
     BodyDev bd{0u, px, py, 0.0f, a, 0.0f, 0.0f};
     switch (shape_tag) {
@@ -1220,6 +1228,7 @@ int32_t phieng_world_add_body(World2 *world, float px, float py, uint32_t shape_
         return -1;
     }
     world->bodies.push_back(bd);
+    world->body_meta.push_back(snap_body(bd, density, friction, restitution));
     return world->next_body_id++;
 }
 

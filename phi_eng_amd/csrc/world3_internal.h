@@ -57,7 +57,7 @@ struct SnapMeta {
 };
 
 struct SnapBody {
-    uint32_t shape = 0;  // PHIENG_SHAPE3_SPHERE / BOX / CAPSULE
+    uint32_t shape = 0;  // PHIENG_SHAPE3_SPHERE / BOX / CAPSULE; 2D: circle 0, box 3 (the BodyDev shapes)
     float pos[3] = {0.0f, 0.0f, 0.0f};
     float a = 0.0f, b = 0.0f, c = 0.0f;  // radius | half extents | radius, half length
     float friction = 0.4f, restitution = 0.1f, density = 1.0f;  // material.rs:31-36
@@ -68,6 +68,8 @@ struct SnapBody {
 };
 
 struct SnapWorld {
+    int dims = 3;          // 2: a World2 (World2Dto, snapshot.rs:3048-3113; z components unused)
+    float cell_size = 1.0f;  // World2's broad-phase cell (world.rs:334), kept verbatim
     float gravity[3] = {0.0f, -9.81f, 0.0f};
     float time = 0.0f;
     SnapMeta meta;
@@ -77,7 +79,12 @@ struct SnapWorld {
     std::vector<uint8_t> phase;
 };
 
+struct World2;
+World3 *world2_base(World2 *w);
+World2 *world2_of(World3 *w);  // w must have been made as a World2
+
 // The world as a snapshot (reads the particles back from the device).
 void world3_snapshot(World3 *w, SnapWorld &out);
-// A new world from a snapshot, or null with the reason in err.
+// A new world from a snapshot (a World2 when s.dims == 2), or null with the
+// reason in err.
 World3 *world3_from_snapshot(const SnapWorld &s, std::string &err);

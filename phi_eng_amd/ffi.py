@@ -158,6 +158,8 @@ SIGNATURES = {
     "phieng_world_fluid_count": (_U32, [_P]),
     "phieng_world_fluid_clear": (_I32, [_P]),
     "phieng_world_fluid_get": (_U32, [_P, _fp, _U32]),
+    "phieng_world_save": (_I32, [_P, ctypes.c_char_p, ctypes.c_size_t, _U32, _U32]),
+    "phieng_world_load": (_P, [ctypes.c_char_p, ctypes.c_size_t, _U32]),
     "phieng_world_fluid_reserve": (_U32, [_P, _U32]),
     "phieng_world_fluid_get_mass": (_U32, [_P, _fp, _u8p, _U32]),
     "phieng_world_fluid_set_velocity": (_U32, [_P, _fp, _U32]),

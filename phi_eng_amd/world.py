@@ -226,13 +226,25 @@ class World2:
     """Owns one ``World2*`` from ``phieng_world_new`` (ffi.rs:51-53): the 2D
     fluid surface (``phieng_world_*``), arrays of xy pairs."""
 
-    def __init__(self, capacity=None):
+    def __init__(self, capacity=None, _ptr=None):
         self._l = lib()
-        self.ptr = self._l.phieng_world_new()
+        self.ptr = _ptr if _ptr else self._l.phieng_world_new()
         if not self.ptr:
             raise MemoryError("phieng_world_new returned null")
         if capacity is not None:
             self.reserve(capacity)
+
+    def save(self, path, binary=False, fresh=False):
+        """phieng_world_save; returns its code (1, 0, -1 or -2)."""
+        b = os.fsencode(path)
+        return self._l.phieng_world_save(self.ptr, b, len(b), int(binary), int(fresh))
+
+    @classmethod
+    def load(cls, path, fresh=False):
+        """phieng_world_load; a new World2, or None (last_error says why)."""
+        b = os.fsencode(path)
+        ptr = lib().phieng_world_load(b, len(b), int(fresh))
+        return cls(_ptr=ptr) if ptr else None
 
     def close(self):
         if self.ptr:

@@ -90,3 +90,31 @@ def test_phases_adaptive_masses_and_large_counts(tmp_path):
     assert _same(w, r)
     w.close()
     r.close()
+
+
+@pytest.mark.parametrize("binary", [False, True])
+def test_reference_fluid_roundtrip_2d(tmp_path, binary):
+    """snapshot.rs:8680-8721 on the GPU path: a 2D box and a 4 x 3 lattice,
+    60 steps; the loaded world equals the original and so do 60 more steps
+    of each."""
+    from phi_eng_amd.world import SHAPE2_BOX, World2
+    w = World2()
+    assert w.add_body((0.0, -1.0), SHAPE2_BOX, 10.0, 0.5) == 0
+    pts = np.array([(ix * 0.25, iy * 0.25) for iy in range(3) for ix in range(4)], np.float32)
+    assert w.add_particles(pts) == 12
+    for _ in range(60):
+        w.step(1.0 / 60.0)
+    w.sync()
+    path = str(tmp_path / ("w.bin" if binary else "w.json"))
+    assert w.save(path, binary=binary) == 1, ffi.last_error()
+    r = World2.load(path)
+    assert r is not None, ffi.last_error()
+    assert r.count() == 12 and _same(w, r)
+    for _ in range(60):
+        w.step(1.0 / 60.0)
+        r.step(1.0 / 60.0)
+    w.sync()
+    r.sync()
+    assert _same(w, r)
+    w.close()
+    r.close()

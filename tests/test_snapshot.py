@@ -268,3 +268,95 @@ def test_argument_checks(tmp):
     q = tmp("missing.json").encode()
     assert lib.phieng_world3_load(q, len(q), 0) is None
     w.close()
+
+
+# ---------------------------------------------------------------- 2D (World2Dto)
+
+def _world2(cfg=None):
+    from phi_eng_amd.world import SHAPE2_BOX, SHAPE2_CIRCLE, World2
+    w = World2()
+    if cfg is not None:
+        assert w.set_fluid_config(cfg)
+    w.set_gravity((0.5, -9.81))
+    assert w.add_body((0.0, -1.0), SHAPE2_BOX, 10.0, 0.5, density=2.0) == 0
+    assert w.add_body((1.0, 2.0), SHAPE2_CIRCLE, 0.5) == 1
+    return w
+
+
+def decode2(path):
+    """encode_world2 (snapshot.rs:6033-6238) read back with struct."""
+    b = open(path, "rb").read()
+    assert b[:4] == b"PHE1"
+    version, kind, n = struct.unpack_from("<IBI", b, 4)
+    assert kind == 2 and len(b) == 13 + n + 4
+    assert zlib.crc32(b[13:13 + n]) == struct.unpack_from("<I", b, 13 + n)[0]
+    r = Payload(b[13:13 + n])
+    out = dict(schema=version, gravity=r.take("2f"), dt=r.take("f"), sleep=r.take("2f"), solver=r.take("I3f"))
+    out["validation"] = r.take("B5f2B")
+    if version >= 4:
+        out["gravity_config"] = r.take("II2f2f2f")
+    if version >= 5:
+        out["ambient"] = r.take("B2f3fB4f2fB3fIBB")
+        out["time"] = r.take("f")
+    out["next_body"], out["next_joint"], out["cell_size"], nb = r.take("IIfI")
+    bodies = []
+    for _ in range(nb):
+        body = dict(id=r.take("I"), flags=r.take("B"), damping=r.take("2f"), material=r.take("3f"))
+        tag = r.take("B")
+        body["shape"] = (tag, r.take({0: "f", 1: "2f"}[tag]))
+        body["pos"], body["rot"], body["vel"], body["angvel"] = r.take("2f"), r.take("4f"), r.take("2f"), r.take("f")
+        if version >= 4:
+            body["gravity_scale"] = r.take("f")
+        if version >= 5:
+            body["ambient_scale"] = r.take("f")
+        bodies.append(body)
+    out["bodies"] = bodies
+    out["joints"], out["contacts"] = r.take("I"), r.take("I")
+    cfg = dict(head=r.take("4f"))
+    if version >= 5:
+        cfg["opts"] = r.take("4fIBffBf")
+        cfg["phases"] = (r.take("B"), r.take("4f"), r.take("4f"))
+    cfg["tail"] = r.take("4fB")
+    out["fluid_config"] = cfg
+    out["particles"] = r.take("I")
+    assert r.p == len(r.b), "trailing bytes"
+    return out
+
+
+def test_2d_layouts_and_round_trip(tmp):
+    """World2Dto in both formats: 2-vectors, cell_size, circle and box
+    shapes, the identity rotation matrix, a scalar angular velocity; the
+    round trip is a fixed point; a 3D file is refused by the 2D loader."""
+    from phi_eng_amd.world import World2
+    cfg = FluidConfig3Ffi.default()
+    cfg.wall_friction = 0.25
+    w = _world2(cfg)
+    assert w.save(tmp("w2.json")) == 1, ffi.last_error()
+    d = json.load(open(tmp("w2.json")))
+    assert d["schema"] == 5 and d["gravity"] == {"x": 0.5, "y": -9.81} and d["cell_size"] == 1
+    b0, b1 = d["bodies"]
+    assert b0["shape"] == {"type": "box", "half_extents": {"x": 10, "y": 0.5}}
+    assert b1["shape"] == {"type": "circle", "radius": 0.5}
+    assert b0["rotation"] == [1, 0, 0, 1] and b0["angular_velocity"] == 0 and b0["position"] == {"x": 0, "y": -1}
+    assert d["fluid"]["config"]["wall_friction"] == 0.25
+    assert w.save(tmp("w2.bin"), binary=True) == 1
+    e = decode2(tmp("w2.bin"))
+    assert e["schema"] == 5 and e["cell_size"] == 1.0 and e["next_body"] == 2
+    assert [x["shape"] for x in e["bodies"]] == [(1, (10.0, 0.5)), (0, 0.5)]
+    assert e["bodies"][0]["rot"] == (1.0, 0.0, 0.0, 1.0)
+    for ext, binary in (("json", False), ("bin", True)):
+        v = World2.load(tmp("w2." + ext))
+        assert v is not None, ffi.last_error()
+        assert bytes(v.get_fluid_config()) == bytes(w.get_fluid_config())
+        assert v.save(tmp("v2." + ext), binary=binary) == 1
+        assert open(tmp("w2." + ext), "rb").read() == open(tmp("v2." + ext), "rb").read()
+        v.close()
+    w3 = _world()
+    assert w3.save(tmp("w3.bin"), binary=True) == 1
+    assert World2.load(tmp("w3.bin")) is None and "kind" in ffi.last_error()
+    assert World3.load(tmp("w2.bin")) is None and "kind" in ffi.last_error()
+    d["bodies"][1]["shape"] = {"type": "polygon", "vertices": [{"x": 0, "y": 0}]}
+    json.dump(d, open(tmp("p.json"), "w"))
+    assert World2.load(tmp("p.json")) is None and "polygon" in ffi.last_error()
+    w.close()
+    w3.close()
