@@ -227,3 +227,22 @@ def test_adaptive_refuses_surrogate_and_batch():
     assert not w.set_fluid_config(_cfg())
     net.close()
     w.close()
+
+
+def test_adaptive_with_ghosts_and_cfl_substeps():
+    """Adaptive resolution together with boundary ghosts (fluid3.rs:1014-1027
+    runs after the resample) and CFL substeps (a changing count between
+    substeps of one step): 10 steps equal the oracle's."""
+    cfg = _cfg(boundary_particles=1, cfl=0.3, max_substeps=4, stiffness=400.0)
+    cfg.particle_mass = cfg.phase_mass[0]
+    pos = _mixed_scene(8)
+    w = _world3(cfg, pos)
+    for _ in range(10):
+        w.step(1.0 / 60.0)
+    w.sync()
+    assert w.boundary_info()[0] > 0
+    ref = O.run_cap_f32(pos, np.zeros_like(pos), O.masses(cfg, len(pos)), cfg, (0.0, -9.81, 0.0),
+                        O.bodies_from_boxes(TANK3), 1.0 / 60.0, 10, 4096)
+    assert (ref["mass"] != F(0.03)).any()
+    _compare(w, ref, "adaptive + ghosts + cfl")
+    w.close()
