@@ -475,9 +475,14 @@ def bench_extras(a):
     print(json.dumps(out))
 
 
+BATCH_MODES = {0: "kernels", 1: "persistent", 2: "persistent+hipGraph", 3: "kernels+hipGraph"}
+
+
 def bench_batch(a):
-    """256 independent dam scenes (counts log-uniform in [2048, 65536], seed 3),
-    one persistent launch per step from a hipGraph, launch shape autotuned."""
+    """256 independent dam scenes (counts log-uniform in [2048, 65536], seed 3)
+    advanced together; the launch form (eight launches or one persistent
+    launch per substep, each with or without a hipGraph) and the persistent
+    grid are autotuned, and every form is timed."""
     from phi_eng_amd import scenes
     from phi_eng_amd.world import Batch3
     scs = scenes.batch_scenes(count=256, seed=3)
@@ -501,17 +506,33 @@ def bench_batch(a):
         b.step(dt)
     b.sync()
     timing = b.get_timing()
+    b.set_timing(False)
     kern = {k: round(1e3 * ms / c, 2) for k, (ms, c) in timing.items()}
+    # every launch form, each from the state it finds (50 timed steps; the
+    # persistent forms at the occupancy limit of blocks per CU)
+    forms = {}
+    for m, name in BATCH_MODES.items():
+        b.set_mode(m, 0)
+        for _ in range(3):
+            b.step(dt)
+        b.sync()
+        f0 = time.perf_counter()
+        for _ in range(50):
+            b.step(dt)
+        b.sync()
+        f_ms = (time.perf_counter() - f0) * 1e3 / 50
+        forms[name] = {"ms_per_step": round(f_ms, 4), "value": round(n / (f_ms * 1e-3), 1)}
+    b.set_mode(mode, bpc)
     out = {"metric": "particle-steps/sec, batched 3D SPH (256 scenes)", "value": round(n * steps / wall, 1),
            "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
            "ms_per_step": round(wall * 1e3 / steps, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic: 256 seeded jittered dam breaks",
            "config": {"workload": "batch_256_scenes", "particles": n, "scenes": len(scs),
                       "counts": [int(min(sc.n for sc in scs)), int(max(sc.n for sc in scs))], "dt": dt,
-                      "mode": {0: "kernels", 1: "persistent", 2: "persistent+hipGraph"}[mode],
+                      "mode": BATCH_MODES[mode],
                       "blocks_per_cu": bpc, "autotune_ms_per_step": round(tuned_ms, 4)},
            "step_roofline_frac": round(B_ALG_STEP * n / (wall / steps) / (HBM_PEAK_GBS * 1e9), 5),
-           "kernels": kern}
+           "kernels": kern, "launch_forms": forms}
     print(json.dumps(out))
 
 

@@ -197,6 +197,7 @@ typedef struct Batch3 Batch3;
 #define PHIENG_BATCH_KERNELS 0           /* the eight launches of a single world   */
 #define PHIENG_BATCH_PERSISTENT 1        /* one persistent launch per substep      */
 #define PHIENG_BATCH_PERSISTENT_GRAPH 2  /* the persistent launch from a hipGraph  */
+#define PHIENG_BATCH_KERNELS_GRAPH 3     /* the eight launches from a hipGraph     */
 /* Null (with phieng_batch3_last_error) if a world is null, disabled, uses
  * CFL substepping or boundary ghosts, or the total exceeds 2^24 particles. */
 Batch3 *phieng_batch3_new(World3 *const *worlds, uint32_t count);

@@ -90,6 +90,8 @@ struct Batch3 {
         a.rq = e.rq.p;
         a.masks = e.masks.p;
         a.flags = e.flags.p;
+        a.lcnt = e.nl_cnt.p;
+        a.lidx = e.nl_idx.p;
         a.mstride = (uint32_t)e.alloc_particles;
         a.tile_bbox = e.d_bbox.p;
         a.n = e.n_total;
@@ -114,8 +116,45 @@ struct Batch3 {
     void substep() {
         ph::Engine &e = eng;
         if (e.n_total == 0) return;
-        if (mode == PHIENG_BATCH_KERNELS) {
+        if (mode == PHIENG_BATCH_KERNELS || (mode == PHIENG_BATCH_KERNELS_GRAPH && (e.timing || e.rec_flags))) {
             e.substep();
+            return;
+        }
+        if (mode == PHIENG_BATCH_KERNELS_GRAPH) {
+            // the sort and the two passes of one substep, captured once per
+            // tag parity (the tags alternate between two buffers) and
+            // replayed; a sort cut short earlier is cleared outside the graph
+            if (e.counts_dirty) {
+                e.run(ph::KIND_FILL, [&] { return ph::launch_fill_u32(e.stream, e.counts.p, (uint32_t)e.alloc_buckets, 0u); });
+                e.counts_dirty = false;
+            }
+            const int k = e.ct;
+            if (!exec[k] || exec_nbuckets[k] != e.nbuckets) {
+                if (exec[k]) {
+                    (void)hipGraphExecDestroy(exec[k]);
+                    exec[k] = nullptr;
+                }
+                hipGraph_t g = nullptr;
+                PH_CK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    e.sort_phase(nullptr);
+                    e.plain_phase(nullptr);
+                } catch (...) {
+                    (void)hipStreamEndCapture(e.stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                hipError_t ce = hipStreamEndCapture(e.stream, &g);
+                if (ce != hipSuccess) throw ph::HipError("hipStreamEndCapture", ce);
+                hipError_t ie = hipGraphInstantiate(&exec[k], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ie != hipSuccess) throw ph::HipError("hipGraphInstantiate", ie);
+                exec_nbuckets[k] = e.nbuckets;
+            }
+            PH_CK(hipGraphLaunch(exec[k], e.stream));
+            e.counts_dirty = false;
+            e.gen_open = true;
+            e.finish_substep();
             return;
         }
         // The host cannot see whether a persistent launch finished its scan
@@ -286,7 +325,7 @@ void phieng_batch3_free(Batch3 *batch) {
 }
 
 int32_t phieng_batch3_set_mode(Batch3 *b, int32_t mode, uint32_t blocks_per_cu) {
-    if (!b || mode < PHIENG_BATCH_KERNELS || mode > PHIENG_BATCH_PERSISTENT_GRAPH) return 0;
+    if (!b || mode < PHIENG_BATCH_KERNELS || mode > PHIENG_BATCH_KERNELS_GRAPH) return 0;
     b->mode = mode;
     b->blocks_per_cu = blocks_per_cu;
     b->set_grid();
@@ -418,7 +457,7 @@ int32_t phieng_batch3_autotune(Batch3 *b, float dt, uint32_t trial_steps, const 
         PH_CK(hipMemcpyAsync(sv.p, e.vel[e.cs].p, sizeof(float4) * n, hipMemcpyDeviceToDevice, e.stream));
         PH_CK(hipMemcpyAsync(st.p, e.tag[e.ct].p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, e.stream));
         const int ct0 = e.ct;
-        std::vector<std::pair<int, uint32_t>> cands = {{PHIENG_BATCH_KERNELS, 0}};
+        std::vector<std::pair<int, uint32_t>> cands = {{PHIENG_BATCH_KERNELS, 0}, {PHIENG_BATCH_KERNELS_GRAPH, 0}};
         for (int bpc = 1; bpc <= b->max_blocks_per_cu; ++bpc) cands.push_back({PHIENG_BATCH_PERSISTENT_GRAPH, bpc});
         hipEvent_t t0, t1;
         PH_CK(hipEventCreate(&t0));

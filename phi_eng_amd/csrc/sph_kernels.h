@@ -17,6 +17,7 @@ struct PersistArgs {
     uint32_t *keys, *slots, *tmp, *counts, *start, *partial;
     float2 *rq;
     uint32_t *masks, *flags;
+    uint32_t *lcnt, *lidx;  // the neighbour list between the two passes (ListOut)
     uint32_t mstride;
     int32_t *tile_bbox;
     uint32_t n, nbuckets;

@@ -848,14 +848,20 @@ __global__ __launch_bounds__(PH_TILE) void k_step_persistent(ph::PersistArgs a) 
         dev_rank_gather(k, a.ntiles, a.tiles, a.scenes, a.tmp, a.keys, a.start, a.posm_cur, a.vel_cur, a.tag_cur,
                         a.posm_alt, a.vel_alt, a.tag_alt, a.rec_sorted, nullptr);
     if (!grid_barrier(a.bar, G, a.spin_limit)) return;
+    // the density and force passes of a single world (k_density_list<false>,
+    // k_force_list): the neighbour list carries the neighbours between them
+    ph::ListOut lo;
+    lo.cnt = a.lcnt;
+    lo.idx = a.lidx;
+    lo.skin_factor2 = 1.0f;
     for (uint32_t k = blockIdx.x; k < a.ntiles; k += G)
-        dev_density<false>(k, a.ntiles, a.tiles, a.scenes, a.start, a.posm_alt, a.tag_alt, a.rq, a.masks, a.flags,
-                    a.mstride, a.rec_rho, a.rec_p, a.rec_count);
+        dev_density<true, false>(k, a.ntiles, a.tiles, a.scenes, a.start, a.posm_alt, a.tag_alt, a.rq, nullptr,
+                                 nullptr, a.mstride, a.rec_rho, a.rec_p, a.rec_count, lo);
     if (!grid_barrier(a.bar, G, a.spin_limit)) return;
     for (uint32_t k = blockIdx.x; k < a.ntiles; k += G) {
-        dev_force_integrate<false>(k, a.ntiles, a.tiles, a.scenes, a.bodies, a.start, a.posm_alt, a.vel_alt, a.rq,
-                            a.tag_alt, a.masks, a.flags, a.mstride, a.posm_cur, a.vel_cur, a.rec_acc, a.tile_bbox,
-                            nullptr);
+        dev_force_integrate<true>(k, a.ntiles, a.tiles, a.scenes, a.bodies, a.start, a.posm_alt, a.vel_alt, a.rq,
+                                  a.tag_alt, nullptr, nullptr, a.mstride, a.posm_cur, a.vel_cur, a.rec_acc,
+                                  a.tile_bbox, nullptr, GhostDev{}, a.lcnt, reinterpret_cast<const uint4 *>(a.lidx));
         __syncthreads();
     }
 }

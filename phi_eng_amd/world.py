@@ -368,7 +368,7 @@ class World2:
         return dict(pos=xy, density=rho, pressure=p)
 
 
-BATCH_KERNELS, BATCH_PERSISTENT, BATCH_PERSISTENT_GRAPH = 0, 1, 2
+BATCH_KERNELS, BATCH_PERSISTENT, BATCH_PERSISTENT_GRAPH, BATCH_KERNELS_GRAPH = 0, 1, 2, 3
 
 
 class Batch3:

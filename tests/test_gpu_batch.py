@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from phi_eng_amd import scenes
-from phi_eng_amd.world import BATCH_KERNELS, BATCH_PERSISTENT, BATCH_PERSISTENT_GRAPH, Batch3
+from phi_eng_amd.world import BATCH_KERNELS, BATCH_KERNELS_GRAPH, BATCH_PERSISTENT, BATCH_PERSISTENT_GRAPH, Batch3
 
 pytestmark = pytest.mark.gpu
 
@@ -50,7 +50,7 @@ def test_batch_matches_worlds_alone_all_modes(gpu):
     scs = _scenes()
     ref = _alone(scs)
     for mode, bpc in [(BATCH_KERNELS, 0), (BATCH_PERSISTENT, 0), (BATCH_PERSISTENT, 1),
-                      (BATCH_PERSISTENT_GRAPH, 0), (BATCH_PERSISTENT_GRAPH, 2)]:
+                      (BATCH_PERSISTENT_GRAPH, 0), (BATCH_PERSISTENT_GRAPH, 2), (BATCH_KERNELS_GRAPH, 0)]:
         got = _batched(scs, mode, bpc)
         print(f"mode {mode} blocks/CU {bpc}: n={sum(len(g[0]) for g in got)}", flush=True)
         assert _same(got, ref), (mode, bpc)
@@ -64,7 +64,7 @@ def test_autotune_keeps_state_and_results(gpu, tmp_path):
     cache = str(tmp_path / "tune.txt")
     mode, bpc, ms = b.autotune(DT, trial_steps=4, cache_path=cache)
     print(f"autotune picked mode {mode} blocks/CU {bpc} ({ms:.3f} ms/step)", flush=True)
-    assert mode in (BATCH_KERNELS, BATCH_PERSISTENT_GRAPH)
+    assert mode in (BATCH_KERNELS, BATCH_KERNELS_GRAPH, BATCH_PERSISTENT_GRAPH)
     for _ in range(STEPS):
         b.step(DT)
     b.sync()
@@ -83,3 +83,27 @@ def test_batch_rejects_cfl_worlds(gpu):
     w = scenes.make_world(sc)
     with pytest.raises(RuntimeError, match="CFL"):
         Batch3([w])
+
+
+def test_graph_replays_match_eager_every_step(gpu):
+    """The multi-kernel substep captured in a hipGraph, replayed step after
+    step with the host's own work (readback, monitor) between replays:
+    every replay's state equals the eager multi-kernel batch's."""
+    scs = scenes.batch_scenes(count=6, seed=9, nmin=1000, nmax=6000)
+    eager_w = [scenes.make_world(sc) for sc in scs]
+    graph_w = [scenes.make_world(sc) for sc in scs]
+    eager, graph = Batch3(eager_w), Batch3(graph_w)
+    eager.set_mode(BATCH_KERNELS, 0)
+    graph.set_mode(BATCH_KERNELS_GRAPH, 0)
+    for step in range(20):
+        eager.step(DT)
+        graph.step(DT)
+        eager.sync()
+        graph.sync()
+        eager.sync_worlds()
+        graph.sync_worlds()
+        a = [w.get_state() for w in eager_w]
+        b = [w.get_state() for w in graph_w]
+        assert _same(a, b), f"replay {step} differs"
+    eager.close()
+    graph.close()
