@@ -88,8 +88,6 @@ struct Batch3 {
         a.start = e.start.p;
         a.partial = e.partial.p;
         a.rq = e.rq.p;
-        a.masks = e.masks.p;
-        a.flags = e.flags.p;
         a.lcnt = e.nl_cnt.p;
         a.lidx = e.nl_idx.p;
         a.mstride = (uint32_t)e.alloc_particles;

@@ -116,7 +116,6 @@ struct Engine {
     size_t alloc_particles = 0;
     DBuf<uint32_t> keys, slots, tmp;
     DBuf<float2> rq;
-    DBuf<uint32_t> masks, flags;  // per-row neighbour bit masks (9 x N) + row flags (persistent step)
     // Neighbour list of the last density pass: the candidates within
     // (1 + list_skin) h in reference order, four slots per 16-byte group
     // (group g of slot i at uint4 g * alloc_particles + i, SL_CAP / 4 groups

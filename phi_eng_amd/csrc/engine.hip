@@ -304,8 +304,6 @@ void Engine::ensure_particles(uint32_t cap) {
     slots.ensure(want);
     tmp.ensure(want);
     rq.ensure(want);
-    masks.ensure(9 * want);
-    flags.ensure(want);
     nl_cnt.ensure(want);
     nl_idx.ensure((size_t)(SL_CAP + 4) * want);
     if (list_cells) nl_cell.ensure(want);
@@ -509,7 +507,7 @@ void Engine::plain_phase(float4 *acc_sorted, const AuxFuse *af, bool density_don
     if (!density_done)
         run(KIND_DENSITY, [&] {
             return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p,
-                                       masks.p, flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
+                                       (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
                                        rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr, lo);
         });
     // boundary ghosts (fluid3.rs:1018-1027): their density and pressure from
@@ -537,8 +535,8 @@ void Engine::density_phase() {
     lo.idx = nl_idx.p;
     lo.skin_factor2 = 1.0f;
     run(KIND_DENSITY, [&] {
-        return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p, masks.p,
-                                   flags.p, (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
+        return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p,
+                                   (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
                                    rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr, lo);
     });
 }

@@ -341,11 +341,10 @@ __device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__r
     e = start[row + (uint32_t)(xhi - s.x0) + 1];
 }
 
-#define PH_ROW_BITS 32u
-#define PH_SLOW_PATH 0x80000000u
+#define PH_ROW_BITS 32u  // candidates per row tested into one mask word
 
-// Skin list of the surrogate substeps: candidates per particle at most, and
-// the count word's flags.
+// Neighbour list (the plain force pass's, or the surrogate's skin list):
+// entries per particle at most, and the count word's flags.
 #define SL_CAP 96u
 #define SLF_COUNT 0x00FFFFFFu
 #define SLF_SLOW 0x01000000u  // sweep the cells (outliers near the domain, or > SL_CAP candidates)

@@ -16,7 +16,6 @@ struct PersistArgs {
     uint32_t *tag_cur, *tag_alt;
     uint32_t *keys, *slots, *tmp, *counts, *start, *partial;
     float2 *rq;
-    uint32_t *masks, *flags;
     uint32_t *lcnt, *lidx;  // the neighbour list between the two passes (ListOut)
     uint32_t mstride;
     int32_t *tile_bbox;
@@ -45,10 +44,6 @@ hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *ti
                               const float4 *posm_in, const float4 *vel_in, const uint32_t *tag_in,
                               float4 *posm_out, float4 *vel_out, uint32_t *tag_out, uint32_t *rec_sorted,
                               uint32_t *perm_out = nullptr);
-hipError_t launch_density(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
-                          const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
-                          uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
-                          uint32_t *rec_count);
 // Skin list written by the density pass for surrogate substeps.
 struct ListOut {
     int4 *cell = nullptr;
@@ -66,14 +61,7 @@ hipError_t launch_list_radius(hipStream_t st, uint32_t ntiles, const TileDev *ti
                               const float4 *vel, float *tile_vmax, float coef, float skin_min, float *out);
 hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                                const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
-                               uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
-                               uint32_t *rec_count, const ListOut &lo);
-hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
-                                  const BodyDev *bodies, const uint32_t *start, const float4 *posm,
-                                  const float4 *vel, const float2 *rq, const uint32_t *tag, const uint32_t *masks,
-                                  const uint32_t *flags, uint32_t mstride, float4 *posm_out, float4 *vel_out,
-                                  float *rec_acc, int32_t *bbox, float4 *acc_sorted = nullptr,
-                                  const GhostDev &gh = GhostDev{});
+                               uint32_t mstride, float *rec_rho, float *rec_p, uint32_t *rec_count, const ListOut &lo);
 // The force pass over the density pass's neighbour list (ListOut cnt/idx).
 hipError_t launch_force_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                              const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,

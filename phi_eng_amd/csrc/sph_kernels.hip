@@ -254,37 +254,34 @@ __global__ __launch_bounds__(PH_TILE) void k_rank_gather(const TileDev *__restri
 // ascending slot order, i.e. the reference's neighbour order.  Rows with
 // more than 32 candidates finish their tail inline.  rq[i] = (rho_i,
 // p_i / max(rho_i, EPS)^2), the pressure quotient of the force pass
-// (fluid3.rs:1046-1051).  The density results are the same in every mode.
-// !LIST: the row masks and long-row flags are stored for the mask-based
-//   force pass (the batch's persistent kernel).
-// LIST: the neighbours are written to the particle's list instead, in
-//   order, four slots per 16-byte store (ListOut documents the layout);
-//   the count word carries SLF_SLOW / SLF_NONE where the force pass must
-//   sweep the cells or has nothing to sweep.
+// (fluid3.rs:1046-1051).  The neighbours are written to the particle's
+// list in the same order, four slots per 16-byte store (ListOut documents
+// the layout); the count word carries SLF_SLOW / SLF_NONE where the force
+// pass must sweep the cells or has nothing to sweep.
 // SKIN: the list radius is sqrt(skin_factor2) h > h (the surrogate's
 //   Verlet list, from skin_factor2_dev when set); else the list is exactly
-//   the neighbours and reuses the cut-off test's masks.
-template <bool LIST, bool SKIN = LIST>
+//   the neighbours and reuses the cut-off test's masks.  The density
+//   results are the same either way.
+template <bool SKIN>
 __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles,
                                                      const SceneDev *__restrict__ scenes,
                                                      const uint32_t *__restrict__ start,
                                                      const float4 *__restrict__ posm,
                                                      const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
-                                                     uint32_t *__restrict__ masks, uint32_t *__restrict__ flags,
                                                      uint32_t mstride, float *__restrict__ rec_rho,
                                                      float *__restrict__ rec_p, uint32_t *__restrict__ rec_count,
-                                                     const ph::ListOut &lo = ph::ListOut{}) {
+                                                     const ph::ListOut &lo) {
     const TileDev t = tiles[xcd_tile(bid, nbl)];
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     if (i >= s.part_end) return;
     const float4 pi = posm[i];
     float rho = 0.0f;
-    uint32_t cnt = 0, fl = 0;
+    uint32_t cnt = 0;
     uint32_t lk = 0;                          // skin list length
     // skin list entries are written four at a time: group g of this particle
     // (entries 4g .. 4g + 3) is one 16-byte store at lgrp[g * mstride]
-    uint4 *lgrp = LIST ? reinterpret_cast<uint4 *>(lo.idx) + i : nullptr;
+    uint4 *lgrp = reinterpret_cast<uint4 *>(lo.idx) + i;
     uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0;
     auto push = [&](uint32_t j) {
         const uint32_t p = lk & 3u;
@@ -299,7 +296,7 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
     if (finite3(pi.x, pi.y, pi.z)) {
         int32_t cx, cy, cz;
         cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
-        if (LIST && lo.cell) lo.cell[i] = make_int4(cx, cy, cz, 0);
+        if (lo.cell) lo.cell[i] = make_int4(cx, cy, cz, 0);
         const float hg2 = s.h_grid2;
         if (!needs_cell_path(s, start, cx, cy, cz)) {
 #pragma unroll 1
@@ -329,14 +326,11 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                     m |= (r2 < hg2 ? 1u : 0u) << (j - br);
                     if (SKIN) ml |= (r2 < hs2 ? 1u : 0u) << (j - br);
                 }
-                if (LIST && !SKIN) ml = m;
-                if (!LIST) masks[(size_t)r * mstride + i] = m;
-                if (LIST) {
-                    while (ml) {
-                        const uint32_t k = __ffs(ml) - 1;
-                        ml &= ml - 1;
-                        push(br + k);
-                    }
+                if (!SKIN) ml = m;
+                while (ml) {
+                    const uint32_t k = __ffs(ml) - 1;
+                    ml &= ml - 1;
+                    push(br + k);
                 }
                 while (m) {
                     const uint32_t k = __ffs(m) - 1;
@@ -347,7 +341,6 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                     rho += pj.w * poly6(r2, s);
                 }
                 if (er > lim) {
-                    fl |= 1u << r;
                     for (uint32_t j = lim; j < er; ++j) {
                         const float4 pj = posm[j];
                         const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
@@ -355,20 +348,15 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
                             ++cnt;
                             rho += pj.w * poly6(r2, s);
                         }
-                        if (LIST && (SKIN ? r2 < hs2 : r2 < hg2)) {
-                            push(j);
-                        }
+                        if (SKIN ? r2 < hs2 : r2 < hg2) push(j);
                     }
                 }
             }
-            if (LIST) {
-                // the last, partly filled group
-                if ((lk & 3u) != 0u && lk < SL_CAP) lgrp[(size_t)(lk >> 2) * mstride] = make_uint4(q0, q1, q2, q3);
-                lo.cnt[i] = lk <= SL_CAP ? lk : SLF_SLOW;
-            }
+            // the last, partly filled group
+            if ((lk & 3u) != 0u && lk < SL_CAP) lgrp[(size_t)(lk >> 2) * mstride] = make_uint4(q0, q1, q2, q3);
+            lo.cnt[i] = lk <= SL_CAP ? lk : SLF_SLOW;
         } else {
-            fl = PH_SLOW_PATH;
-            if (LIST) lo.cnt[i] = SLF_SLOW;
+            lo.cnt[i] = SLF_SLOW;
             for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
                 const float4 pj = posm[j];
                 const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
@@ -379,10 +367,8 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
             });
         }
     } else {
-        fl = PH_SLOW_PATH;  // no neighbours: the force pass skips the sweep
-        if (LIST) lo.cnt[i] = SLF_NONE;
+        lo.cnt[i] = SLF_NONE;  // no neighbours: the force pass skips the sweep
     }
-    if (!LIST) flags[i] = fl;
     rho = rclampf(rho, 0.0f, s.rho_max);
     const uint32_t tg = tag[i];
     // Indexed through global memory: a lane-varying index into the scene
@@ -399,30 +385,17 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
     }
 }
 
-__global__ __launch_bounds__(PH_TILE) void k_density(const TileDev *__restrict__ tiles,
-                                                     const SceneDev *__restrict__ scenes,
-                                                     const uint32_t *__restrict__ start,
-                                                     const float4 *__restrict__ posm,
-                                                     const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
-                                                     uint32_t *__restrict__ masks, uint32_t *__restrict__ flags,
-                                                     uint32_t mstride, float *__restrict__ rec_rho,
-                                                     float *__restrict__ rec_p, uint32_t *__restrict__ rec_count) {
-    dev_density<false>(blockIdx.x, gridDim.x, tiles, scenes, start, posm, tag, rq, masks, flags, mstride, rec_rho,
-                       rec_p, rec_count);
-}
-
 template <bool SKIN>
 __global__ __launch_bounds__(PH_TILE) void k_density_list(const TileDev *__restrict__ tiles,
                                                           const SceneDev *__restrict__ scenes,
                                                           const uint32_t *__restrict__ start,
                                                           const float4 *__restrict__ posm,
                                                           const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
-                                                          uint32_t *__restrict__ masks, uint32_t *__restrict__ flags,
                                                           uint32_t mstride, float *__restrict__ rec_rho,
                                                           float *__restrict__ rec_p, uint32_t *__restrict__ rec_count,
                                                           ph::ListOut lo) {
-    dev_density<true, SKIN>(blockIdx.x, gridDim.x, tiles, scenes, start, posm, tag, rq, masks, flags, mstride, rec_rho,
-                            rec_p, rec_count, lo);
+    dev_density<SKIN>(blockIdx.x, gridDim.x, tiles, scenes, start, posm, tag, rq, mstride, rec_rho, rec_p, rec_count,
+                      lo);
 }
 
 // ---------------------------------------------------------------- K5
@@ -485,19 +458,17 @@ __device__ __forceinline__ void pair_terms(const SceneDev &s, const float4 &pi, 
 // Force + XSPH over the neighbours the density pass found (same state,
 // same test), then integrate, speed cap, finite guards and the boundary
 // clamp of this particle (fluid3.rs:1088-1179, Jacobi).  The neighbours come
-// from the density pass's per-row bit masks, or (LIST) from its neighbour
-// list: the candidates within (1 + skin) h in reference order, four per
-// 16-byte group (group g of slot i at lgrp[g * mstride + i]), re-tested
-// here against h.  The list takes one loop over a particle's own entries
-// instead of nine row loops whose trip counts differ between lanes.
-template <bool LIST>
+// from the density pass's neighbour list: the candidates within (1 + skin) h
+// in reference order, four per 16-byte group (group g of slot i at
+// lgrp[g * mstride + i]), re-tested here against h.  The list takes one loop
+// over a particle's own entries instead of nine row loops whose trip counts
+// differ between lanes.
 __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const BodyDev *__restrict__ bodies,
     const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
-    const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ masks,
-    const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
+    const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ lcnt,
+    const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox,
-    float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}, const uint32_t *__restrict__ lcnt = nullptr,
-    const uint4 *__restrict__ lgrp = nullptr, const AuxFuse &af = AuxFuse{}) {
+    float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}, const AuxFuse &af = AuxFuse{}) {
     const uint32_t tile = xcd_tile(bid, nbl);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
@@ -513,7 +484,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         const float q_i = rqi.y;
         PairAcc acc;
         const float hg2 = s.h_grid2;
-        if (LIST) {
+        {
             const uint32_t cw = lcnt[i];
             // one list entry: the self entry and those outside h add nothing
             auto entry = [&](uint32_t j, const float4 &pj, const float4 &vj, const float2 &qj) {
@@ -545,59 +516,6 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
                     entry(j2, p2, v2, q2);
                     entry(j3, p3, v3, q3);
                 }
-            }
-        } else if (finite3(pi.x, pi.y, pi.z)) {
-            const uint32_t fl = flags[i];
-            int32_t cx, cy, cz;
-            cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
-            if (!(fl & PH_SLOW_PATH)) {
-#pragma unroll 1
-                for (int r = 0; r < 9; ++r) {
-                    uint32_t br, er;
-                    row_range(s, start, cx, cy, cz, r, br, er);
-                    uint32_t m = masks[(size_t)r * mstride + i];
-                    m &= ~(i - br < PH_ROW_BITS ? 1u << (i - br) : 0u);  // no self pair
-                    if (m) {
-                        // Software pipeline: the next neighbour's loads are in
-                        // flight while this pair is evaluated.
-                        uint32_t j = br + __ffs(m) - 1;
-                        m &= m - 1;
-                        float4 pj = posm[j], vj = vel[j];
-                        float2 qj = rq[j];
-                        while (true) {
-                            const bool more = m != 0;
-                            uint32_t jn = j;
-                            if (more) {
-                                jn = br + __ffs(m) - 1;
-                                m &= m - 1;
-                            }
-                            const float4 pn = posm[jn], vn = vel[jn];
-                            const float2 qn = rq[jn];
-                            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                            pair_terms(s, pi, vi, rho_i, q_i, pj, vj, qj, r2, acc);
-                            if (!more) break;
-                            j = jn;
-                            pj = pn;
-                            vj = vn;
-                            qj = qn;
-                        }
-                    }
-                    if (fl & (1u << r)) {
-                        for (uint32_t j = br + PH_ROW_BITS; j < er; ++j) {
-                            if (j == i) continue;
-                            const float4 pj = posm[j];
-                            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                            if (r2 < hg2) pair_terms(s, pi, vi, rho_i, q_i, pj, vel[j], rq[j], r2, acc);
-                        }
-                    }
-                }
-            } else {
-                for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
-                    if (j == i) return;
-                    const float4 pj = posm[j];
-                    const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                    if (r2 < hg2) pair_terms(s, pi, vi, rho_i, q_i, pj, vel[j], rq[j], r2, acc);
-                });
             }
         }
         if (s.ghost_count) ghost_force(s, gh, pi, q_i, acc.fx, acc.fy, acc.fz);
@@ -633,7 +551,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             rec_acc[3 * id + 1] = ay;
             rec_acc[3 * id + 2] = az;
         }
-        if (LIST && af.aux_out) {
+        if (af.aux_out) {
             const float4 ap = af.aux_in[af.perm[i]];
             const uint32_t tg = tag[i];
             const float dq = rqi.y - ap.y;
@@ -666,7 +584,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         by1 = max(by1, __shfl_xor(by1, d, 64));
         bz1 = max(bz1, __shfl_xor(bz1, d, 64));
     }
-    const bool judge = LIST && af.tile_err;
+    const bool judge = af.tile_err != nullptr;
     if (judge)
         for (int d = 32; d >= 1; d >>= 1) {
             jerr += __shfl_xor(jerr, d, 64);
@@ -735,23 +653,14 @@ __global__ __launch_bounds__(256) void k_list_radius(const float *__restrict__ t
     }
 }
 
-__global__ __launch_bounds__(PH_TILE) void k_force_integrate(const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const BodyDev *__restrict__ bodies,
-    const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
-    const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ masks,
-    const uint32_t *__restrict__ flags, uint32_t mstride, float4 *__restrict__ posm_out,
-    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, float4 *__restrict__ acc_sorted,
-    GhostDev gh) {
-    dev_force_integrate<false>(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh);
-}
-
 __global__ __launch_bounds__(PH_TILE) void k_force_list(const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes,
     const BodyDev *__restrict__ bodies, const uint32_t *__restrict__ start, const float4 *__restrict__ posm,
     const float4 *__restrict__ vel, const float2 *__restrict__ rq, const uint32_t *__restrict__ tag,
     const uint32_t *__restrict__ lcnt, const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, float4 *__restrict__ acc_sorted,
     GhostDev gh, AuxFuse af) {
-    dev_force_integrate<true>(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, nullptr, nullptr,
-                              mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh, lcnt, lgrp, af);
+    dev_force_integrate(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, lcnt, lgrp, mstride,
+                        posm_out, vel_out, rec_acc, bbox, acc_sorted, gh, af);
 }
 
 // Boundary ghost density and pressure from the fluid (fluid3.rs:906-920):
@@ -855,13 +764,13 @@ __global__ __launch_bounds__(PH_TILE) void k_step_persistent(ph::PersistArgs a) 
     lo.idx = a.lidx;
     lo.skin_factor2 = 1.0f;
     for (uint32_t k = blockIdx.x; k < a.ntiles; k += G)
-        dev_density<true, false>(k, a.ntiles, a.tiles, a.scenes, a.start, a.posm_alt, a.tag_alt, a.rq, nullptr,
-                                 nullptr, a.mstride, a.rec_rho, a.rec_p, a.rec_count, lo);
+        dev_density<false>(k, a.ntiles, a.tiles, a.scenes, a.start, a.posm_alt, a.tag_alt, a.rq, a.mstride,
+                           a.rec_rho, a.rec_p, a.rec_count, lo);
     if (!grid_barrier(a.bar, G, a.spin_limit)) return;
     for (uint32_t k = blockIdx.x; k < a.ntiles; k += G) {
-        dev_force_integrate<true>(k, a.ntiles, a.tiles, a.scenes, a.bodies, a.start, a.posm_alt, a.vel_alt, a.rq,
-                                  a.tag_alt, nullptr, nullptr, a.mstride, a.posm_cur, a.vel_cur, a.rec_acc,
-                                  a.tile_bbox, nullptr, GhostDev{}, a.lcnt, reinterpret_cast<const uint4 *>(a.lidx));
+        dev_force_integrate(k, a.ntiles, a.tiles, a.scenes, a.bodies, a.start, a.posm_alt, a.vel_alt, a.rq,
+                            a.tag_alt, a.lcnt, reinterpret_cast<const uint4 *>(a.lidx), a.mstride, a.posm_cur,
+                            a.vel_cur, a.rec_acc, a.tile_bbox, nullptr);
         __syncthreads();
     }
 }
@@ -1054,33 +963,16 @@ hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *ti
                        posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted, perm_out);
     return hipGetLastError();
 }
-hipError_t launch_density(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
-                          const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
-                          uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
-                          uint32_t *rec_count) {
-    hipLaunchKernelGGL(k_density, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, rq, masks,
-                       flags, mstride, rec_rho, rec_p, rec_count);
-    return hipGetLastError();
-}
 hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                                const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
-                               uint32_t *masks, uint32_t *flags, uint32_t mstride, float *rec_rho, float *rec_p,
-                               uint32_t *rec_count, const ListOut &lo) {
+                               uint32_t mstride, float *rec_rho, float *rec_p, uint32_t *rec_count,
+                               const ListOut &lo) {
     if (lo.skin_factor2 != 1.0f || lo.skin_factor2_dev)
         hipLaunchKernelGGL(k_density_list<true>, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag,
-                           rq, masks, flags, mstride, rec_rho, rec_p, rec_count, lo);
+                           rq, mstride, rec_rho, rec_p, rec_count, lo);
     else
         hipLaunchKernelGGL(k_density_list<false>, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag,
-                           rq, masks, flags, mstride, rec_rho, rec_p, rec_count, lo);
-    return hipGetLastError();
-}
-hipError_t launch_force_integrate(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
-                                  const BodyDev *bodies, const uint32_t *start, const float4 *posm,
-                                  const float4 *vel, const float2 *rq, const uint32_t *tag, const uint32_t *masks,
-                                  const uint32_t *flags, uint32_t mstride, float4 *posm_out, float4 *vel_out,
-                                  float *rec_acc, int32_t *bbox, float4 *acc_sorted, const GhostDev &gh) {
-    hipLaunchKernelGGL(k_force_integrate, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm,
-                       vel, rq, tag, masks, flags, mstride, posm_out, vel_out, rec_acc, bbox, acc_sorted, gh);
+                           rq, mstride, rec_rho, rec_p, rec_count, lo);
     return hipGetLastError();
 }
 hipError_t launch_force_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
