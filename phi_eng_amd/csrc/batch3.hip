@@ -90,7 +90,7 @@ struct Batch3 {
         a.rq = e.rq.p;
         a.lcnt = e.nl_cnt.p;
         a.lidx = e.nl_idx.p;
-        a.mstride = (uint32_t)e.alloc_particles;
+        a.mstride = e.list_stride();
         a.tile_bbox = e.d_bbox.p;
         a.n = e.n_total;
         a.nbuckets = e.nbuckets;

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -179,6 +180,12 @@ struct Engine {
     ~Engine();
 
     void init_device();
+    // Row stride of the neighbour list (entries of group g of slot i sit at
+    // g * stride + i): the particle count rounded up to 256, so a list
+    // spans no more rows than the particles fill whatever the capacity.
+    uint32_t list_stride() const {
+        return (uint32_t)std::min<size_t>(alloc_particles, ((size_t)n_total + 255) & ~(size_t)255);
+    }
     // Make room for `cap` particles, keeping the current state.
     void ensure_particles(uint32_t cap);
     void build_tiles();

@@ -507,7 +507,7 @@ void Engine::plain_phase(float4 *acc_sorted, const AuxFuse *af, bool density_don
     if (!density_done)
         run(KIND_DENSITY, [&] {
             return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p,
-                                       (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
+                                       list_stride(), rd ? rec_rho.p : nullptr,
                                        rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr, lo);
         });
     // boundary ghosts (fluid3.rs:1018-1027): their density and pressure from
@@ -519,7 +519,7 @@ void Engine::plain_phase(float4 *acc_sorted, const AuxFuse *af, bool density_don
         });
     run(KIND_FORCE, [&] {
         return launch_force_list(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p, vel[ns].p, rq.p,
-                                 tag[nt_].p, nl_cnt.p, nl_idx.p, (uint32_t)alloc_particles, posm[cs].p, vel[cs].p,
+                                 tag[nt_].p, nl_cnt.p, nl_idx.p, list_stride(), posm[cs].p, vel[cs].p,
                                  ra ? rec_acc.p : nullptr, d_bbox.p, acc_sorted, ghosts ? ghost_dev() : GhostDev{},
                                  af ? *af : AuxFuse{});
     });
@@ -536,7 +536,7 @@ void Engine::density_phase() {
     lo.skin_factor2 = 1.0f;
     run(KIND_DENSITY, [&] {
         return launch_density_list(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p,
-                                   (uint32_t)alloc_particles, rd ? rec_rho.p : nullptr,
+                                   list_stride(), rd ? rec_rho.p : nullptr,
                                    rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr, lo);
     });
 }

@@ -50,8 +50,9 @@ struct ResampleArgs {
 
 // Launches the whole pass on the stream (no host synchronisation).  After
 // it, ctr[1] > pcap means the candidate store was too small (grow it to
-// ctr[1] and run again); else the new count is ctr[3] + ctr[2] and the new
-// set is in posm_out / vel_out / tag_out.
+// ctr[1] and run again); else the new count is ctr[3] + ctr[2] and, unless
+// nothing changed (ctr[3] == n and ctr[2] == 0), the new set is in
+// posm_out / vel_out / tag_out.
 hipError_t launch_resample(hipStream_t st, const ResampleArgs &a);
 
 // Per particle, in index order: its mass and phase (n slots of the state).

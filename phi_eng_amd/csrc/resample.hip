@@ -240,6 +240,8 @@ __global__ __launch_bounds__(PH_TILE) void k_rs_count(ResampleArgs a) {
 }
 
 __global__ __launch_bounds__(PH_TILE) void k_rs_write(ResampleArgs a) {
+    // nothing merged or split: the caller keeps the sorted state as it is
+    if (a.ctr[3] == a.n && a.ctr[2] == 0) return;
     const uint32_t p = blockIdx.x * PH_TILE + threadIdx.x;
     if (p >= a.n || a.rK[p + 1] == a.rK[p]) return;  // absorbed
     const uint32_t s = a.slot_of[p];

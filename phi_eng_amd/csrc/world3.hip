@@ -411,7 +411,7 @@ struct World3 {
 
     bool list_usable() const {
         return eng.nl_cell.p && sl_rec[0].p && list_sorts == eng.sorts_done && list_epoch == eng.layout_epoch &&
-               list_stride == eng.alloc_particles;
+               list_stride == eng.list_stride();
     }
 
     // Plain substep that also refreshes the history; optionally judged
@@ -458,7 +458,7 @@ struct World3 {
             // the skin list of the order this substep sorted into
             list_sorts = e.sorts_done;
             list_epoch = e.layout_epoch;
-            list_stride = (uint32_t)e.alloc_particles;
+            list_stride = e.list_stride();
         }
         if (collect) {
             SurCollectArgs a{};
