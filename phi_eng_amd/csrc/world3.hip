@@ -275,7 +275,10 @@ struct World3 {
             domain_dirty = true;
         }
         if (before.h_grid != s.h_grid) domain_dirty = true;
-        if (e.n_total != n || e.tiles.empty()) {
+        // the tiles cover [part_begin, part_end): rebuilt when the count
+        // changed since they were laid out (an add or a load lays them out
+        // before the scene range is updated here)
+        if (e.n_total != n || e.tiles.empty() || before.part_end != s.part_end) {
             e.n_total = n;
             e.build_tiles();
         }

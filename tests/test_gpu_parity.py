@@ -120,6 +120,49 @@ def test_cfl_substeps_follow_oracle(gpu, oracle_lib):
     w.close()
 
 
+def test_add_particles_after_steps(gpu, oracle_lib):
+    """Particles added to a world that has stepped (the tile count grows
+    from 7 to 8 and to 9 on the second add): every step after each add equals
+    the oracle's Jacobi substep of the same particle set bit for bit."""
+    from phi_eng_amd.ffi import FluidConfig3Ffi
+    from phi_eng_amd.world import World3
+    O = oracle_lib
+    cfg = FluidConfig3Ffi.default()
+    boxes = [((2.0, -1.0, 2.0), (3.0, 0.5, 3.0)), ((-1.0, 2.0, 2.0), (0.5, 3.5, 3.5)),
+             ((5.0, 2.0, 2.0), (0.5, 3.5, 3.5)), ((2.0, 2.0, -1.0), (3.5, 3.5, 0.5)),
+             ((2.0, 2.0, 5.0), (3.5, 3.5, 0.5))]
+    bodies = O.bodies_from_boxes(boxes)
+    rng = np.random.default_rng(21)
+
+    def block(n_side, lo):
+        g = np.arange(n_side) * 0.3
+        p = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + np.asarray(lo)
+        return (p + rng.uniform(-0.02, 0.02, p.shape)).astype(np.float32)
+
+    w = World3()
+    assert w.set_fluid_config(cfg)
+    for c, he in boxes:
+        assert w.add_static_box(c, he) >= 0
+    rp = block(12, (0.2, 0.2, 0.2))
+    rv = np.zeros_like(rp)
+    assert w.add_particles(rp) == len(rp)
+    dt = 1.0 / 60.0
+    for more, steps in ((block(6, (2.3, 1.0, 2.3)), 0), (None, 5), (block(7, (0.4, 2.5, 0.4)), 0), (None, 5)):
+        if more is not None:
+            assert w.add_particles(more) == len(more)
+            rp = np.concatenate([rp, more])
+            rv = np.concatenate([rv, np.zeros_like(more)])
+        for k in range(steps):
+            w.step(dt)
+            r = O.substep_f32(rp, rv, cfg, (0.0, -9.81, 0.0), bodies, dt, jacobi=True)
+            rp, rv = r["pos"], r["vel"]
+            p, v = w.get_state()
+            print(f"[add] {len(p)} particles, step {k}", flush=True)
+            assert np.array_equal(p.view(np.uint32), rp.view(np.uint32)), f"positions, {len(p)} particles"
+            assert np.array_equal(v.view(np.uint32), rv.view(np.uint32)), f"velocities, {len(p)} particles"
+    w.close()
+
+
 def test_run_twice_bit_identical(gpu):
     """fluid3.rs:1398-1416 determinism, on the GPU path."""
     sc = scenes.small_dam(10, seed=15)
