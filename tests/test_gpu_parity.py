@@ -122,8 +122,9 @@ def test_cfl_substeps_follow_oracle(gpu, oracle_lib):
 
 def test_add_particles_after_steps(gpu, oracle_lib):
     """Particles added to a world that has stepped (the tile count grows
-    from 7 to 8 and to 9 on the second add): every step after each add equals
-    the oracle's Jacobi substep of the same particle set bit for bit."""
+    from 7 to 8 and to 9 on the second add), then the fluid cleared and a
+    new set added (3 tiles): every step equals the oracle's Jacobi substep
+    of the same particle set bit for bit."""
     from phi_eng_amd.ffi import FluidConfig3Ffi
     from phi_eng_amd.world import World3
     O = oracle_lib
@@ -147,8 +148,14 @@ def test_add_particles_after_steps(gpu, oracle_lib):
     rv = np.zeros_like(rp)
     assert w.add_particles(rp) == len(rp)
     dt = 1.0 / 60.0
-    for more, steps in ((block(6, (2.3, 1.0, 2.3)), 0), (None, 5), (block(7, (0.4, 2.5, 0.4)), 0), (None, 5)):
-        if more is not None:
+    plan = ((block(6, (2.3, 1.0, 2.3)), 0), (None, 5), (block(7, (0.4, 2.5, 0.4)), 0), (None, 5), ("clear", 0),
+            (block(9, (1.0, 0.3, 1.0)), 4))
+    for more, steps in plan:
+        if isinstance(more, str):
+            w.clear()
+            rp = np.zeros((0, 3), np.float32)
+            rv = np.zeros((0, 3), np.float32)
+        elif more is not None:
             assert w.add_particles(more) == len(more)
             rp = np.concatenate([rp, more])
             rv = np.concatenate([rv, np.zeros_like(more)])
