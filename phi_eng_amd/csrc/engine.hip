@@ -473,7 +473,7 @@ void Engine::sort_phase(uint32_t *perm_out) {
     });
     run(KIND_SCAN_REDUCE, [&] { return launch_scan_reduce(stream, counts.p, nbuckets, partial.p); });
     run(KIND_SCAN_PARTIALS, [&] {
-        return launch_scan_partials(stream, partial.p, nbuckets, start.p, d_bbox.p, (uint32_t)scenes.size());
+        return launch_scan_partials(stream, partial.p, nbuckets, start.p);
     });
     run(KIND_SCAN_APPLY, [&] { return launch_scan_apply(stream, counts.p, nbuckets, partial.p, start.p); });
     counts_dirty = false;

@@ -300,7 +300,7 @@ hipError_t launch_resample(hipStream_t st, const ResampleArgs &a) {
     for (int k = 0; k < 2; ++k) {
         uint32_t *c = k ? a.cE : a.cK, *r = k ? a.rE : a.rK;
         if ((e = launch_scan_reduce(st, c, a.n, a.partial)) != hipSuccess) return e;
-        if ((e = launch_scan_partials(st, a.partial, a.n, r, nullptr, 0u)) != hipSuccess) return e;
+        if ((e = launch_scan_partials(st, a.partial, a.n, r)) != hipSuccess) return e;
         if ((e = launch_scan_apply(st, c, a.n, a.partial, r)) != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_rs_count, dim3(g), dim3(PH_TILE), 0, st, a);

@@ -33,8 +33,7 @@ hipError_t launch_cell_key(hipStream_t st, uint32_t ntiles, const TileDev *tiles
                            const float4 *posm, const uint32_t *tag, uint32_t *keys, uint32_t *slots,
                            uint32_t *counts, int32_t *rec_cells);
 hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t nb, uint32_t *partial);
-hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start,
-                                int32_t *bbox, uint32_t nscenes);
+hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start);
 hipError_t launch_scan_apply(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
                              uint32_t *start);
 hipError_t launch_scatter(hipStream_t st, uint32_t n, const uint32_t *keys, const uint32_t *slots,

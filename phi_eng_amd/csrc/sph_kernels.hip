@@ -935,10 +935,7 @@ hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t n
                        partial);
     return hipGetLastError();
 }
-hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start,
-                                int32_t *bbox, uint32_t nscenes) {
-    (void)bbox;
-    (void)nscenes;
+hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start) {
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlock), 0, st, partial, scan_partials_needed(nb),
                        start, nb);
     return hipGetLastError();

@@ -42,8 +42,8 @@ struct SurAcc {
 // rsqrt, fused multiply-adds): the surrogate is judged on its density
 // error, not on bit parity.
 template <bool UMASS = false>
-__device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, const float4 &pi, const float4 &vi,
-                                         float4 mt, const float4 &ra, const float4 &rb, SurAcc &a) {
+__device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, const float4 &pi, float4 mt,
+                                         const float4 &ra, const float4 &rb, SurAcc &a) {
     const float dx = ra.x - pi.x, dy = ra.y - pi.y, dz = ra.z - pi.z;
     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const bool in = ok && r2 < s.h_grid2;
@@ -74,7 +74,6 @@ __device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, 
     a.dz = fmaf(md, dz, a.dz);
     // XSPH: sum m_j W_ij v_j (mw = 0 off the neighbourhood); the numerator
     // sum m_j W_ij (v_j - v_i) is this minus v_i rho, formed once per particle
-    (void)vi;
     a.sx = fmaf(rb.x, mw, a.sx);
     a.sy = fmaf(rb.y, mw, a.sy);
     a.sz = fmaf(rb.z, mw, a.sz);
@@ -85,8 +84,7 @@ __device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, 
 __device__ __forceinline__ void sur_sweep(const SceneDev &s, const uint32_t *__restrict__ start,
                                           const float4 *__restrict__ posm, const float4 *__restrict__ vel,
                                           const float4 *__restrict__ aux, const uint32_t *__restrict__ tag,
-                                          float4 mt, uint32_t i, const float4 &pi, const float4 &vi,
-                                          SurAcc &a) {
+                                          float4 mt, uint32_t i, const float4 &pi, SurAcc &a) {
     if (!finite3(pi.x, pi.y, pi.z)) return;
     int32_t cx, cy, cz;
     cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
@@ -94,7 +92,7 @@ __device__ __forceinline__ void sur_sweep(const SceneDev &s, const uint32_t *__r
         const float4 pj = posm[j], vj = vel[j], aj = aux[j];
         const float4 ra = make_float4(pj.x, pj.y, pj.z, pack_pq(aj.y, tag[j] >> 24));
         const float4 rb = make_float4(vj.x, vj.y, vj.z, aj.z);
-        sur_cand(s, true, j == i, pi, vi, mt, ra, rb, a);
+        sur_cand(s, true, j == i, pi, mt, ra, rb, a);
     });
 }
 
@@ -215,12 +213,11 @@ template <bool UMASS>
 __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t *__restrict__ start,
                                                const float4 *__restrict__ posm_p, const BufF4 &rec, float4 mt,
                                                uint32_t cw, const int4 &cell, const uint4 *__restrict__ grp_p,
-                                               uint32_t mstride, uint32_t i, const float4 &pi, const float4 &vi,
-                                               SurAcc &a) {
+                                               uint32_t mstride, uint32_t i, const float4 &pi, SurAcc &a) {
     if (cw & SLF_NONE) return;
     if (cw & SLF_SLOW) {
         for_each_candidate(s, start, posm_p, cell.x, cell.y, cell.z,
-                           [&](uint32_t j) { sur_cand<UMASS>(s, true, j == i, pi, vi, mt, rec[2 * j], rec[2 * j + 1], a); });
+                           [&](uint32_t j) { sur_cand<UMASS>(s, true, j == i, pi, mt, rec[2 * j], rec[2 * j + 1], a); });
         return;
     }
     const uint32_t n = cw & SLF_COUNT;
@@ -247,10 +244,10 @@ __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t
         // can be consumed while this one is still in flight
         row += mstride;
         const uint4 nxt = load_row(row);
-        sur_cand<UMASS>(s, true, j0 == i, pi, vi, mt, a0, b0, a);
-        sur_cand<UMASS>(s, o1, j1 == i, pi, vi, mt, a1, b1, a);
-        sur_cand<UMASS>(s, o2, j2 == i, pi, vi, mt, a2, b2, a);
-        sur_cand<UMASS>(s, o3, j3 == i, pi, vi, mt, a3, b3, a);
+        sur_cand<UMASS>(s, true, j0 == i, pi, mt, a0, b0, a);
+        sur_cand<UMASS>(s, o1, j1 == i, pi, mt, a1, b1, a);
+        sur_cand<UMASS>(s, o2, j2 == i, pi, mt, a2, b2, a);
+        sur_cand<UMASS>(s, o3, j3 == i, pi, mt, a3, b3, a);
         grp = nxt;
     }
 }
@@ -279,9 +276,9 @@ __global__ __launch_bounds__(PH_TILE) void k_sur_collect(SurCollectArgs a) {
     if (id % a.stride != a.phase) return;
     const uint32_t k = a.sample_base + id / a.stride;
     if (k >= a.max_samples) return;
-    const float4 pi = a.posm[i], vi = a.vel[i], ai = a.aux[i];
+    const float4 pi = a.posm[i], ai = a.aux[i];
     SurAcc acc;
-    sur_sweep(s, a.start, a.posm, a.vel, a.aux, a.tag, a.mass, i, pi, vi, acc);
+    sur_sweep(s, a.start, a.posm, a.vel, a.aux, a.tag, a.mass, i, pi, acc);
     const float rho0 = a.scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
     SurFeat f;
     sur_features(s, rho0, a.qs, acc, ai.x, ai.z, f);
@@ -346,7 +343,7 @@ __global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
         const float4 mt = make_float4(a.mass.x, a.mass.y, a.mass.z, a.mass.w);
         SurAcc acc;
         sur_list_sweep<UMASS>(s, a.start, a.posm, rec, mt, cw, cl, reinterpret_cast<const uint4 *>(a.idx), a.mstride, i,
-                              pi, vi, acc);
+                              pi, acc);
         const float rho0 = a.scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
         sur_features(s, rho0, a.qs, acc, ai.x, ai.z, f);
         const float irl = __builtin_amdgcn_rcpf(rmaxf(ai.x, PH_EPS));

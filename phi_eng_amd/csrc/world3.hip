@@ -1244,6 +1244,8 @@ int32_t phieng_world_get_fluid_config(const World2 *world, FluidConfig2Ffi *out)
     return world ? phieng_world3_get_fluid_config(as3(world), out) : 0;
 }
 
+}  // extern "C"
+
 namespace {
 // xy pairs -> xyz triples in the z = 0 plane (after the finite check of the
 // caller's values, so a rejected call stays atomic)
@@ -1262,6 +1264,8 @@ bool all_finite(const float *v, size_t k) {
     return true;
 }
 }  // namespace
+
+extern "C" {
 
 uint32_t phieng_world_fluid_add(World2 *world, const float *positions, uint32_t count) {
     if (!world || !positions) return 0;
