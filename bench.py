@@ -46,10 +46,11 @@ def _args():
                     help="HBM bytes per launch per kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE (optional)")
     ap.add_argument("--valu", default=os.path.join(ROOT, "profiles", "valu.json"),
                     help="VALU wave instructions per launch per kernel from rocprofv3 SQ_INSTS_VALU (optional)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "scale8m", "extras"],
+    ap.add_argument("--workload", default="plain", choices=["plain", "batch", "scale8m", "extras", "dam2d"],
                     help="plain: the 1M dam break (the headline); batch: 256 uneven scenes in one launch per step; "
                          "scale8m: 8,388,608 particles, two colliding columns, plain vs surrogate; extras: the "
-                         "1M dam break with adaptive resolution, and snapshot save / load of the 1M world")
+                         "1M dam break with adaptive resolution, and snapshot save / load of the 1M world; "
+                         "dam2d: the 4,096-particle 2D dam break alone (GPU and CPU)")
     ap.add_argument("--calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (rocprof counter scale)")
     ap.add_argument("--sur-steps", type=int, default=200,
@@ -293,6 +294,8 @@ def main():
         return bench_scale(a)
     if a.workload == "extras":
         return bench_extras(a)
+    if a.workload == "dam2d":
+        return bench_dam2d(a)
     from phi_eng_amd import scenes
     from phi_eng_amd.ffi import last_error
 
@@ -394,6 +397,21 @@ def main():
     if a.sur_steps > 0:
         out["surrogate"] = bench_surrogate(a, value)
     print(json.dumps(out))
+
+
+def bench_dam2d(a):
+    """The 2D dam break (4,096 particles, 1,000 steps at dt 1/600) alone: the
+    whole workload on the GPU, and on one CPU core unless --no-cpu."""
+    gpu = bench_gpu_2d()
+    line = {"metric": "particle-steps/sec, 2D dam break (4,096 particles, 1,000 steps)", "value": gpu["value"],
+            "unit": "particle-steps/s", "n_gpus": 1, "steps": gpu["steps"], "warmup": 5,
+            "ms_per_step": gpu["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic: seeded jittered 64 x 64 lattice",
+            "config": {"workload": gpu["workload"], "particles": gpu["particles"], "dt": gpu["dt"]},
+            "contained": gpu["contained"]}
+    if not a.no_cpu:
+        line["cpu_baseline"] = bench_cpu_2d()
+    print(json.dumps(line), flush=True)
 
 
 def bench_extras(a):
