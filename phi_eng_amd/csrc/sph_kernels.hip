@@ -868,12 +868,16 @@ __global__ __launch_bounds__(PH_TILE) void k_cell_bbox(uint32_t n, const float4 
         z1 = max(z1, __shfl_xor(z1, d, 64));
     }
     if ((threadIdx.x & 63) == 0 && x0 != INT32_MAX) {
-        atomicMin(bb + 0, x0);
-        atomicMin(bb + 1, y0);
-        atomicMin(bb + 2, z0);
-        atomicMax(bb + 3, x1);
-        atomicMax(bb + 4, y1);
-        atomicMax(bb + 5, z1);
+        // the six words only move toward the result, so a word that already
+        // holds a value at least as tight as this wave's needs no atomic
+        // (millions of same-address atomics otherwise serialise)
+        auto rd = [&](int k) { return __hip_atomic_load(bb + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+        if (x0 < rd(0)) atomicMin(bb + 0, x0);
+        if (y0 < rd(1)) atomicMin(bb + 1, y0);
+        if (z0 < rd(2)) atomicMin(bb + 2, z0);
+        if (x1 > rd(3)) atomicMax(bb + 3, x1);
+        if (y1 > rd(4)) atomicMax(bb + 4, y1);
+        if (z1 > rd(5)) atomicMax(bb + 5, z1);
     }
 }
 
