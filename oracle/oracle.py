@@ -272,6 +272,37 @@ def pressure(rho, rest, k, gamma):
     return lib().oracle3_pressure(rho, rest, k, gamma)
 
 
+def artificial_pressure(delta, v_rel, r_sq, h, rho_bar, c, alpha):
+    """fluid3.rs:254-269 (Monaghan viscous pressure term)."""
+    l = lib()
+    l.oracle3_artificial_pressure.restype = ctypes.c_float
+    d = np.asarray(delta, dtype=np.float32)
+    v = np.asarray(v_rel, dtype=np.float32)
+    return l.oracle3_artificial_pressure(_p(d), _p(v), *[ctypes.c_float(x) for x in (r_sq, h, rho_bar, c, alpha)])
+
+
+def cohesion(dist, h):
+    """fluid3.rs:272-278."""
+    l = lib()
+    l.oracle3_cohesion.restype = ctypes.c_float
+    return l.oracle3_cohesion(ctypes.c_float(dist), ctypes.c_float(h))
+
+
+def artificial_pressure2(delta, v_rel, r_sq, h, rho_bar, c, alpha):
+    """fluid.rs:302-317."""
+    l = lib2()
+    l.oracle2_artificial_pressure.restype = ctypes.c_float
+    args = [delta[0], delta[1], v_rel[0], v_rel[1], r_sq, h, rho_bar, c, alpha]
+    return l.oracle2_artificial_pressure(*[ctypes.c_float(x) for x in args])
+
+
+def cohesion2(dist, h):
+    """fluid.rs:320-327."""
+    l = lib2()
+    l.oracle2_cohesion.restype = ctypes.c_float
+    return l.oracle2_cohesion(ctypes.c_float(dist), ctypes.c_float(h))
+
+
 def cfl_substeps(vel, cfg, dt):
     vel = np.ascontiguousarray(vel, dtype=np.float32)
     return lib().oracle3_cfl_substeps(ctypes.c_uint32(len(vel)), _p(vel), ctypes.byref(cfg), ctypes.c_float(dt))

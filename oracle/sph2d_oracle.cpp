@@ -20,7 +20,10 @@
 // brute-force neighbour counts and densities, the lattice closed form, the
 // two-particle force closed form and momentum antisymmetry, and the
 // reference's behaviour tests restated (dam break contained :1225-1256,
-// determinism :1292-1309, hydrostatic band :1676-1719).
+// determinism :1292-1309, hydrostatic band :1676-1719).  The optional pair
+// terms (artificial viscosity, cohesion, XSPH) are pinned by the reference's
+// sign tests (fluid.rs:1333-1360) and two-particle closed forms of a whole
+// substep in tests/test_oracle_pair_terms.py.
 // Parity unpinned: glam 0.29.3 Vec2 association order ((x*x) + (y*y)) and
 // the last ulp of libm powf (taken in f64 and rounded, as in the 3D oracle).
 #include <algorithm>
@@ -663,5 +666,11 @@ void oracle2_spiky_grad(float rx, float ry, float r, float h, float *out2) {
 float oracle2_pressure(float density, float rest, float k, float gamma) {
     return Fluid2::pressure(density, rest, k, gamma);
 }
+// fluid.rs:302-327 (pinned by fluid.rs:1333-1360 and two-particle closed forms).
+float oracle2_artificial_pressure(float dx, float dy, float vx, float vy, float r_sq, float h, float rho_bar, float c,
+                                  float alpha) {
+    return Fluid2::artificial_pressure(dx, dy, vx, vy, r_sq, h, rho_bar, c, alpha);
+}
+float oracle2_cohesion(float dist, float h) { return Fluid2::cohesion(dist, h); }
 
 }  // extern "C"

@@ -19,6 +19,11 @@
 //       lattice closed form (interior rho = m * sum W).
 //   force / acceleration           — two-particle closed form, momentum
 //       antisymmetry sum_i m_i F_i = 0, FP64 twin agreement.
+//   artificial_pressure / cohesion / XSPH — the reference's sign tests
+//       (fluid3.rs:1426-1453), two-particle closed forms of each term over a
+//       whole substep (unequal masses; Jacobi and in-place integrate),
+//       sum m_i F_i = 0 (AV) and sum F_i = 0 (cohesion) on a cloud
+//       (tests/test_oracle_pair_terms.py).
 //   integrate + boundary clamp     — containment, count preserved,
 //       run-twice determinism, hydrostatic band (fluid3.rs:1313-1347,
 //       1398-1416, 1560-1605), single-particle closed forms.
@@ -933,6 +938,14 @@ float oracle3_pressure(float density, float rest, float k, float gamma) {
     return Fluid<float>::eos(density, rest, k, gamma);
 }
 double oracle3_poly6_f64(double r2, double h) { return Fluid<double>::poly6(r2, h); }
+// fluid3.rs:254-269 and :272-278 (pinned by the reference's own sign test,
+// fluid3.rs:1418-1455, and by two-particle closed forms of the whole step).
+float oracle3_artificial_pressure(const float *delta3, const float *vrel3, float r_sq, float h, float rho_bar,
+                                  float c, float alpha) {
+    return Fluid<float>::artificial_pressure({delta3[0], delta3[1], delta3[2]}, {vrel3[0], vrel3[1], vrel3[2]}, r_sq,
+                                             h, rho_bar, c, alpha);
+}
+float oracle3_cohesion(float dist, float h) { return Fluid<float>::cohesion(dist, h); }
 uint32_t oracle3_cfl_substeps(uint32_t n, const float *vel, const FluidConfig3Ffi *cfg, float dt) {
     Fluid<float> f(*cfg);
     std::vector<float> p(3 * (size_t)n, 0.0f), m(n, 1.0f);
