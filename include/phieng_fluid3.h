@@ -202,11 +202,18 @@ typedef struct Batch3 Batch3;
  * CFL substepping or boundary ghosts, or the total exceeds 2^24 particles. */
 Batch3 *phieng_batch3_new(World3 *const *worlds, uint32_t count);
 void phieng_batch3_free(Batch3 *batch);
-/* Launch configuration; blocks_per_cu = 0 uses the occupancy limit. */
+/* Launch configuration; blocks_per_cu = 0 uses the occupancy limit.  Also
+ * clears the persistent launch's grid barrier after a time-out. */
 int32_t phieng_batch3_set_mode(Batch3 *batch, int32_t mode, uint32_t blocks_per_cu);
-/* Advance every world by dt (same rules as phieng_world3_step). 1 / 0. */
+/* Bounded wait of the persistent launch's grid barrier: a block that spins
+ * more than `spins` times raises the abort flag and every block leaves at
+ * its next barrier (default 2^22). */
+int32_t phieng_batch3_set_spin_limit(Batch3 *batch, uint32_t spins);
+/* Advance every world by dt (same rules as phieng_world3_step). 1 / 0;
+ * 0 once a sync has seen a timed-out barrier, until phieng_batch3_set_mode. */
 int32_t phieng_batch3_step(Batch3 *batch, float dt);
-/* Wait for the queued steps; 0 if a grid barrier timed out. */
+/* Wait for the queued steps; 0 if a grid barrier timed out (the batch's
+ * state is then invalid). */
 int32_t phieng_batch3_sync(Batch3 *batch);
 /* Write the batch state back into its worlds. 1 / 0. */
 int32_t phieng_batch3_sync_worlds(Batch3 *batch);

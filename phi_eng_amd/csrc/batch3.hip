@@ -324,15 +324,43 @@ void phieng_batch3_free(Batch3 *batch) {
 
 int32_t phieng_batch3_set_mode(Batch3 *b, int32_t mode, uint32_t blocks_per_cu) {
     if (!b || mode < PHIENG_BATCH_KERNELS || mode > PHIENG_BATCH_KERNELS_GRAPH) return 0;
-    b->mode = mode;
-    b->blocks_per_cu = blocks_per_cu;
-    b->set_grid();
-    b->drop_graphs();
+    try {
+        b->mode = mode;
+        b->blocks_per_cu = blocks_per_cu;
+        b->set_grid();
+        b->drop_graphs();
+        // A timed-out grid barrier leaves its arrival count and abort flag
+        // set; a new mode starts from clear barrier words (a fill kernel on
+        // the stream, outside any captured graph).
+        // A launch that left at a barrier may also have counted keys into the
+        // bucket counts without the scan that re-zeroes them.
+        ph::Engine &e = b->eng;
+        e.run(ph::KIND_FILL, [&] { return ph::launch_fill_u32(e.stream, b->bar.p, 4u, 0u); });
+        if (e.counts.p)
+            e.run(ph::KIND_FILL, [&] { return ph::launch_fill_u32(e.stream, e.counts.p, (uint32_t)e.alloc_buckets, 0u); });
+        e.counts_dirty = false;
+        PH_CK(hipStreamSynchronize(e.stream));
+        b->abort_seen = false;
+        return 1;
+    } catch (const std::exception &ex) {
+        set_err("phieng_batch3_set_mode", ex.what());
+        return 0;
+    }
+}
+
+int32_t phieng_batch3_set_spin_limit(Batch3 *b, uint32_t spins) {
+    if (!b) return 0;
+    b->spin_limit = spins;
+    b->drop_graphs();  // captured launches carry the old limit
     return 1;
 }
 
 int32_t phieng_batch3_step(Batch3 *b, float dt) {
     if (!b || !std::isfinite(dt) || dt <= 0.0f) return 0;
+    if (b->abort_seen) {
+        set_err("phieng_batch3_step", "a grid barrier timed out earlier; call phieng_batch3_set_mode to recover");
+        return 0;
+    }
     try {
         b->step(std::min(dt, 1.0f / 15.0f));
         return 1;

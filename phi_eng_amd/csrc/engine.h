@@ -224,6 +224,11 @@ struct Engine {
     DBuf<unsigned long long> rs_ctr;
     unsigned long long *rs_host = nullptr;  // pinned copy of rs_ctr
     DBuf<uint32_t> rs_flag, rs_items, rs_minT, rs_pick, rs_state, rs_cK, rs_cE, rs_rK, rs_rE, rs_slot, rs_partial;
+    // First size of the merge-candidate store (0: the particle capacity).
+    // The store grows when a pass overflows it; the environment variable
+    // PHIENG_RESAMPLE_STORE_INITIAL sets a small first size so tests can
+    // force that grow-and-rerun path.
+    size_t rs_items_initial = 0;
     DBuf<uint4> rs_takers;
     DBuf<float4> rs_mv;
     // surrogate history (previous rho, P = p/rho^2, its change) per slot

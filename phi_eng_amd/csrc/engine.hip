@@ -1,6 +1,8 @@
 // engine.hip — step orchestration of the device-resident SPH state.
 #include "engine.h"
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -266,6 +268,7 @@ void Engine::init_device() {
     PH_CK(hipEventCreateWithFlags(&mon_event, hipEventDisableTiming));
     PH_CK(hipHostMalloc((void **)&h_monitor, 8 * sizeof(int32_t) * 4096, hipHostMallocDefault));
     d_vmax.ensure(1);
+    if (const char *e = std::getenv("PHIENG_RESAMPLE_STORE_INITIAL")) rs_items_initial = std::strtoull(e, nullptr, 10);
 }
 
 void Engine::ensure_particles(uint32_t cap) {
@@ -564,7 +567,7 @@ uint32_t Engine::resample_phase(float merge_hi, float split_lo, float min_mass, 
     rs_partial.ensure(scan_partials_needed((uint32_t)m) + 1);
     rs_takers.ensure(m);
     rs_mv.ensure(m);
-    if (rs_items.n < m) rs_items.ensure(m);
+    if (!rs_items.p) rs_items.ensure(rs_items_initial ? rs_items_initial : m);
     ResampleArgs a;
     a.tiles = d_tiles.p;
     a.scenes = d_scenes.p;

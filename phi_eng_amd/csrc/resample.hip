@@ -218,7 +218,13 @@ __device__ __forceinline__ void merged(const ResampleArgs &a, uint32_t s, float4
     }
 }
 
+// A pass whose candidate store overflowed (ctr[1] > pcap) writes nothing
+// after k_rs_find: its takers' candidate runs are incomplete, so the host
+// grows the store and runs the whole pass again.  k_rs_mark leaves cK / cE
+// at zero (the scans re-zero them), k_rs_count and k_rs_write return, and
+// the particle arrays stay as they were.
 __global__ __launch_bounds__(PH_TILE) void k_rs_mark(ResampleArgs a) {
+    if (a.ctr[1] > a.pcap) return;
     const uint32_t s = blockIdx.x * PH_TILE + threadIdx.x;
     if (s >= a.n) return;
     const uint32_t id = a.tag[s] & PH_ID_MASK;
@@ -232,6 +238,7 @@ __global__ __launch_bounds__(PH_TILE) void k_rs_mark(ResampleArgs a) {
 }
 
 __global__ __launch_bounds__(PH_TILE) void k_rs_count(ResampleArgs a) {
+    if (a.ctr[1] > a.pcap) return;
     const uint32_t p = blockIdx.x * PH_TILE + threadIdx.x;
     if (p == 0) a.ctr[3] = a.rK[a.n];
     if (p >= a.n) return;
@@ -240,6 +247,7 @@ __global__ __launch_bounds__(PH_TILE) void k_rs_count(ResampleArgs a) {
 }
 
 __global__ __launch_bounds__(PH_TILE) void k_rs_write(ResampleArgs a) {
+    if (a.ctr[1] > a.pcap) return;
     // nothing merged or split: the caller keeps the sorted state as it is
     if (a.ctr[3] == a.n && a.ctr[2] == 0) return;
     const uint32_t p = blockIdx.x * PH_TILE + threadIdx.x;

@@ -187,7 +187,10 @@ __device__ __forceinline__ void dev_scatter(uint32_t bid, uint32_t nbl, uint32_t
                                                      uint32_t *__restrict__ tmp) {
     const uint32_t i = bid * PH_TILE + threadIdx.x;
     if (i >= n) return;
-    tmp[start[keys[i]] + slots[i]] = i;
+    // in range whenever the counts started at zero; the test keeps stale
+    // counts (a sort cut short) from writing outside the table
+    const uint32_t d = start[keys[i]] + slots[i];
+    if (d < n) tmp[d] = i;
 }
 
 __global__ __launch_bounds__(PH_TILE) void k_scatter(uint32_t n, const uint32_t *__restrict__ keys,
@@ -211,8 +214,9 @@ __device__ __forceinline__ void dev_rank_gather(uint32_t bid, uint32_t nbl, cons
     const uint32_t q = t.first + threadIdx.x;
     if (q >= s.part_end) return;
     const uint32_t src = tmp[q];
+    if (src < s.part_begin || src >= s.part_end) return;  // only after a sort cut short (see dev_scatter)
     const uint32_t key = keys[src];
-    const uint32_t b = start[key], e = start[key + 1];
+    const uint32_t b = min(start[key], s.part_end), e = min(start[key + 1], s.part_end);
     const uint32_t mytag = tag_in[src];
     const uint32_t myid = mytag & PH_ID_MASK;
     uint32_t rank = 0;

@@ -117,6 +117,7 @@ SIGNATURES = {
     "phieng_batch3_new": (_P, [ctypes.POINTER(_P), _U32]),
     "phieng_batch3_free": (None, [_P]),
     "phieng_batch3_set_mode": (_I32, [_P, _I32, _U32]),
+    "phieng_batch3_set_spin_limit": (_I32, [_P, _U32]),
     "phieng_batch3_step": (_I32, [_P, _F]),
     "phieng_batch3_sync": (_I32, [_P]),
     "phieng_batch3_sync_worlds": (_I32, [_P]),

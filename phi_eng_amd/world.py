@@ -400,6 +400,9 @@ class Batch3:
     def set_mode(self, mode, blocks_per_cu=0):
         self._ck(self._l.phieng_batch3_set_mode(self.ptr, int(mode), int(blocks_per_cu)))
 
+    def set_spin_limit(self, spins):
+        self._ck(self._l.phieng_batch3_set_spin_limit(self.ptr, int(spins)))
+
     def step(self, dt):
         self._ck(self._l.phieng_batch3_step(self.ptr, float(dt)))
 

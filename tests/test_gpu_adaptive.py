@@ -246,3 +246,24 @@ def test_adaptive_with_ghosts_and_cfl_substeps():
     assert (ref["mass"] != F(0.03)).any()
     _compare(w, ref, "adaptive + ghosts + cfl")
     w.close()
+
+
+def test_candidate_store_regrow_keeps_parity(monkeypatch):
+    """The merge-candidate store starts at 4 entries (test hook), so the
+    first merge pass overflows it: that pass must write nothing, the host
+    grows the store and runs the pass again, and the substep still equals
+    the oracle bit for bit (the particle arrays are untouched by the
+    overflowed pass)."""
+    monkeypatch.setenv("PHIENG_RESAMPLE_STORE_INITIAL", "4")
+    cfg = _cfg()
+    pos = _mixed_scene(3)
+    rng = np.random.default_rng(4)
+    vel = rng.normal(0, 0.5, pos.shape).astype(F)
+    w = _world3(cfg, pos, vel)
+    mass = O.masses(cfg, len(pos))
+    ref = O.substep_cap_f32(pos, vel, mass, cfg, (0.0, -9.81, 0.0), O.bodies_from_boxes(TANK3), 1.0 / 60.0, 4096)
+    assert (ref["mass"] > F(0.03)).sum() > 4  # more merges than the first store holds
+    w.step(1.0 / 60.0)
+    w.sync()
+    _compare(w, ref, "regrown store")
+    w.close()

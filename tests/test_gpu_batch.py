@@ -107,3 +107,30 @@ def test_graph_replays_match_eager_every_step(gpu):
         assert _same(a, b), f"replay {step} differs"
     eager.close()
     graph.close()
+
+
+def test_barrier_timeout_is_reported_and_recovered(gpu):
+    """The persistent launch's grid barrier waits a bounded number of spins;
+    with the limit at zero a block that has to wait gives up at once, every
+    block leaves at its next barrier and the launch ends normally.  The next
+    sync reports it, steps are refused until set_mode clears the barrier,
+    and the batch then steps again."""
+    scs = _scenes(4, seed=47)
+    ws = [scenes.make_world(sc) for sc in scs]
+    b = Batch3(ws)
+    b.set_mode(BATCH_PERSISTENT, 0)
+    b.set_spin_limit(0)
+    for _ in range(3):
+        b.step(DT)
+    with pytest.raises(RuntimeError, match="timed out"):
+        b.sync()
+    with pytest.raises(RuntimeError, match="timed out earlier"):
+        b.step(DT)
+    b.set_spin_limit(1 << 22)
+    b.set_mode(BATCH_PERSISTENT, 0)
+    for _ in range(3):
+        b.step(DT)
+    b.sync()
+    b.close()
+    for w in ws:
+        w.close()

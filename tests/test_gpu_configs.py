@@ -1,0 +1,163 @@
+"""The three larger configurations on the GPU, checked against the oracle
+and by size-independent properties.
+
+- 256 batched scenes, particle counts log-uniform in [2,048, 65,536]
+  (seed 3): one step of the whole batch in every launch form, and after
+  the autotuner, bit for bit against the oracle's Jacobi step of each of
+  the 16 smallest and 16 largest scenes run alone (fluid3.rs:990-1184).
+- The learned pressure term in the loop of the 1,048,576-particle dam
+  break for 200 steps (plain judge every 10 substeps): surrogate substeps
+  run, and the mean density error against the plain solver from the same
+  start stays below 1 %.
+- The 8,388,608-particle two-column scene: one full-size step against the
+  oracle (cell keys, sorted order and neighbour counts exact; density,
+  pressure and acceleration within 1e-5), then 300 steps keep the particle
+  count, finiteness, containment and the speed cap.
+"""
+import numpy as np
+import pytest
+
+from phi_eng_amd import scenes
+from phi_eng_amd.world import (BATCH_KERNELS, BATCH_KERNELS_GRAPH, BATCH_PERSISTENT, BATCH_PERSISTENT_GRAPH,
+                               RECORD_DENSITY, Batch3, Surrogate3)
+
+pytestmark = pytest.mark.gpu
+
+DT = 1e-4
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def c3():
+    """The 256 scenes, and the oracle's one-step result for the 16 smallest
+    and 16 largest of them."""
+    from oracle import oracle as O
+    scs = scenes.batch_scenes(count=256, seed=3)
+    sizes = np.array([sc.n for sc in scs])
+    assert sizes.min() >= 2048 and sizes.max() <= 65536 * 1.05
+    order = np.argsort(sizes, kind="stable")
+    pick = list(order[:16]) + list(order[-16:])
+    ref = {}
+    for k in pick:
+        sc = scs[k]
+        r = O.substep_f32(sc.positions, np.zeros_like(sc.positions), sc.config, sc.gravity,
+                          O.bodies_from_boxes(sc.boxes), DT, jacobi=True)
+        ref[k] = (r["pos"], r["vel"])
+    print(f"256 scenes, {sizes.sum()} particles; checked scenes {sizes[order[0]]}..{sizes[order[15]]} and "
+          f"{sizes[order[-16]]}..{sizes[order[-1]]} particles", flush=True)
+    return scs, ref
+
+
+def _batch_step_and_check(scs, ref, mode=None, tune_dir=None):
+    ws = [scenes.make_world(sc) for sc in scs]
+    b = Batch3(ws)
+    label = f"mode {mode}"
+    if tune_dir is not None:
+        m, bpc, ms = b.autotune(DT, trial_steps=3, cache_path=str(tune_dir / "tune.txt"))
+        label = f"autotuned: mode {m}, {bpc} blocks/CU, {ms:.3f} ms/step"
+    else:
+        b.set_mode(mode, 0)
+    b.step(DT)
+    b.sync()
+    b.sync_worlds()
+    bad = []
+    for k, (p_ref, v_ref) in ref.items():
+        p, v = ws[k].get_state()
+        if not (np.array_equal(_bits(p), _bits(p_ref)) and np.array_equal(_bits(v), _bits(v_ref))):
+            bad.append(k)
+    print(f"[{label}] {len(ref)} scenes checked, {len(bad)} differ", flush=True)
+    b.close()
+    for w in ws:
+        w.close()
+    assert not bad, f"{label}: scenes {bad} differ from the oracle"
+
+
+@pytest.mark.parametrize("mode", [BATCH_KERNELS, BATCH_KERNELS_GRAPH, BATCH_PERSISTENT, BATCH_PERSISTENT_GRAPH])
+def test_c3_batch_of_256_scenes_vs_oracle(gpu, c3, mode):
+    scs, ref = c3
+    _batch_step_and_check(scs, ref, mode=mode)
+
+
+def test_c3_autotuned_batch_vs_oracle(gpu, c3, tmp_path):
+    scs, ref = c3
+    _batch_step_and_check(scs, ref, tune_dir=tmp_path)
+
+
+def _train_small_network():
+    """Train on samples of a plain GPU trajectory of a half-scale dam."""
+    tr = scenes.dam_break(32, 64, 64, box=(6.4, 3.2, 3.2), seed=4, name="train")
+    tr.config.stiffness = 1.8e6
+    wt = scenes.make_world(tr)
+    net = Surrogate3(seed=4)
+    wt.collect_samples(net, every=10, stride=16)
+    for _ in range(600):
+        wt.step(tr.dt)
+    wt.sync()
+    wt.close()
+    first, last, base = net.train(epochs=4, batch=4096, lr=2e-3, seed=1)
+    print(f"trained on {net.num_samples()} samples: loss {first:.4f} -> {last:.4f} (network-free {base:.4f})",
+          flush=True)
+    return net
+
+
+def _density_after(sc, steps, net=None):
+    w = scenes.make_world(sc)
+    if net is not None:
+        w.set_surrogate(net, check_every=10, max_density_error=0.01)
+    for _ in range(steps):
+        w.step(sc.dt)
+    w.sync()
+    st = w.surrogate_stats() if net is not None else None
+    w.set_surrogate(None)
+    w.set_record(RECORD_DENSITY)
+    w.step(sc.dt)
+    rho = w.get_record()["density"].astype(np.float64)
+    pos, vel = w.get_state()
+    w.close()
+    return rho, st, pos, vel
+
+
+def test_c4_surrogate_in_the_loop_at_1m(gpu):
+    net = _train_small_network()
+    sc = scenes.c2_dam_break_1m()
+    steps = 200
+    ref, _, _, _ = _density_after(sc, steps)
+    rho, st, pos, vel = _density_after(sc, steps, net)
+    net.close()
+    e = np.abs(rho - ref) / np.maximum(ref, 1e-6)
+    print(f"1M surrogate, {steps} steps, judge every 10: {st}; density error mean {e.mean():.3e} "
+          f"p99 {np.percentile(e, 99):.3e}", flush=True)
+    assert st["surrogate_steps"] > 0.8 * steps
+    assert e.mean() < 0.01
+    assert np.isfinite(pos).all() and np.isfinite(vel).all()
+
+
+def test_c5_two_columns_8m_step_vs_oracle(gpu, oracle_lib):
+    from tests.test_gpu_parity import _compare_step
+    sc = scenes.c5_two_columns_8m()
+    w = scenes.make_world(sc)
+    for _ in range(5):
+        w.step(sc.dt)
+    _compare_step(w, sc, oracle_lib, "8.4M after 5 steps")
+    w.close()
+
+
+def test_c5_two_columns_8m_300_steps(gpu):
+    sc = scenes.c5_two_columns_8m()
+    w = scenes.make_world(sc)
+    for _ in range(300):
+        w.step(sc.dt)
+    w.sync()
+    pos, vel = w.get_state()
+    assert len(pos) == sc.n == w.count()
+    assert np.isfinite(pos).all() and np.isfinite(vel).all()
+    pr = sc.config.particle_radius
+    for ax in range(3):
+        assert (pos[:, ax] >= pr - 1e-4).all() and (pos[:, ax] <= 12.8 - pr + 1e-4).all()
+    assert np.linalg.norm(vel, axis=1).max() <= sc.config.max_velocity * (1 + 1e-6)
+    # the columns collapse toward the middle: mean height falls, x spreads
+    assert pos[:, 1].mean() < sc.positions[:, 1].mean()
+    w.close()

@@ -3,10 +3,12 @@ fluid block :582-636, save :3344-3380, load :4545-4615): the reference's
 3D fluid round-trip test on the GPU path, and larger states (several
 phases, masses changed by adaptive resolution, more than MAX_PARTICLES_3D
 particles)."""
+import json
+
 import numpy as np
 import pytest
 
-from phi_eng_amd import ffi
+from phi_eng_amd import ffi, scenes
 from phi_eng_amd.ffi import FluidConfig3Ffi
 from phi_eng_amd.world import World3
 
@@ -76,6 +78,7 @@ def test_phases_adaptive_masses_and_large_counts(tmp_path):
     for binary in (False, True):
         path = str(tmp_path / ("a.bin" if binary else "a.json"))
         assert w.save(path, binary=binary) == 1, ffi.last_error()
+        _decoded_equals_state(w, path, binary)
         r = World3.load(path)
         assert r is not None, ffi.last_error()
         assert bytes(r.get_fluid_config()) == bytes(w.get_fluid_config())
@@ -90,6 +93,59 @@ def test_phases_adaptive_masses_and_large_counts(tmp_path):
     assert _same(w, r)
     w.close()
     r.close()
+
+
+def _decoded_equals_state(w, path, binary, dim=3):
+    """The file's particle block, read by the independent decoders of
+    tests/test_snapshot.py (struct / numpy for PHE1, json for JSON), equals
+    the world's state in index order bit for bit: position, velocity, mass
+    and phase (FluidParticle3Dto, snapshot.rs:582-592; encode_fluid3
+    :6662-6677)."""
+    from tests import test_snapshot as ts
+    if binary:
+        d = ts.decode(path) if dim == 3 else ts.decode2(path)
+        got = d["particles"]  # no mass / phase below schema 5 (SNAPSHOT_SCHEMA_AMBIENT)
+        count = d["particle_count"]
+    else:
+        with open(path) as f:
+            got = ts.json_particles(json.load(f), dim)
+        count = len(got["pos"])
+    p, v = w.get_state()
+    m, ph = w.get_mass()
+    assert count == len(p)
+    pairs = [("pos", got["pos"], p), ("vel", got["vel"], v)]
+    if "mass" in got:
+        pairs.append(("mass", got["mass"], m))
+        assert np.array_equal(got["phase"], ph)
+    for name, a, b in pairs:
+        assert a.shape == b.shape, name
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32)), name
+    return count
+
+
+@pytest.mark.parametrize("binary", [True, False])
+def test_particle_block_of_a_1m_world(tmp_path, binary):
+    """A saved 1,048,576-particle dam break: the decoded particle block is
+    the GPU state bit for bit.  The count is above MAX_PARTICLES_3D (4,096),
+    which the reference's decode_fluid3 refuses (snapshot.rs:6679-6684): such
+    a file round-trips on this path only."""
+    from tests import test_snapshot as ts
+    sc = scenes.c2_dam_break_1m()
+    w = scenes.make_world(sc)
+    for _ in range(3):
+        w.step(sc.dt)
+    w.sync()
+    path = str(tmp_path / ("m.bin" if binary else "m.json"))
+    assert w.save(path, binary=binary) == 1, ffi.last_error()
+    count = _decoded_equals_state(w, path, binary)
+    assert count > ts.MAX_PARTICLES_3D
+    print(f"{count} particles decoded ({'PHE1' if binary else 'JSON'}); beyond the reference's "
+          f"{ts.MAX_PARTICLES_3D}-particle limit", flush=True)
+    r = World3.load(path)
+    assert r is not None, ffi.last_error()
+    assert _same(w, r)
+    r.close()
+    w.close()
 
 
 @pytest.mark.parametrize("binary", [False, True])
@@ -107,6 +163,7 @@ def test_reference_fluid_roundtrip_2d(tmp_path, binary):
     w.sync()
     path = str(tmp_path / ("w.bin" if binary else "w.json"))
     assert w.save(path, binary=binary) == 1, ffi.last_error()
+    _decoded_equals_state(w, path, binary, dim=2)
     r = World2.load(path)
     assert r is not None, ffi.last_error()
     assert r.count() == 12 and _same(w, r)

@@ -9,6 +9,8 @@ import os
 import struct
 import zlib
 
+import numpy as np
+
 import pytest
 
 from phi_eng_amd import ffi
@@ -83,9 +85,42 @@ def decode(path):
         cfg["phases"] = (r.take("B"), r.take("4f"), r.take("4f"))
     cfg["tail"] = r.take("4fB")
     out["fluid_config"] = cfg
-    out["particles"] = r.take("I")
+    out["particle_count"], out["particles"] = fluid_particles(r, version, 3)
     assert r.p == len(r.b), "trailing bytes"
     return out
+
+
+MAX_PARTICLES_3D = 4096  # crates/dynamics/src/fluid3.rs:28; decode_fluid3 (snapshot.rs:6679-6684) refuses more
+
+
+def fluid_particles(r, version, dim):
+    """The particle block of encode_fluid3 / encode_fluid2 (snapshot.rs:
+    6662-6677): a u32 count, then per particle dim position and dim velocity
+    f32 values and, from schema 5 (SNAPSHOT_SCHEMA_AMBIENT), its mass (f32)
+    and phase (u8), packed.  Returns (count, dict of arrays)."""
+    n = r.take("I")
+    fields = [("pos", "<f4", (dim,)), ("vel", "<f4", (dim,))]
+    if version >= 5:
+        fields += [("mass", "<f4"), ("phase", "u1")]
+    dt = np.dtype(fields)  # packed: no padding between fields
+    a = np.frombuffer(r.b, dtype=dt, count=n, offset=r.p)
+    r.p += n * dt.itemsize
+    out = dict(pos=np.array(a["pos"]).reshape(-1, dim), vel=np.array(a["vel"]).reshape(-1, dim))
+    if version >= 5:
+        out["mass"], out["phase"] = np.array(a["mass"]), np.array(a["phase"])
+    return n, out
+
+
+def json_particles(d, dim):
+    """The same fields from the JSON form (FluidParticle3Dto / 2Dto,
+    snapshot.rs:582-592: position and velocity objects, mass, phase)."""
+    ps = d["fluid"]["particles"]
+    ax = "xyz"[:dim]
+    pos = np.array([[p["position"][c] for c in ax] for p in ps], np.float64).astype(np.float32)
+    vel = np.array([[p["velocity"][c] for c in ax] for p in ps], np.float64).astype(np.float32)
+    mass = np.array([p.get("mass", 0.0) for p in ps], np.float64).astype(np.float32)
+    phase = np.array([p.get("phase", 0) for p in ps], np.uint8)
+    return dict(pos=pos.reshape(-1, dim), vel=vel.reshape(-1, dim), mass=mass, phase=phase)
 
 
 def test_default_world_json_layout(tmp):
@@ -124,7 +159,7 @@ def test_binary_layout_and_schema_5(tmp):
     assert a["schema"] == 3 and a["next_body"] == 3 and a["joints"] == 0 and a["contacts"] == 0
     assert [b["shape"][0] for b in a["bodies"]] == [1, 0, 2]
     assert a["bodies"][0]["flags"] == 1 and a["bodies"][0]["rot"] == (0.0, 0.0, 0.0, 1.0)
-    assert a["particles"] == 0
+    assert a["particle_count"] == 0
     cfg = FluidConfig3Ffi.default()
     cfg.gamma = 7.0
     cfg.phase_count = 2
@@ -318,7 +353,7 @@ def decode2(path):
         cfg["phases"] = (r.take("B"), r.take("4f"), r.take("4f"))
     cfg["tail"] = r.take("4fB")
     out["fluid_config"] = cfg
-    out["particles"] = r.take("I")
+    out["particle_count"], out["particles"] = fluid_particles(r, version, 2)
     assert r.p == len(r.b), "trailing bytes"
     return out
 
