@@ -37,7 +37,7 @@ def c3():
     from oracle import oracle as O
     scs = scenes.batch_scenes(count=256, seed=3)
     sizes = np.array([sc.n for sc in scs])
-    assert sizes.min() >= 2048 and sizes.max() <= 65536 * 1.05
+    assert sizes.min() >= 0.95 * 2048 and sizes.max() <= 1.05 * 65536  # lattices round the drawn counts
     order = np.argsort(sizes, kind="stable")
     pick = list(order[:16]) + list(order[-16:])
     ref = {}
@@ -155,8 +155,9 @@ def test_c5_two_columns_8m_300_steps(gpu):
     assert len(pos) == sc.n == w.count()
     assert np.isfinite(pos).all() and np.isfinite(vel).all()
     pr = sc.config.particle_radius
-    for ax in range(3):
+    for ax in (0, 2):  # walls at 0 and 12.8
         assert (pos[:, ax] >= pr - 1e-4).all() and (pos[:, ax] <= 12.8 - pr + 1e-4).all()
+    assert (pos[:, 1] >= pr - 1e-4).all()  # floor; the tank is open above
     assert np.linalg.norm(vel, axis=1).max() <= sc.config.max_velocity * (1 + 1e-6)
     # the columns collapse toward the middle: mean height falls, x spreads
     assert pos[:, 1].mean() < sc.positions[:, 1].mean()
