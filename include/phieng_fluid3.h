@@ -247,6 +247,14 @@ int32_t phieng_surrogate3_get_params(Surrogate3 *s, float *out, uint32_t n);
 int32_t phieng_surrogate3_set_params(Surrogate3 *s, const float *in, uint32_t n);
 /* y = MLP(x) for `rows` rows (x: rows x 16, y: rows x 16). */
 int32_t phieng_surrogate3_mlp_forward(Surrogate3 *s, const float *x, uint32_t rows, float *y);
+/* phieng_surrogate3_mlp_forward without any allocation: no counterpart in
+ * the reference.  For callers that cannot allocate device memory (tight
+ * serving loops, code that must not call hipMalloc).  Inputs and outputs are
+ * staged in a static per-GPU scratch array of 8 MiB, so rows must be at most
+ * 65,536 (larger calls return 0 with phieng_surrogate3_last_error).  Calls
+ * that overlap on one GPU share that array and race.  Same results as
+ * phieng_surrogate3_mlp_forward, bit for bit. */
+int32_t phieng_surrogate3_mlp_forward_noalloc(Surrogate3 *s, const float *x, uint32_t rows, float *y);
 /* grad = d(sum dy . MLP(x)) / d params. */
 int32_t phieng_surrogate3_mlp_backward(Surrogate3 *s, const float *x, const float *dy, uint32_t rows,
                                        float *grad);

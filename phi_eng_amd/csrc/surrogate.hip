@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -14,6 +15,35 @@
 #include "engine.h"
 #include "mlp.h"
 #include "surrogate.h"
+
+// ---------------------------------------------------------------- scratch
+// Staging of phieng_surrogate3_mlp_forward_noalloc: the input rows, then the
+// output rows, of at most SUR_NOALLOC_MAX_ROWS rows (8 MiB).  Each GPU the
+// library runs on holds its own copy until the process ends.  Host code
+// never passes this name to a kernel or a copy: it uses the device address
+// that scratch_on_device() looked up for the current GPU.
+#define SUR_NOALLOC_MAX_ROWS 65536u
+static __device__ __align__(256) float scratch[(size_t)SUR_NOALLOC_MAX_ROWS * (MLP_IN + MLP_OUT)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be the current
+// device; looked up once per device and kept.
+float *scratch_on_device(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        PH_CK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("hipGetSymbolAddress returned a null scratch address");
+        g_scratch_dev[device] = static_cast<float *>(p);
+    }
+    return g_scratch_dev[device];
+}
+}  // namespace
 
 // ---------------------------------------------------------------- kernels
 // Plain MLP evaluation of `rows` input rows (fp32 in, rounded to bf16).
@@ -137,6 +167,7 @@ void Surrogate3::init(uint32_t seed) {
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) throw std::runtime_error("no HIP device available");
     PH_CK(hipGetDevice(&device));
     PH_CK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    (void)scratch_on_device(device);  // set-up of the allocation-free forward on this device
     params.ensure(MLP_NPARAM);
     grad.ensure(MLP_NPARAM);
     mom.ensure(MLP_NPARAM);
@@ -302,6 +333,36 @@ int32_t phieng_surrogate3_mlp_forward(Surrogate3 *s, const float *x, uint32_t ro
         return 1;
     } catch (const std::exception &e) {
         set_err("phieng_surrogate3_mlp_forward", e.what());
+        return 0;
+    }
+}
+
+int32_t phieng_surrogate3_mlp_forward_noalloc(Surrogate3 *s, const float *x, uint32_t rows, float *y) {
+    if (!s || !x || !y || rows == 0) {
+        set_err("phieng_surrogate3_mlp_forward_noalloc", "null argument or zero rows");
+        return 0;
+    }
+    if (rows > SUR_NOALLOC_MAX_ROWS) {
+        set_err("phieng_surrogate3_mlp_forward_noalloc",
+                ("rows = " + std::to_string(rows) + " exceeds the scratch maximum of " +
+                 std::to_string(SUR_NOALLOC_MAX_ROWS) + " rows").c_str());
+        return 0;
+    }
+    try {
+        // No allocation: inputs and outputs are staged in this GPU's static
+        // scratch (looked up at Surrogate3 set-up).  Calls that overlap on
+        // one GPU share that array and race.
+        float *sx = scratch_on_device(s->device);
+        float *sy = sx + (size_t)rows * MLP_IN;
+        PH_CK(hipMemcpyAsync(sx, x, sizeof(float) * rows * MLP_IN, hipMemcpyHostToDevice, s->stream));
+        hipLaunchKernelGGL(k_mlp_forward, dim3((rows + MLP_ROWS - 1) / MLP_ROWS), dim3(MLP_ROWS), 0, s->stream, sx,
+                           rows, s->dev(), sy);
+        PH_CK(hipGetLastError());
+        PH_CK(hipMemcpyAsync(y, sy, sizeof(float) * rows * MLP_OUT, hipMemcpyDeviceToHost, s->stream));
+        PH_CK(hipStreamSynchronize(s->stream));
+        return 1;
+    } catch (const std::exception &e) {
+        set_err("phieng_surrogate3_mlp_forward_noalloc", e.what());
         return 0;
     }
 }

@@ -132,6 +132,7 @@ SIGNATURES = {
     "phieng_surrogate3_get_params": (_I32, [_P, _fp, _U32]),
     "phieng_surrogate3_set_params": (_I32, [_P, _fp, _U32]),
     "phieng_surrogate3_mlp_forward": (_I32, [_P, _fp, _U32, _fp]),
+    "phieng_surrogate3_mlp_forward_noalloc": (_I32, [_P, _fp, _U32, _fp]),
     "phieng_surrogate3_mlp_backward": (_I32, [_P, _fp, _fp, _U32, _fp]),
     "phieng_surrogate3_add_samples": (_I32, [_P, _fp, _U32]),
     "phieng_surrogate3_num_samples": (_U32, [_P]),
