@@ -327,6 +327,7 @@ def main():
     t3 = time.perf_counter()
     timing = w.get_timing()
     w.set_timing(False)
+    nstats = w.neighbour_stats()
     kernels = {k: {"avg_us": 1e3 * ms / cnt, "launches": int(cnt)} for k, (ms, cnt) in timing.items()}
     dom = max((k for k in kernels if k in ALG_BYTES and ALG_BYTES[k]), key=lambda k: kernels[k]["avg_us"])
     avg_s = kernels[dom]["avg_us"] * 1e-6
@@ -389,6 +390,7 @@ def main():
         "step_roofline_frac": round(step_frac, 5),
         "kernels": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
         "timed_with_events_ms_per_step": round((t3 - t2) * 1e3 / a.steps, 4),
+        "neighbour_list": dict(nstats, kind="verlet", skin=0.1),
         "cpu_baseline": cpu,
         "cpu_dam_break_2d": cpu2d,
         "gpu_dam_break_2d": gpu2d,

@@ -109,6 +109,10 @@ int32_t phieng_world_sync(World2 *world);
 /* Boundary ghosts (fluid.rs:661-819), as phieng_world3_boundary_info / _get
  * with xy pairs. */
 int32_t phieng_world_boundary_info(const World2 *world, uint32_t *count, uint64_t *regens);
+/* Extensions: the 2D forms of phieng_world3_set_neighbour_list and
+ * phieng_world3_neighbour_stats. */
+int32_t phieng_world_set_neighbour_list(World2 *world, int32_t verlet, float skin);
+int32_t phieng_world_neighbour_stats(World2 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired);
 uint32_t phieng_world_boundary_get(const World2 *world, float *xy, float *density, float *pressure, uint32_t max);
 
 #ifdef __cplusplus

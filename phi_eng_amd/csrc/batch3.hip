@@ -114,6 +114,11 @@ struct Batch3 {
     void substep() {
         ph::Engine &e = eng;
         if (e.n_total == 0) return;
+        // Verlet lists in the eager multi-kernel form only; the captured and
+        // persistent forms reorder particles without keeping the rebuild
+        // slots, so the lists are invalid after them.
+        e.vl_enabled = mode == PHIENG_BATCH_KERNELS;
+        if (!e.vl_enabled) e.vl_valid = false;
         if (mode == PHIENG_BATCH_KERNELS || (mode == PHIENG_BATCH_KERNELS_GRAPH && (e.timing || e.rec_flags))) {
             e.substep();
             return;
@@ -329,6 +334,7 @@ int32_t phieng_batch3_set_mode(Batch3 *b, int32_t mode, uint32_t blocks_per_cu) 
         b->blocks_per_cu = blocks_per_cu;
         b->set_grid();
         b->drop_graphs();
+        b->eng.vl_valid = false;
         // A timed-out grid barrier leaves its arrival count and abort flag
         // set; a new mode starts from clear barrier words (a fill kernel on
         // the stream, outside any captured graph).
@@ -510,6 +516,7 @@ int32_t phieng_batch3_autotune(Batch3 *b, float dt, uint32_t trial_steps, const 
             PH_CK(hipMemcpyAsync(e.vel[e.cs].p, sv.p, sizeof(float4) * n, hipMemcpyDeviceToDevice, e.stream));
             PH_CK(hipMemcpyAsync(e.tag[ct0].p, st.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, e.stream));
             e.ct = ct0;
+            e.vl_valid = false;  // the Verlet rows described the trial's state
         }
         PH_CK(hipStreamSynchronize(e.stream));
         (void)hipEventDestroy(t0);

@@ -37,6 +37,7 @@ enum Kind : int {
     KIND_SUR_COLLECT,
     KIND_GHOST,
     KIND_RESAMPLE,
+    KIND_VDECIDE,
     KIND_COUNT
 };
 const char *kind_name(int k);
@@ -193,8 +194,28 @@ struct Engine {
     void upload_tables();
     void ensure_record();
     // One substep of every scene (dt per scene in SceneDev::dt):
-    // sort_phase + plain_phase + finish_substep.
+    // sort_phase + plain_phase + finish_substep, or substep_verlet.
     void substep();
+    // The plain substep over Verlet lists (verlet.hip): the decision, the
+    // sort (its kernels return at once unless the device decided to
+    // rebuild), the density pass walking (or rebuilding) the rows and
+    // putting rows out of order back in order, the force pass over the
+    // neighbour list.
+    void substep_verlet();
+    // Verlet lists: used by substep() when enabled (not for a world with
+    // boundary ghosts, whose density pass reads this substep's cells).
+    // vl_valid says that the rows and the displacement bound describe the
+    // current state; every other way of moving or reordering particles
+    // clears it, and the next Verlet substep then rebuilds.
+    bool vl_enabled = true;
+    float vl_skin = 0.1f;
+    bool vl_valid = false;
+    uint64_t vl_epoch = 0;
+    uint32_t vl_n = 0;
+    DBuf<uint32_t> vl_cnt, vl_idx, vl_word[2], vl_tile_q;
+    DBuf<float> vl_tile_dmax;
+    DBuf<VerletDev> vl_state;
+    bool vl_state_ready = false;
     void sort_phase(uint32_t *perm_out);
     // Density pass (writing the neighbour list), boundary ghosts, force pass
     // (with af, also the surrogate's history, records and judge sums).

@@ -40,7 +40,7 @@ const char *kind_name(int k) {
         "cell_key", "scan_reduce", "scan_partials", "scan_apply", "scatter", "rank_gather",
         "density",  "force_integrate", "vmax", "export", "monitor", "append", "cell_bbox",
         "fill", "set_velocity", "batch_persistent", "surrogate_step", "surrogate_history",
-        "surrogate_collect", "boundary_ghosts", "resample"};
+        "surrogate_collect", "boundary_ghosts", "resample", "verlet_decide"};
     return (k >= 0 && k < KIND_COUNT) ? names[k] : "";
 }
 
@@ -453,13 +453,102 @@ void Engine::upload_tables() {
 
 void Engine::substep() {
     if (n_total == 0 || tiles.empty()) return;
+    if (vl_enabled && !(scenes.size() == 1 && scenes[0].ghost_count)) {
+        substep_verlet();
+        return;
+    }
     sort_phase(nullptr);
     plain_phase(nullptr);
     finish_substep();
 }
 
+void Engine::substep_verlet() {
+    const uint32_t nt = (uint32_t)tiles.size();
+    const uint32_t n = n_total;
+    const size_t m = alloc_particles;
+    const bool rc = rec_flags & PHIENG_RECORD_CELLS, rd = rec_flags & PHIENG_RECORD_DENSITY,
+               ra = rec_flags & PHIENG_RECORD_ACCEL;
+    // buffers first, so no allocation failure can cut the substep short
+    vl_cnt.ensure(m);
+    vl_idx.ensure((size_t)VL_CAP * m);
+    vl_word[0].ensure(m);
+    vl_word[1].ensure(m);
+    vl_state.ensure(1);
+    const size_t ntl = std::max<size_t>(nt, 1);
+    if (vl_tile_q.n < ntl || !vl_state_ready) {
+        vl_tile_q.ensure(ntl);
+        vl_tile_dmax.ensure(ntl);
+        run(KIND_FILL, [&] { return launch_fill_u32(stream, vl_tile_q.p, (uint32_t)vl_tile_q.n, 0u); });
+    }
+    if (!vl_state_ready) {
+        run(KIND_FILL, [&] {
+            return launch_fill_u32(stream, reinterpret_cast<uint32_t *>(vl_state.p),
+                                   (uint32_t)(sizeof(VerletDev) / sizeof(uint32_t)), 0u);
+        });
+        vl_state_ready = true;
+    }
+    // Rows built for another particle set or layout cannot be walked; the
+    // cell record is the sorted state's, so it sorts every substep.
+    const bool force = !vl_valid || vl_epoch != layout_epoch || vl_n != n || rc;
+    float hmin = INFINITY;
+    for (const SceneDev &s : scenes) hmin = std::min(hmin, s.h_grid);
+    const float skin = std::min(std::max(vl_skin, 0.0f), 0.5f);
+    const float thr = 0.45f * skin * hmin;
+    const int ns = 1 - cs, nt_ = 1 - ct;
+    const uint32_t *gate = reinterpret_cast<const uint32_t *>(vl_state.p);  // VerletDev::rebuild
+    vl_valid = false;  // until this substep is queued whole
+    run(KIND_VDECIDE, [&] {
+        return launch_vl_decide(stream, vl_state.p, vl_tile_dmax.p, vl_tile_q.p, nt, force ? 1u : 0u, thr);
+    });
+    if (counts_dirty)
+        run(KIND_FILL, [&] { return launch_fill_u32(stream, counts.p, (uint32_t)alloc_buckets, 0u); });
+    counts_dirty = true;
+    ++sorts_done;
+    run(KIND_CELL_KEY, [&] {
+        return launch_cell_key(stream, nt, d_tiles.p, d_scenes.p, posm[cs].p, tag[ct].p, keys.p, slots.p, counts.p,
+                               rc ? rec_cells.p : nullptr, vl_state.p, vl_word[0].p);
+    });
+    run(KIND_SCAN_REDUCE, [&] { return launch_scan_reduce(stream, counts.p, nbuckets, partial.p, gate); });
+    run(KIND_SCAN_PARTIALS, [&] { return launch_scan_partials(stream, partial.p, nbuckets, start.p, gate); });
+    run(KIND_SCAN_APPLY, [&] { return launch_scan_apply(stream, counts.p, nbuckets, partial.p, start.p, gate); });
+    counts_dirty = false;
+    run(KIND_SCATTER, [&] { return launch_scatter(stream, n, keys.p, slots.p, start.p, tmp.p, gate); });
+    run(KIND_RANK_GATHER, [&] {
+        return launch_rank_gather(stream, nt, d_tiles.p, d_scenes.p, tmp.p, keys.p, start.p, posm[cs].p, vel[cs].p,
+                                  tag[ct].p, posm[ns].p, vel[ns].p, tag[nt_].p, rc ? rec_sorted.p : nullptr, nullptr,
+                                  vl_state.p, vl_word[0].p, vl_word[1].p);
+    });
+    VerletArgs va;
+    va.vword = vl_word[1].p;
+    va.vcnt = vl_cnt.p;
+    va.vidx = vl_idx.p;
+    va.vstride = list_stride();
+    va.f2 = (1.0f + skin) * (1.0f + skin);
+    va.skin = skin;
+    va.vs = vl_state.p;
+    va.tile_q = vl_tile_q.p;
+    va.lidx = nl_idx.p;
+    va.lstride = list_stride();
+    run(KIND_DENSITY, [&] {
+        return launch_density_verlet(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p,
+                                     rd ? rec_rho.p : nullptr, rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr,
+                                     nl_cnt.p, va);
+    });
+    gen_open = true;
+    run(KIND_FORCE, [&] {
+        return launch_force_verlet(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p, vel[ns].p,
+                                   rq.p, tag[nt_].p, nl_cnt.p, nl_idx.p, list_stride(), posm[cs].p, vel[cs].p,
+                                   ra ? rec_acc.p : nullptr, d_bbox.p, GhostDev{}, vl_tile_dmax.p, vl_state.p);
+    });
+    finish_substep();
+    vl_valid = true;
+    vl_epoch = layout_epoch;
+    vl_n = n;
+}
+
 // K1 .. K3: keys, bucket scan, scatter, rank + gather into generation 1-cs.
 void Engine::sort_phase(uint32_t *perm_out) {
+    vl_valid = false;  // reorders particles without keeping the rebuild slots
     const uint32_t nt = (uint32_t)tiles.size();
     const uint32_t n = n_total;
     const bool rc = rec_flags & PHIENG_RECORD_CELLS;
@@ -634,6 +723,7 @@ void Engine::finish_substep() {
 
 void Engine::recover() {
     (void)hipGetLastError();
+    vl_valid = false;
     if (gen_open) {
         ct = 1 - ct;
         gen_open = false;

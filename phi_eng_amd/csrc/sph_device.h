@@ -350,6 +350,50 @@ __device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__r
 #define SLF_SLOW 0x01000000u  // sweep the cells (outliers near the domain, or > SL_CAP candidates)
 #define SLF_NONE 0x02000000u  // non-finite position: no neighbours
 
+// ---------------------------------------------------------------- Verlet list
+// A plain world keeps, per particle, every particle within (1 + skin) h of
+// it at the last rebuild (a Verlet list), so a substep tests ~45 listed
+// candidates instead of the ~216 of its 27 cells.  Between rebuilds the
+// particles keep their slots (the sort is skipped), so a row holds slots
+// directly.  The reference's neighbour condition and order are recovered
+// from one word per particle, its neighbour word: the original index (24
+// bits) and the cell coordinates mod 4.  Two particles within h lie at most
+// two cells apart per axis, so the difference mod 4 tells the offset
+// exactly: 2 means not among the 27 cells, otherwise -1, 0, 1; and the
+// reference visits neighbours by offset (oz, oy, ox), then by index
+// (fluid3.rs:301-314, 326-338).
+#define VL_CAP 96u                 // entries per row (4 to a 16-byte group)
+#define VLF_SLOW 0x01000000u       // row unusable (longer than VL_CAP, or a per-cell particle)
+#define VL_FAR 0x1p24f             // |x / h| from which cells are not compared mod 4
+#define VL_NOT27 0xFFFFFFFFu
+
+struct VerletDev {
+    uint32_t rebuild;         // this substep sorts and rebuilds every row
+    float dacc;               // bound on any particle's displacement since the last rebuild
+    uint32_t slow;            // a particle needs the per-cell path: rebuild next substep too
+    uint32_t pad;
+    unsigned long long rebuilds, repaired, substeps;  // statistics
+};
+
+__device__ __forceinline__ uint32_t vl_word(uint32_t id, int32_t cx, int32_t cy, int32_t cz) {
+    return (id & PH_ID_MASK) | (((uint32_t)cx & 3u) << 24) | (((uint32_t)cy & 3u) << 26) |
+           (((uint32_t)cz & 3u) << 28);
+}
+// Visiting key of j among i's neighbours: (cell offset rank, index), or
+// VL_NOT27 when j's cell is not one of i's 27.  Valid for pairs within h.
+__device__ __forceinline__ uint32_t vl_key(uint32_t wi, uint32_t wj) {
+    const uint32_t dx = ((wj >> 24) - (wi >> 24)) & 3u, dy = ((wj >> 26) - (wi >> 26)) & 3u,
+                   dz = ((wj >> 28) - (wi >> 28)) & 3u;
+    if (dx == 2u || dy == 2u || dz == 2u) return VL_NOT27;
+    // offsets -1, 0, 1 are stored as 3, 0, 1; rank them 0, 1, 2
+    const uint32_t rx = (dx + 1u) & 3u, ry = (dy + 1u) & 3u, rz = (dz + 1u) & 3u;
+    return (((rz * 3u + ry) * 3u + rx) << 24) | (wj & PH_ID_MASK);
+}
+__device__ __forceinline__ bool vl_far(float x, float y, float z, float h) {
+    const float f = VL_FAR * h;
+    return !(fabsf(x) < f && fabsf(y) < f && fabsf(z) < f);
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8),
 // each with its own L2.  Give every XCD one contiguous run of tiles, so
 // the neighbour rows a tile reads were mostly fetched by tiles on the same

@@ -29,20 +29,62 @@ struct PersistArgs {
 
 uint32_t scan_partials_needed(uint32_t nbuckets);
 
+// vs (may be null): a Verlet substep; the neighbour words go to vword and
+// the bucket keys are made only when vs->rebuild.
 hipError_t launch_cell_key(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                            const float4 *posm, const uint32_t *tag, uint32_t *keys, uint32_t *slots,
-                           uint32_t *counts, int32_t *rec_cells);
-hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t nb, uint32_t *partial);
-hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start);
+                           uint32_t *counts, int32_t *rec_cells, const VerletDev *vs = nullptr,
+                           uint32_t *vword = nullptr);
+// gate (may be null): the kernel runs only when *gate is nonzero.
+hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t nb, uint32_t *partial,
+                              const uint32_t *gate = nullptr);
+hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start,
+                                const uint32_t *gate = nullptr);
 hipError_t launch_scan_apply(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
-                             uint32_t *start);
+                             uint32_t *start, const uint32_t *gate = nullptr);
 hipError_t launch_scatter(hipStream_t st, uint32_t n, const uint32_t *keys, const uint32_t *slots,
-                          const uint32_t *start, uint32_t *tmp);
+                          const uint32_t *start, uint32_t *tmp, const uint32_t *gate = nullptr);
 hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                               const uint32_t *tmp, const uint32_t *keys, const uint32_t *start,
                               const float4 *posm_in, const float4 *vel_in, const uint32_t *tag_in,
                               float4 *posm_out, float4 *vel_out, uint32_t *tag_out, uint32_t *rec_sorted,
-                              uint32_t *perm_out = nullptr);
+                              uint32_t *perm_out = nullptr, const VerletDev *vs = nullptr,
+                              const uint32_t *vword_in = nullptr, uint32_t *vword_out = nullptr);
+// The force pass of a Verlet substep: the list-based pass over the
+// neighbour list the Verlet density pass wrote, also recording the
+// substep's largest displacement in vs.
+hipError_t launch_force_verlet(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                               const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
+                               const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
+                               uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
+                               const GhostDev &gh, float *tile_dmax, VerletDev *vs);
+// Verlet substep (verlet.hip).  k_vl_decide (one block) decides whether
+// this substep sorts and rebuilds every row: on request (force), after a
+// particle needed the per-cell path, or when the displacement bound (the
+// last substep's largest per-tile displacement added up) reaches thr.  The
+// density pass then either sweeps the cells within (1 + skin) h of every
+// particle, writing its row, or walks its row; a wave puts rows whose
+// neighbours within h are out of visiting order back in order itself.
+struct VerletArgs {
+    const uint32_t *vword;   // neighbour word of each slot (this substep's order)
+    uint32_t *vcnt;          // per row: entry count (| VLF_SLOW)
+    uint32_t *vidx;          // rows: group g of slot i at uint4 g * vstride + i
+    uint32_t vstride;
+    float f2;                // (1 + skin)^2
+    float skin;
+    VerletDev *vs;
+    uint32_t *tile_q;        // per tile: rows put in order (statistics; zeroed by k_vl_decide)
+    // the neighbours within h, for the force pass (ListOut layout: group g
+    // of slot i at uint4 g * lstride + i; the count word in fcnt)
+    uint32_t *lidx;
+    uint32_t lstride;
+};
+hipError_t launch_vl_decide(hipStream_t st, VerletDev *vs, const float *tile_dmax, uint32_t *tile_q, uint32_t ntiles,
+                            uint32_t force, float thr);
+hipError_t launch_density_verlet(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                                 const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
+                                 float *rec_rho, float *rec_p, uint32_t *rec_count, uint32_t *fcnt,
+                                 const VerletArgs &va);
 // Skin list written by the density pass for surrogate substeps.
 struct ListOut {
     int4 *cell = nullptr;

@@ -100,6 +100,9 @@ __device__ __forceinline__ void dev_cell_key(uint32_t bid, uint32_t nbl, const T
     }
 }
 
+// With a Verlet list (vs non-null): every particle's neighbour word
+// (original index, cell mod 4) into vword; the bucket keys and counts only
+// on a substep that sorts (vs->rebuild, decided by k_vl_decide).
 __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict__ tiles,
                                                       const SceneDev *__restrict__ scenes,
                                                       const float4 *__restrict__ posm,
@@ -107,7 +110,31 @@ __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict_
                                                       uint32_t *__restrict__ keys,
                                                       uint32_t *__restrict__ slots,
                                                       uint32_t *__restrict__ counts,
-                                                      int32_t *__restrict__ rec_cells) {
+                                                      int32_t *__restrict__ rec_cells, const VerletDev *vs,
+                                                      uint32_t *__restrict__ vword) {
+    if (vs) {
+        const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+        const SceneDev &s = scenes[t.scene];
+        const uint32_t i = t.first + threadIdx.x;
+        if (i < s.part_end) {
+            const float4 p = posm[i];
+            int32_t cx, cy, cz;
+            cell_of(p.x, p.y, p.z, s.h_grid, cx, cy, cz);
+            vword[i] = vl_word(tag[i], cx, cy, cz);
+        }
+        if (!vs->rebuild) {
+            if (rec_cells && i < s.part_end) {
+                const float4 p = posm[i];
+                int32_t cx, cy, cz;
+                cell_of(p.x, p.y, p.z, s.h_grid, cx, cy, cz);
+                const uint32_t id = tag[i] & PH_ID_MASK;
+                rec_cells[3 * id] = cx;
+                rec_cells[3 * id + 1] = cy;
+                rec_cells[3 * id + 2] = cz;
+            }
+            return;
+        }
+    }
     dev_cell_key(blockIdx.x, gridDim.x, tiles, scenes, posm, tag, keys, slots, counts, rec_cells);
 }
 
@@ -124,8 +151,12 @@ __device__ __forceinline__ void dev_scan_reduce(uint32_t bid, uint32_t nbl, cons
     if (threadIdx.x == 0) partial[bid] = tot;
 }
 
+// gate (may be null): run only when *gate is nonzero (a Verlet substep that
+// sorts); the launch sequence stays the same either way.
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__restrict__ counts,
-                                                            uint32_t nb, uint32_t *__restrict__ partial) {
+                                                            uint32_t nb, uint32_t *__restrict__ partial,
+                                                            const uint32_t *gate) {
+    if (gate && !*gate) return;
     dev_scan_reduce(blockIdx.x, gridDim.x, counts, nb, partial);
 }
 
@@ -145,7 +176,9 @@ __device__ __forceinline__ void dev_scan_partials(uint32_t bid, uint32_t nbl, ui
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t *__restrict__ partial, uint32_t np,
-                                                              uint32_t *__restrict__ start, uint32_t nb) {
+                                                              uint32_t *__restrict__ start, uint32_t nb,
+                                                              const uint32_t *gate) {
+    if (gate && !*gate) return;
     dev_scan_partials(blockIdx.x, gridDim.x, partial, np, start, nb);
 }
 
@@ -176,7 +209,8 @@ __device__ __forceinline__ void dev_scan_apply(uint32_t bid, uint32_t nbl, uint3
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t *__restrict__ counts, uint32_t nb,
                                                            const uint32_t *__restrict__ partial,
-                                                           uint32_t *__restrict__ start) {
+                                                           uint32_t *__restrict__ start, const uint32_t *gate) {
+    if (gate && !*gate) return;
     dev_scan_apply(blockIdx.x, gridDim.x, counts, nb, partial, start);
 }
 
@@ -196,7 +230,8 @@ __device__ __forceinline__ void dev_scatter(uint32_t bid, uint32_t nbl, uint32_t
 __global__ __launch_bounds__(PH_TILE) void k_scatter(uint32_t n, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ slots,
                                                      const uint32_t *__restrict__ start,
-                                                     uint32_t *__restrict__ tmp) {
+                                                     uint32_t *__restrict__ tmp, const uint32_t *gate) {
+    if (gate && !*gate) return;
     dev_scatter(blockIdx.x, gridDim.x, n, keys, slots, start, tmp);
 }
 
@@ -208,11 +243,22 @@ __device__ __forceinline__ void dev_rank_gather(uint32_t bid, uint32_t nbl, cons
     const uint32_t *__restrict__ keys, const uint32_t *__restrict__ start, const float4 *__restrict__ posm_in,
     const float4 *__restrict__ vel_in, const uint32_t *__restrict__ tag_in, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted,
-    uint32_t *__restrict__ perm_out) {
+    uint32_t *__restrict__ perm_out, const VerletDev *vs = nullptr, const uint32_t *__restrict__ vword_in = nullptr,
+    uint32_t *__restrict__ vword_out = nullptr) {
     const TileDev t = tiles[xcd_tile(bid, nbl)];
     const SceneDev s = scenes[t.scene];
     const uint32_t q = t.first + threadIdx.x;
     if (q >= s.part_end) return;
+    if (vs && !vs->rebuild) {
+        // a Verlet substep that keeps the order: the same slots, copied
+        posm_out[q] = posm_in[q];
+        vel_out[q] = vel_in[q];
+        tag_out[q] = tag_in[q];
+        vword_out[q] = vword_in[q];
+        if (rec_sorted) rec_sorted[q] = tag_in[q] & PH_ID_MASK;
+        if (perm_out) perm_out[q] = q;
+        return;
+    }
     const uint32_t src = tmp[q];
     if (src < s.part_begin || src >= s.part_end) return;  // only after a sort cut short (see dev_scatter)
     const uint32_t key = keys[src];
@@ -242,13 +288,16 @@ __device__ __forceinline__ void dev_rank_gather(uint32_t bid, uint32_t nbl, cons
     tag_out[d] = mytag;
     if (rec_sorted) rec_sorted[d] = myid;
     if (perm_out) perm_out[d] = src;
+    if (vword_out) vword_out[d] = vword_in[src];
 }
 
 __global__ __launch_bounds__(PH_TILE) void k_rank_gather(const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const uint32_t *__restrict__ tmp,
     const uint32_t *__restrict__ keys, const uint32_t *__restrict__ start, const float4 *__restrict__ posm_in,
     const float4 *__restrict__ vel_in, const uint32_t *__restrict__ tag_in, float4 *__restrict__ posm_out,
-    float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted, uint32_t *__restrict__ perm_out) {
-    dev_rank_gather(blockIdx.x, gridDim.x, tiles, scenes, tmp, keys, start, posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted, perm_out);
+    float4 *__restrict__ vel_out, uint32_t *__restrict__ tag_out, uint32_t *__restrict__ rec_sorted, uint32_t *__restrict__ perm_out,
+    const VerletDev *vs, const uint32_t *__restrict__ vword_in, uint32_t *__restrict__ vword_out) {
+    dev_rank_gather(blockIdx.x, gridDim.x, tiles, scenes, tmp, keys, start, posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted, perm_out,
+                    vs, vword_in, vword_out);
 }
 
 
@@ -467,12 +516,16 @@ __device__ __forceinline__ void pair_terms(const SceneDev &s, const float4 &pi, 
 // lgrp[g * mstride + i]), re-tested here against h.  The list takes one loop
 // over a particle's own entries instead of nine row loops whose trip counts
 // differ between lanes.
+// VL: a Verlet substep; the pass also records the substep's largest
+// displacement for the next rebuild decision.
+template <bool VL = false>
 __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes, const BodyDev *__restrict__ bodies,
     const uint32_t *__restrict__ start, const float4 *__restrict__ posm, const float4 *__restrict__ vel,
     const float2 *__restrict__ rq, const uint32_t *__restrict__ tag, const uint32_t *__restrict__ lcnt,
     const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox,
-    float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}, const AuxFuse &af = AuxFuse{}) {
+    float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}, const AuxFuse &af = AuxFuse{},
+    float *__restrict__ tile_dmax = nullptr, VerletDev *vs = nullptr) {
     const uint32_t tile = xcd_tile(bid, nbl);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
@@ -480,6 +533,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
     const bool active = i < s.part_end;
     int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bz0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN, bz1 = INT32_MIN;
     float jerr = 0.0f, jcnt = 0.0f;  // judge sums (AuxFuse)
+    float dmax2 = 0.0f;               // largest squared displacement (VL)
     if (active) {
         const float4 pi = posm[i];
         const float4 vi = vel[i];
@@ -546,6 +600,12 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             z = pi.z;
         }
         clamp_to_bodies(s, bodies, x, y, z, ux, uy, uz);
+        if (VL) {
+            const float d2 = dot3(x - pi.x, y - pi.y, z - pi.z);
+            dmax2 = d2 == d2 ? d2 : 0.0f;  // a non-finite particle is nobody's neighbour
+            // cells this far out saturate; the next substep sweeps cells
+            if (finite3(x, y, z) && vl_far(x, y, z, s.h_grid)) atomicOr(&vs->slow, 1u);
+        }
         posm_out[i] = make_float4(x, y, z, pi.w);
         vel_out[i] = make_float4(ux, uy, uz, 0.0f);
         if (acc_sorted) acc_sorted[i] = make_float4(ax - s.gx, ay - s.gy, az - s.gz, 0.0f);
@@ -588,6 +648,8 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         by1 = max(by1, __shfl_xor(by1, d, 64));
         bz1 = max(bz1, __shfl_xor(bz1, d, 64));
     }
+    if (VL)
+        for (int d = 32; d >= 1; d >>= 1) dmax2 = fmaxf(dmax2, __shfl_xor(dmax2, d, 64));
     const bool judge = af.tile_err != nullptr;
     if (judge)
         for (int d = 32; d >= 1; d >>= 1) {
@@ -606,8 +668,11 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         wb[w][5] = bz1;
         wj[w][0] = jerr;
         wj[w][1] = jcnt;
+        if (VL) wj[w][0] = dmax2;
     }
     __syncthreads();
+    if (VL && threadIdx.x == 0)
+        tile_dmax[tile] = fmaxf(fmaxf(wj[0][0], wj[1][0]), fmaxf(wj[2][0], wj[3][0]));
     if (threadIdx.x < 6) {
         const int c = threadIdx.x;
         int32_t v = wb[0][c];
@@ -665,6 +730,16 @@ __global__ __launch_bounds__(PH_TILE) void k_force_list(const TileDev *__restric
     GhostDev gh, AuxFuse af) {
     dev_force_integrate(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, lcnt, lgrp, mstride,
                         posm_out, vel_out, rec_acc, bbox, acc_sorted, gh, af);
+}
+
+__global__ __launch_bounds__(PH_TILE) void k_force_verlet(const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes,
+    const BodyDev *__restrict__ bodies, const uint32_t *__restrict__ start, const float4 *__restrict__ posm,
+    const float4 *__restrict__ vel, const float2 *__restrict__ rq, const uint32_t *__restrict__ tag,
+    const uint32_t *__restrict__ lcnt, const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
+    float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, GhostDev gh,
+    float *__restrict__ tile_dmax, VerletDev *vs) {
+    dev_force_integrate<true>(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, lcnt, lgrp,
+                              mstride, posm_out, vel_out, rec_acc, bbox, nullptr, gh, AuxFuse{}, tile_dmax, vs);
 }
 
 // Boundary ghost density and pressure from the fluid (fluid3.rs:906-920):
@@ -928,9 +1003,9 @@ static inline uint32_t blocks_for(uint32_t n) { return (n + PH_TILE - 1) / PH_TI
 
 hipError_t launch_cell_key(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                            const float4 *posm, const uint32_t *tag, uint32_t *keys, uint32_t *slots,
-                           uint32_t *counts, int32_t *rec_cells) {
+                           uint32_t *counts, int32_t *rec_cells, const VerletDev *vs, uint32_t *vword) {
     hipLaunchKernelGGL(k_cell_key, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, posm, tag, keys, slots,
-                       counts, rec_cells);
+                       counts, rec_cells, vs, vword);
     return hipGetLastError();
 }
 
@@ -938,34 +1013,48 @@ uint32_t scan_partials_needed(uint32_t nbuckets) {
     return (nbuckets + kScanBlock * kScanItems - 1) / (kScanBlock * kScanItems);
 }
 
-hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t nb, uint32_t *partial) {
+hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t nb, uint32_t *partial,
+                              const uint32_t *gate) {
     hipLaunchKernelGGL(k_scan_reduce, dim3(scan_partials_needed(nb)), dim3(kScanBlock), 0, st, counts, nb,
-                       partial);
+                       partial, gate);
     return hipGetLastError();
 }
-hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start) {
+hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, uint32_t *start,
+                                const uint32_t *gate) {
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlock), 0, st, partial, scan_partials_needed(nb),
-                       start, nb);
+                       start, nb, gate);
     return hipGetLastError();
 }
 hipError_t launch_scan_apply(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
-                             uint32_t *start) {
+                             uint32_t *start, const uint32_t *gate) {
     hipLaunchKernelGGL(k_scan_apply, dim3(scan_partials_needed(nb)), dim3(kScanBlock), 0, st, counts, nb,
-                       partial, start);
+                       partial, start, gate);
     return hipGetLastError();
 }
 hipError_t launch_scatter(hipStream_t st, uint32_t n, const uint32_t *keys, const uint32_t *slots,
-                          const uint32_t *start, uint32_t *tmp) {
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, n, keys, slots, start, tmp);
+                          const uint32_t *start, uint32_t *tmp, const uint32_t *gate) {
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, n, keys, slots, start, tmp, gate);
     return hipGetLastError();
 }
 hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                               const uint32_t *tmp, const uint32_t *keys, const uint32_t *start,
                               const float4 *posm_in, const float4 *vel_in, const uint32_t *tag_in,
                               float4 *posm_out, float4 *vel_out, uint32_t *tag_out, uint32_t *rec_sorted,
-                              uint32_t *perm_out) {
+                              uint32_t *perm_out, const VerletDev *vs, const uint32_t *vword_in,
+                              uint32_t *vword_out) {
     hipLaunchKernelGGL(k_rank_gather, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, tmp, keys, start,
-                       posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted, perm_out);
+                       posm_in, vel_in, tag_in, posm_out, vel_out, tag_out, rec_sorted, perm_out, vs, vword_in,
+                       vword_out);
+    return hipGetLastError();
+}
+hipError_t launch_force_verlet(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                               const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
+                               const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
+                               uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
+                               const GhostDev &gh, float *tile_dmax, VerletDev *vs) {
+    hipLaunchKernelGGL(k_force_verlet, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm, vel,
+                       rq, tag, lcnt, reinterpret_cast<const uint4 *>(lidx), mstride, posm_out, vel_out, rec_acc,
+                       bbox, gh, tile_dmax, vs);
     return hipGetLastError();
 }
 hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,

@@ -1,0 +1,434 @@
+// verlet.hip — neighbour search of a plain substep over Verlet lists.
+//
+// The reference finds a particle's neighbours by sweeping its 27 cells
+// every substep (fluid3.rs:319-340): ~216 candidates for ~33 neighbours.
+// Here each particle keeps a row of every particle that was within
+// (1 + skin) h of it at the last rebuild.  While no particle has moved more
+// than 0.45 skin h since then (the bound k_vl_decide keeps), every particle
+// within h of it is in its row, and the row is walked instead of the cells.
+//
+// Between rebuilds the particles keep their slots: the sort's kernels are
+// launched but return at once, and the gather copies the state unchanged,
+// so a row holds slots and the host's launch sequence never depends on the
+// device's decision.  Results stay those of the reference, bit for bit:
+// - a neighbour is a particle j with d2 < h * h (the reference's f32 test)
+//   whose cell is one of i's 27; with the cells mod 4 in the neighbour
+//   words, vl_key decides that exactly;
+// - the reference sums in cell-offset order (oz, oy, ox), then by original
+//   index; a row is walked in its stored order and the keys of the
+//   neighbours found must increase.  If not (a neighbour changed cell since
+//   the row was last ordered) the wave sorts the row by key, stores it
+//   back in that order and sums again;
+// - a rebuild sorts the state (cell, then index) and sweeps the cells
+//   within (1 + skin) h in ascending slot order, which is the reference
+//   order, summing the density over the 27 cells in the same sweep.  A
+//   particle the sweep cannot serve (its ball leaves the grid while
+//   outliers exist, or more than VL_CAP candidates) takes the per-cell path
+//   of the list-free pass, and the next substep sorts and rebuilds again.
+// Every form writes the particle's neighbours within h, in reference order,
+// to the list the force pass walks (as the list-free density pass does).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "sph_kernels.h"
+
+namespace {
+
+// fluid3.rs:646-655 after the sum: clamp, equation of state, the force
+// pass's pressure quotient, and the parity records (same operations as the
+// list-free density pass).
+__device__ __forceinline__ void density_out(const SceneDev &s, const SceneDev *__restrict__ scenes, uint32_t scene,
+                                            uint32_t i, float rho, uint32_t cnt, const uint32_t *__restrict__ tag,
+                                            float2 *__restrict__ rq, float *__restrict__ rec_rho,
+                                            float *__restrict__ rec_p, uint32_t *__restrict__ rec_count) {
+    rho = rclampf(rho, 0.0f, s.rho_max);
+    const uint32_t tg = tag[i];
+    const float rho0 = scenes[scene].phase_rho0[min(tg >> 24, 3u)];
+    const float p = eos(rho, rho0, s.stiffness, s.gamma);
+    const float rc = rmaxf(rho, PH_EPS);
+    rq[i] = make_float2(rho, p / (rc * rc));
+    if (rec_rho) {
+        const uint32_t id = tg & PH_ID_MASK;
+        rec_rho[id] = rho;
+        rec_p[id] = p;
+        rec_count[id] = cnt;
+    }
+}
+
+// The neighbour list handed to the force pass: the slots within h in
+// reference order, four to a 16-byte group (group g of slot i at
+// g * stride + i), at most SL_CAP.
+struct ListPush {
+    uint4 *g;
+    uint32_t stride;
+    uint32_t k = 0;
+    uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+    __device__ __forceinline__ ListPush(uint32_t *base, uint32_t stride_, uint32_t i)
+        : g(reinterpret_cast<uint4 *>(base) + i), stride(stride_) {}
+    __device__ __forceinline__ void push(uint32_t j) {
+        const uint32_t p = k & 3u;
+        q0 = p == 0u ? j : q0;
+        q1 = p == 1u ? j : q1;
+        q2 = p == 2u ? j : q2;
+        q3 = p == 3u ? j : q3;
+        if (p == 3u && k < SL_CAP) g[(size_t)(k >> 2) * stride] = make_uint4(q0, q1, q2, q3);
+        ++k;
+    }
+    // the last, partly filled group; the count word for the force pass
+    __device__ __forceinline__ uint32_t finish() {
+        if ((k & 3u) != 0u && k < SL_CAP) g[(size_t)(k >> 2) * stride] = make_uint4(q0, q1, q2, q3);
+        return k <= SL_CAP ? k : SLF_SLOW;
+    }
+};
+
+// The per-cell sum (reference order, outlier cells included).
+__device__ __forceinline__ float density_cells(const SceneDev &s, const uint32_t *__restrict__ start,
+                                               const float4 *__restrict__ posm, const float4 &pi, int32_t cx,
+                                               int32_t cy, int32_t cz, uint32_t &cnt) {
+    float rho = 0.0f;
+    for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
+        const float4 pj = posm[j];
+        const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+        if (r2 < s.h_grid2) {
+            ++cnt;
+            rho += pj.w * poly6(r2, s);
+        }
+    });
+    return rho;
+}
+
+// Cells per axis that the ball of radius (1 + skin) h around x can reach:
+// the neighbour cells, and the next ones when x lies within the skin (plus
+// a margin for the rounding of x / h) of a face.  Far from the origin the
+// fraction is not resolved and both are taken.
+__device__ __forceinline__ void reach(float x, float h, int32_t c, float skin, int64_t &lo, int64_t &hi) {
+    const float q = x / h;
+    const float fr = q - floorf(q);
+    const float m = skin + 0.01f + fabsf(q) * 0x1p-22f;
+    const bool wide = !(fabsf(q) < 0x1p22f);
+    lo = (int64_t)c - 1 - ((wide || fr < m) ? 1 : 0);
+    hi = (int64_t)c + 1 + ((wide || fr > 1.0f - m) ? 1 : 0);
+}
+
+}  // namespace
+
+// One block: this substep's decision, from the last substep's per-tile
+// largest squared displacements (per-tile stores, so no same-address
+// atomics contend across the XCDs); the per-tile counts of rows put in
+// order go to the statistics and restart at zero.
+__global__ __launch_bounds__(PH_TILE) void k_vl_decide(VerletDev *vs, const float *__restrict__ tile_dmax,
+                                                       uint32_t *__restrict__ tile_q, uint32_t nt, uint32_t force,
+                                                       float thr) {
+    float dm = 0.0f;
+    uint32_t qs = 0;
+    for (uint32_t k = threadIdx.x; k < nt; k += PH_TILE) {
+        if (!force) dm = fmaxf(dm, tile_dmax[k]);
+        qs += tile_q[k];
+        tile_q[k] = 0u;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        dm = fmaxf(dm, __shfl_xor(dm, d, 64));
+        qs += __shfl_xor(qs, d, 64);
+    }
+    __shared__ float wdm[PH_TILE / 64];
+    __shared__ uint32_t wqs[PH_TILE / 64];
+    if ((threadIdx.x & 63) == 0) {
+        wdm[threadIdx.x >> 6] = dm;
+        wqs[threadIdx.x >> 6] = qs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < PH_TILE / 64; ++w) {
+            dm = fmaxf(dm, wdm[w]);
+            qs += wqs[w];
+        }
+        const float d = force ? 0.0f : vs->dacc + sqrtf(dm);
+        const bool rb = force || vs->slow || !(d <= thr);  // a NaN bound rebuilds
+        vs->rebuild = rb ? 1u : 0u;
+        vs->dacc = rb ? 0.0f : d;
+        vs->slow = 0u;
+        vs->rebuilds += rb ? 1ull : 0ull;
+        vs->repaired += qs;
+        vs->substeps += 1ull;
+    }
+}
+
+// The rebuild form of a particle: sweep the cells within (1 + skin) h,
+// writing the row (every particle within (1 + skin) h, ascending slot) and
+// summing the density over the 27 cells in the same sweep.
+__device__ __forceinline__ void density_rebuild(const SceneDev &s, const SceneDev *__restrict__ scenes,
+                                                uint32_t scene, const uint32_t *__restrict__ start,
+                                                const float4 *__restrict__ posm, const uint32_t *__restrict__ tag,
+                                                float2 *__restrict__ rq, float *__restrict__ rec_rho,
+                                                float *__restrict__ rec_p, uint32_t *__restrict__ rec_count,
+                                                uint32_t *__restrict__ fcnt, const ph::VerletArgs &va, uint32_t i,
+                                                const float4 &pi) {
+    uint32_t cnt = 0;
+    float rho = 0.0f;
+    ListPush lp(va.lidx, va.lstride, i);
+    const float hg2 = s.h_grid2;
+    int32_t cx, cy, cz;
+    cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
+    int64_t xlo, xhi, ylo, yhi, zlo, zhi;
+    reach(pi.x, s.h_grid, cx, va.skin, xlo, xhi);
+    reach(pi.y, s.h_grid, cy, va.skin, ylo, yhi);
+    reach(pi.z, s.h_grid, cz, va.skin, zlo, zhi);
+    const bool inside = in_domain(s, xlo, ylo, zlo) && in_domain(s, xhi, yhi, zhi);
+    const uint32_t ob = start[s.bucket_base + s.ncells], oe = start[s.bucket_base + s.ncells + 1];
+    if ((!inside && ob != oe) || vl_far(pi.x, pi.y, pi.z, s.h_grid)) {
+        // the ball leaves the grid while outliers exist, or the cells
+        // saturate: the per-cell path, and the next substep sorts again
+        va.vcnt[i] = VLF_SLOW;
+        fcnt[i] = SLF_SLOW;
+        atomicOr(&va.vs->slow, 1u);
+        rho = density_cells(s, start, posm, pi, cx, cy, cz, cnt);
+        density_out(s, scenes, scene, i, rho, cnt, tag, rq, rec_rho, rec_p, rec_count);
+        return;
+    }
+    const float R2 = hg2 * va.f2;
+    uint32_t lk = 0;
+    uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+    uint4 *lg = reinterpret_cast<uint4 *>(va.vidx) + i;
+    auto push = [&](uint32_t j) {
+        const uint32_t p = lk & 3u;
+        q0 = p == 0u ? j : q0;
+        q1 = p == 1u ? j : q1;
+        q2 = p == 2u ? j : q2;
+        q3 = p == 3u ? j : q3;
+        if (p == 3u && lk < VL_CAP) lg[(size_t)(lk >> 2) * va.vstride] = make_uint4(q0, q1, q2, q3);
+        ++lk;
+    };
+    for (int64_t z = max(zlo, (int64_t)s.z0); z <= min(zhi, (int64_t)s.z0 + s.nz - 1); ++z) {
+        for (int64_t y = max(ylo, (int64_t)s.y0); y <= min(yhi, (int64_t)s.y0 + s.ny - 1); ++y) {
+            const int64_t xl = max(xlo, (int64_t)s.x0), xh = min(xhi, (int64_t)s.x0 + s.nx - 1);
+            if (xl > xh) continue;
+            const uint32_t row = s.bucket_base + (uint32_t)(((z - s.z0) * s.ny + (y - s.y0)) * s.nx);
+            const uint32_t b = start[row + (uint32_t)(xl - s.x0)], e = start[row + (uint32_t)(xh - s.x0) + 1];
+            // the part of this row inside the 27 cells (empty off them)
+            uint32_t b27 = 0, e27 = 0;
+            if (llabs(z - cz) <= 1 && llabs(y - cy) <= 1) {
+                const int64_t al = max((int64_t)cx - 1, (int64_t)s.x0),
+                              ah = min((int64_t)cx + 1, (int64_t)s.x0 + s.nx - 1);
+                if (al <= ah) {
+                    b27 = start[row + (uint32_t)(al - s.x0)];
+                    e27 = start[row + (uint32_t)(ah - s.x0) + 1];
+                }
+            }
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 pj = posm[j];
+                const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                if (r2 < R2) push(j);
+                if (j >= b27 && j < e27 && r2 < hg2) {
+                    ++cnt;
+                    rho += pj.w * poly6(r2, s);
+                    lp.push(j);
+                }
+            }
+        }
+    }
+    if ((lk & 3u) != 0u && lk < VL_CAP) lg[(size_t)(lk >> 2) * va.vstride] = make_uint4(q0, q1, q2, q3);
+    const uint32_t fc = lp.finish();
+    if (lk > VL_CAP || fc == SLF_SLOW) {
+        va.vcnt[i] = VLF_SLOW;
+        atomicOr(&va.vs->slow, 1u);
+    } else {
+        va.vcnt[i] = lk;
+    }
+    fcnt[i] = fc;
+    density_out(s, scenes, scene, i, rho, cnt, tag, rq, rec_rho, rec_p, rec_count);
+}
+
+// Within one wave: make LDS stores of every lane visible to the loads of
+// every lane that follow.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The rebuild substep's density pass (returns at once on a walk substep).
+__global__ __launch_bounds__(PH_TILE) void k_density_rebuild(const TileDev *__restrict__ tiles,
+                                                             const SceneDev *__restrict__ scenes,
+                                                             const uint32_t *__restrict__ start,
+                                                             const float4 *__restrict__ posm,
+                                                             const uint32_t *__restrict__ tag,
+                                                             float2 *__restrict__ rq, float *__restrict__ rec_rho,
+                                                             float *__restrict__ rec_p,
+                                                             uint32_t *__restrict__ rec_count,
+                                                             uint32_t *__restrict__ fcnt, ph::VerletArgs va) {
+    if (va.vs->rebuild == 0u) return;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const TileDev t = tiles[tile];
+    const SceneDev s = scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    if (i >= s.part_end) return;
+    const float4 pi = posm[i];
+    if (!finite3(pi.x, pi.y, pi.z)) {
+        // fluid3.rs:302-304: no neighbours, density 0; an empty row
+        va.vcnt[i] = 0u;
+        fcnt[i] = SLF_NONE;
+        density_out(s, scenes, t.scene, i, 0.0f, 0u, tag, rq, rec_rho, rec_p, rec_count);
+        return;
+    }
+    density_rebuild(s, scenes, t.scene, start, posm, tag, rq, rec_rho, rec_p, rec_count, fcnt, va, i, pi);
+}
+
+// The walk substep's density pass (returns at once on a rebuild substep).
+__global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restrict__ tiles,
+                                                          const SceneDev *__restrict__ scenes,
+                                                          const float4 *__restrict__ posm,
+                                                          const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
+                                                          float *__restrict__ rec_rho, float *__restrict__ rec_p,
+                                                          uint32_t *__restrict__ rec_count,
+                                                          uint32_t *__restrict__ fcnt, ph::VerletArgs va) {
+    // per wave: one row being put in order (slots, keys, contributions)
+    __shared__ uint32_t s_key[PH_TILE / 64][VL_CAP];
+    __shared__ uint32_t s_uj[PH_TILE / 64][VL_CAP], s_slot[PH_TILE / 64][VL_CAP];
+    __shared__ float s_uc[PH_TILE / 64][VL_CAP], s_c[PH_TILE / 64][VL_CAP];
+    if (va.vs->rebuild != 0u) return;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const TileDev t = tiles[tile];
+    const SceneDev s = scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    const bool act = i < s.part_end;
+    const float4 pi = act ? posm[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool fin = finite3(pi.x, pi.y, pi.z);
+    // Walk (every lane of the wave stays to the end: the repair below is
+    // the whole wave's).  The row's entries, four per 16-byte group; for
+    // each, the position and the neighbour word (independent loads).
+    const float hg2 = s.h_grid2;
+    const bool mine = act && fin;
+    if (act && !fin) {
+        fcnt[i] = SLF_NONE;
+        density_out(s, scenes, t.scene, i, 0.0f, 0u, tag, rq, rec_rho, rec_p, rec_count);
+    }
+    uint4 *rows = reinterpret_cast<uint4 *>(va.vidx);
+    // never VLF_SLOW here (such a state sorts and rebuilds); bounded anyway
+    const uint32_t n = mine ? min(va.vcnt[i], VL_CAP) : 0u;
+    const uint32_t wi = mine ? va.vword[i] : 0u;
+    bool bad = false;
+    uint32_t cnt = 0, fc = 0;
+    float rho = 0.0f;
+    {
+        ListPush lp(va.lidx, va.lstride, i);
+        uint32_t next = 0;  // the next neighbour's key must be at least this
+        auto one = [&](uint32_t j, const float4 &pj, uint32_t wj) {
+            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+            const uint32_t key = vl_key(wi, wj);
+            if (r2 < hg2 && key != VL_NOT27) {
+                bad |= key < next;
+                next = key + 1u;
+                ++cnt;
+                rho += pj.w * poly6(r2, s);
+                lp.push(j);
+            }
+        };
+        const uint4 *row = rows + i;
+#pragma unroll 1
+        for (uint32_t k = 0; k < n; k += 4) {
+            const uint4 e = *row;
+            row += va.vstride;
+            const bool v1 = k + 1 < n, v2 = k + 2 < n, v3 = k + 3 < n;
+            const uint32_t j0 = e.x, j1 = v1 ? e.y : e.x, j2 = v2 ? e.z : e.x, j3 = v3 ? e.w : e.x;
+            const float4 p0 = posm[j0], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
+            const uint32_t w0 = va.vword[j0], w1 = va.vword[j1], w2 = va.vword[j2], w3 = va.vword[j3];
+            one(j0, p0, w0);
+            if (v1) one(j1, p1, w1);
+            if (v2) one(j2, p2, w2);
+            if (v3) one(j3, p3, w3);
+        }
+        fc = lp.finish();
+    }
+    // Rows whose neighbours within h are out of visiting order (one of them
+    // changed cell): the wave puts each in order in turn.  Every lane takes
+    // entries lane and lane + 64, ranks them by key (ties, only between
+    // entries off the 27 cells, by entry index), and the row is stored back
+    // in that order; then the owner sums in that order.
+    const unsigned long long badm = __ballot(mine && bad);
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    if (badm != 0ull && lane == 0) atomicAdd(&va.tile_q[tile], (uint32_t)__popcll(badm));  // statistics
+    for (unsigned long long mm = badm; mm; mm &= mm - 1ull) {
+        const int b = __ffsll((long long)mm) - 1;
+        const uint32_t ri = __shfl(i, b, 64), rn = __shfl(n, b, 64), rw = __shfl(wi, b, 64);
+        const float px = __shfl(pi.x, b, 64), py = __shfl(pi.y, b, 64), pz = __shfl(pi.z, b, 64);
+        const uint32_t *ru = reinterpret_cast<const uint32_t *>(rows);
+        // entries lane and lane + 64: slot with bit 31 set for a neighbour
+        // within h, its contribution, its key
+#pragma unroll 1
+        for (uint32_t e = lane; e < rn; e += 64u) {
+            const uint32_t j = ru[((size_t)(e >> 2) * va.vstride + ri) * 4u + (e & 3u)];
+            const float4 pj = posm[j];
+            const uint32_t key = vl_key(rw, va.vword[j]);
+            const float r2 = dot3(pj.x - px, pj.y - py, pj.z - pz);
+            const bool in = r2 < hg2 && key != VL_NOT27;
+            s_key[w][e] = key;
+            s_uj[w][e] = j | (in ? 0x80000000u : 0u);
+            s_uc[w][e] = in ? pj.w * poly6(r2, s) : 0.0f;
+        }
+        wave_sync();
+#pragma unroll 1
+        for (uint32_t e = lane; e < rn; e += 64u) {
+            const uint32_t ke = s_key[w][e];
+            uint32_t r = 0;
+            for (uint32_t m = 0; m < rn; ++m) {
+                const uint32_t km = s_key[w][m];
+                r += (km < ke || (km == ke && m < e)) ? 1u : 0u;
+            }
+            s_slot[w][r] = s_uj[w][e];
+            s_c[w][r] = s_uc[w][e];
+        }
+        wave_sync();
+        // the row in visiting order, one 16-byte group per lane
+        const uint32_t g0 = lane * 4u;
+        if (g0 < rn) {
+            const uint32_t e0 = s_slot[w][g0] & 0x7FFFFFFFu;
+            const uint32_t e1 = g0 + 1 < rn ? s_slot[w][g0 + 1] & 0x7FFFFFFFu : 0u,
+                           e2 = g0 + 2 < rn ? s_slot[w][g0 + 2] & 0x7FFFFFFFu : 0u,
+                           e3 = g0 + 3 < rn ? s_slot[w][g0 + 3] & 0x7FFFFFFFu : 0u;
+            rows[(size_t)lane * va.vstride + ri] = make_uint4(e0, e1, e2, e3);
+        }
+        if (lane == (uint32_t)b) {
+            rho = 0.0f;
+            cnt = 0;
+            ListPush lp(va.lidx, va.lstride, ri);
+            for (uint32_t m = 0; m < rn; ++m) {
+                const uint32_t e = s_slot[w][m];
+                if (e & 0x80000000u) {
+                    ++cnt;
+                    rho += s_c[w][m];
+                    lp.push(e & 0x7FFFFFFFu);
+                }
+            }
+            fc = lp.finish();
+        }
+        wave_sync();
+    }
+    if (mine) {
+        fcnt[i] = fc;
+        density_out(s, scenes, t.scene, i, rho, cnt, tag, rq, rec_rho, rec_p, rec_count);
+    }
+}
+
+namespace ph {
+
+hipError_t launch_vl_decide(hipStream_t st, VerletDev *vs, const float *tile_dmax, uint32_t *tile_q, uint32_t ntiles,
+                            uint32_t force, float thr) {
+    hipLaunchKernelGGL(k_vl_decide, dim3(1), dim3(PH_TILE), 0, st, vs, tile_dmax, tile_q, ntiles, force, thr);
+    return hipGetLastError();
+}
+
+hipError_t launch_density_verlet(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                                 const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
+                                 float *rec_rho, float *rec_p, uint32_t *rec_count, uint32_t *fcnt,
+                                 const VerletArgs &va) {
+    hipLaunchKernelGGL(k_density_rebuild, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, rq,
+                       rec_rho, rec_p, rec_count, fcnt, va);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_density_walk, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, posm, tag, rq, rec_rho,
+                       rec_p, rec_count, fcnt, va);
+    return hipGetLastError();
+}
+
+}  // namespace ph

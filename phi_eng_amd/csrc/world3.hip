@@ -510,6 +510,7 @@ struct World3 {
     // read, 1 - cs written, then becomes current (tags keep their order).
     void substep_surrogate() {
         ph::Engine &e = eng;
+        e.vl_valid = false;  // moves particles without the Verlet displacement bound
         SurStepArgs a{};
         a.use_net = use_net;
         a.tiles = e.d_tiles.p;
@@ -1088,6 +1089,33 @@ int32_t phieng_world3_surrogate_stats(const World3 *world, uint64_t *sur_steps, 
     return 1;
 }
 
+int32_t phieng_world3_set_neighbour_list(World3 *world, int32_t verlet, float skin) {
+    if (!world || !(skin >= 0.0f && skin <= 0.5f)) return 0;
+    world->eng.vl_enabled = verlet != 0;
+    world->eng.vl_skin = skin;
+    world->eng.vl_valid = false;  // the next substep rebuilds with the new radius
+    return 1;
+}
+
+int32_t phieng_world3_neighbour_stats(World3 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired) {
+    if (!world) return 0;
+    try {
+        VerletDev v{};
+        ph::Engine &e = world->eng;
+        if (e.vl_state.p && e.vl_state_ready) {
+            PH_CK(hipMemcpyAsync(&v, e.vl_state.p, sizeof(v), hipMemcpyDeviceToHost, e.stream));
+            PH_CK(hipStreamSynchronize(e.stream));
+        }
+        if (substeps) *substeps = v.substeps;
+        if (rebuilds) *rebuilds = v.rebuilds;
+        if (repaired) *repaired = v.repaired;
+        return 1;
+    } catch (const std::exception &ex) {
+        set_error("phieng_world3_neighbour_stats", ex.what());
+        return 0;
+    }
+}
+
 int32_t phieng_world3_boundary_info(const World3 *world, uint32_t *count, uint64_t *regens) {
     if (!world) return 0;
     if (count) *count = (uint32_t)world->ghost_gen.size();
@@ -1370,6 +1398,14 @@ uint32_t phieng_world_fluid_get_record(const World2 *world, int32_t *cell2, uint
 }
 
 int32_t phieng_world_sync(World2 *world) { return world ? phieng_world3_sync(as3(world)) : 0; }
+
+int32_t phieng_world_set_neighbour_list(World2 *world, int32_t verlet, float skin) {
+    return world ? phieng_world3_set_neighbour_list(world2_base(world), verlet, skin) : 0;
+}
+
+int32_t phieng_world_neighbour_stats(World2 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired) {
+    return world ? phieng_world3_neighbour_stats(world2_base(world), substeps, rebuilds, repaired) : 0;
+}
 
 int32_t phieng_world_boundary_info(const World2 *world, uint32_t *count, uint64_t *regens) {
     return world ? phieng_world3_boundary_info(as3(world), count, regens) : 0;

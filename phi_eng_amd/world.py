@@ -168,6 +168,16 @@ class World3:
         self._l.phieng_world3_boundary_info(self.ptr, ctypes.byref(c), ctypes.byref(r))
         return c.value, r.value
 
+    def set_neighbour_list(self, verlet=True, skin=0.1):
+        if not self._l.phieng_world3_set_neighbour_list(self.ptr, 1 if verlet else 0, float(skin)):
+            raise ValueError("neighbour list settings rejected")
+
+    def neighbour_stats(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        if not self._l.phieng_world3_neighbour_stats(self.ptr, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)):
+            raise RuntimeError(last_error())
+        return {"substeps": a.value, "rebuilds": b.value, "repaired": c.value}
+
     def boundary(self):
         """Ghosts in generation order: dict(pos (G, 3), density, pressure)."""
         g, _ = self.boundary_info()
@@ -355,6 +365,16 @@ class World2:
         c, r = ctypes.c_uint32(), ctypes.c_uint64()
         self._l.phieng_world_boundary_info(self.ptr, ctypes.byref(c), ctypes.byref(r))
         return c.value, r.value
+
+    def set_neighbour_list(self, verlet=True, skin=0.1):
+        if not self._l.phieng_world_set_neighbour_list(self.ptr, 1 if verlet else 0, float(skin)):
+            raise ValueError("neighbour list settings rejected")
+
+    def neighbour_stats(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        if not self._l.phieng_world_neighbour_stats(self.ptr, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)):
+            raise RuntimeError(last_error())
+        return {"substeps": a.value, "rebuilds": b.value, "repaired": c.value}
 
     def boundary(self):
         """Ghosts in generation order: dict(pos (G, 2), density, pressure)."""
