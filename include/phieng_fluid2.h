@@ -52,9 +52,25 @@ void phieng_world_step(World2 *world, float dt);
 void phieng_world_set_gravity(World2 *world, float gx, float gy);
 /* ffi.rs:1159-1203 — circle (a = radius) or box (a, b = half extents) at
  * angle 0; returns the body id or -1.  Static bodies only: a dynamic body
- * returns -1 (no two-way coupling on the GPU path). */
+ * returns -1 (the 2D fluid's two-way coupling, fluid.rs, is not on the
+ * GPU path; 3D worlds couple). */
 int32_t phieng_world_add_body(World2 *world, float pos_x, float pos_y, uint32_t shape_tag, float a, float b,
                               float density, uint8_t is_static, float friction, float restitution);
+/* #[repr(C)] BodyDesc2D (ffi.rs:1205-1222), 36 bytes. */
+typedef struct BodyDesc2D {
+    float pos_x, pos_y;
+    uint32_t shape_tag;
+    float a, b;
+    float density;
+    uint8_t is_static;
+    uint8_t _pad[3];
+    float friction, restitution;
+} BodyDesc2D;
+/* ffi.rs:1224-1290 — each descriptor validated and added as
+ * phieng_world_add_body; out_ids[i] = id or -1 (no rollback).  Returns the
+ * number added; 0 for null world, descs or out_ids; n capped at
+ * 1,000,000. */
+uint32_t phieng_world_add_bodies(World2 *world, const BodyDesc2D *descs, uint32_t n, int32_t *out_ids);
 /* ffi.rs:999-1011 — 1 on success, 0 on null or invalid input (world
  * unchanged); validation ffi.rs:773-840.  adaptive_resolution merges and
  * splits particles each substep as fluid.rs:826-891 does (split cap: the

@@ -66,17 +66,48 @@ World3 *phieng_world3_new(void);
 /* ffi.rs:3209-3213 — null is a no-op. */
 void phieng_world3_free(World3 *world);
 /* ffi.rs:3221-3226 — null, non-finite or dt <= 0 is a no-op; dt is
- * clamped to 1/15 s. Runs the fluid phase of World3::step
- * (crates/dynamics/src/world.rs:2670-2697, 2809-2812) on the GPU. */
+ * clamped to 1/15 s. Runs World3::step (crates/dynamics/src/world.rs:
+ * 2670-2697, 2766-2812) as far as the fluid and its bodies go: the gravity
+ * kick of dynamic bodies, the fluid phase on the GPU with its two-way
+ * coupling (fluid3.rs:1141-1213), the bodies' integration.  The rigid
+ * contact solver (body-body contacts, joints) is not part of this
+ * library: dynamic bodies meet only the fluid. */
 void phieng_world3_step(World3 *world, float dt);
 /* ffi.rs:3234-3244 — non-finite components are ignored. */
 void phieng_world3_set_gravity(World3 *world, float gx, float gy, float gz);
-/* ffi.rs:3703-3760 — returns the body id or -1.  Static bodies only:
- * a dynamic body (is_static == 0) returns -1 because two-way coupling
- * with moving bodies is not implemented on the GPU path. */
+/* ffi.rs:3703-3760 — returns the body id or -1.  A dynamic body
+ * (is_static == 0) gets the mass and inertia of RigidBody3::set_shape
+ * (body.rs:1266-1300) and is coupled both ways to the fluid; it is refused
+ * (-1, reason in phieng_last_error) while the world runs the learned
+ * surrogate, collects samples or has adaptive_resolution on. */
 int32_t phieng_world3_add_body(World3 *world, float pos_x, float pos_y, float pos_z,
                                uint32_t shape_tag, float a, float b, float c, float density,
                                uint8_t is_static, float friction, float restitution);
+/* #[repr(C)] BodyDesc3D (ffi.rs:3765-3784), 44 bytes. */
+typedef struct BodyDesc3D {
+    float pos_x, pos_y, pos_z;
+    uint32_t shape_tag;
+    float a, b, c;
+    float density;
+    uint8_t is_static;
+    uint8_t _pad[3];
+    float friction, restitution;
+} BodyDesc3D;
+/* ffi.rs:3786-3860 — each descriptor validated and added as
+ * phieng_world3_add_body; out_ids[i] = id or -1 (no rollback).  Returns
+ * the number added; 0 for null world, descs or out_ids; n capped at
+ * 1,000,000. */
+uint32_t phieng_world3_add_bodies(World3 *world, const BodyDesc3D *descs, uint32_t n, int32_t *out_ids);
+/* ffi.rs:3387-3393 — -1 for null. */
+int32_t phieng_world3_body_count(World3 *world);
+/* ffi.rs:3885-3910 — position (3) and rotation quaternion x, y, z, w (4);
+ * 1 on success, 0 on null or an unknown id. */
+int32_t phieng_world3_get_body_transform(World3 *world, uint32_t body_id, float *out_pos, float *out_quat);
+/* ffi.rs:3917-3937 — 1 on success, 0 on null or an unknown id. */
+int32_t phieng_world3_get_body_velocity(World3 *world, uint32_t body_id, float *out_vel);
+/* ffi.rs:1639-1657 — radians per second; 1 on success, 0 on null or an
+ * unknown id. */
+int32_t phieng_world3_get_body_angular_velocity(World3 *world, uint32_t body_id, float *out_w);
 /* ffi.rs:1032-1044 — 1 on success, 0 on null or invalid input (world
  * unchanged); validation rules ffi.rs:857-914.  Also returns 0 for
  * boundary_particles or adaptive_resolution set while a surrogate is

@@ -18,6 +18,30 @@ class OracleBody3(ctypes.Structure):
                 ("c", ctypes.c_float)]
 
 
+class OracleRigid3(ctypes.Structure):
+    """A body of oracle3_world_run_f32: shape 0 sphere / 1 box / 2 capsule,
+    density, static flag and motion state (rotation x, y, z, w)."""
+    _fields_ = [("shape", ctypes.c_uint32), ("a", ctypes.c_float), ("b", ctypes.c_float), ("c", ctypes.c_float),
+                ("density", ctypes.c_float), ("is_static", ctypes.c_uint32), ("pos", ctypes.c_float * 3),
+                ("rot", ctypes.c_float * 4), ("vel", ctypes.c_float * 3), ("ang", ctypes.c_float * 3),
+                ("linear_damping", ctypes.c_float), ("angular_damping", ctypes.c_float),
+                ("gravity_scale", ctypes.c_float)]
+
+
+def rigid(shape, center, a, b=0.0, c=0.0, density=1.0, is_static=True):
+    """A body at rest with the identity rotation and RigidBody3::new's damping."""
+    r = OracleRigid3()
+    r.shape = shape
+    r.a, r.b, r.c = a, b, c
+    r.density = density
+    r.is_static = 1 if is_static else 0
+    r.pos[:] = [float(v) for v in center]
+    r.rot[:] = [0.0, 0.0, 0.0, 1.0]
+    r.linear_damping = r.angular_damping = 0.01
+    r.gravity_scale = 1.0
+    return r
+
+
 _lib = None
 
 
@@ -38,6 +62,8 @@ def lib():
         _lib.oracle3_spiky_grad.argtypes = [fp, ctypes.c_float, ctypes.c_float, fp]
         _lib.oracle3_cfl_substeps.restype = ctypes.c_uint32
         _lib.oracle3_cfl_substeps.argtypes = [ctypes.c_uint32, fp, ctypes.c_void_p, ctypes.c_float]
+        _lib.oracle3_sample_pressure.restype = ctypes.c_float
+        _lib.oracle3_surface_samples.restype = ctypes.c_uint32
     return _lib
 
 
@@ -477,3 +503,77 @@ def boundary2_f32(pos, mass, cfg, bodies):
     p = np.zeros(g, np.float32)
     l.oracle2_boundary_f32(len(pos), _p(pos), _p(mass), ctypes.byref(cfg), arr, nb, _p(xy), _p(rho), _p(p), g)
     return dict(pos=xy, density=rho, pressure=p)
+
+
+def world_run_f32(pos, vel, cfg, gravity, bodies, dt, steps, jacobi=True, phases=None):
+    """`steps` World3 steps with the fluid coupled to `bodies` (a list of
+    OracleRigid3, updated in place); returns (pos, vel, substeps of the last step)."""
+    n = len(pos)
+    pos = np.ascontiguousarray(pos, dtype=np.float32).copy()
+    vel = np.ascontiguousarray(vel, dtype=np.float32).copy()
+    m = masses(cfg, n, phases)
+    ph = None if phases is None else np.ascontiguousarray(phases, dtype=np.uint8)
+    g = np.asarray(gravity, dtype=np.float32)
+    arr = (OracleRigid3 * max(1, len(bodies)))(*bodies)
+    sub = lib().oracle3_world_run_f32(
+        ctypes.c_uint32(n), _p(pos), _p(vel), _p(m), _p(ph, ctypes.c_uint8), ctypes.byref(cfg), _p(g), arr,
+        ctypes.c_uint32(len(bodies)), ctypes.c_float(dt), ctypes.c_uint32(steps), ctypes.c_int(int(jacobi)))
+    for k in range(len(bodies)):
+        ctypes.memmove(ctypes.byref(bodies[k]), ctypes.byref(arr[k]), ctypes.sizeof(OracleRigid3))
+    return pos, vel, sub
+
+
+def cos_sin(angle):
+    c, s = ctypes.c_float(), ctypes.c_float()
+    lib().oracle3_cos_sin(ctypes.c_float(angle), ctypes.byref(c), ctypes.byref(s))
+    return c.value, s.value
+
+
+def quat_mul(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    out = np.zeros(4, np.float32)
+    lib().oracle3_quat_mul(_p(a), _p(b), _p(out))
+    return out
+
+
+def quat_normalize(a):
+    a = np.asarray(a, np.float32)
+    out = np.zeros(4, np.float32)
+    lib().oracle3_quat_normalize(_p(a), _p(out))
+    return out
+
+
+def quat_rotate(q, v):
+    q = np.asarray(q, np.float32)
+    v = np.asarray(v, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().oracle3_quat_rotate(_p(q), _p(v), _p(out))
+    return out
+
+
+def body_mass(shape, a, b=0.0, c=0.0, density=1.0):
+    """(mass, inverse mass, body-frame inverse inertia 3x3) of a dynamic body."""
+    m, im = ctypes.c_float(), ctypes.c_float()
+    ii = np.zeros(9, np.float32)
+    lib().oracle3_body_mass(ctypes.c_uint32(shape), ctypes.c_float(a), ctypes.c_float(b), ctypes.c_float(c),
+                            ctypes.c_float(density), ctypes.byref(m), ctypes.byref(im), _p(ii))
+    return m.value, im.value, ii.reshape(3, 3).T  # columns -> matrix
+
+
+def surface_samples(body):
+    pts = np.zeros((64, 3), np.float32)
+    nrm = np.zeros((64, 3), np.float32)
+    area = np.zeros(64, np.float32)
+    k = lib().oracle3_surface_samples(ctypes.byref(body), _p(pts), _p(nrm), _p(area), 64)
+    return pts[:k], nrm[:k], area[:k]
+
+
+def sample_pressure(pos, mass, density, pressure, cfg, point):
+    pos = np.ascontiguousarray(pos, np.float32)
+    mass = np.ascontiguousarray(mass, np.float32)
+    density = np.ascontiguousarray(density, np.float32)
+    pressure = np.ascontiguousarray(pressure, np.float32)
+    pt = np.asarray(point, np.float32)
+    return lib().oracle3_sample_pressure(ctypes.c_uint32(len(pos)), _p(pos), _p(mass), _p(density), _p(pressure),
+                                         ctypes.byref(cfg), _p(pt))

@@ -145,6 +145,10 @@ struct Fluid {
     // MAX_PARTICLES_3D (fluid3.rs:28), the split cap of resample(); a world
     // whose capacity was lifted passes its capacity instead.
     uint32_t cap = 4096;
+    // With dynamic bodies: replaces the static clamp at the end of a
+    // substep (the coupled clamp, the second grid rebuild and buoyancy).
+    void (*tail)(void *ctx, float dt) = nullptr;
+    void *tail_ctx = nullptr;
 
     explicit Fluid(const FluidConfig3Ffi &c) : cfg(c), n(0) {}
 
@@ -653,6 +657,10 @@ struct Fluid {
                 }
             }
         }
+        if (tail) {
+            tail(tail_ctx, (float)dt);
+            return;
+        }
         for (uint32_t i = 0; i < n; ++i) clamp_particle(i, bodies, nbodies);
         // fluid3.rs:1184-1213: the second grid rebuild only feeds buoyancy of
         // dynamic bodies, which this path does not have.
@@ -700,6 +708,390 @@ struct Fluid {
     }
 };
 
+// ---------------------------------------------------------------- dynamic bodies
+// The rigid side of the two-way coupling (fluid3.rs:1141-1213), restated
+// from the reference with the f32 operations of its math library, glam
+// 0.29.3 (not vendored; Vec3 and Mat3 in their scalar forms, Quat in its
+// SSE2 form, association orders kept):
+//   Quat * Vec3   v (w w - b.b) + b (2 (b.v)) + (b x v)(2 w),  b = (x, y, z);
+//   Quat * Quat   per lane (w-term + x-term) + (y-term + z-term);
+//   normalize     q / sqrt((xx + zz) + (yy + ww));
+//   Mat3          columns; M v = (c0 vx + c1 vy) + c2 vz; inverse from the
+//                 cross products of the columns and det = c2 . (c0 x c1).
+// A body whose quaternion is exactly (0, 0, 0, 1) skips the rotation (the
+// product reproduces its operand up to the sign of zero components).
+
+struct Quat {
+    float x, y, z, w;
+};
+struct Mat3 {
+    V3<float> c0, c1, c2;
+};
+
+inline V3<float> crossf(V3<float> a, V3<float> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline V3<float> quat_apply(Quat q, V3<float> v) {
+    V3<float> b{q.x, q.y, q.z};
+    float b2 = dot(b, b);
+    return add(add(muls(v, q.w * q.w - b2), muls(b, dot(b, v) * 2.0f)), muls(crossf(b, v), q.w * 2.0f));
+}
+inline Quat quat_mul(Quat a, Quat b) {
+    return {(a.w * b.x + a.x * b.w) + (a.y * b.z - a.z * b.y), (a.w * b.y - a.x * b.z) + (a.y * b.w + a.z * b.x),
+            (a.w * b.z + a.x * b.y) + (a.z * b.w - a.y * b.x), (a.w * b.w - a.x * b.x) - (a.y * b.y + a.z * b.z)};
+}
+inline Quat quat_normalize(Quat q) {
+    float l = std::sqrt((q.x * q.x + q.z * q.z) + (q.y * q.y + q.w * q.w));
+    return {q.x / l, q.y / l, q.z / l, q.w / l};
+}
+inline V3<float> mat_apply(const Mat3 &m, V3<float> v) {
+    return add(add(muls(m.c0, v.x), muls(m.c1, v.y)), muls(m.c2, v.z));
+}
+inline Mat3 mat_mul(const Mat3 &a, const Mat3 &b) { return {mat_apply(a, b.c0), mat_apply(a, b.c1), mat_apply(a, b.c2)}; }
+inline Mat3 mat_t(const Mat3 &m) {
+    return {{m.c0.x, m.c1.x, m.c2.x}, {m.c0.y, m.c1.y, m.c2.y}, {m.c0.z, m.c1.z, m.c2.z}};
+}
+inline Mat3 mat_inverse(const Mat3 &m) {
+    V3<float> t0 = crossf(m.c1, m.c2), t1 = crossf(m.c2, m.c0), t2 = crossf(m.c0, m.c1);
+    float inv = 1.0f / dot(m.c2, t2);
+    return mat_t({muls(t0, inv), muls(t1, inv), muls(t2, inv)});
+}
+inline Mat3 mat_from_quat(Quat q) {
+    float x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
+    float xx = q.x * x2, xy = q.x * y2, xz = q.x * z2, yy = q.y * y2, yz = q.y * z2, zz = q.z * z2;
+    float wx = q.w * x2, wy = q.w * y2, wz = q.w * z2;
+    return {{1.0f - (yy + zz), xy + wz, xz - wy}, {xy - wz, 1.0f - (xx + zz), yz + wx}, {xz + wy, yz - wx, 1.0f - (xx + yy)}};
+}
+
+// crates/math/src/trig.rs:129-164: reduction by pi/2 (two-part constant,
+// f64), sin and cos Taylor polynomials to x^11 / x^12 on [-pi/4, pi/4],
+// quadrant from the integer n mod 4.
+inline void trig_cos_sin(float angle, float &c, float &s) {
+    if (!std::isfinite(angle)) {
+        c = s = NAN;
+        return;
+    }
+    const double a = angle;
+    const double n = std::round(a * (2.0 / 3.14159265358979323846));
+    const double r = (a - n * 1.570796012878418) - n * 3.13916478589249e-07;
+    const double z = r * r;
+    const double sp = -1.0 / 6.0 + z * (1.0 / 120.0 + z * (-1.0 / 5040.0 + z * (1.0 / 362880.0 + z * (-1.0 / 39916800.0))));
+    const double sv = r * (1.0 + z * sp);
+    const double cq =
+        -0.5 + z * (1.0 / 24.0 + z * (-1.0 / 720.0 + z * (1.0 / 40320.0 + z * (-1.0 / 3628800.0 + z * (1.0 / 479001600.0)))));
+    const double cv = 1.0 + z * cq;
+    double qd = std::fmod(n, 4.0);
+    if (qd < 0.0) qd += 4.0;
+    const int quadrant = (int)qd;
+    switch (quadrant) {
+        case 1: c = (float)-sv; s = (float)cv; break;
+        case 2: c = (float)-cv; s = (float)-sv; break;
+        case 3: c = (float)sv; s = (float)-cv; break;
+        default: c = (float)cv; s = (float)sv; break;
+    }
+}
+
+struct RigidO {
+    uint32_t shape = 1;
+    float a = 0, b = 0, c = 0;
+    bool is_static = true, sleeping = false;
+    V3<float> pos{0, 0, 0};
+    Quat rot{0, 0, 0, 1};
+    V3<float> vel{0, 0, 0}, ang{0, 0, 0};
+    float mass = 1, inv_mass = 1;
+    Mat3 inv_inertia{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    float linear_damping = 0.01f, angular_damping = 0.01f, gravity_scale = 1.0f;
+
+    bool identity() const { return rot.x == 0.0f && rot.y == 0.0f && rot.z == 0.0f && rot.w == 1.0f; }
+    V3<float> rotate(V3<float> v) const { return identity() ? v : quat_apply(rot, v); }
+    V3<float> unrotate(V3<float> v) const { return identity() ? v : quat_apply({-rot.x, -rot.y, -rot.z, rot.w}, v); }
+    V3<float> to_world(V3<float> l) const { return add(rotate(l), pos); }
+
+    // body.rs:1266-1300, mass_properties_3d (body.rs:811-841),
+    // InertiaTensor3::capsule (math.rs:148-160)
+    void shape_mass(float density) {
+        const float PI = kPiF, k43 = 4.0f / 3.0f;
+        float m, ix, iy, iz;
+        if (shape == 0) {
+            m = k43 * PI * a * a * a * density;
+            ix = iy = iz = 0.4f * m * a * a;
+        } else if (shape == 1) {
+            float w = a * 2.0f, h = b * 2.0f, d = c * 2.0f;
+            m = w * h * d * density;
+            ix = m * (h * h + d * d) / 12.0f;
+            iy = m * (w * w + d * d) / 12.0f;
+            iz = m * (w * w + h * h) / 12.0f;
+        } else {
+            float r = a, hl = b, len2 = hl * 2.0f;
+            m = (PI * r * r * len2 + k43 * PI * r * r * r) * density;
+            float vc = PI * r * r * len2, vs = k43 * PI * r * r * r;
+            float den = m / rmax<float>(vc + vs, 1e-12f);
+            float mc = den * vc, ms = den * vs;
+            ix = iz = mc * (3.0f * r * r + len2 * len2) / 12.0f + ms * (0.4f * r * r + hl * hl);
+            iy = 0.5f * mc * r * r + 0.4f * ms * r * r;
+        }
+        mass = rmax<float>(m, kEps);
+        if (is_static) {
+            inv_mass = 0.0f;
+            inv_inertia = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        } else {
+            inv_mass = 1.0f / mass;
+            inv_inertia = mat_inverse({{ix, 0, 0}, {0, iy, 0}, {0, 0, iz}});
+        }
+    }
+    Mat3 world_inv_inertia() const {
+        Mat3 r = mat_from_quat(rot);
+        return mat_mul(mat_mul(r, inv_inertia), mat_t(r));
+    }
+    void apply_impulse(V3<float> j, V3<float> point) {  // body.rs:1349-1361
+        if (is_static) return;
+        sleeping = false;
+        vel = add(vel, muls(j, inv_mass));
+        ang = add(ang, mat_apply(world_inv_inertia(), crossf(sub(point, pos), j)));
+    }
+    void apply_force(V3<float> f, V3<float> point, float dt) {  // body.rs:1334-1346
+        if (is_static) return;
+        sleeping = false;
+        vel = add(vel, muls(muls(f, inv_mass), dt));
+        ang = add(ang, muls(mat_apply(world_inv_inertia(), crossf(sub(point, pos), f)), dt));
+    }
+    void integrate(float dt) {  // body.rs:1390-1406, 1372-1387
+        if (is_static || sleeping) return;
+        vel = muls(vel, 1.0f - std::fmin(linear_damping * dt, 1.0f));
+        ang = muls(ang, 1.0f - std::fmin(angular_damping * dt, 1.0f));
+        pos = add(pos, muls(vel, dt));
+        float angle = len(ang) * dt;
+        if (angle > 1e-6f) {
+            V3<float> axis = divs(ang, len(ang));
+            float ch, sh;
+            trig_cos_sin(angle * 0.5f, ch, sh);
+            rot = quat_normalize(quat_mul({axis.x * sh, axis.y * sh, axis.z * sh, ch}, rot));
+        }
+    }
+    Body as_body() const { return {shape, pos.x, pos.y, pos.z, a, shape == 0 ? 0.0f : b, shape == 1 ? c : 0.0f}; }
+};
+
+// fluid3.rs:471-562 with the body's rotation; also the contact point.
+inline bool collide_rigid(V3<float> p, float pr, const RigidO &bd, V3<float> &normal, float &depth,
+                          V3<float> &point) {
+    if (bd.shape == PHIENG_SHAPE3_SPHERE) {
+        V3<float> delta = sub(p, bd.pos);
+        float dist = len(delta);
+        if (dist >= bd.a + pr) return false;
+        normal = dist > kEps ? divs(delta, dist) : V3<float>{1, 0, 0};
+        depth = bd.a + pr - dist;
+        point = add(bd.pos, muls(normal, bd.a));
+        return true;
+    }
+    V3<float> local = bd.unrotate(sub(p, bd.pos));
+    if (bd.shape == PHIENG_SHAPE3_BOX) {
+        V3<float> cl{rclamp(local.x, -bd.a, bd.a), rclamp(local.y, -bd.b, bd.b), rclamp(local.z, -bd.c, bd.c)};
+        V3<float> diff = sub(local, cl);
+        if (dot(diff, diff) > kEpsSq) {
+            float dist = len(diff);
+            if (dist >= pr) return false;
+            normal = bd.rotate(divs(diff, dist));
+            depth = pr - dist;
+            point = bd.to_world(cl);
+            return true;
+        }
+        float dx = bd.a - std::fabs(local.x), dy = bd.b - std::fabs(local.y), dz = bd.c - std::fabs(local.z);
+        V3<float> axis;
+        float d;
+        if (dx <= dy && dx <= dz) {
+            axis = {1, 0, 0};
+            d = dx;
+        } else if (dy <= dz) {
+            axis = {0, 1, 0};
+            d = dy;
+        } else {
+            axis = {0, 0, 1};
+            d = dz;
+        }
+        normal = bd.rotate(axis);
+        depth = pr + d;
+        point = bd.to_world(cl);
+        return true;
+    }
+    if (bd.shape == PHIENG_SHAPE3_CAPSULE) {
+        float r = bd.a, hl = bd.b;
+        float sy = rclamp(local.y, -hl, hl);
+        V3<float> seg{0, sy, 0};
+        V3<float> diff = sub(local, seg);
+        if (dot(diff, diff) > kEpsSq) {
+            float dist = len(diff);
+            if (dist >= r + pr) return false;
+            normal = bd.rotate(divs(diff, dist));
+            depth = r + pr - dist;
+            float rcp = 1.0f / len(diff);
+            V3<float> u = (std::isfinite(rcp) && rcp > 0.0f) ? muls(diff, rcp) : V3<float>{0, 0, 0};
+            point = bd.to_world(add(seg, muls(u, r)));
+            return true;
+        }
+        V3<float> radial{local.x, 0, local.z};
+        V3<float> dir{1, 0, 0};
+        if (dot(radial, radial) > kEpsSq) {
+            float rcp = 1.0f / len(radial);
+            dir = (std::isfinite(rcp) && rcp > 0.0f) ? muls(radial, rcp) : V3<float>{0, 0, 0};
+        }
+        normal = bd.rotate(dir);
+        depth = pr + r - len(diff);
+        point = bd.to_world(add(seg, muls(dir, r)));
+        return true;
+    }
+    return false;
+}
+
+struct SampleO {
+    V3<float> point, normal;
+    float area;
+};
+
+// fluid3.rs:375-465 (both faces of a box axis sample the plane at -half
+// extent, as the reference's table has it)
+inline void surface_samples_o(const RigidO &bd, std::vector<SampleO> &out) {
+    out.clear();
+    const float PI = kPiF;
+    auto put = [&](V3<float> l, V3<float> nrm, float area) { out.push_back({bd.to_world(l), bd.rotate(nrm), area}); };
+    if (bd.shape == 0) {
+        V3<float> d[14] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+        int k = 6;
+        for (float sx : {1.0f, -1.0f})
+            for (float sy : {1.0f, -1.0f})
+                for (float sz : {1.0f, -1.0f}) {
+                    V3<float> v{sx, sy, sz};
+                    float rcp = 1.0f / len(v);
+                    d[k++] = muls(v, rcp);
+                }
+        float area = 4.0f * PI * bd.a * bd.a / 14.0f;
+        for (int q = 0; q < 14; ++q) put(muls(d[q], bd.a), d[q], area);
+    } else if (bd.shape == 1) {
+        float area = 8.0f * (bd.a * bd.b + bd.b * bd.c + bd.c * bd.a) / 24.0f;
+        const float t[4][2] = {{-0.75f, -0.75f}, {0.75f, -0.75f}, {-0.75f, 0.75f}, {0.75f, 0.75f}};
+        for (int f = 0; f < 6; ++f) {
+            int axis = f / 2;
+            float sgn = (f % 2) ? -1.0f : 1.0f;
+            V3<float> nrm = axis == 0 ? V3<float>{0, sgn, 0} : axis == 1 ? V3<float>{sgn, 0, 0} : V3<float>{0, 0, sgn};
+            for (int k = 0; k < 4; ++k) {
+                V3<float> l = axis == 0   ? V3<float>{bd.a * t[k][0], -bd.b, bd.c * t[k][1]}
+                              : axis == 1 ? V3<float>{-bd.a, bd.b * t[k][0], bd.c * t[k][1]}
+                                          : V3<float>{bd.a * t[k][0], bd.b * t[k][1], -bd.c};
+                put(l, nrm, area);
+            }
+        }
+    } else if (bd.shape == 2) {
+        float r = bd.a, hl = bd.b;
+        float side = 4.0f * PI * r * hl / 16.0f, cap = 2.0f * PI * r * r;
+        for (float sy : {-hl * 0.5f, hl * 0.5f})
+            for (int k = 0; k < 8; ++k) {
+                float an = 2.0f * PI * (float)k / 8.0f;
+                V3<float> dir{std::cos(an), 0.0f, std::sin(an)};
+                put(add(muls(dir, r), muls(V3<float>{0, 1, 0}, sy)), dir, side);
+            }
+        for (float sg : {-1.0f, 1.0f}) put(muls(muls(V3<float>{0, 1, 0}, sg), hl + r), muls(V3<float>{0, 1, 0}, sg), cap);
+    }
+}
+
+// The world and its coupled fluid.
+struct CoupledWorld {
+    Fluid<float> f;
+    std::vector<RigidO> bodies;
+    std::vector<Body> statics;  // for the ghosts (static bodies only, fluid3.rs:724)
+    V3<float> g;
+    std::vector<SampleO> samp;
+    explicit CoupledWorld(const FluidConfig3Ffi &c) : f(c) {
+        f.tail = &CoupledWorld::tail_fn;
+        f.tail_ctx = this;
+    }
+    static void tail_fn(void *ctx, float dt) { static_cast<CoupledWorld *>(ctx)->coupled_tail(dt); }
+
+    // fluid3.rs:345-369 on the current grid
+    float sample_pressure(V3<float> point) const {
+        const float h = f.h_kernel();
+        const int32_t cx = sat_i32((double)std::floor(point.x / h)), cy = sat_i32((double)std::floor(point.y / h)),
+                      cz = sat_i32((double)std::floor(point.z / h));
+        float p = 0.0f;
+        for (int oz = -1; oz <= 1; ++oz)
+            for (int oy = -1; oy <= 1; ++oy)
+                for (int ox = -1; ox <= 1; ++ox) {
+                    auto it = f.grid.find(Key{cx + ox, cy + oy, cz + oz});
+                    if (it == f.grid.end()) continue;
+                    for (uint32_t j : it->second) {
+                        V3<float> d = sub(f.pos[j], point);
+                        float r2 = dot(d, d);
+                        if (r2 < h * h) p += f.mass[j] * (f.pressure[j] / rmax<float>(f.density[j], kEps)) *
+                                             Fluid<float>::poly6(r2, h);
+                    }
+                }
+        return rmax<float>(p, 0.0f);
+    }
+
+    // fluid3.rs:1141-1213 after the integrate of a substep
+    void coupled_tail(float dt) {
+        const float pr = rmax<float>(f.cfg.particle_radius, 0.0f);
+        for (uint32_t i = 0; i < f.n; ++i) {
+            for (RigidO &bd : bodies) {
+                V3<float> nrm, pt;
+                float depth;
+                if (!collide_rigid(f.pos[i], pr, bd, nrm, depth, pt)) continue;
+                if (depth <= 0.0f) continue;
+                f.pos[i] = add(f.pos[i], muls(nrm, depth));
+                float vn = dot(f.vel[i], nrm);
+                if (vn < 0.0f) {
+                    f.vel[i] = sub(f.vel[i], muls(nrm, vn));
+                    if (!bd.is_static) bd.sleeping = false;
+                    bd.apply_impulse(muls(nrm, vn * f.mass[i]), pt);
+                    if (f.cfg.wall_friction > 0.0f) {
+                        float mu = rclamp<float>(f.cfg.wall_friction, 0.0f, 1.0f);
+                        V3<float> vt = f.vel[i];
+                        V3<float> wv = bd.vel;
+                        V3<float> wvt = sub(wv, muls(nrm, dot(wv, nrm)));
+                        V3<float> dv = muls(sub(wvt, vt), mu);
+                        f.vel[i] = add(f.vel[i], dv);
+                        bd.apply_impulse(muls(V3<float>{-dv.x, -dv.y, -dv.z}, f.mass[i]), pt);
+                    }
+                }
+            }
+        }
+        f.rebuild_grid();
+        const float drag = rmax<float>(f.cfg.drag, 0.0f);
+        for (RigidO &bd : bodies) {
+            if (bd.is_static) continue;
+            surface_samples_o(bd, samp);
+            if (samp.empty()) continue;
+            bool wet = false;
+            for (const SampleO &sm : samp) {
+                float p = sample_pressure(sm.point);
+                if (p <= 0.0f) continue;
+                wet = true;
+                V3<float> r = sub(sm.point, bd.pos);
+                V3<float> vrel = sub(add(bd.vel, crossf(bd.ang, r)), V3<float>{0, 0, 0});
+                float speed = len(vrel);
+                V3<float> force = muls(V3<float>{-sm.normal.x, -sm.normal.y, -sm.normal.z}, p * sm.area);
+                if (speed > kEps) force = sub(force, muls(vrel, drag * sm.area * speed));
+                bd.apply_force(force, sm.point, dt);
+            }
+            if (wet) bd.sleeping = false;
+        }
+    }
+
+    // World3::step_inner (world.rs:2766-2812) without the rigid solver
+    void step(float dt, bool jacobi) {
+        for (RigidO &bd : bodies)
+            if (!bd.is_static && !bd.sleeping) bd.vel = add(bd.vel, muls(g, bd.gravity_scale * dt));
+        if (f.cfg.enabled && f.n > 0 && dt > 0.0f) {
+            statics.clear();
+            for (const RigidO &bd : bodies)
+                if (bd.is_static) statics.push_back(bd.as_body());
+            uint32_t sub = f.cfl_substeps(dt);
+            float sdt = dt / (float)sub;
+            for (uint32_t k = 0; k < sub; ++k)
+                f.substep(g, sdt, jacobi, statics.data(), (uint32_t)statics.size(), nullptr, nullptr, nullptr);
+        }
+        for (RigidO &bd : bodies) bd.integrate(dt);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -709,6 +1101,151 @@ typedef struct OracleBody3 {
     float px, py, pz;
     float a, b, c;
 } OracleBody3;
+
+// A body of the coupled world step: shape (0 sphere, 1 box, 2 capsule),
+// density (the mass properties follow as RigidBody3::set_shape), static
+// flag and the motion state (rotation x, y, z, w), read and written back.
+typedef struct OracleRigid3 {
+    uint32_t shape;
+    float a, b, c;
+    float density;
+    uint32_t is_static;
+    float pos[3], rot[4], vel[3], ang[3];
+    float linear_damping, angular_damping, gravity_scale;
+} OracleRigid3;
+
+static RigidO rigid_in(const OracleRigid3 &o) {
+    RigidO r;
+    r.shape = o.shape;
+    r.a = o.a;
+    r.b = o.shape == 0 ? 0.0f : o.b;
+    r.c = o.shape == 1 ? o.c : 0.0f;
+    r.is_static = o.is_static != 0;
+    r.pos = {o.pos[0], o.pos[1], o.pos[2]};
+    r.rot = {o.rot[0], o.rot[1], o.rot[2], o.rot[3]};
+    r.vel = {o.vel[0], o.vel[1], o.vel[2]};
+    r.ang = {o.ang[0], o.ang[1], o.ang[2]};
+    r.linear_damping = o.linear_damping;
+    r.angular_damping = o.angular_damping;
+    r.gravity_scale = o.gravity_scale;
+    r.shape_mass(o.density);
+    return r;
+}
+
+// World3::step (world.rs:2670-2697, 2766-2812) for `steps` steps of dt
+// without the rigid contact solver: the gravity kick of the dynamic bodies,
+// the fluid (fluid3.rs:591-607) coupled to every body (fluid3.rs:1141-1213),
+// the bodies' integration.  Particles and bodies are updated in place.
+// Returns the substeps of the last step.
+int oracle3_world_run_f32(uint32_t n, float *pos, float *vel, const float *mass, const uint8_t *phase,
+                          const FluidConfig3Ffi *cfg, const float *g3, OracleRigid3 *bodies, uint32_t nbodies,
+                          float dt, uint32_t steps, int jacobi) {
+    CoupledWorld w(*cfg);
+    w.f.load(n, pos, vel, mass, phase, nullptr);
+    w.g = {g3[0], g3[1], g3[2]};
+    for (uint32_t k = 0; k < nbodies; ++k) w.bodies.push_back(rigid_in(bodies[k]));
+    uint32_t sub = 0;
+    for (uint32_t s = 0; s < steps; ++s) {
+        sub = (cfg->enabled && n > 0 && dt > 0.0f) ? w.f.cfl_substeps(dt) : 0;
+        w.step(dt, jacobi != 0);
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        pos[3 * i] = w.f.pos[i].x;
+        pos[3 * i + 1] = w.f.pos[i].y;
+        pos[3 * i + 2] = w.f.pos[i].z;
+        vel[3 * i] = w.f.vel[i].x;
+        vel[3 * i + 1] = w.f.vel[i].y;
+        vel[3 * i + 2] = w.f.vel[i].z;
+    }
+    for (uint32_t k = 0; k < nbodies; ++k) {
+        const RigidO &r = w.bodies[k];
+        OracleRigid3 &o = bodies[k];
+        o.pos[0] = r.pos.x;
+        o.pos[1] = r.pos.y;
+        o.pos[2] = r.pos.z;
+        o.rot[0] = r.rot.x;
+        o.rot[1] = r.rot.y;
+        o.rot[2] = r.rot.z;
+        o.rot[3] = r.rot.w;
+        o.vel[0] = r.vel.x;
+        o.vel[1] = r.vel.y;
+        o.vel[2] = r.vel.z;
+        o.ang[0] = r.ang.x;
+        o.ang[1] = r.ang.y;
+        o.ang[2] = r.ang.z;
+    }
+    return (int)sub;
+}
+
+void oracle3_cos_sin(float angle, float *c, float *s) { trig_cos_sin(angle, *c, *s); }
+void oracle3_quat_mul(const float *a, const float *b, float *out) {
+    Quat q = quat_mul({a[0], a[1], a[2], a[3]}, {b[0], b[1], b[2], b[3]});
+    out[0] = q.x;
+    out[1] = q.y;
+    out[2] = q.z;
+    out[3] = q.w;
+}
+void oracle3_quat_normalize(const float *a, float *out) {
+    Quat q = quat_normalize({a[0], a[1], a[2], a[3]});
+    out[0] = q.x;
+    out[1] = q.y;
+    out[2] = q.z;
+    out[3] = q.w;
+}
+void oracle3_quat_rotate(const float *q, const float *v, float *out) {
+    V3<float> r = quat_apply({q[0], q[1], q[2], q[3]}, {v[0], v[1], v[2]});
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+// Mass, inverse mass and the body-frame inverse inertia (9, columns) of a
+// dynamic body of the shape (RigidBody3::set_shape).
+void oracle3_body_mass(uint32_t shape, float a, float b, float c, float density, float *mass, float *inv_mass,
+                       float *inv_inertia9) {
+    OracleRigid3 o{};
+    o.shape = shape;
+    o.a = a;
+    o.b = b;
+    o.c = c;
+    o.density = density;
+    o.rot[3] = 1.0f;
+    RigidO r = rigid_in(o);
+    *mass = r.mass;
+    *inv_mass = r.inv_mass;
+    const V3<float> cols[3] = {r.inv_inertia.c0, r.inv_inertia.c1, r.inv_inertia.c2};
+    for (int k = 0; k < 3; ++k) {
+        inv_inertia9[3 * k] = cols[k].x;
+        inv_inertia9[3 * k + 1] = cols[k].y;
+        inv_inertia9[3 * k + 2] = cols[k].z;
+    }
+}
+// The body's buoyancy samples (fluid3.rs:375-465): points, outward
+// normals (3 floats each) and areas, up to max; returns the count.
+uint32_t oracle3_surface_samples(const OracleRigid3 *body, float *points, float *normals, float *areas, uint32_t max) {
+    RigidO r = rigid_in(*body);
+    std::vector<SampleO> ss;
+    surface_samples_o(r, ss);
+    for (uint32_t k = 0; k < ss.size() && k < max; ++k) {
+        points[3 * k] = ss[k].point.x;
+        points[3 * k + 1] = ss[k].point.y;
+        points[3 * k + 2] = ss[k].point.z;
+        normals[3 * k] = ss[k].normal.x;
+        normals[3 * k + 1] = ss[k].normal.y;
+        normals[3 * k + 2] = ss[k].normal.z;
+        areas[k] = ss[k].area;
+    }
+    return (uint32_t)ss.size();
+}
+// sample_pressure (fluid3.rs:345-369) at a point for a given particle
+// state (density and pressure as the density pass left them).
+float oracle3_sample_pressure(uint32_t n, const float *pos, const float *mass, const float *density,
+                              const float *pressure, const FluidConfig3Ffi *cfg, const float *point3) {
+    CoupledWorld w(*cfg);
+    w.f.load(n, pos, nullptr, mass, nullptr, density);
+    for (uint32_t i = 0; i < n; ++i) w.f.pressure[i] = pressure[i];
+    w.f.rebuild_grid();
+    return w.sample_pressure({point3[0], point3[1], point3[2]});
+}
 
 // One f32 substep in place (fluid3.rs:990-1184).  jacobi = 0 reproduces the
 // reference's Gauss-Seidel integrate.  Optional outputs (null to skip):

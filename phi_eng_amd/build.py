@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libphieng_hip.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["sph_kernels.hip", "engine.hip", "world3.hip", "batch3.hip", "surrogate.hip", "surrogate_step.hip", "resample.hip", "snapshot.hip", "verlet.hip"]
+SOURCES = ["sph_kernels.hip", "engine.hip", "world3.hip", "batch3.hip", "surrogate.hip", "surrogate_step.hip", "resample.hip", "snapshot.hip", "verlet.hip", "coupling.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PHIENG_ARCH", "gfx950")

@@ -98,7 +98,8 @@ void make_ghosts(const SceneHost &h, std::vector<float4> &out) {
         if (out.size() >= kMaxGhosts) throw std::runtime_error("boundary ghosts exceed 4,194,304");
         out.push_back(make_float4(x + b.px, y + b.py, z + b.pz, gm));
     };
-    for (const BodyDev &b : h.bodies) {  // the C ABI only holds static bodies
+    for (const BodyDev &b : h.bodies) {
+        if (b.flags & BODY_DYNAMIC) continue;  // ghosts of static bodies only (fluid3.rs:724)
         if (h.dims == 2) {
             if (b.shape == 0u) {  // circle ring, fluid.rs:673-688
                 const uint32_t m = runs((2.0f * kPiF * b.a) / spacing, 0.0f);
@@ -195,6 +196,7 @@ void choose_domain(SceneDev &s, const int32_t *pb, uint32_t np, const std::vecto
     int64_t ulo[3] = {dlo[0], dlo[1], dlo[2]}, uhi[3] = {dhi[0], dhi[1], dhi[2]};
     const float h = s.h_grid;
     for (const BodyDev &b : bodies) {
+        if (b.flags & BODY_DYNAMIC) continue;  // they move; the domain follows the particles
         float ex = b.a, ey = b.b, ez = b.c;
         if (b.shape == 0u) ey = ez = b.a;
         if (b.shape == 3u) ez = 0.0f;

@@ -14,8 +14,10 @@
 // switches.  Every feature of a higher schema (thermal, collision tiers,
 // nuclear, sensors, quantum, motors, electromagnetism) lies outside the
 // GPU fluid path, so a file of schema 8 or more is refused, as are
-// dynamic or rotated bodies, hulls, joints and contacts.  Fields the GPU
+// rotated static bodies, dynamic bodies of a 2D world, hulls, joints and
+// contacts.  Fields the GPU
 // step does not read (solver, validation, CCD, ...) are kept verbatim.
+// Dynamic bodies of a 3D world keep their rotation and velocities.
 // Load resets density to rest and pressure to 0 (apply_fluid3,
 // :4545-4615), which the GPU step recomputes anyway, and gives a particle
 // of mass 0 the config's particle_mass.  Unlike the reference, a file may
@@ -275,7 +277,7 @@ std::string to_json(const SnapWorld &w) {
         o.item();
         o.open('{');
         o.ku("id", k);
-        o.kb("is_static", true);
+        o.kb("is_static", b.is_static);
         o.kb("is_sleeping", b.sleeping);
         o.kf("linear_damping", b.linear_damping);
         o.kf("angular_damping", b.angular_damping);
@@ -313,13 +315,22 @@ std::string to_json(const SnapWorld &w) {
         o.close('}');
         o.kv3("position", b.pos);
         o.key("rotation");
-        // 2D: the columns of Mat2::IDENTITY (body2_dto); 3D: the identity quaternion
-        o.s += w.dims == 2 ? "[1.0, 0.0, 0.0, 1.0]" : "[0.0, 0.0, 0.0, 1.0]";
-        o.kv3("velocity", zero3);
+        // 2D: the columns of Mat2::IDENTITY (body2_dto); 3D: the quaternion x, y, z, w
+        if (w.dims == 2 || b.is_static) {
+            o.s += w.dims == 2 ? "[1.0, 0.0, 0.0, 1.0]" : "[0.0, 0.0, 0.0, 1.0]";
+        } else {
+            o.s += '[';
+            for (int q = 0; q < 4; ++q) {
+                if (q) o.s += ", ";
+                o.f(b.rot[q]);
+            }
+            o.s += ']';
+        }
+        o.kv3("velocity", w.dims == 2 ? zero3 : b.vel);
         if (w.dims == 2)
             o.kf("angular_velocity", 0.0f);
         else
-            o.kv3("angular_velocity", zero3);
+            o.kv3("angular_velocity", b.angvel);
         if (b.gravity_scale != 1.0f) o.kf("gravity_scale", b.gravity_scale);
         if (b.ambient_scale != 1.0f) o.kf("ambient_scale", b.ambient_scale);
         o.close('}');
@@ -596,15 +607,23 @@ void add_body(SnapWorld &w, const BodyIn &in, SnapBody b) {
     check_finite(b.restitution, "material.restitution");
     check_finite(b.density, "material.density");
     if (in.id != w.bodies.size()) fail("body id out of order");
-    if (!in.is_static) fail("a dynamic body: two-way coupling is not implemented on the GPU path");
+    if (!in.is_static && w.dims == 2) fail("a dynamic body in a 2D world: the GPU path couples 3D worlds only");
     if (!in.has_shape) fail("a body without a shape");
     if (in.shape_tag > (w.dims == 2 ? 1u : 2u)) fail("a hull, mesh or polygon body: not a GPU boundary shape");
     const bool identity = w.dims == 2 ? in.rot[0] == 1.0f && in.rot[1] == 0.0f && in.rot[2] == 0.0f && in.rot[3] == 1.0f
                                       : in.rot[0] == 0.0f && in.rot[1] == 0.0f && in.rot[2] == 0.0f && in.rot[3] > 0.0f;
-    if (!identity) fail("a rotated static body: the GPU boundary has identity rotations only");
-    if (in.vel[0] != 0.0f || in.vel[1] != 0.0f || in.vel[2] != 0.0f || in.angvel[0] != 0.0f ||
-        in.angvel[1] != 0.0f || in.angvel[2] != 0.0f)
+    if (in.is_static && !identity) fail("a rotated static body: the GPU boundary has identity rotations only");
+    if (in.is_static && (in.vel[0] != 0.0f || in.vel[1] != 0.0f || in.vel[2] != 0.0f || in.angvel[0] != 0.0f ||
+                         in.angvel[1] != 0.0f || in.angvel[2] != 0.0f))
         fail("a moving static body");
+    b.is_static = in.is_static;
+    if (!in.is_static) {
+        for (int k = 0; k < 4; ++k) b.rot[k] = in.rot[k];
+        for (int k = 0; k < 3; ++k) {
+            b.vel[k] = in.vel[k];
+            b.angvel[k] = in.angvel[k];
+        }
+    }
     for (int k = 0; k < 3; ++k) check_finite(in.dims[k], "shape");
     const bool ok = in.shape_tag == 0   ? in.dims[0] > 0.0f
                     : in.shape_tag == 1 ? in.dims[0] > 0.0f && in.dims[1] > 0.0f && (w.dims == 2 || in.dims[2] > 0.0f)
@@ -1026,7 +1045,7 @@ std::vector<uint8_t> to_binary(const SnapWorld &w) {
     for (size_t k = 0; k < w.bodies.size(); ++k) {
         const SnapBody &b = w.bodies[k];
         o.u32((uint32_t)k);
-        o.u8(1u | (b.sleeping ? 2u : 0u));
+        o.u8((b.is_static ? 1u : 0u) | (b.sleeping ? 2u : 0u));
         o.f32(b.linear_damping);
         o.f32(b.angular_damping);
         o.f32(b.friction);
@@ -1051,16 +1070,19 @@ std::vector<uint8_t> to_binary(const SnapWorld &w) {
         }
         o.v3(b.pos);
         // rotation: 3D the quaternion x, y, z, w; 2D the columns of the matrix
-        o.f32(w.dims == 2 ? 1.0f : 0.0f);
-        o.f32(0.0f);
-        o.f32(0.0f);
-        o.f32(1.0f);
         const float z3[3] = {0.0f, 0.0f, 0.0f};
-        o.v3(z3);  // velocity
-        if (w.dims == 2)
+        if (w.dims == 2) {
+            o.f32(1.0f);
+            o.f32(0.0f);
+            o.f32(0.0f);
+            o.f32(1.0f);
+            o.v3(z3);     // velocity
             o.f32(0.0f);  // angular velocity
-        else
-            o.v3(z3);
+        } else {
+            for (int q = 0; q < 4; ++q) o.f32(b.rot[q]);
+            o.v3(b.vel);
+            o.v3(b.angvel);
+        }
         if (schema >= kSchemaGravity) o.f32(b.gravity_scale);
         if (schema >= kSchemaAmbient) o.f32(b.ambient_scale);
     }

@@ -15,11 +15,19 @@
 #define PH_TILE 256             // particles per tile (= threads per block)
 #define PH_MAX_BODIES 64
 
-// Static collision body (identity rotation; the C ABI cannot rotate one).
+#define PH_HD __host__ __device__ __forceinline__
+
+// Collision body.  Static bodies have the identity rotation (the C ABI
+// cannot rotate one); a dynamic body carries its rotation (x, y, z, w),
+// flagged BODY_ROTATED once it is not exactly the identity.
+#define BODY_DYNAMIC 1u
+#define BODY_ROTATED 2u
 struct BodyDev {
     uint32_t shape;  // 0 sphere, 1 box, 2 capsule (ffi.rs:3193-3195); 3 box of a 2D world
     float px, py, pz;
     float a, b, c;
+    float qx, qy, qz, qw;
+    uint32_t flags;
 };
 
 // Per-scene parameters: grid domain, bucket range, particle range and the
@@ -56,13 +64,13 @@ struct TileDev {
 };
 
 // Rust f32::max: a NaN operand loses.
-__device__ __forceinline__ float rmaxf(float a, float b) {
+PH_HD float rmaxf(float a, float b) {
     if (a != a) return b;
     if (b != b) return a;
     return a > b ? a : b;
 }
 // Rust f32::clamp: NaN passes through.
-__device__ __forceinline__ float rclampf(float v, float lo, float hi) {
+PH_HD float rclampf(float v, float lo, float hi) {
     if (v < lo) v = lo;
     if (v > hi) v = hi;
     return v;
@@ -74,11 +82,11 @@ __device__ __forceinline__ int32_t sat_i32(float v) {
     if (v < -2147483648.0f) return INT32_MIN;
     return (int32_t)v;
 }
-__device__ __forceinline__ bool finite3(float x, float y, float z) {
-    return isfinite(x) && isfinite(y) && isfinite(z);
+PH_HD bool finite3(float x, float y, float z) {
+    return __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z);
 }
 // glam Vec3::length_squared: (x*x + y*y) + z*z.
-__device__ __forceinline__ float dot3(float x, float y, float z) { return x * x + y * y + z * z; }
+PH_HD float dot3(float x, float y, float z) { return x * x + y * y + z * z; }
 
 // fluid3.rs:220-228 (coefficient precomputed on the host in f32).
 __device__ __forceinline__ float poly6(float r2, const SceneDev &s) {
@@ -179,10 +187,51 @@ __device__ __forceinline__ void for_each_candidate(const SceneDev &s, const uint
             }
 }
 
-// fluid3.rs:471-585 for identity-rotation shapes.  Returns false when
-// separated.
-__device__ __forceinline__ bool collide(float px, float py, float pz, float pr, const BodyDev &b,
-                                        float &nx, float &ny, float &nz, float &depth) {
+// glam 0.29 Quat * Vec3 (the SSE2 form the reference's x86-64 build runs,
+// crates/dynamics via glam's Quat): v (w w - b.b) + b (2 (b.v)) + (b x v)(2 w),
+// b = (x, y, z), summed left to right.
+PH_HD void quat_rotate(float qx, float qy, float qz, float qw, float vx, float vy, float vz, float &ox, float &oy,
+                       float &oz) {
+    const float b2 = qx * qx + qy * qy + qz * qz;
+    const float s0 = qw * qw - b2;
+    const float d2 = (qx * vx + qy * vy + qz * vz) * 2.0f;
+    const float w2 = qw * 2.0f;
+    const float cx = qy * vz - qz * vy, cy = qz * vx - qx * vz, cz = qx * vy - qy * vx;
+    ox = vx * s0 + qx * d2 + cx * w2;
+    oy = vy * s0 + qy * d2 + cy * w2;
+    oz = vz * s0 + qz * d2 + cz * w2;
+}
+
+// The body frame: local = R^-1 (p - c), and back (world = R l + c for a
+// point, R l for a direction).  Without BODY_ROTATED the rotation is the
+// identity, whose quaternion product reproduces its operand (up to the
+// sign of zero components), so it is skipped.
+struct BodyFrame {
+    const BodyDev &b;
+    PH_HD bool rot() const { return (b.flags & BODY_ROTATED) != 0u; }
+    PH_HD void to_local(float px, float py, float pz, float &lx, float &ly, float &lz) const {
+        lx = px - b.px;
+        ly = py - b.py;
+        lz = pz - b.pz;
+        if (rot()) quat_rotate(-b.qx, -b.qy, -b.qz, b.qw, lx, ly, lz, lx, ly, lz);
+    }
+    PH_HD void dir(float &x, float &y, float &z) const {
+        if (rot()) quat_rotate(b.qx, b.qy, b.qz, b.qw, x, y, z, x, y, z);
+    }
+    PH_HD void point(float lx, float ly, float lz, float &x, float &y, float &z) const {
+        dir(lx, ly, lz);
+        x = lx + b.px;
+        y = ly + b.py;
+        z = lz + b.pz;
+    }
+};
+
+// fluid3.rs:471-585 (collide_particle_body): outward normal, depth and,
+// when cp is given, the contact point on the body surface (cp[0..2]).
+// Returns false when separated.
+PH_HD bool collide(float px, float py, float pz, float pr, const BodyDev &b, float &nx, float &ny, float &nz,
+                   float &depth, float *cp = nullptr) {
+    const BodyFrame f{b};
     if (b.shape == 0u) {  // sphere
         const float dx = px - b.px, dy = py - b.py, dz = pz - b.pz;
         const float dist = sqrtf(dot3(dx, dy, dz));
@@ -197,6 +246,11 @@ __device__ __forceinline__ bool collide(float px, float py, float pz, float pr, 
             nz = 0.0f;
         }
         depth = b.a + pr - dist;
+        if (cp) {
+            cp[0] = b.px + nx * b.a;
+            cp[1] = b.py + ny * b.a;
+            cp[2] = b.pz + nz * b.a;
+        }
         return true;
     }
     if (b.shape == 3u) {  // 2D box in the z = 0 plane (fluid.rs:481-517), identity rotation
@@ -226,7 +280,8 @@ __device__ __forceinline__ bool collide(float px, float py, float pz, float pr, 
         return true;
     }
     if (b.shape == 1u) {  // box
-        const float lx = px - b.px, ly = py - b.py, lz = pz - b.pz;
+        float lx, ly, lz;
+        f.to_local(px, py, pz, lx, ly, lz);
         const float cxl = rclampf(lx, -b.a, b.a), cyl = rclampf(ly, -b.b, b.b), czl = rclampf(lz, -b.c, b.c);
         const float dx = lx - cxl, dy = ly - cyl, dz = lz - czl;
         if (dot3(dx, dy, dz) > PH_EPS_SQ) {
@@ -235,7 +290,9 @@ __device__ __forceinline__ bool collide(float px, float py, float pz, float pr, 
             nx = dx / dist;
             ny = dy / dist;
             nz = dz / dist;
+            f.dir(nx, ny, nz);
             depth = pr - dist;
+            if (cp) f.point(cxl, cyl, czl, cp[0], cp[1], cp[2]);
             return true;
         }
         // Centre inside: least-penetration axis, always +axis (fluid3.rs:514-520).
@@ -251,10 +308,13 @@ __device__ __forceinline__ bool collide(float px, float py, float pz, float pr, 
             nz = 1.0f;
             depth = pr + ez;
         }
+        f.dir(nx, ny, nz);
+        if (cp) f.point(cxl, cyl, czl, cp[0], cp[1], cp[2]);
         return true;
     }
     if (b.shape == 2u) {  // capsule along local y
-        const float lx = px - b.px, ly = py - b.py, lz = pz - b.pz;
+        float lx, ly, lz;
+        f.to_local(px, py, pz, lx, ly, lz);
         const float sy = rclampf(ly, -b.b, b.b);
         const float dx = lx - 0.0f, dy = ly - sy, dz = lz - 0.0f;
         if (dot3(dx, dy, dz) > PH_EPS_SQ) {
@@ -263,39 +323,59 @@ __device__ __forceinline__ bool collide(float px, float py, float pz, float pr, 
             nx = dx / dist;
             ny = dy / dist;
             nz = dz / dist;
+            f.dir(nx, ny, nz);
             depth = b.a + pr - dist;
+            if (cp) {
+                // seg + diff.normalize_or_zero() * r (glam: diff * (1 / |diff|))
+                const float rcp = 1.0f / dist;
+                const bool ok = __builtin_isfinite(rcp) && rcp > 0.0f;
+                const float ux = ok ? dx * rcp : 0.0f, uy = ok ? dy * rcp : 0.0f, uz = ok ? dz * rcp : 0.0f;
+                f.point(0.0f + ux * b.a, sy + uy * b.a, 0.0f + uz * b.a, cp[0], cp[1], cp[2]);
+            }
             return true;
         }
+        float rx = 1.0f, ry = 0.0f, rz = 0.0f;
         if (dot3(lx, 0.0f, lz) > PH_EPS_SQ) {
             const float rcp = 1.0f / sqrtf(dot3(lx, 0.0f, lz));  // glam normalize_or_zero
-            if (isfinite(rcp) && rcp > 0.0f) {
-                nx = lx * rcp;
-                ny = 0.0f * rcp;
-                nz = lz * rcp;
+            if (__builtin_isfinite(rcp) && rcp > 0.0f) {
+                rx = lx * rcp;
+                ry = 0.0f * rcp;
+                rz = lz * rcp;
             } else {
-                nx = ny = nz = 0.0f;
+                rx = ry = rz = 0.0f;
             }
-        } else {
-            nx = 1.0f;
-            ny = 0.0f;
-            nz = 0.0f;
         }
+        nx = rx;
+        ny = ry;
+        nz = rz;
+        f.dir(nx, ny, nz);
         depth = pr + b.a - sqrtf(dot3(dx, dy, dz));
+        if (cp) f.point(0.0f + rx * b.a, sy + ry * b.a, 0.0f + rz * b.a, cp[0], cp[1], cp[2]);
         return true;
     }
     return false;
 }
 
-// fluid3.rs:1141-1179 for one particle against the scene's static bodies.
+// fluid3.rs:1141-1179 for one particle against the scene's bodies, static
+// ones (their impulses are no-ops, body.rs:1349-1352, and their velocity
+// is zero).  A dynamic body in contact (depth > 0) takes this particle's
+// impulses in particle order and its friction reads the body's running
+// velocity: with `dynamic` given the loop stops there and sets *dynamic
+// (the caller keeps the state from before the loop and hands the particle
+// to the ordered host pass); without it, a dynamic body acts as a static one.
 __device__ __forceinline__ void clamp_to_bodies(const SceneDev &s, const BodyDev *__restrict__ bodies,
                                                 float &x, float &y, float &z, float &vx, float &vy,
-                                                float &vz) {
+                                                float &vz, bool *dynamic = nullptr) {
     const float pr = rmaxf(s.particle_radius, 0.0f);
     for (uint32_t k = 0; k < s.body_count; ++k) {
         const BodyDev b = bodies[s.body_begin + k];
         float nx, ny, nz, depth;
         if (!collide(x, y, z, pr, b, nx, ny, nz, depth)) continue;
         if (depth <= 0.0f) continue;
+        if (dynamic && (b.flags & BODY_DYNAMIC)) {
+            *dynamic = true;
+            return;
+        }
         x = x + nx * depth;
         y = y + ny * depth;
         z = z + nz * depth;
@@ -502,5 +582,10 @@ struct AuxFuse {
     float4 *aux_out = nullptr;        // null: nothing fused
     float4 *rec_out = nullptr;        // may be null
     float2 *tile_err = nullptr;
+    // particles in contact with a dynamic body: defer[0] counts them,
+    // defer[1 + k] (k < defer_cap) holds their slots; their state is left
+    // as before the boundary clamp (null: no dynamic bodies)
+    uint32_t *defer = nullptr;
+    uint32_t defer_cap = 0;
 };
 

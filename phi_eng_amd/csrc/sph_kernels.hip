@@ -599,7 +599,25 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             y = pi.y;
             z = pi.z;
         }
-        clamp_to_bodies(s, bodies, x, y, z, ux, uy, uz);
+        if (af.defer) {
+            // a particle touching a dynamic body leaves the clamp to the
+            // ordered host pass, from this state
+            const float x0 = x, y0 = y, z0 = z, u0 = ux, v0 = uy, w0 = uz;
+            bool dyn = false;
+            clamp_to_bodies(s, bodies, x, y, z, ux, uy, uz, &dyn);
+            if (dyn) {
+                x = x0;
+                y = y0;
+                z = z0;
+                ux = u0;
+                uy = v0;
+                uz = w0;
+                const uint32_t k = atomicAdd(af.defer, 1u);
+                if (k < af.defer_cap) af.defer[1 + k] = i;
+            }
+        } else {
+            clamp_to_bodies(s, bodies, x, y, z, ux, uy, uz);
+        }
         if (VL) {
             const float d2 = dot3(x - pi.x, y - pi.y, z - pi.z);
             dmax2 = d2 == d2 ? d2 : 0.0f;  // a non-finite particle is nobody's neighbour

@@ -15,14 +15,18 @@
 
 #include "../../include/phieng_fluid2.h"
 #include "../../include/phieng_fluid3.h"
+#include "coupling.h"
 #include "engine.h"
 #include "resample.h"
+#include "rigid3.h"
 #include "sph_kernels.h"
 #include "surrogate.h"
 #include "surrogate_step.h"
 #include "world3_internal.h"
 
 static_assert(sizeof(FluidConfig3Ffi) == 112, "FluidConfig3Ffi must match ffi.rs:743-767");
+static_assert(sizeof(BodyDesc3D) == 44, "BodyDesc3D must match ffi.rs:3765-3784");
+static_assert(sizeof(BodyDesc2D) == 36, "BodyDesc2D must match ffi.rs:1205-1222");
 static_assert(offsetof(FluidConfig3Ffi, max_substeps) == 32, "layout");
 static_assert(offsetof(FluidConfig3Ffi, adaptive_resolution) == 36, "layout");
 static_assert(offsetof(FluidConfig3Ffi, boundary_particles) == 48, "layout");
@@ -123,6 +127,20 @@ struct World3 {
     float g[3] = {0.0f, -9.81f, 0.0f};  // World3::new (world.rs)
     std::vector<BodyDev> bodies;
     std::vector<SnapBody> body_meta;  // material, damping, ... of each body, for snapshots
+    // Rigid state of each body (a static body's never changes); the
+    // dynamic ones are advanced by the world step and coupled both ways to
+    // the fluid (fluid3.rs:1141-1213).
+    std::vector<ph::Rigid3> rigid;
+    uint32_t n_dynamic = 0;
+    // coupling buffers: deferred slots, their staging, sample points and
+    // pressure terms
+    ph::DBuf<uint32_t> cp_defer, cp_count;
+    ph::DBuf<float4> cp_stage, cp_bound, cp_spos;
+    ph::DBuf<uint2> cp_range;
+    ph::DBuf<int4> cp_scell;
+    ph::DBuf<uint4> cp_rec;
+    uint32_t cp_rec_cap = 4096;
+    uint64_t n_deferred = 0, n_terms = 0;
     SnapMeta meta;                    // world fields the GPU step does not read, for snapshots
     float cell_size = 1.0f;           // a World2's broad-phase cell, kept for its snapshots
     int32_t next_body_id = 0;
@@ -184,6 +202,7 @@ struct World3 {
         mixf(cfg.particle_mass);
         mixf(cfg.rest_density);
         for (const BodyDev &b : bodies) {
+            if (b.flags & BODY_DYNAMIC) continue;  // static bodies only (fluid3.rs:673)
             mixf(b.px);
             mixf(b.py);
             if (dims == 2) {
@@ -270,9 +289,14 @@ struct World3 {
         s.body_count = (uint32_t)bodies.size();
         if (e.bodies.size() != bodies.size() ||
             (bodies.size() && std::memcmp(e.bodies.data(), bodies.data(), sizeof(BodyDev) * bodies.size()))) {
+            // a dynamic body's move leaves the domain (it follows the particles)
+            bool statics_same = e.bodies.size() == bodies.size();
+            for (size_t k = 0; statics_same && k < bodies.size(); ++k)
+                if (!(bodies[k].flags & BODY_DYNAMIC))
+                    statics_same = std::memcmp(&e.bodies[k], &bodies[k], sizeof(BodyDev)) == 0;
             e.bodies = bodies;
             e.bodies_dirty = true;
-            domain_dirty = true;
+            if (!statics_same) domain_dirty = true;
         }
         if (before.h_grid != s.h_grid) domain_dirty = true;
         // the tiles cover [part_begin, part_end): rebuilt when the count
@@ -321,7 +345,117 @@ struct World3 {
 
     // World3::step fluid phase (world.rs:2670-2697, 2809-2812) and
     // FluidState3::step_fluid (fluid3.rs:591-607).
+    // phieng_world3_add_body (ffi.rs:3703-3760): the shape and density
+    // checks, then a static body or a dynamic one with the mass properties
+    // of RigidBody3::set_shape.  Dynamic bodies are refused while the fluid
+    // runs the learned surrogate or adaptive resolution (their substeps do
+    // not couple).
+    int32_t add_body3(float px, float py, float pz, uint32_t shape_tag, float a, float b, float c, float density,
+                      uint8_t is_static, float friction, float restitution, const char *where) {
+        if (!std::isfinite(px) || !std::isfinite(py) || !std::isfinite(pz)) return -1;
+        BodyDev bd{shape_tag, px, py, pz, a, b, c};
+        switch (shape_tag) {
+            case PHIENG_SHAPE3_SPHERE:
+                if (!(a > 0.0f && std::isfinite(a))) return -1;
+                bd.b = bd.c = 0.0f;
+                break;
+            case PHIENG_SHAPE3_BOX:
+                if (!(a > 0.0f && b > 0.0f && c > 0.0f && std::isfinite(a) && std::isfinite(b) && std::isfinite(c)))
+                    return -1;
+                break;
+            case PHIENG_SHAPE3_CAPSULE:
+                if (!(a > 0.0f && b > 0.0f && std::isfinite(a) && std::isfinite(b))) return -1;
+                bd.c = 0.0f;
+                break;
+            default:
+                return -1;
+        }
+        if (!std::isfinite(density) || density <= 0.0f) return -1;
+        if (!is_static && (sur || col || cfg.adaptive_resolution)) {
+            set_error(where, "a dynamic body with the learned surrogate or adaptive resolution: those substeps do "
+                             "not couple to bodies");
+            return -1;
+        }
+        if (bodies.size() >= PH_MAX_BODIES) {
+            set_error(where, "too many bodies");
+            return -1;
+        }
+        ph::Rigid3 r;
+        r.is_static = is_static != 0;
+        r.pos = ph::f3(px, py, pz);
+        r.set_shape(shape_tag, bd.a, bd.b, bd.c, density);
+        if (!r.is_static) {
+            bd = r.dev();
+            ++n_dynamic;
+        }
+        bodies.push_back(bd);
+        rigid.push_back(r);
+        SnapBody sb = snap_body(bd, density, friction, restitution);
+        sb.is_static = r.is_static;
+        body_meta.push_back(sb);
+        return next_body_id++;
+    }
+
+    // phieng_world_add_body (ffi.rs:1149-1203) for a World2: a circle (a
+    // disc in the z = 0 plane) or a box.  Friction and restitution are
+    // sanitised, never rejected, by set_material (body.rs:627-637), and
+    // static walls take wall_friction from the config; they are kept for
+    // snapshots.  A 2D world's bodies are static: the 2D fluid's coupling
+    // (fluid.rs) is not built on the GPU path.
+    int32_t add_body2(float px, float py, uint32_t shape_tag, float a, float b, float density, uint8_t is_static,
+                      float friction, float restitution, const char *where) {
+        if (!std::isfinite(px) || !std::isfinite(py)) return -1;
+        BodyDev bd{0u, px, py, 0.0f, a, 0.0f, 0.0f};
+        switch (shape_tag) {
+            case PHIENG_SHAPE2_CIRCLE:
+                if (!(a > 0.0f && std::isfinite(a))) return -1;
+                bd.shape = 0u;
+                break;
+            case PHIENG_SHAPE2_BOX:
+                if (!(a > 0.0f && b > 0.0f && std::isfinite(a) && std::isfinite(b))) return -1;
+                bd.shape = 3u;
+                bd.b = b;
+                break;
+            default:
+                return -1;
+        }
+        if (!std::isfinite(density) || density <= 0.0f) return -1;
+        if (!is_static) {
+            set_error(where, "dynamic bodies in a 2D world: the GPU path couples 3D worlds only");
+            return -1;
+        }
+        if (bodies.size() >= PH_MAX_BODIES) {
+            set_error(where, "too many bodies");
+            return -1;
+        }
+        ph::Rigid3 r;
+        r.shape = bd.shape;
+        r.pos = ph::f3(px, py, 0.0f);
+        r.a = bd.a;
+        r.b = bd.b;
+        bodies.push_back(bd);
+        rigid.push_back(r);
+        body_meta.push_back(snap_body(bd, density, friction, restitution));
+        return next_body_id++;
+    }
+
     void step(float dt) {
+        // World3::step_inner (world.rs:2766-2812) as far as bodies move
+        // without the rigid contact solver: the gravity kick, the fluid
+        // with its two-way coupling, the integration.
+        if (n_dynamic) {
+            const ph::F3 gv = ph::f3(g[0], g[1], g[2]);
+            for (ph::Rigid3 &r : rigid) r.gravity_kick(gv, dt);
+            for (size_t k = 0; k < rigid.size(); ++k)
+                if (!rigid[k].is_static) bodies[k] = rigid[k].dev();
+        }
+        step_fluid(dt);
+        if (n_dynamic)
+            for (ph::Rigid3 &r : rigid) r.integrate(dt);
+        time += dt;
+    }
+
+    void step_fluid(float dt) {
         if (cfg.enabled && n > 0 && dt > 0.0f) {
             eng.init_device();
             (void)hipGetLastError();  // a failed call before this step has already reported
@@ -336,6 +470,10 @@ struct World3 {
             if (sub != 1) sync_scene(sdt);
             for (uint32_t k = 0; k < sub; ++k) {
                 if (cfg.boundary_particles || eng.scenes[0].ghost_count) update_ghosts();
+                if (n_dynamic) {
+                    substep_coupled(sdt);
+                    continue;
+                }
                 if (cfg.adaptive_resolution) {
                     substep_adaptive(sdt);
                     continue;
@@ -360,7 +498,203 @@ struct World3 {
                 ++step_index;
             }
         }
-        time += dt;
+    }
+
+    // One plain substep coupled to the dynamic bodies.  The force pass
+    // leaves particles touching a dynamic body unclamped; the host replays
+    // the reference's clamp loop for them in particle order (the bodies'
+    // impulses land in that order and wall friction reads the running body
+    // velocity), then the pressure is sampled on each dynamic body's
+    // surface from the clamped state (the reference's second grid rebuild)
+    // and the buoyancy and drag forces are applied in sample order.
+    void substep_coupled(float sdt) {
+        ph::Engine &e = eng;
+        const size_t m = e.alloc_particles;
+        cp_defer.ensure(m + 1);
+        e.run(ph::KIND_FILL, [&] { return ph::launch_fill_u32(e.stream, cp_defer.p, 1u, 0u); });
+        e.sort_phase(nullptr);
+        AuxFuse af;
+        af.defer = cp_defer.p;
+        af.defer_cap = (uint32_t)m;
+        e.plain_phase(nullptr, &af);
+        e.finish_substep();
+        uint32_t cnt = 0;
+        PH_CK(hipMemcpyAsync(&cnt, cp_defer.p, sizeof(uint32_t), hipMemcpyDeviceToHost, e.stream));
+        PH_CK(hipStreamSynchronize(e.stream));
+        cnt = std::min<uint32_t>(cnt, n);
+        if (cnt) replay_clamp(cnt);
+        buoyancy(sdt);
+    }
+
+    // fluid3.rs:1141-1179 for the deferred particles, ascending index.
+    void replay_clamp(uint32_t cnt) {
+        ph::Engine &e = eng;
+        cp_stage.ensure(2 * (size_t)cnt);
+        e.run(ph::KIND_EXPORT, [&] {
+            return ph::launch_defer_gather(e.stream, cp_defer.p, cnt, e.posm[e.cs].p, e.vel[e.cs].p, e.tag[e.ct].p,
+                                           cp_stage.p);
+        });
+        std::vector<float4> st(2 * (size_t)cnt);
+        PH_CK(hipMemcpyAsync(st.data(), cp_stage.p, sizeof(float4) * st.size(), hipMemcpyDeviceToHost, e.stream));
+        PH_CK(hipStreamSynchronize(e.stream));
+        std::vector<std::pair<uint32_t, uint32_t>> order(cnt);
+        for (uint32_t k = 0; k < cnt; ++k) {
+            uint32_t tg;
+            std::memcpy(&tg, &st[2 * k + 1].w, 4);
+            order[k] = {tg & PH_ID_MASK, k};
+        }
+        std::sort(order.begin(), order.end());
+        const float pr = cfg.particle_radius > 0.0f ? cfg.particle_radius : 0.0f;
+        const float mu = std::min(std::max(cfg.wall_friction, 0.0f), 1.0f);
+        for (const auto &ok : order) {
+            float4 &P = st[2 * ok.second];
+            float4 &V = st[2 * ok.second + 1];
+            float x = P.x, y = P.y, z = P.z, vx = V.x, vy = V.y, vz = V.z;
+            const float mass = P.w;
+            for (size_t k = 0; k < bodies.size(); ++k) {
+                float nx, ny, nz, depth, cp[3];
+                if (!collide(x, y, z, pr, bodies[k], nx, ny, nz, depth, cp)) continue;
+                if (depth <= 0.0f) continue;
+                x = x + nx * depth;
+                y = y + ny * depth;
+                z = z + nz * depth;
+                const float vn = vx * nx + vy * ny + vz * nz;
+                if (!(vn < 0.0f)) continue;
+                vx = vx - nx * vn;
+                vy = vy - ny * vn;
+                vz = vz - nz * vn;
+                ph::Rigid3 &r = rigid[k];
+                const ph::F3 n3 = ph::f3(nx, ny, nz), point = ph::f3(cp[0], cp[1], cp[2]);
+                if (!r.is_static) r.sleeping = false;
+                r.apply_impulse(n3 * (vn * mass), point);
+                if (cfg.wall_friction > 0.0f) {
+                    const ph::F3 wv = r.vel;  // zero for a static body
+                    const float wd = wv.x * nx + wv.y * ny + wv.z * nz;
+                    const float wtx = wv.x - nx * wd, wty = wv.y - ny * wd, wtz = wv.z - nz * wd;
+                    const float dvx = (wtx - vx) * mu, dvy = (wty - vy) * mu, dvz = (wtz - vz) * mu;
+                    vx = vx + dvx;
+                    vy = vy + dvy;
+                    vz = vz + dvz;
+                    r.apply_impulse(-ph::f3(dvx, dvy, dvz) * mass, point);
+                }
+            }
+            P.x = x;
+            P.y = y;
+            P.z = z;
+            V.x = vx;
+            V.y = vy;
+            V.z = vz;
+        }
+        PH_CK(hipMemcpyAsync(cp_stage.p, st.data(), sizeof(float4) * st.size(), hipMemcpyHostToDevice, e.stream));
+        e.run(ph::KIND_EXPORT, [&] {
+            return ph::launch_defer_scatter(e.stream, cp_defer.p, cnt, cp_stage.p, e.posm[e.cs].p, e.vel[e.cs].p);
+        });
+        n_deferred += cnt;
+    }
+
+    // fluid3.rs:1180-1213: pressure sampled on each dynamic body's surface,
+    // then per sample (skipping dry ones) Archimedes' -n p A and the
+    // Morison drag -v_rel drag A |v_rel|, applied in order.
+    void buoyancy(float sdt) {
+        ph::Engine &e = eng;
+        std::vector<uint32_t> who;
+        std::vector<std::vector<ph::Sample3>> smp;
+        std::vector<float4> bound, spos;
+        std::vector<uint2> range;
+        std::vector<int4> scell;
+        const float hk = std::max(cfg.smoothing_length, PH_EPS);
+        auto cellf = [&](float v) {
+            const float q = std::floor(v / hk);
+            if (q != q) return 0;
+            if (q >= 2147483648.0f) return INT32_MAX;
+            if (q < -2147483648.0f) return INT32_MIN;
+            return (int32_t)q;
+        };
+        for (size_t k = 0; k < rigid.size(); ++k) {
+            if (rigid[k].is_static) continue;
+            std::vector<ph::Sample3> ss;
+            ph::surface_samples(rigid[k], ss);
+            if (ss.empty()) continue;
+            const float R = (rigid[k].bound() + hk) * 1.001f + 1e-6f;
+            bound.push_back(make_float4(rigid[k].pos.x, rigid[k].pos.y, rigid[k].pos.z, R * R));
+            range.push_back(make_uint2((uint32_t)spos.size(), (uint32_t)ss.size()));
+            for (const ph::Sample3 &q : ss) {
+                spos.push_back(make_float4(q.point.x, q.point.y, q.point.z, 0.0f));
+                scell.push_back(make_int4(cellf(q.point.x), cellf(q.point.y), cellf(q.point.z), 0));
+            }
+            who.push_back((uint32_t)k);
+            smp.push_back(std::move(ss));
+        }
+        if (who.empty()) return;
+        cp_bound.ensure(bound.size());
+        cp_range.ensure(range.size());
+        cp_spos.ensure(spos.size());
+        cp_scell.ensure(scell.size());
+        cp_count.ensure(1);
+        PH_CK(hipMemcpyAsync(cp_bound.p, bound.data(), sizeof(float4) * bound.size(), hipMemcpyHostToDevice, e.stream));
+        PH_CK(hipMemcpyAsync(cp_range.p, range.data(), sizeof(uint2) * range.size(), hipMemcpyHostToDevice, e.stream));
+        PH_CK(hipMemcpyAsync(cp_spos.p, spos.data(), sizeof(float4) * spos.size(), hipMemcpyHostToDevice, e.stream));
+        PH_CK(hipMemcpyAsync(cp_scell.p, scell.data(), sizeof(int4) * scell.size(), hipMemcpyHostToDevice, e.stream));
+        uint32_t want = 0;
+        for (;;) {
+            cp_rec.ensure(cp_rec_cap);
+            e.run(ph::KIND_FILL, [&] { return ph::launch_fill_u32(e.stream, cp_count.p, 1u, 0u); });
+            ph::SampleArgs a;
+            a.scenes = e.d_scenes.p;
+            a.n = n;
+            a.posm = e.posm[e.cs].p;
+            a.rq = e.rq.p;
+            a.tag = e.tag[e.ct].p;
+            a.bound = cp_bound.p;
+            a.range = cp_range.p;
+            a.nbodies = (uint32_t)bound.size();
+            a.spos = cp_spos.p;
+            a.scell = cp_scell.p;
+            a.rec = cp_rec.p;
+            a.count = cp_count.p;
+            a.cap = cp_rec_cap;
+            e.run(ph::KIND_EXPORT, [&] { return ph::launch_sample_terms(e.stream, a); });
+            PH_CK(hipMemcpyAsync(&want, cp_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost, e.stream));
+            PH_CK(hipStreamSynchronize(e.stream));
+            if (want <= cp_rec_cap) break;
+            cp_rec_cap = want + want / 2;  // the record store was too small: grow, run again
+        }
+        std::vector<uint4> rec(want);
+        if (want) {
+            PH_CK(hipMemcpyAsync(rec.data(), cp_rec.p, sizeof(uint4) * want, hipMemcpyDeviceToHost, e.stream));
+            PH_CK(hipStreamSynchronize(e.stream));
+        }
+        n_terms += want;
+        // per sample, the terms in the reference's visiting order
+        std::sort(rec.begin(), rec.end(), [](const uint4 &p, const uint4 &q) {
+            return p.x != q.x ? p.x < q.x : p.y < q.y;
+        });
+        std::vector<float> pres(spos.size(), 0.0f);
+        for (const uint4 &r : rec) {
+            float t;
+            std::memcpy(&t, &r.z, 4);
+            pres[r.x] = pres[r.x] + t;
+        }
+        for (float &p : pres) p = p > 0.0f ? p : 0.0f;  // p.max(0.0): NaN gives 0
+        const float drag = std::max(cfg.drag, 0.0f);
+        for (size_t b = 0; b < who.size(); ++b) {
+            ph::Rigid3 &r = rigid[who[b]];
+            bool wet = false;
+            for (size_t q = 0; q < smp[b].size(); ++q) {
+                const float p = pres[range[b].x + q];
+                if (p <= 0.0f) continue;
+                wet = true;
+                const ph::Sample3 &sm = smp[b][q];
+                const ph::F3 rr = sm.point - r.pos;
+                const ph::F3 wind = ph::f3(meta.drag_wind[0], meta.drag_wind[1], meta.drag_wind[2]);
+                const ph::F3 v_rel = (r.vel + ph::cross(r.ang, rr)) - wind;
+                const float speed = ph::length(v_rel);
+                ph::F3 force = -sm.normal * (p * sm.area);
+                if (speed > PH_EPS) force = force - v_rel * (drag * sm.area * speed);
+                r.apply_force(force, sm.point, sdt);
+            }
+            if (wet) r.sleeping = false;
+        }
     }
 
     ~World3() {
@@ -709,6 +1043,26 @@ void world3_snapshot(World3 *w, SnapWorld &out) {
     out.time = w->time;
     out.meta = w->meta;
     out.bodies = w->body_meta;
+    for (size_t k = 0; k < out.bodies.size() && k < w->rigid.size(); ++k) {
+        const ph::Rigid3 &r = w->rigid[k];
+        SnapBody &b = out.bodies[k];
+        b.is_static = r.is_static;
+        if (r.is_static) continue;
+        b.pos[0] = r.pos.x;
+        b.pos[1] = r.pos.y;
+        b.pos[2] = r.pos.z;
+        b.rot[0] = r.rot.x;
+        b.rot[1] = r.rot.y;
+        b.rot[2] = r.rot.z;
+        b.rot[3] = r.rot.w;
+        b.vel[0] = r.vel.x;
+        b.vel[1] = r.vel.y;
+        b.vel[2] = r.vel.z;
+        b.angvel[0] = r.ang.x;
+        b.angvel[1] = r.ang.y;
+        b.angvel[2] = r.ang.z;
+        b.sleeping = r.sleeping;
+    }
     out.cfg = w->cfg;
     const uint32_t n = w->n;
     out.pos.assign(3 * (size_t)n, 0.0f);
@@ -749,7 +1103,32 @@ World3 *world3_from_snapshot(const SnapWorld &s, std::string &err) {
     w->time = s.time;
     w->meta = s.meta;
     for (const SnapBody &b : s.bodies) {
-        w->bodies.push_back(BodyDev{b.shape, b.pos[0], b.pos[1], b.pos[2], b.a, b.b, b.c});
+        ph::Rigid3 r;
+        r.is_static = b.is_static;
+        r.pos = ph::f3(b.pos[0], b.pos[1], b.pos[2]);
+        r.rot.x = b.rot[0];
+        r.rot.y = b.rot[1];
+        r.rot.z = b.rot[2];
+        r.rot.w = b.rot[3];
+        r.vel = ph::f3(b.vel[0], b.vel[1], b.vel[2]);
+        r.ang = ph::f3(b.angvel[0], b.angvel[1], b.angvel[2]);
+        r.linear_damping = b.linear_damping;
+        r.angular_damping = b.angular_damping;
+        r.gravity_scale = b.gravity_scale;
+        r.sleeping = b.sleeping;
+        if (w->dims == 3) r.set_shape(b.shape, b.a, b.b, b.c, b.density);
+        else {
+            r.shape = b.shape;
+            r.a = b.a;
+            r.b = b.b;
+        }
+        BodyDev bd{b.shape, b.pos[0], b.pos[1], b.pos[2], b.a, b.b, b.c};
+        if (!r.is_static) {
+            bd = r.dev();
+            ++w->n_dynamic;
+        }
+        w->bodies.push_back(bd);
+        w->rigid.push_back(r);
         w->body_meta.push_back(b);
         ++w->next_body_id;
     }
@@ -803,37 +1182,56 @@ void phieng_world3_set_gravity(World3 *world, float gx, float gy, float gz) {
 
 int32_t phieng_world3_add_body(World3 *world, float px, float py, float pz, uint32_t shape_tag, float a, float b,
                                float c, float density, uint8_t is_static, float friction, float restitution) {
-    if (!world || !std::isfinite(px) || !std::isfinite(py) || !std::isfinite(pz)) return -1;
-    BodyDev bd{shape_tag, px, py, pz, a, b, c};
-    switch (shape_tag) {
-        case PHIENG_SHAPE3_SPHERE:
-            if (!(a > 0.0f && std::isfinite(a))) return -1;
-            bd.b = bd.c = 0.0f;
-            break;
-        case PHIENG_SHAPE3_BOX:
-            if (!(a > 0.0f && b > 0.0f && c > 0.0f && std::isfinite(a) && std::isfinite(b) && std::isfinite(c)))
-                return -1;
-            break;
-        case PHIENG_SHAPE3_CAPSULE:
-            if (!(a > 0.0f && b > 0.0f && std::isfinite(a) && std::isfinite(b))) return -1;
-            bd.c = 0.0f;
-            break;
-        default:
-            return -1;
-    }
-    if (!std::isfinite(density) || density <= 0.0f) return -1;
-    if (!is_static) {
-        set_error("phieng_world3_add_body", "dynamic bodies are not supported by the GPU fluid path");
-        return -1;
-    }
-    if (world->bodies.size() >= PH_MAX_BODIES) {
-        set_error("phieng_world3_add_body", "too many bodies");
-        return -1;
-    }
-    world->bodies.push_back(bd);
-    world->body_meta.push_back(snap_body(bd, density, friction, restitution));
-    return world->next_body_id++;
+    if (!world) return -1;
+    return world->add_body3(px, py, pz, shape_tag, a, b, c, density, is_static, friction, restitution,
+                            "phieng_world3_add_body");
 }
+
+uint32_t phieng_world3_add_bodies(World3 *world, const BodyDesc3D *descs, uint32_t n, int32_t *out_ids) {
+    if (!world || !descs || !out_ids) return 0;
+    n = std::min<uint32_t>(n, kMaxPerCall);
+    uint32_t added = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const BodyDesc3D &d = descs[i];
+        out_ids[i] = world->add_body3(d.pos_x, d.pos_y, d.pos_z, d.shape_tag, d.a, d.b, d.c, d.density, d.is_static,
+                                      d.friction, d.restitution, "phieng_world3_add_bodies");
+        if (out_ids[i] >= 0) ++added;
+    }
+    return added;
+}
+
+int32_t phieng_world3_get_body_transform(World3 *world, uint32_t body_id, float *out_pos, float *out_quat) {
+    if (!world || !out_pos || !out_quat || body_id >= world->rigid.size()) return 0;
+    const ph::Rigid3 &r = world->rigid[body_id];
+    out_pos[0] = r.pos.x;
+    out_pos[1] = r.pos.y;
+    out_pos[2] = r.pos.z;
+    out_quat[0] = r.rot.x;
+    out_quat[1] = r.rot.y;
+    out_quat[2] = r.rot.z;
+    out_quat[3] = r.rot.w;
+    return 1;
+}
+
+int32_t phieng_world3_get_body_velocity(World3 *world, uint32_t body_id, float *out_vel) {
+    if (!world || !out_vel || body_id >= world->rigid.size()) return 0;
+    const ph::Rigid3 &r = world->rigid[body_id];
+    out_vel[0] = r.vel.x;
+    out_vel[1] = r.vel.y;
+    out_vel[2] = r.vel.z;
+    return 1;
+}
+
+int32_t phieng_world3_get_body_angular_velocity(World3 *world, uint32_t body_id, float *out_w) {
+    if (!world || !out_w || body_id >= world->rigid.size()) return 0;
+    const ph::Rigid3 &r = world->rigid[body_id];
+    out_w[0] = r.ang.x;
+    out_w[1] = r.ang.y;
+    out_w[2] = r.ang.z;
+    return 1;
+}
+
+int32_t phieng_world3_body_count(World3 *world) { return world ? (int32_t)world->rigid.size() : -1; }
 
 int32_t phieng_world3_set_fluid_config(World3 *world, const FluidConfig3Ffi *cfg) {
     if (!world || !cfg) return 0;
@@ -846,6 +1244,10 @@ int32_t phieng_world3_set_fluid_config(World3 *world, const FluidConfig3Ffi *cfg
     if (cfg->boundary_particles && world->sur) {
         set_error("phieng_world3_set_fluid_config",
                   "boundary_particles with the learned surrogate attached: the surrogate does not model ghosts");
+        return 0;
+    }
+    if (cfg->adaptive_resolution && world->n_dynamic) {
+        set_error("phieng_world3_set_fluid_config", "adaptive_resolution with dynamic bodies: its substeps do not couple");
         return 0;
     }
     FluidConfig3Ffi c = *cfg;
@@ -1036,6 +1438,10 @@ int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check
         set_error("phieng_world3_set_surrogate", "the surrogate keeps a fixed particle set (adaptive_resolution)");
         return 0;
     }
+    if (s && world->n_dynamic) {
+        set_error("phieng_world3_set_surrogate", "the surrogate's substeps do not couple to dynamic bodies");
+        return 0;
+    }
     world->sur = s;
     world->check_every = check_every ? check_every : 1;
     world->max_err = max_density_error;
@@ -1050,6 +1456,10 @@ int32_t phieng_world3_collect_samples(World3 *world, Surrogate3 *s, uint32_t eve
     if (!world || (s && (every < 1 || stride < 1))) return 0;
     if (s && world->cfg.adaptive_resolution) {
         set_error("phieng_world3_collect_samples", "sample collection keeps a fixed particle set (adaptive_resolution)");
+        return 0;
+    }
+    if (s && world->n_dynamic) {
+        set_error("phieng_world3_collect_samples", "sample collection does not run coupled substeps (dynamic bodies)");
         return 0;
     }
     world->col = s;
@@ -1231,36 +1641,22 @@ void phieng_world_set_gravity(World2 *world, float gx, float gy) {
 
 int32_t phieng_world_add_body(World2 *world, float px, float py, uint32_t shape_tag, float a, float b, float density,
                               uint8_t is_static, float friction, float restitution) {
-    // friction / restitution are sanitised, never rejected, by set_material
-    // (body.rs:627-637), and static walls take wall_friction from the config;
-    // they are kept for snapshots only
-    if (!world || !std::isfinite(px) || !std::isfinite(py)) return -1;
-    BodyDev bd{0u, px, py, 0.0f, a, 0.0f, 0.0f};
-    switch (shape_tag) {
-        case PHIENG_SHAPE2_CIRCLE:  // a disc in the plane: the sphere test restricted to z = 0
-            if (!(a > 0.0f && std::isfinite(a))) return -1;
-            bd.shape = 0u;
-            break;
-        case PHIENG_SHAPE2_BOX:
-            if (!(a > 0.0f && b > 0.0f && std::isfinite(a) && std::isfinite(b))) return -1;
-            bd.shape = 3u;
-            bd.b = b;
-            break;
-        default:
-            return -1;
+    if (!world) return -1;
+    return world2_base(world)->add_body2(px, py, shape_tag, a, b, density, is_static, friction, restitution,
+                                         "phieng_world_add_body");
+}
+
+uint32_t phieng_world_add_bodies(World2 *world, const BodyDesc2D *descs, uint32_t n, int32_t *out_ids) {
+    if (!world || !descs || !out_ids) return 0;
+    n = std::min<uint32_t>(n, kMaxPerCall);
+    uint32_t added = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const BodyDesc2D &d = descs[i];
+        out_ids[i] = world2_base(world)->add_body2(d.pos_x, d.pos_y, d.shape_tag, d.a, d.b, d.density, d.is_static,
+                                                   d.friction, d.restitution, "phieng_world_add_bodies");
+        if (out_ids[i] >= 0) ++added;
     }
-    if (!std::isfinite(density) || density <= 0.0f) return -1;
-    if (!is_static) {
-        set_error("phieng_world_add_body", "dynamic bodies are not supported by the GPU fluid path");
-        return -1;
-    }
-    if (world->bodies.size() >= PH_MAX_BODIES) {
-        set_error("phieng_world_add_body", "too many bodies");
-        return -1;
-    }
-    world->bodies.push_back(bd);
-    world->body_meta.push_back(snap_body(bd, density, friction, restitution));
-    return world->next_body_id++;
+    return added;
 }
 
 int32_t phieng_world_set_fluid_config(World2 *world, const FluidConfig2Ffi *cfg) {

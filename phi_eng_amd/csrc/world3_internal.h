@@ -65,6 +65,11 @@ struct SnapBody {
     float linear_damping = 0.01f, angular_damping = 0.01f;
     bool sleeping = false;
     float gravity_scale = 1.0f, ambient_scale = 1.0f;
+    // a dynamic body's motion state (3D worlds); a static body keeps the
+    // identity rotation and zero velocities
+    bool is_static = true;
+    float rot[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+    float vel[3] = {0.0f, 0.0f, 0.0f}, angvel[3] = {0.0f, 0.0f, 0.0f};
 };
 
 struct SnapWorld {

@@ -75,6 +75,25 @@ class FluidConfig3Ffi(ctypes.Structure):
 
 assert ctypes.sizeof(FluidConfig3Ffi) == 112
 
+
+class BodyDesc3D(ctypes.Structure):
+    """#[repr(C)] BodyDesc3D (crates/engine/src/ffi.rs:3765-3784), 44 bytes."""
+    _fields_ = [("pos_x", ctypes.c_float), ("pos_y", ctypes.c_float), ("pos_z", ctypes.c_float),
+                ("shape_tag", ctypes.c_uint32), ("a", ctypes.c_float), ("b", ctypes.c_float),
+                ("c", ctypes.c_float), ("density", ctypes.c_float), ("is_static", ctypes.c_uint8),
+                ("_pad", ctypes.c_uint8 * 3), ("friction", ctypes.c_float), ("restitution", ctypes.c_float)]
+
+
+class BodyDesc2D(ctypes.Structure):
+    """#[repr(C)] BodyDesc2D (crates/engine/src/ffi.rs:1205-1222), 36 bytes."""
+    _fields_ = [("pos_x", ctypes.c_float), ("pos_y", ctypes.c_float), ("shape_tag", ctypes.c_uint32),
+                ("a", ctypes.c_float), ("b", ctypes.c_float), ("density", ctypes.c_float),
+                ("is_static", ctypes.c_uint8), ("_pad", ctypes.c_uint8 * 3), ("friction", ctypes.c_float),
+                ("restitution", ctypes.c_float)]
+
+
+assert ctypes.sizeof(BodyDesc3D) == 44 and ctypes.sizeof(BodyDesc2D) == 36
+
 _P = ctypes.c_void_p
 _F = ctypes.c_float
 _U32 = ctypes.c_uint32
@@ -95,6 +114,11 @@ SIGNATURES = {
     "phieng_world3_step": (None, [_P, _F]),
     "phieng_world3_set_gravity": (None, [_P, _F, _F, _F]),
     "phieng_world3_add_body": (_I32, [_P, _F, _F, _F, _U32, _F, _F, _F, _F, ctypes.c_uint8, _F, _F]),
+    "phieng_world3_add_bodies": (_U32, [_P, ctypes.POINTER(BodyDesc3D), _U32, _i32p]),
+    "phieng_world3_body_count": (_I32, [_P]),
+    "phieng_world3_get_body_transform": (_I32, [_P, _U32, _fp, _fp]),
+    "phieng_world3_get_body_velocity": (_I32, [_P, _U32, _fp]),
+    "phieng_world3_get_body_angular_velocity": (_I32, [_P, _U32, _fp]),
     "phieng_world3_set_fluid_config": (_I32, [_P, _cfgp]),
     "phieng_world3_get_fluid_config": (_I32, [_P, _cfgp]),
     "phieng_world3_fluid_add": (_U32, [_P, _fp, _U32]),
@@ -155,6 +179,7 @@ SIGNATURES = {
     "phieng_world_step": (None, [_P, _F]),
     "phieng_world_set_gravity": (None, [_P, _F, _F]),
     "phieng_world_add_body": (_I32, [_P, _F, _F, _U32, _F, _F, _F, ctypes.c_uint8, _F, _F]),
+    "phieng_world_add_bodies": (_U32, [_P, ctypes.POINTER(BodyDesc2D), _U32, _i32p]),
     "phieng_world_set_fluid_config": (_I32, [_P, _cfgp]),
     "phieng_world_get_fluid_config": (_I32, [_P, _cfgp]),
     "phieng_world_fluid_add": (_U32, [_P, _fp, _U32]),

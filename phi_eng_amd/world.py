@@ -18,6 +18,10 @@ def _fptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
+def _i32ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
 class World3:
     """Owns one ``World3*`` from ``phieng_world3_new`` (ffi.rs:3199)."""
 
@@ -76,10 +80,40 @@ class World3:
         return self._l.phieng_world3_add_body(self.ptr, *map(float, center), SHAPE3_BOX,
                                               *map(float, half_extents), 1.0, 1, 0.5, 0.0)
 
-    def add_body(self, center, shape, a, b=0.0, c=0.0, density=1.0, is_static=1):
+    def add_body(self, center, shape, a, b=0.0, c=0.0, density=1.0, is_static=1, friction=0.5, restitution=0.0):
         return self._l.phieng_world3_add_body(self.ptr, *map(float, center), int(shape), float(a),
                                               float(b), float(c), float(density), int(is_static),
-                                              0.5, 0.0)
+                                              float(friction), float(restitution))
+
+    def add_bodies(self, descs):
+        """descs: sequence of dicts with the BodyDesc3D fields; returns (added, ids)."""
+        from .ffi import BodyDesc3D
+        arr = (BodyDesc3D * max(1, len(descs)))()
+        for k, d in enumerate(descs):
+            for key, v in d.items():
+                setattr(arr[k], key, v)
+        ids = np.full(max(1, len(descs)), -2, np.int32)
+        added = self._l.phieng_world3_add_bodies(self.ptr, arr, len(descs), _i32ptr(ids))
+        return added, ids[:len(descs)]
+
+    def body_count(self):
+        return self._l.phieng_world3_body_count(self.ptr)
+
+    def body_transform(self, body_id):
+        """(position (3,), quaternion x, y, z, w (4,)) or None for an unknown id."""
+        p = np.zeros(3, np.float32)
+        q = np.zeros(4, np.float32)
+        if not self._l.phieng_world3_get_body_transform(self.ptr, int(body_id), _fptr(p), _fptr(q)):
+            return None
+        return p, q
+
+    def body_velocity(self, body_id):
+        v = np.zeros(3, np.float32)
+        w = np.zeros(3, np.float32)
+        if not self._l.phieng_world3_get_body_velocity(self.ptr, int(body_id), _fptr(v)):
+            return None
+        self._l.phieng_world3_get_body_angular_velocity(self.ptr, int(body_id), _fptr(w))
+        return v, w
 
     def reserve(self, capacity):
         return self._l.phieng_world3_fluid_reserve(self.ptr, int(capacity))

@@ -120,6 +120,10 @@ def test_null_tolerance():
     assert lib.phieng_world3_fluid_reserve(None, 10) == 0
     assert lib.phieng_world3_fluid_get_mass(None, fp, None, 1) == 0
     assert lib.phieng_world3_sync(None) == 0
+    assert lib.phieng_world3_body_count(None) == -1
+    assert lib.phieng_world3_get_body_transform(None, 0, fp, fp) == 0
+    assert lib.phieng_world3_get_body_velocity(None, 0, fp) == 0
+    assert lib.phieng_world3_get_body_angular_velocity(None, 0, fp) == 0
     with World3() as w:
         assert lib.phieng_world3_fluid_add(w.ptr, None, 1) == 0
         assert lib.phieng_world3_fluid_get(w.ptr, None, 1) == 0
@@ -142,9 +146,91 @@ def test_add_body_rules():
         assert w.add_body((0, 0, 0), 7, 1.0) == -1
         assert w.add_body((float("nan"), 0, 0), 0, 1.0) == -1
         assert w.add_body((0, 0, 0), 0, 1.0, density=0.0) == -1
-        # moving bodies are not part of the GPU fluid path: refused, not ignored
+        # a dynamic body (coupled both ways to the fluid) takes the next id
+        assert w.add_body((0, 2, 0), 0, 1.0, is_static=0) == 3
+        assert w.body_count() == 4
+        p, q = w.body_transform(3)
+        assert list(p) == [0, 2, 0] and list(q) == [0, 0, 0, 1]
+        assert w.body_transform(4) is None
+    with World3() as w:
+        cfg = FluidConfig3Ffi.default()
+        cfg.adaptive_resolution = 1
+        assert w.set_fluid_config(cfg)
+        # adaptive resolution's substeps do not couple: refused, not ignored
         assert w.add_body((0, 0, 0), 0, 1.0, is_static=0) == -1
         assert "dynamic" in ffi.last_error()
+        cfg.adaptive_resolution = 0
+        assert w.set_fluid_config(cfg)
+        assert w.add_body((0, 0, 0), 0, 1.0, is_static=0) == 0
+        cfg.adaptive_resolution = 1
+        assert not w.set_fluid_config(cfg)
+
+
+def test_add_bodies_bulk():
+    """ffi.rs:3786-3860 and 1224-1290: every descriptor validated as
+    add_body, -1 for a rejected one without rollback, the count added."""
+    from phi_eng_amd.ffi import BodyDesc2D, BodyDesc3D
+    lib = ffi.lib()
+    with World3() as w:
+        descs = [dict(pos_x=1.0, pos_y=2.0, pos_z=3.0, shape_tag=1, a=0.5, b=0.5, c=0.5, density=1.0, is_static=1),
+                 dict(pos_x=0.0, pos_y=0.0, pos_z=0.0, shape_tag=1, a=0.5, b=-1.0, c=0.5, density=1.0, is_static=1),
+                 dict(pos_x=0.0, pos_y=5.0, pos_z=0.0, shape_tag=0, a=0.3, density=2.0, is_static=0),
+                 dict(pos_x=float("inf"), pos_y=0.0, pos_z=0.0, shape_tag=0, a=0.3, density=1.0, is_static=1),
+                 dict(pos_x=0.0, pos_y=1.0, pos_z=0.0, shape_tag=2, a=0.2, b=0.4, density=1.0, is_static=1),
+                 dict(pos_x=0.0, pos_y=1.0, pos_z=0.0, shape_tag=9, a=0.2, density=1.0, is_static=1)]
+        added, ids = w.add_bodies(descs)
+        assert added == 3
+        assert list(ids) == [0, -1, 1, -1, 2, -1]
+        assert w.body_count() == 3
+        assert list(w.body_transform(0)[0]) == [1, 2, 3]
+        assert list(w.body_transform(1)[0]) == [0, 5, 0]
+        arr = (BodyDesc3D * 1)()
+        out = np.zeros(1, np.int32)
+        op = out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        assert lib.phieng_world3_add_bodies(None, arr, 1, op) == 0
+        assert lib.phieng_world3_add_bodies(w.ptr, None, 1, op) == 0
+        assert lib.phieng_world3_add_bodies(w.ptr, arr, 1, None) == 0
+        assert lib.phieng_world3_add_bodies(w.ptr, arr, 0, op) == 0
+    w2 = lib.phieng_world_new()
+    try:
+        arr = (BodyDesc2D * 3)()
+        arr[0].pos_x, arr[0].pos_y, arr[0].shape_tag, arr[0].a, arr[0].density, arr[0].is_static = 1, 2, 0, 0.5, 1, 1
+        arr[1].pos_x, arr[1].pos_y, arr[1].shape_tag, arr[1].a, arr[1].b, arr[1].density, arr[1].is_static = \
+            0, 0, 1, 1.0, 0.5, 1.0, 1
+        arr[2].pos_x, arr[2].shape_tag, arr[2].a, arr[2].density, arr[2].is_static = 0, 0, 0.5, 1.0, 0  # dynamic
+        out = np.full(3, -2, np.int32)
+        n = lib.phieng_world_add_bodies(w2, arr, 3, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        assert n == 2 and list(out) == [0, 1, -1]
+        assert "2D" in ffi.last_error()
+    finally:
+        lib.phieng_world_free(w2)
+
+
+def test_dynamic_body_free_fall_matches_oracle(oracle_lib):
+    """With an empty fluid the step is host-only: the gravity kick and the
+    integration of a dynamic body (gravity.rs:355-364, body.rs:1390-1406)
+    equal the oracle's bit for bit, and a spinning capsule's rotation too
+    (integrate_rotation, compute_cos_sin)."""
+    O = oracle_lib
+    with World3() as w:
+        b = w.add_body((0.0, 5.0, 0.0), 2, 0.2, 0.5, density=2.0, is_static=0)
+        ref = [O.rigid(2, (0.0, 5.0, 0.0), 0.2, 0.5, density=2.0, is_static=False)]
+        dt = np.float32(1.0 / 60.0)
+        for k in range(40):
+            w.step(dt)
+            O.world_run_f32(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32), w.get_fluid_config(),
+                            (0.0, -9.81, 0.0), ref, dt, 1)
+        p, q = w.body_transform(b)
+        v, om = w.body_velocity(b)
+        assert np.array_equal(p.view(np.uint32), np.array(ref[0].pos[:], np.float32).view(np.uint32))
+        assert np.array_equal(v.view(np.uint32), np.array(ref[0].vel[:], np.float32).view(np.uint32))
+        assert list(q) == [0, 0, 0, 1] and list(om) == [0, 0, 0]
+        # free fall with linear damping 0.01 (RigidBody3::new): v_k = (v_k-1 + g dt)(1 - 0.01 dt)
+        vy = np.float32(0.0)
+        for k in range(40):
+            vy = np.float32(np.float32(vy + np.float32(-9.81) * np.float32(1.0 * dt)) *
+                            np.float32(1.0 - np.float32(0.01 * dt)))
+        assert v[1] == vy
 
 
 def test_fluid_add_rejects_non_finite_atomically():

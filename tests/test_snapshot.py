@@ -265,7 +265,7 @@ def test_hostile_input_rejected(tmp):
     assert rejected(bytes(v)) and "schema 9" in ffi.last_error()
     d = json.load(open(tmp("ok.json")))
     for mutate, why in [
-        (lambda d: d["bodies"][1].update(is_static=False), "dynamic"),
+        (lambda d: d["bodies"][1].update(velocity={"x": 1.0, "y": 0.0, "z": 0.0}), "moving static"),
         (lambda d: d["bodies"][1].update(rotation=[0, 0.6, 0, 0.8]), "rotated"),
         (lambda d: d["bodies"][1].update(id=5), "out of order"),
         (lambda d: d.update(next_body_id=7), "next_body_id"),
@@ -395,3 +395,36 @@ def test_2d_layouts_and_round_trip(tmp):
     assert World2.load(tmp("p.json")) is None and "polygon" in ffi.last_error()
     w.close()
     w3.close()
+
+
+@pytest.mark.parametrize("binary", [False, True])
+def test_dynamic_body_state_round_trip(tmp, binary):
+    """A dynamic body keeps its state through save / load (body3_dto,
+    load_body3: is_static false, position, rotation, velocities); the
+    loaded world steps on exactly as the saved one (host-only steps of an
+    empty fluid), and save(load(save(w))) == save(w)."""
+    w = _world()
+    b = w.add_body((0.5, 3.0, -0.5), 1, 0.2, 0.3, 0.4, density=1.5, is_static=0)
+    dt = np.float32(1.0 / 60.0)
+    for _ in range(7):
+        w.step(dt)
+    ext = "bin" if binary else "json"
+    assert w.save(tmp("d1." + ext), binary=binary) == 1, ffi.last_error()
+    if not binary:
+        d = json.load(open(tmp("d1." + ext)))
+        jb = d["bodies"][b]
+        assert jb["is_static"] is False
+        assert abs(jb["velocity"]["y"] - float(w.body_velocity(b)[0][1])) < 1e-6
+    v = World3.load(tmp("d1." + ext))
+    assert v is not None, ffi.last_error()
+    for x, y in ((w.body_transform(b), v.body_transform(b)), (w.body_velocity(b), v.body_velocity(b))):
+        for a1, a2 in zip(x, y):
+            assert np.array_equal(a1.view(np.uint32), a2.view(np.uint32))
+    assert v.save(tmp("d2." + ext), binary=binary) == 1
+    assert open(tmp("d1." + ext), "rb").read() == open(tmp("d2." + ext), "rb").read()
+    for _ in range(5):
+        w.step(dt)
+        v.step(dt)
+    assert np.array_equal(w.body_transform(b)[0].view(np.uint32), v.body_transform(b)[0].view(np.uint32))
+    w.close()
+    v.close()
