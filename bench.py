@@ -3,8 +3,10 @@
 
 Prints one JSON line (particle-steps/s over the timed steps, device-resident
 state, synthetic seeded input), with the HBM roofline of the dominant kernel
-from HIP-event kernel timings and the single-core CPU oracle timed on a
-bounded sample of the same scene.
+from HIP-event kernel timings, the FP64 error of one GPU substep, and the
+CPU oracle timed on a bounded sample of the same scene: on all host cores
+(OpenMP Jacobi, cpu_baseline) and on one core (the reference's Gauss-Seidel
+form, cpu_baseline_1core).
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--sur-steps 200]
 
@@ -16,6 +18,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -40,7 +43,10 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="CPU oracle sample length (steps)")
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="sample length (steps) of the one-core CPU baseline (the reference's Gauss-Seidel form)")
+    ap.add_argument("--cpu-par-steps", type=int, default=10,
+                    help="sample length (steps) of the all-cores CPU baseline (OpenMP Jacobi)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="HBM bytes per launch per kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE (optional)")
@@ -51,6 +57,8 @@ def _args():
                          "scale8m: 8,388,608 particles, two colliding columns, plain vs surrogate; extras: the "
                          "1M dam break with adaptive resolution, and snapshot save / load of the 1M world; "
                          "dam2d: the 4,096-particle 2D dam break alone (GPU and CPU)")
+    ap.add_argument("--no-2d", action="store_true",
+                    help="skip the 2D dam break timed beside the plain workload (profiled runs)")
     ap.add_argument("--calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (rocprof counter scale)")
     ap.add_argument("--sur-steps", type=int, default=200,
@@ -108,6 +116,32 @@ def bench_surrogate(a, plain_value):
         "data": "synthetic: seeded jittered lattice dam breaks; network trained in this run",
     })
     return res
+
+
+def fp64_error(w, sc):
+    """Max and p99 over all particles of one GPU substep's density (relative)
+    and acceleration (relative to max(|a|, |g| + sum_j |m_j term grad W_ij| /
+    rho_i), the cancellation-aware scale) against the FP64 twin of the same
+    substep from the same state."""
+    from oracle import oracle as O
+    from phi_eng_amd.world import RECORD_ACCEL, RECORD_DENSITY
+    pos, vel = w.get_state()
+    w.set_record(RECORD_DENSITY | RECORD_ACCEL)
+    w.step(sc.dt)
+    rec = w.get_record()
+    w.set_record(0)
+    t0 = time.perf_counter()
+    r = O.substep_f64(pos, vel, sc.config, sc.gravity, O.bodies_from_boxes(sc.boxes), sc.dt)
+    t1 = time.perf_counter()
+    drho = np.abs(rec["density"].astype(np.float64) - r["density"]) / np.maximum(np.abs(r["density"]), 1e-30)
+    g = float(np.linalg.norm(sc.gravity))
+    scale = np.maximum(np.linalg.norm(r["accel"], axis=1), g + r["scale"])
+    dacc = np.abs(rec["accel"].astype(np.float64) - r["accel"]).max(axis=1) / scale
+    return {"density_rel_max": float(drho.max()), "density_rel_p99": float(np.percentile(drho, 99)),
+            "accel_rel_max": float(dacc.max()), "accel_rel_p99": float(np.percentile(dacc, 99)),
+            "particles": int(len(pos)), "twin_seconds": round(t1 - t0, 2),
+            "what": "one substep of the 1M dam break from the state after the timed run; acceleration error "
+                    "relative to max(|a|, |g| + sum |m term grad W| / rho)"}
 
 
 def bench_cpu_2d():
@@ -352,23 +386,38 @@ def main():
                             "frac": round(wips / VALU_PEAK_WIPS, 4)}
     step_frac = B_ALG_STEP * n / (wall / a.steps) / (HBM_PEAK_GBS * 1e9)
 
-    cpu = None
-    if not a.no_cpu and a.cpu_steps > 0:
+    # FP64 error of the GPU result: one recorded substep from the current
+    # state against the oracle's FP64 twin of the same substep.
+    fp64 = None if a.no_cpu else fp64_error(w, sc)
+
+    cpu = cpu1 = None
+    if not a.no_cpu:
         from oracle import oracle as O
         bodies = O.bodies_from_boxes(sc.boxes)
         pos, vel = w.get_state()
-        c0 = time.perf_counter()
-        O.run_f32(pos, vel, sc.config, sc.gravity, bodies, sc.dt, a.cpu_steps, jacobi=False)
-        c1 = time.perf_counter()
-        cpu = {"value": round(n * a.cpu_steps / (c1 - c0), 1), "unit": "particle-steps/s", "cores": 1,
-               "kind": "port",
-               "sample": f"{a.cpu_steps} steps of the same {n}-particle scene (state after the GPU run), "
-                         "f32 hash-grid Gauss-Seidel restatement of fluid3.rs, 1 thread"}
+        if a.cpu_par_steps > 0:
+            threads = O.max_threads()
+            c0 = time.perf_counter()
+            O.run_f32_par(pos, vel, sc.config, sc.gravity, bodies, sc.dt, a.cpu_par_steps, threads=threads)
+            c1 = time.perf_counter()
+            cpu = {"value": round(n * a.cpu_par_steps / (c1 - c0), 1), "unit": "particle-steps/s",
+                   "cores": threads, "nproc": os.cpu_count(), "kind": "port",
+                   "sample": f"{a.cpu_par_steps} steps of the same {n}-particle scene (state after the GPU run), "
+                             f"the oracle's f32 Jacobi restatement of fluid3.rs with its per-particle loops on "
+                             f"{threads} OpenMP threads (grid rebuild on one)"}
+        if a.cpu_steps > 0:
+            c0 = time.perf_counter()
+            O.run_f32(pos, vel, sc.config, sc.gravity, bodies, sc.dt, a.cpu_steps, jacobi=False)
+            c1 = time.perf_counter()
+            cpu1 = {"value": round(n * a.cpu_steps / (c1 - c0), 1), "unit": "particle-steps/s", "cores": 1,
+                    "kind": "port",
+                    "sample": f"{a.cpu_steps} steps of the same {n}-particle scene (state after the GPU run), "
+                              "f32 hash-grid Gauss-Seidel restatement of fluid3.rs, 1 thread"}
 
     cpu2d = None
-    if not a.no_cpu:
+    if not a.no_cpu and not a.no_2d:
         cpu2d = bench_cpu_2d()
-    gpu2d = bench_gpu_2d()
+    gpu2d = None if a.no_2d else bench_gpu_2d()
 
     out = {
         "metric": METRIC,
@@ -392,6 +441,8 @@ def main():
         "timed_with_events_ms_per_step": round((t3 - t2) * 1e3 / a.steps, 4),
         "neighbour_list": dict(nstats, kind="verlet", skin=0.1),
         "cpu_baseline": cpu,
+        "cpu_baseline_1core": cpu1,
+        "fp64_err": fp64,
         "cpu_dam_break_2d": cpu2d,
         "gpu_dam_break_2d": gpu2d,
     }
@@ -510,7 +561,7 @@ def bench_batch(a):
     b = Batch3(worlds)
     n = b.count()
     dt = 1e-4
-    cache = os.path.join(ROOT, "profiles", "batch_tune_cache.txt")
+    cache = os.environ.get("PHIENG_TUNE_CACHE", os.path.join(tempfile.gettempdir(), "phieng_batch_tune_cache.txt"))
     mode, bpc, tuned_ms = b.autotune(dt, trial_steps=10, cache_path=cache)
     for _ in range(a.warmup):
         b.step(dt)

@@ -127,12 +127,14 @@ def substep_f64(pos, vel, cfg, gravity, bodies, dt, phases=None, mass=None):
     acc = np.zeros((n, 3))
     opos = np.zeros((n, 3))
     ovel = np.zeros((n, 3))
+    scale = np.zeros(n)
     arr, nb = bodies
     d = ctypes.c_double
     lib().oracle3_substep_f64(
         ctypes.c_uint32(n), _p(pos), _p(vel), _p(m), _p(ph, ctypes.c_uint8), ctypes.byref(cfg), _p(g), arr,
-        ctypes.c_uint32(nb), ctypes.c_float(dt), _p(rho, d), _p(p, d), _p(acc, d), _p(opos, d), _p(ovel, d))
-    return dict(pos=opos, vel=ovel, density=rho, pressure=p, accel=acc)
+        ctypes.c_uint32(nb), ctypes.c_float(dt), _p(rho, d), _p(p, d), _p(acc, d), _p(opos, d), _p(ovel, d),
+        _p(scale, d))
+    return dict(pos=opos, vel=ovel, density=rho, pressure=p, accel=acc, scale=scale)
 
 
 def boundary_f32(pos, mass, cfg, bodies):
@@ -577,3 +579,23 @@ def sample_pressure(pos, mass, density, pressure, cfg, point):
     pt = np.asarray(point, np.float32)
     return lib().oracle3_sample_pressure(ctypes.c_uint32(len(pos)), _p(pos), _p(mass), _p(density), _p(pressure),
                                          ctypes.byref(cfg), _p(pt))
+
+
+def run_f32_par(pos, vel, cfg, gravity, bodies, dt, steps, threads=0, phases=None):
+    """run_f32 in the Jacobi form with the per-particle loops on `threads`
+    OpenMP threads (0: all); equal to run_f32(..., jacobi=True) bit for bit."""
+    n = len(pos)
+    pos = np.ascontiguousarray(pos, dtype=np.float32).copy()
+    vel = np.ascontiguousarray(vel, dtype=np.float32).copy()
+    m = masses(cfg, n, phases)
+    ph = None if phases is None else np.ascontiguousarray(phases, dtype=np.uint8)
+    g = np.asarray(gravity, dtype=np.float32)
+    arr, nb = bodies
+    sub = lib().oracle3_run_f32_par(
+        ctypes.c_uint32(n), _p(pos), _p(vel), _p(m), _p(ph, ctypes.c_uint8), ctypes.byref(cfg), _p(g), arr,
+        ctypes.c_uint32(nb), ctypes.c_float(dt), ctypes.c_uint32(steps), ctypes.c_int(int(threads)))
+    return pos, vel, sub
+
+
+def max_threads():
+    return lib().oracle3_max_threads()

@@ -37,6 +37,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <omp.h>
+
 #include <algorithm>
 #include <unordered_map>
 #include <vector>
@@ -149,6 +151,11 @@ struct Fluid {
     // substep (the coupled clamp, the second grid rebuild and buoyancy).
     void (*tail)(void *ctx, float dt) = nullptr;
     void *tail_ctx = nullptr;
+    // Jacobi substeps only: run the per-particle loops of the density, the
+    // forces, the integrate and the static clamp on OpenMP threads (each
+    // particle's result is independent of the others', so they equal the
+    // one-thread results bit for bit).
+    bool par = false;
 
     explicit Fluid(const FluidConfig3Ffi &c) : cfg(c), n(0) {}
 
@@ -350,7 +357,10 @@ struct Fluid {
 
     // fluid3.rs:639-655
     void compute_density_pressure(T h, uint32_t *out_count) {
+#pragma omp parallel if (par)
+        {
         std::vector<uint32_t> nb;
+#pragma omp for schedule(static, 256)
         for (uint32_t i = 0; i < n; ++i) {
             neighbors(i, nb);
             V3<T> pi = pos[i];
@@ -362,6 +372,8 @@ struct Fluid {
             if (out_count) out_count[i] = (uint32_t)nb.size();
             density[i] = rclamp<T>(rho, (T)0, (T)3.0 * rmax<T>((T)cfg.rest_density, (T)kEps));
         }
+        }
+#pragma omp parallel for if (par) schedule(static, 1024)
         for (uint32_t i = 0; i < n; ++i) {
             T rho0 = (T)cfg.phase_rest_density[std::min<uint32_t>(phase[i], 3u)];
             pressure[i] = eos(density[i], rho0, (T)cfg.stiffness, (T)cfg.gamma);
@@ -372,7 +384,10 @@ struct Fluid {
     void compute_forces(T h, double *scale_out) {
         forces.assign(n, V3<T>{0, 0, 0});
         T c_s = std::sqrt(rmax<T>((T)cfg.stiffness, (T)0));
+#pragma omp parallel if (par)
+        {
         std::vector<uint32_t> nb;
+#pragma omp for schedule(static, 256)
         for (uint32_t i = 0; i < n; ++i) {
             neighbors(i, nb);
             V3<T> pi = pos[i], vi = vel[i];
@@ -421,6 +436,7 @@ struct Fluid {
             }
             forces[i] = f;
             if (scale_out) scale_out[i] = mag / (double)rho_i;
+        }
         }
     }
 
@@ -631,15 +647,20 @@ struct Fluid {
         std::vector<uint32_t> nb;
         if (jacobi) {
             std::vector<V3<T>> npos(n), nvel(n);
+#pragma omp parallel if (par)
+            {
+            std::vector<uint32_t> lnb;
+#pragma omp for schedule(static, 256)
             for (uint32_t i = 0; i < n; ++i) {
-                neighbors(i, nb);
+                neighbors(i, lnb);
                 V3<T> a;
-                integrate_one(i, nb, pos, vel, g, dt, h, nvel[i], npos[i], &a);
+                integrate_one(i, lnb, pos, vel, g, dt, h, nvel[i], npos[i], &a);
                 if (acc_out) {
                     acc_out[3 * i] = a.x;
                     acc_out[3 * i + 1] = a.y;
                     acc_out[3 * i + 2] = a.z;
                 }
+            }
             }
             pos.swap(npos);
             vel.swap(nvel);
@@ -661,6 +682,7 @@ struct Fluid {
             tail(tail_ctx, (float)dt);
             return;
         }
+#pragma omp parallel for if (par && jacobi) schedule(static, 1024)
         for (uint32_t i = 0; i < n; ++i) clamp_particle(i, bodies, nbodies);
         // fluid3.rs:1184-1213: the second grid rebuild only feeds buoyancy of
         // dynamic bodies, which this path does not have.
@@ -1325,6 +1347,36 @@ int oracle3_run_f32(uint32_t n, float *pos, float *vel, const float *mass, const
     return (int)sub;
 }
 
+// oracle3_run_f32 with the Jacobi integrate on `threads` OpenMP threads
+// (0: the OpenMP default).  The grid rebuild stays on one thread.
+int oracle3_run_f32_par(uint32_t n, float *pos, float *vel, const float *mass, const uint8_t *phase,
+                        const FluidConfig3Ffi *cfg, const float *g3, const OracleBody3 *bodies, uint32_t nbodies,
+                        float dt, uint32_t steps, int threads) {
+    if (threads > 0) omp_set_num_threads(threads);
+    Fluid<float> f(*cfg);
+    f.par = true;
+    f.load(n, pos, vel, mass, phase, nullptr);
+    uint32_t sub = 0;
+    for (uint32_t s = 0; s < steps; ++s) {
+        if (!cfg->enabled || n == 0 || !(dt > 0.0f)) break;
+        sub = f.cfl_substeps(dt);
+        float sdt = dt / (float)sub;
+        for (uint32_t k = 0; k < sub; ++k)
+            f.substep({g3[0], g3[1], g3[2]}, sdt, true, reinterpret_cast<const Body *>(bodies), nbodies, nullptr,
+                      nullptr, nullptr);
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        pos[3 * i] = f.pos[i].x;
+        pos[3 * i + 1] = f.pos[i].y;
+        pos[3 * i + 2] = f.pos[i].z;
+        vel[3 * i] = f.vel[i].x;
+        vel[3 * i + 1] = f.vel[i].y;
+        vel[3 * i + 2] = f.vel[i].z;
+    }
+    return (int)sub;
+}
+int oracle3_max_threads(void) { return omp_get_max_threads(); }
+
 // The variable-count forms (adaptive resolution changes the particle
 // count).  *n is the count in and out; every array holds `room` particles
 // (pos, vel 3 each); mass and phase are updated with the state; cap is the
@@ -1404,12 +1456,13 @@ int oracle3_run_cap_f32(uint32_t *n, uint32_t cap, uint32_t room, float *pos, fl
 int oracle3_substep_f64(uint32_t n, const float *pos, const float *vel, const float *mass,
                         const uint8_t *phase, const FluidConfig3Ffi *cfg, const float *g3,
                         const OracleBody3 *bodies, uint32_t nbodies, float dt, double *out_rho,
-                        double *out_p, double *out_acc3, double *out_pos3, double *out_vel3) {
+                        double *out_p, double *out_acc3, double *out_pos3, double *out_vel3, double *out_scale) {
     Fluid<double> f(*cfg);
+    f.par = true;  // Jacobi: the per-particle loops are independent
     f.load(n, pos, vel, mass, phase, nullptr);
     std::vector<double> acc(3 * (size_t)n);
     f.substep({g3[0], g3[1], g3[2]}, (double)dt, true, reinterpret_cast<const Body *>(bodies), nbodies,
-              acc.data(), nullptr, nullptr);
+              acc.data(), nullptr, out_scale);
     for (uint32_t i = 0; i < n; ++i) {
         if (out_rho) out_rho[i] = f.density[i];
         if (out_p) out_p[i] = f.pressure[i];

@@ -212,7 +212,8 @@ struct Engine {
     bool vl_valid = false;
     uint64_t vl_epoch = 0;
     uint32_t vl_n = 0;
-    DBuf<uint32_t> vl_cnt, vl_idx, vl_word[2], vl_tile_q;
+    DBuf<uint32_t> vl_cnt, vl_idx, vl_word[2], vl_tile_q, vl_halo, vl_halo_n;
+    DBuf<uint4> vl_row16;
     DBuf<float> vl_tile_dmax;
     DBuf<VerletDev> vl_state;
     bool vl_state_ready = false;

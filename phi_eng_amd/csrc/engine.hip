@@ -475,11 +475,14 @@ void Engine::substep_verlet() {
     vl_idx.ensure((size_t)VL_CAP * m);
     vl_word[0].ensure(m);
     vl_word[1].ensure(m);
+    vl_row16.ensure((size_t)(VL_CAP / 8) * m);
     vl_state.ensure(1);
     const size_t ntl = std::max<size_t>(nt, 1);
     if (vl_tile_q.n < ntl || !vl_state_ready) {
         vl_tile_q.ensure(ntl);
         vl_tile_dmax.ensure(ntl);
+        vl_halo.ensure(ntl * VL_HALO_CAP);
+        vl_halo_n.ensure(ntl);
         run(KIND_FILL, [&] { return launch_fill_u32(stream, vl_tile_q.p, (uint32_t)vl_tile_q.n, 0u); });
     }
     if (!vl_state_ready) {
@@ -529,6 +532,10 @@ void Engine::substep_verlet() {
     va.skin = skin;
     va.vs = vl_state.p;
     va.tile_q = vl_tile_q.p;
+    va.tag_mirror = tag[ct].p;
+    va.halo = vl_halo.p;
+    va.halo_n = vl_halo_n.p;
+    va.rows16 = vl_row16.p;
     va.lidx = nl_idx.p;
     va.lstride = list_stride();
     run(KIND_DENSITY, [&] {
@@ -540,7 +547,8 @@ void Engine::substep_verlet() {
     run(KIND_FORCE, [&] {
         return launch_force_verlet(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p, vel[ns].p,
                                    rq.p, tag[nt_].p, nl_cnt.p, nl_idx.p, list_stride(), posm[cs].p, vel[cs].p,
-                                   ra ? rec_acc.p : nullptr, d_bbox.p, GhostDev{}, vl_tile_dmax.p, vl_state.p);
+                                   ra ? rec_acc.p : nullptr, d_bbox.p, GhostDev{}, vl_tile_dmax.p, vl_state.p,
+                                   vl_word[1].p);
     });
     finish_substep();
     vl_valid = true;

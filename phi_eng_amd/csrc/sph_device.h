@@ -446,6 +446,12 @@ __device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__r
 #define VLF_SLOW 0x01000000u       // row unusable (longer than VL_CAP, or a per-cell particle)
 #define VL_FAR 0x1p24f             // |x / h| from which cells are not compared mod 4
 #define VL_NOT27 0xFFFFFFFFu
+// A tile's halo: the union of its rows, staged in LDS by the walk.  At most
+// VL_HALO_CAP particles over a slot span of at most VL_HALO_SPAN; a tile
+// beyond either walks from global memory (VL_HALO_NONE).
+#define VL_HALO_CAP 1792u
+#define VL_HALO_SPAN 65536u
+#define VL_HALO_NONE 0xFFFFFFFFu
 
 struct VerletDev {
     uint32_t rebuild;         // this substep sorts and rebuilds every row

@@ -51,13 +51,15 @@ hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *ti
                               uint32_t *perm_out = nullptr, const VerletDev *vs = nullptr,
                               const uint32_t *vword_in = nullptr, uint32_t *vword_out = nullptr);
 // The force pass of a Verlet substep: the list-based pass over the
-// neighbour list the Verlet density pass wrote, also recording the
-// substep's largest displacement in vs.
+// neighbour list the Verlet density pass wrote, also recording each tile's
+// largest squared displacement (tile_dmax), flagging positions beyond the
+// reach of the mod-4 cell words (vs->slow), and writing each slot's next
+// neighbour word (vword_next).
 hipError_t launch_force_verlet(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                                const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
                                const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
                                uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
-                               const GhostDev &gh, float *tile_dmax, VerletDev *vs);
+                               const GhostDev &gh, float *tile_dmax, VerletDev *vs, uint32_t *vword_next);
 // Verlet substep (verlet.hip).  k_vl_decide (one block) decides whether
 // this substep sorts and rebuilds every row: on request (force), after a
 // particle needed the per-cell path, or when the displacement bound (the
@@ -74,6 +76,14 @@ struct VerletArgs {
     float skin;
     VerletDev *vs;
     uint32_t *tile_q;        // per tile: rows put in order (statistics; zeroed by k_vl_decide)
+    uint32_t *tag_mirror;    // a rebuild copies the sorted tags here (the other tag generation)
+    // halo form of the rows: per tile the sorted slots of its rows' union
+    // (halo[tile * VL_HALO_CAP + k], count halo_n[tile] or VL_HALO_NONE),
+    // and the rows as 16-bit halo indices, eight to a 16-byte group (group g
+    // of slot i at rows16[g * vstride + i])
+    uint32_t *halo;
+    uint32_t *halo_n;
+    uint4 *rows16;
     // the neighbours within h, for the force pass (ListOut layout: group g
     // of slot i at uint4 g * lstride + i; the count word in fcnt)
     uint32_t *lidx;

@@ -100,9 +100,10 @@ __device__ __forceinline__ void dev_cell_key(uint32_t bid, uint32_t nbl, const T
     }
 }
 
-// With a Verlet list (vs non-null): every particle's neighbour word
-// (original index, cell mod 4) into vword; the bucket keys and counts only
-// on a substep that sorts (vs->rebuild, decided by k_vl_decide).
+// With a Verlet list (vs non-null) the kernel works only on a substep that
+// sorts (vs->rebuild, decided by k_vl_decide): every particle's neighbour
+// word (original index, cell mod 4) into vword, then the bucket keys and
+// counts.  (The cell record forces a sort on every substep.)
 __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict__ tiles,
                                                       const SceneDev *__restrict__ scenes,
                                                       const float4 *__restrict__ posm,
@@ -113,6 +114,8 @@ __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict_
                                                       int32_t *__restrict__ rec_cells, const VerletDev *vs,
                                                       uint32_t *__restrict__ vword) {
     if (vs) {
+        // between rebuilds the force pass has written the words already
+        if (!vs->rebuild) return;
         const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
         const SceneDev &s = scenes[t.scene];
         const uint32_t i = t.first + threadIdx.x;
@@ -121,18 +124,6 @@ __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict_
             int32_t cx, cy, cz;
             cell_of(p.x, p.y, p.z, s.h_grid, cx, cy, cz);
             vword[i] = vl_word(tag[i], cx, cy, cz);
-        }
-        if (!vs->rebuild) {
-            if (rec_cells && i < s.part_end) {
-                const float4 p = posm[i];
-                int32_t cx, cy, cz;
-                cell_of(p.x, p.y, p.z, s.h_grid, cx, cy, cz);
-                const uint32_t id = tag[i] & PH_ID_MASK;
-                rec_cells[3 * id] = cx;
-                rec_cells[3 * id + 1] = cy;
-                rec_cells[3 * id + 2] = cz;
-            }
-            return;
         }
     }
     dev_cell_key(blockIdx.x, gridDim.x, tiles, scenes, posm, tag, keys, slots, counts, rec_cells);
@@ -250,12 +241,12 @@ __device__ __forceinline__ void dev_rank_gather(uint32_t bid, uint32_t nbl, cons
     const uint32_t q = t.first + threadIdx.x;
     if (q >= s.part_end) return;
     if (vs && !vs->rebuild) {
-        // a Verlet substep that keeps the order: the same slots, copied
+        // A Verlet substep that keeps the order: the same slots.  Positions
+        // and velocities are copied; both tag generations already hold the
+        // sorted tags (the rebuild's density pass mirrors them) and the
+        // force pass has written the next words.
         posm_out[q] = posm_in[q];
         vel_out[q] = vel_in[q];
-        tag_out[q] = tag_in[q];
-        vword_out[q] = vword_in[q];
-        if (rec_sorted) rec_sorted[q] = tag_in[q] & PH_ID_MASK;
         if (perm_out) perm_out[q] = q;
         return;
     }
@@ -525,7 +516,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
     const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox,
     float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}, const AuxFuse &af = AuxFuse{},
-    float *__restrict__ tile_dmax = nullptr, VerletDev *vs = nullptr) {
+    float *__restrict__ tile_dmax = nullptr, VerletDev *vs = nullptr, uint32_t *__restrict__ vword_next = nullptr) {
     const uint32_t tile = xcd_tile(bid, nbl);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
@@ -654,6 +645,9 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             by1 = by0;
             bz1 = bz0;
         }
+        // the next substep's neighbour word of this slot (a non-finite
+        // particle is nobody's neighbour; its word is never compared)
+        if (VL) vword_next[i] = vl_word(tag[i], bx0, by0, bz0);
     }
     // Cell bounding box of the advanced particles for the domain monitor:
     // one plain store per tile (same-address atomics from every wave would
@@ -755,9 +749,10 @@ __global__ __launch_bounds__(PH_TILE) void k_force_verlet(const TileDev *__restr
     const float4 *__restrict__ vel, const float2 *__restrict__ rq, const uint32_t *__restrict__ tag,
     const uint32_t *__restrict__ lcnt, const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, GhostDev gh,
-    float *__restrict__ tile_dmax, VerletDev *vs) {
+    float *__restrict__ tile_dmax, VerletDev *vs, uint32_t *__restrict__ vword_next) {
     dev_force_integrate<true>(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, lcnt, lgrp,
-                              mstride, posm_out, vel_out, rec_acc, bbox, nullptr, gh, AuxFuse{}, tile_dmax, vs);
+                              mstride, posm_out, vel_out, rec_acc, bbox, nullptr, gh, AuxFuse{}, tile_dmax, vs,
+                              vword_next);
 }
 
 // Boundary ghost density and pressure from the fluid (fluid3.rs:906-920):
@@ -1069,10 +1064,10 @@ hipError_t launch_force_verlet(hipStream_t st, uint32_t ntiles, const TileDev *t
                                const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
                                const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
                                uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
-                               const GhostDev &gh, float *tile_dmax, VerletDev *vs) {
+                               const GhostDev &gh, float *tile_dmax, VerletDev *vs, uint32_t *vword_next) {
     hipLaunchKernelGGL(k_force_verlet, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm, vel,
                        rq, tag, lcnt, reinterpret_cast<const uint4 *>(lidx), mstride, posm_out, vel_out, rec_acc,
-                       bbox, gh, tile_dmax, vs);
+                       bbox, gh, tile_dmax, vs, vword_next);
     return hipGetLastError();
 }
 hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,

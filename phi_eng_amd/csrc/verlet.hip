@@ -35,6 +35,8 @@
 
 namespace {
 
+constexpr uint32_t kDecideThreads = 1024, kDecideUnroll = 8;  // 8,192 tiles in one round
+
 // fluid3.rs:646-655 after the sum: clamp, equation of state, the force
 // pass's pressure quotient, and the parity records (same operations as the
 // list-free density pass).
@@ -111,35 +113,65 @@ __device__ __forceinline__ void reach(float x, float h, int32_t c, float skin, i
     hi = (int64_t)c + 1 + ((wide || fr > 1.0f - m) ? 1 : 0);
 }
 
+// Exclusive prefix sum over a PH_TILE-thread block; total to every thread.
+__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t &total) {
+    __shared__ uint32_t wsum[PH_TILE / 64];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t inc = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, 64);
+        if ((int)lane >= d) inc += y;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+    total = 0;
+    for (uint32_t k = 0; k < PH_TILE / 64; ++k) {
+        if (k < w) off += wsum[k];
+        total += wsum[k];
+    }
+    __syncthreads();
+    return off + inc - v;
+}
+
 }  // namespace
 
 // One block: this substep's decision, from the last substep's per-tile
 // largest squared displacements (per-tile stores, so no same-address
 // atomics contend across the XCDs); the per-tile counts of rows put in
 // order go to the statistics and restart at zero.
-__global__ __launch_bounds__(PH_TILE) void k_vl_decide(VerletDev *vs, const float *__restrict__ tile_dmax,
-                                                       uint32_t *__restrict__ tile_q, uint32_t nt, uint32_t force,
-                                                       float thr) {
+__global__ __launch_bounds__(kDecideThreads) void k_vl_decide(VerletDev *vs, const float *__restrict__ tile_dmax,
+                                                              uint32_t *__restrict__ tile_q, uint32_t nt,
+                                                              uint32_t force, float thr) {
+    // every load of the block issued before any reduction: one round trip
     float dm = 0.0f;
     uint32_t qs = 0;
-    for (uint32_t k = threadIdx.x; k < nt; k += PH_TILE) {
+#pragma unroll
+    for (uint32_t r = 0; r < kDecideUnroll; ++r) {
+        const uint32_t k = r * kDecideThreads + threadIdx.x;
+        if (k < nt) {
+            if (!force) dm = fmaxf(dm, tile_dmax[k]);
+            qs += tile_q[k];
+        }
+    }
+    for (uint32_t k = kDecideUnroll * kDecideThreads + threadIdx.x; k < nt; k += kDecideThreads) {
         if (!force) dm = fmaxf(dm, tile_dmax[k]);
         qs += tile_q[k];
-        tile_q[k] = 0u;
     }
+    for (uint32_t k = threadIdx.x; k < nt; k += kDecideThreads) tile_q[k] = 0u;
     for (int d = 32; d >= 1; d >>= 1) {
         dm = fmaxf(dm, __shfl_xor(dm, d, 64));
         qs += __shfl_xor(qs, d, 64);
     }
-    __shared__ float wdm[PH_TILE / 64];
-    __shared__ uint32_t wqs[PH_TILE / 64];
+    __shared__ float wdm[kDecideThreads / 64];
+    __shared__ uint32_t wqs[kDecideThreads / 64];
     if ((threadIdx.x & 63) == 0) {
         wdm[threadIdx.x >> 6] = dm;
         wqs[threadIdx.x >> 6] = qs;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < PH_TILE / 64; ++w) {
+        for (uint32_t w = 1; w < kDecideThreads / 64; ++w) {
             dm = fmaxf(dm, wdm[w]);
             qs += wqs[w];
         }
@@ -263,6 +295,8 @@ __global__ __launch_bounds__(PH_TILE) void k_density_rebuild(const TileDev *__re
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     if (i >= s.part_end) return;
+    // both tag generations hold the sorted tags until the next rebuild
+    va.tag_mirror[i] = tag[i];
     const float4 pi = posm[i];
     if (!finite3(pi.x, pi.y, pi.z)) {
         // fluid3.rs:302-304: no neighbours, density 0; an empty row
@@ -275,6 +309,104 @@ __global__ __launch_bounds__(PH_TILE) void k_density_rebuild(const TileDev *__re
 }
 
 // The walk substep's density pass (returns at once on a rebuild substep).
+// After a rebuild, one block per tile: the sorted union of the tile's rows
+// (its halo) from a bitmap over the rows' slot span, and every row entry
+// rewritten as its rank in the halo (16 bits).  A tile whose span or halo
+// is too large, or with a row that takes the per-cell path, keeps the
+// global rows (VL_HALO_NONE).
+__global__ __launch_bounds__(PH_TILE) void k_halo_build(const TileDev *__restrict__ tiles,
+                                                        const SceneDev *__restrict__ scenes, ph::VerletArgs va) {
+    constexpr uint32_t kWords = VL_HALO_SPAN / 32u, kPer = kWords / PH_TILE;
+    __shared__ uint32_t bm[kWords];  // bit (slot - lo)
+    __shared__ uint32_t pw[kWords];  // halo rank of the word's first set bit
+    __shared__ uint32_t red[3][PH_TILE / 64];
+    if (va.vs->rebuild == 0u) return;
+    const uint32_t tile = blockIdx.x;
+    const TileDev t = tiles[tile];
+    const SceneDev &s = scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t *ru = va.vidx;
+    auto entry = [&](uint32_t m) { return ru[((size_t)(m >> 2) * va.vstride + i) * 4u + (m & 3u)]; };
+    const uint32_t raw = i < s.part_end ? va.vcnt[i] : 0u;
+    const bool slow = (raw & VLF_SLOW) != 0u;
+    const uint32_t n = slow ? 0u : min(raw, VL_CAP);
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    for (uint32_t m = 0; m < n; ++m) {
+        const uint32_t j = entry(m);
+        lo = min(lo, j);
+        hi = max(hi, j);
+    }
+    uint32_t bad = slow ? 1u : 0u;
+    for (int d = 32; d >= 1; d >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor(lo, d, 64));
+        hi = max(hi, (uint32_t)__shfl_xor(hi, d, 64));
+        bad |= (uint32_t)__shfl_xor(bad, d, 64);
+    }
+    if (lane == 0) {
+        red[0][w] = lo;
+        red[1][w] = hi;
+        red[2][w] = bad;
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < PH_TILE / 64; ++k) {
+        lo = min(lo, red[0][k]);
+        hi = max(hi, red[1][k]);
+        bad |= red[2][k];
+    }
+    if (bad || (hi >= lo && hi - lo >= VL_HALO_SPAN)) {
+        if (threadIdx.x == 0) va.halo_n[tile] = VL_HALO_NONE;
+        return;
+    }
+    if (hi < lo) {  // no entries at all
+        if (threadIdx.x == 0) va.halo_n[tile] = 0u;
+        return;
+    }
+    const uint32_t nw = ((hi - lo) >> 5) + 1u;
+    for (uint32_t k = threadIdx.x; k < nw; k += PH_TILE) bm[k] = 0u;
+    __syncthreads();
+    for (uint32_t m = 0; m < n; ++m) {
+        const uint32_t d = entry(m) - lo;
+        atomicOr(&bm[d >> 5], 1u << (d & 31u));
+    }
+    __syncthreads();
+    // ranks: thread k owns words [k kPer, (k + 1) kPer)
+    uint32_t c = 0;
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t k = threadIdx.x * kPer + q;
+        if (k < nw) c += __popc(bm[k]);
+    }
+    uint32_t total;
+    const uint32_t base = block_scan_u32(c, total);
+    if (total > VL_HALO_CAP) {
+        if (threadIdx.x == 0) va.halo_n[tile] = VL_HALO_NONE;
+        return;
+    }
+    uint32_t r = base;
+    uint32_t *hl = va.halo + (size_t)tile * VL_HALO_CAP;
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t k = threadIdx.x * kPer + q;
+        if (k >= nw) break;
+        pw[k] = r;
+        for (uint32_t bits = bm[k]; bits; bits &= bits - 1u) hl[r++] = lo + k * 32u + (uint32_t)(__ffs(bits) - 1);
+    }
+    __syncthreads();
+    uint4 *r16 = va.rows16 + i;
+    for (uint32_t m = 0; m < n; m += 8) {
+        uint32_t v[4] = {0u, 0u, 0u, 0u};
+        for (uint32_t q = 0; q < 8 && m + q < n; ++q) {
+            const uint32_t d = entry(m + q) - lo;
+            const uint32_t rank = pw[d >> 5] + __popc(bm[d >> 5] & ((1u << (d & 31u)) - 1u));
+            v[q >> 1] |= rank << (16u * (q & 1u));
+        }
+        r16[(size_t)(m >> 3) * va.vstride] = make_uint4(v[0], v[1], v[2], v[3]);
+    }
+    if (threadIdx.x == 0) va.halo_n[tile] = total;
+}
+
+// The walk substep's density pass (returns at once on a rebuild substep).
+// A tile with a halo copies its halo's positions, words and slots to LDS
+// and walks 16-bit rows; other tiles walk 32-bit rows of slots.
 __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restrict__ tiles,
                                                           const SceneDev *__restrict__ scenes,
                                                           const float4 *__restrict__ posm,
@@ -282,7 +414,9 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
                                                           float *__restrict__ rec_rho, float *__restrict__ rec_p,
                                                           uint32_t *__restrict__ rec_count,
                                                           uint32_t *__restrict__ fcnt, ph::VerletArgs va) {
-    // per wave: one row being put in order (slots, keys, contributions)
+    __shared__ float4 h_p[VL_HALO_CAP];                  // position and mass
+    __shared__ uint32_t h_w[VL_HALO_CAP], h_s[VL_HALO_CAP];  // neighbour word, slot
+    // per wave: one row being put in order (entries, keys, contributions)
     __shared__ uint32_t s_key[PH_TILE / 64][VL_CAP];
     __shared__ uint32_t s_uj[PH_TILE / 64][VL_CAP], s_slot[PH_TILE / 64][VL_CAP];
     __shared__ float s_uc[PH_TILE / 64][VL_CAP], s_c[PH_TILE / 64][VL_CAP];
@@ -292,11 +426,23 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     const bool act = i < s.part_end;
+    const uint32_t H = va.halo_n[tile];
+    const bool halo = H != VL_HALO_NONE;  // the same for the whole block
+    if (halo) {
+        const uint32_t *hl = va.halo + (size_t)tile * VL_HALO_CAP;
+        for (uint32_t k = threadIdx.x; k < H; k += PH_TILE) {
+            const uint32_t j = hl[k];
+            h_p[k] = posm[j];
+            h_w[k] = va.vword[j];
+            h_s[k] = j;
+        }
+        __syncthreads();
+    }
     const float4 pi = act ? posm[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const bool fin = finite3(pi.x, pi.y, pi.z);
     // Walk (every lane of the wave stays to the end: the repair below is
-    // the whole wave's).  The row's entries, four per 16-byte group; for
-    // each, the position and the neighbour word (independent loads).
+    // the whole wave's).  For each entry, the position and the neighbour
+    // word (independent loads).
     const float hg2 = s.h_grid2;
     const bool mine = act && fin;
     if (act && !fin) {
@@ -324,19 +470,36 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
                 lp.push(j);
             }
         };
-        const uint4 *row = rows + i;
+        if (halo) {
+            const uint4 *row = va.rows16 + i;
 #pragma unroll 1
-        for (uint32_t k = 0; k < n; k += 4) {
-            const uint4 e = *row;
-            row += va.vstride;
-            const bool v1 = k + 1 < n, v2 = k + 2 < n, v3 = k + 3 < n;
-            const uint32_t j0 = e.x, j1 = v1 ? e.y : e.x, j2 = v2 ? e.z : e.x, j3 = v3 ? e.w : e.x;
-            const float4 p0 = posm[j0], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
-            const uint32_t w0 = va.vword[j0], w1 = va.vword[j1], w2 = va.vword[j2], w3 = va.vword[j3];
-            one(j0, p0, w0);
-            if (v1) one(j1, p1, w1);
-            if (v2) one(j2, p2, w2);
-            if (v3) one(j3, p3, w3);
+            for (uint32_t k = 0; k < n; k += 8) {
+                const uint4 e = *row;
+                row += va.vstride;
+                const uint32_t ws[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+                for (uint32_t q = 0; q < 8; ++q) {
+                    if (k + q < n) {
+                        const uint32_t x = (ws[q >> 1] >> (16u * (q & 1u))) & 0xFFFFu;
+                        one(h_s[x], h_p[x], h_w[x]);
+                    }
+                }
+            }
+        } else {
+            const uint4 *row = rows + i;
+#pragma unroll 1
+            for (uint32_t k = 0; k < n; k += 4) {
+                const uint4 e = *row;
+                row += va.vstride;
+                const bool v1 = k + 1 < n, v2 = k + 2 < n, v3 = k + 3 < n;
+                const uint32_t j0 = e.x, j1 = v1 ? e.y : e.x, j2 = v2 ? e.z : e.x, j3 = v3 ? e.w : e.x;
+                const float4 p0 = posm[j0], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
+                const uint32_t w0 = va.vword[j0], w1 = va.vword[j1], w2 = va.vword[j2], w3 = va.vword[j3];
+                one(j0, p0, w0);
+                if (v1) one(j1, p1, w1);
+                if (v2) one(j2, p2, w2);
+                if (v3) one(j3, p3, w3);
+            }
         }
         fc = lp.finish();
     }
@@ -352,18 +515,28 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
         const int b = __ffsll((long long)mm) - 1;
         const uint32_t ri = __shfl(i, b, 64), rn = __shfl(n, b, 64), rw = __shfl(wi, b, 64);
         const float px = __shfl(pi.x, b, 64), py = __shfl(pi.y, b, 64), pz = __shfl(pi.z, b, 64);
-        const uint32_t *ru = reinterpret_cast<const uint32_t *>(rows);
-        // entries lane and lane + 64: slot with bit 31 set for a neighbour
-        // within h, its contribution, its key
+        const uint32_t *ru = reinterpret_cast<const uint32_t *>(halo ? va.rows16 : rows);
+        // entries lane and lane + 64: the entry (slot, or halo index) with
+        // bit 31 set for a neighbour within h, its contribution, its key
 #pragma unroll 1
         for (uint32_t e = lane; e < rn; e += 64u) {
-            const uint32_t j = ru[((size_t)(e >> 2) * va.vstride + ri) * 4u + (e & 3u)];
-            const float4 pj = posm[j];
-            const uint32_t key = vl_key(rw, va.vword[j]);
+            uint32_t x;
+            float4 pj;
+            uint32_t wj;
+            if (halo) {
+                x = (ru[((size_t)(e >> 3) * va.vstride + ri) * 4u + ((e & 7u) >> 1)] >> (16u * (e & 1u))) & 0xFFFFu;
+                pj = h_p[x];
+                wj = h_w[x];
+            } else {
+                x = ru[((size_t)(e >> 2) * va.vstride + ri) * 4u + (e & 3u)];
+                pj = posm[x];
+                wj = va.vword[x];
+            }
+            const uint32_t key = vl_key(rw, wj);
             const float r2 = dot3(pj.x - px, pj.y - py, pj.z - pz);
             const bool in = r2 < hg2 && key != VL_NOT27;
             s_key[w][e] = key;
-            s_uj[w][e] = j | (in ? 0x80000000u : 0u);
+            s_uj[w][e] = x | (in ? 0x80000000u : 0u);
             s_uc[w][e] = in ? pj.w * poly6(r2, s) : 0.0f;
         }
         wave_sync();
@@ -380,13 +553,23 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
         }
         wave_sync();
         // the row in visiting order, one 16-byte group per lane
-        const uint32_t g0 = lane * 4u;
-        if (g0 < rn) {
-            const uint32_t e0 = s_slot[w][g0] & 0x7FFFFFFFu;
-            const uint32_t e1 = g0 + 1 < rn ? s_slot[w][g0 + 1] & 0x7FFFFFFFu : 0u,
-                           e2 = g0 + 2 < rn ? s_slot[w][g0 + 2] & 0x7FFFFFFFu : 0u,
-                           e3 = g0 + 3 < rn ? s_slot[w][g0 + 3] & 0x7FFFFFFFu : 0u;
-            rows[(size_t)lane * va.vstride + ri] = make_uint4(e0, e1, e2, e3);
+        if (halo) {
+            const uint32_t g0 = lane * 8u;
+            if (g0 < rn) {
+                uint32_t v[4] = {0u, 0u, 0u, 0u};
+                for (uint32_t q = 0; q < 8 && g0 + q < rn; ++q)
+                    v[q >> 1] |= (s_slot[w][g0 + q] & 0xFFFFu) << (16u * (q & 1u));
+                va.rows16[(size_t)lane * va.vstride + ri] = make_uint4(v[0], v[1], v[2], v[3]);
+            }
+        } else {
+            const uint32_t g0 = lane * 4u;
+            if (g0 < rn) {
+                const uint32_t e0 = s_slot[w][g0] & 0x7FFFFFFFu;
+                const uint32_t e1 = g0 + 1 < rn ? s_slot[w][g0 + 1] & 0x7FFFFFFFu : 0u,
+                               e2 = g0 + 2 < rn ? s_slot[w][g0 + 2] & 0x7FFFFFFFu : 0u,
+                               e3 = g0 + 3 < rn ? s_slot[w][g0 + 3] & 0x7FFFFFFFu : 0u;
+                rows[(size_t)lane * va.vstride + ri] = make_uint4(e0, e1, e2, e3);
+            }
         }
         if (lane == (uint32_t)b) {
             rho = 0.0f;
@@ -397,7 +580,7 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
                 if (e & 0x80000000u) {
                     ++cnt;
                     rho += s_c[w][m];
-                    lp.push(e & 0x7FFFFFFFu);
+                    lp.push(halo ? h_s[e & 0xFFFFu] : (e & 0x7FFFFFFFu));
                 }
             }
             fc = lp.finish();
@@ -414,7 +597,7 @@ namespace ph {
 
 hipError_t launch_vl_decide(hipStream_t st, VerletDev *vs, const float *tile_dmax, uint32_t *tile_q, uint32_t ntiles,
                             uint32_t force, float thr) {
-    hipLaunchKernelGGL(k_vl_decide, dim3(1), dim3(PH_TILE), 0, st, vs, tile_dmax, tile_q, ntiles, force, thr);
+    hipLaunchKernelGGL(k_vl_decide, dim3(1), dim3(kDecideThreads), 0, st, vs, tile_dmax, tile_q, ntiles, force, thr);
     return hipGetLastError();
 }
 
@@ -425,6 +608,9 @@ hipError_t launch_density_verlet(hipStream_t st, uint32_t ntiles, const TileDev 
     hipLaunchKernelGGL(k_density_rebuild, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, rq,
                        rec_rho, rec_p, rec_count, fcnt, va);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_halo_build, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, va);
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_density_walk, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, posm, tag, rq, rec_rho,
                        rec_p, rec_count, fcnt, va);

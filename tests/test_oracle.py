@@ -320,3 +320,20 @@ def test_golden_fixtures_reproduce(O):
         out = make_golden.recompute(g)
         for k in ("density", "pressure", "accel", "count", "pos_out", "vel_out"):
             assert np.array_equal(out[k], g[k]), (f, k)
+
+
+def test_openmp_jacobi_equals_serial(oracle_lib):
+    """The all-cores baseline (oracle3_run_f32_par) is the one-thread Jacobi
+    oracle on OpenMP threads: equal bit for bit, with AV and cohesion on."""
+    from phi_eng_amd import scenes
+    O = oracle_lib
+    sc = scenes.small_dam(10, seed=41)
+    sc.config.artificial_viscosity = 0.3
+    sc.config.surface_tension = 0.1
+    rng = np.random.default_rng(41)
+    vel = rng.uniform(-0.5, 0.5, sc.positions.shape).astype(np.float32)
+    b = O.bodies_from_boxes(sc.boxes)
+    a = O.run_f32(sc.positions, vel, sc.config, sc.gravity, b, 5e-4, 4, jacobi=True)
+    p = O.run_f32_par(sc.positions, vel, sc.config, sc.gravity, b, 5e-4, 4, threads=4)
+    assert np.array_equal(a[0].view(np.uint32), p[0].view(np.uint32))
+    assert np.array_equal(a[1].view(np.uint32), p[1].view(np.uint32))
