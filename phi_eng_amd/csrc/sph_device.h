@@ -576,6 +576,32 @@ __device__ __forceinline__ float pack_pq(float q, uint32_t phase) {
     return __uint_as_float((__float_as_uint(q) & ~3u) | min(phase, 3u));
 }
 
+// The other 8 bytes of a neighbour record: the velocity as three halves
+// (rounded toward zero, so magnitudes saturate at 65504 instead of
+// overflowing) and dP as a bfloat16 (rounded to nearest even; it keeps the
+// f32 exponent range, which a pressure quotient of a light fluid needs).
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint2 pack_vd(float vx, float vy, float vz, float dq) {
+    const uint32_t u = __float_as_uint(dq);
+    const uint32_t bq = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+    const uint32_t hz = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(vz, 0.0f)) & 0xffffu;
+    return make_uint2(__builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(vx, vy)), hz | bq);
+}
+__device__ __forceinline__ void unpack_vd(uint2 w, float &vx, float &vy, float &vz, float &dq) {
+    const half2_t a = __builtin_bit_cast(half2_t, w.x), b = __builtin_bit_cast(half2_t, w.y);
+    vx = (float)a.x;
+    vy = (float)a.y;
+    vz = (float)b.x;
+    dq = __uint_as_float(w.y & 0xffff0000u);
+}
+
+// Neighbour records of one generation of the surrogate's state: per slot
+// j, pos[j] = (x, y, z, P | phase) and vd[j] = pack_vd(v, dP).
+struct SurRec {
+    float4 *pos = nullptr;
+    uint2 *vd = nullptr;
+};
+
 // What a judging plain substep of a surrogate world also writes from the
 // force pass (one pass instead of a history gather and an update kernel):
 // the new history (rho, P, P - P_prev) of each sorted slot, with P_prev read
@@ -586,7 +612,7 @@ struct AuxFuse {
     const uint32_t *perm = nullptr;   // sorted slot -> slot before the sort
     const float4 *aux_in = nullptr;   // history in the order before the sort
     float4 *aux_out = nullptr;        // null: nothing fused
-    float4 *rec_out = nullptr;        // may be null
+    SurRec rec_out;                   // pos may be null: no records
     float2 *tile_err = nullptr;
     // particles in contact with a dynamic body: defer[0] counts them,
     // defer[1 + k] (k < defer_cap) holds their slots; their state is left

@@ -629,9 +629,9 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             const uint32_t tg = tag[i];
             const float dq = rqi.y - ap.y;
             af.aux_out[i] = make_float4(rqi.x, rqi.y, dq, 0.0f);
-            if (af.rec_out) {
-                af.rec_out[2 * i] = make_float4(x, y, z, pack_pq(rqi.y, tg >> 24));
-                af.rec_out[2 * i + 1] = make_float4(ux, uy, uz, dq);
+            if (af.rec_out.pos) {
+                af.rec_out.pos[i] = make_float4(x, y, z, pack_pq(rqi.y, tg >> 24));
+                af.rec_out.vd[i] = pack_vd(ux, uy, uz, dq);
             }
             const float r = rqi.x / scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
             if (r >= 0.9f) {

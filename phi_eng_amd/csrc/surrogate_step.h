@@ -35,13 +35,14 @@ struct SurStepArgs {
     const uint32_t *cnt;   // skin list: candidate count | SLF_ flags
     const uint32_t *idx;   // skin list: candidates 4g..4g+3 of particle i as uint4 g * mstride + i
     uint32_t mstride;
-    // neighbour records of the current state, 2 float4 per slot:
-    // (x, y, z, P with the phase in its 2 low bits), (v, dP)
-    const float4 *rec;
+    // neighbour records of the current state: (x, y, z, P with the phase
+    // in its 2 low bits) and (v, dP) as halves and a bfloat16
+    SurRec rec;
     float4 mass;  // particle mass of phases 0..3
     MlpDev mlp;
     float qs;
-    float4 *posm_out, *vel_out, *aux_out, *rec_out;
+    float4 *posm_out, *vel_out, *aux_out;
+    SurRec rec_out;
 };
 
 namespace ph {
@@ -51,7 +52,7 @@ hipError_t launch_gather_aux(hipStream_t st, uint32_t n, const uint32_t *perm, c
 hipError_t launch_aux_update(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                              const float2 *rq, const float4 *aux_prev, const uint32_t *tag, float4 *aux_out,
                              float2 *tile_err, const float4 *posm_new = nullptr, const float4 *vel_new = nullptr,
-                             float4 *rec_out = nullptr);
+                             SurRec rec_out = SurRec());
 hipError_t launch_err_reduce(hipStream_t st, const float2 *tile_err, uint32_t ntiles, float *out);
 hipError_t launch_aux_reset(hipStream_t st, uint32_t n, const uint32_t *tag, float4 rho0, float4 *aux);
 }  // namespace ph

@@ -22,6 +22,18 @@
 #include "surrogate.h"
 #include "surrogate_step.h"
 
+// Indexed loads of a structured buffer: the record stride sits in the
+// resource and the slot is the index, so no offset arithmetic is spent per
+// load, and an index at or past the resource's count reads 0.  These are
+// LLVM's buffer-load intrinsics, which clang exposes as builtins only in
+// their raw (offset) form.
+typedef float sur_f4v __attribute__((ext_vector_type(4)));
+typedef uint32_t sur_u2v __attribute__((ext_vector_type(2)));
+__device__ sur_f4v sur_load_idx_b128(__amdgpu_buffer_rsrc_t, int, int, int, int) __asm(
+    "llvm.amdgcn.struct.ptr.buffer.load.v4f32");
+__device__ sur_u2v sur_load_idx_b64(__amdgpu_buffer_rsrc_t, int, int, int, int) __asm(
+    "llvm.amdgcn.struct.ptr.buffer.load.v2i32");
+
 namespace {
 
 struct SurAcc {
@@ -33,17 +45,16 @@ struct SurAcc {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;  // sum m_j W_ij v_j (the XSPH sum is this - v_i rho)
 };
 
-
 // One candidate of particle i, branch-free: contributes when ok and
-// r2 < h^2; self = (j == i).  ra = (x, y, z, P|phase), rb = (v, dP) of the
-// candidate; its mass is the phase's entry of mt.  Candidates come from
+// r2 < h^2.  ra = (x, y, z, P|phase), v and dq = dP of the candidate; its
+// mass is the phase's entry of mt.  Candidates come from
 // grid cells, which hold finite positions only, so offsets and data are
 // finite and a zero weight zeroes a term.  Fast math throughout (hardware
 // rsqrt, fused multiply-adds): the surrogate is judged on its density
 // error, not on bit parity.
 template <bool UMASS = false>
-__device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, const float4 &pi, float4 mt,
-                                         const float4 &ra, const float4 &rb, SurAcc &a) {
+__device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, const float4 &pi, float4 mt, const float4 &ra,
+                                         float vx, float vy, float vz, float dq, SurAcc &a) {
     const float dx = ra.x - pi.x, dy = ra.y - pi.y, dz = ra.z - pi.z;
     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const bool in = ok && r2 < s.h_grid2;
@@ -55,17 +66,18 @@ __device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, 
         const float m01 = p1 ? mt.y : mt.x, m23 = p1 ? mt.w : mt.z;
         m = p2 ? m23 : m01;
     }
-    const float mw = in ? m * (s.poly6_c * d * d * d) : 0.0f;
+    const float mw = in ? (m * s.poly6_c) * (d * d * d) : 0.0f;
     a.cnt += in ? 1u : 0u;
     a.rho += mw;
-    const bool nb = in && !self && r2 > PH_EPS_SQ;
+    // the particle itself (and any coincident one) sits at r2 = 0: no gradient
+    const bool nb = in && r2 > PH_EPS_SQ;
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float hr = s.h - r2 * inv_r;
     const float mg = nb ? -(m * s.spiky_c) * (hr * hr * inv_r) : 0.0f;  // m_j grad W = mg (x_j - x_i)
     a.gx = fmaf(mg, dx, a.gx);
     a.gy = fmaf(mg, dy, a.gy);
     a.gz = fmaf(mg, dz, a.gz);
-    const float mh = mg * ra.w, md = mg * rb.w;
+    const float mh = mg * ra.w, md = mg * dq;
     a.hx = fmaf(mh, dx, a.hx);
     a.hy = fmaf(mh, dy, a.hy);
     a.hz = fmaf(mh, dz, a.hz);
@@ -74,9 +86,9 @@ __device__ __forceinline__ void sur_cand(const SceneDev &s, bool ok, bool self, 
     a.dz = fmaf(md, dz, a.dz);
     // XSPH: sum m_j W_ij v_j (mw = 0 off the neighbourhood); the numerator
     // sum m_j W_ij (v_j - v_i) is this minus v_i rho, formed once per particle
-    a.sx = fmaf(rb.x, mw, a.sx);
-    a.sy = fmaf(rb.y, mw, a.sy);
-    a.sz = fmaf(rb.z, mw, a.sz);
+    a.sx = fmaf(vx, mw, a.sx);
+    a.sy = fmaf(vy, mw, a.sy);
+    a.sz = fmaf(vz, mw, a.sz);
 }
 
 // Sweep over particle i's grid candidates (collection on the gathered
@@ -91,8 +103,7 @@ __device__ __forceinline__ void sur_sweep(const SceneDev &s, const uint32_t *__r
     for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
         const float4 pj = posm[j], vj = vel[j], aj = aux[j];
         const float4 ra = make_float4(pj.x, pj.y, pj.z, pack_pq(aj.y, tag[j] >> 24));
-        const float4 rb = make_float4(vj.x, vj.y, vj.z, aj.z);
-        sur_cand(s, true, j == i, pi, mt, ra, rb, a);
+        sur_cand(s, true, pi, mt, ra, vj.x, vj.y, vj.z, aj.z, a);
     });
 }
 
@@ -190,34 +201,45 @@ __device__ __forceinline__ void sur_integrate(const SceneDev &s, const BodyDev *
     vo = make_float4(ux, uy, uz, 0.0f);
 }
 
-// float4 loads through a buffer resource: 32-bit offsets (no 64-bit
-// address arithmetic per load), and reads past `count` elements return 0.
-struct BufF4 {
-    __amdgpu_buffer_rsrc_t r;
-    __device__ BufF4(const float4 *p, uint32_t count)
-        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(p), (short)0, (int)(count * 16u), 0x00020000)) {}
-    __device__ __forceinline__ float4 operator[](uint32_t j) const {
-        return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, j * 16u, 0, 0));
+// The two record arrays of one generation, indexed by slot.
+struct RecBuf {
+    __amdgpu_buffer_rsrc_t p, v;
+    __device__ RecBuf(const SurRec &r, uint32_t count)
+        : p(__builtin_amdgcn_make_buffer_rsrc(r.pos, (short)16, (int)count, 0x00020000)),
+          v(__builtin_amdgcn_make_buffer_rsrc(r.vd, (short)8, (int)count, 0x00020000)) {}
+    __device__ __forceinline__ float4 pos(uint32_t j) const {
+        return __builtin_bit_cast(float4, sur_load_idx_b128(p, (int)j, 0, 0, 0));
+    }
+    __device__ __forceinline__ uint2 vd(uint32_t j) const {
+        return __builtin_bit_cast(uint2, sur_load_idx_b64(v, (int)j, 0, 0, 0));
     }
 };
+
+template <bool UMASS>
+__device__ __forceinline__ void sur_rec_cand(const SceneDev &s, bool ok, const float4 &pi, float4 mt,
+                                             const float4 &ra, uint2 rb, SurAcc &a) {
+    float vx, vy, vz, dq;
+    unpack_vd(rb, vx, vy, vz, dq);
+    sur_cand<UMASS>(s, ok, pi, mt, ra, vx, vy, vz, dq, a);
+}
 
 // Sweep over the skin list of particle i: its recorded candidates, a group
 // of four at a time with their eight record loads issued together while the
 // next group's slots are already in flight; the per-cell path when the list
-// could not hold them.  rec holds the 32-byte record of slot j at float4s
-// 2j, 2j + 1.  Group g (candidates 4g .. 4g + 3) of particle i is the
-// 16 bytes at grp[g * mstride + i]; the list holds one spare group row, so
-// the look-ahead load after the last group stays inside it, and every load
-// is unconditional (slots past the count are replaced by i and masked).
+// could not hold them.  Group g (candidates 4g .. 4g + 3) of particle i is
+// the 16 bytes at grp[g * mstride + i]; the list holds one spare group row,
+// so the look-ahead load after the last group stays inside it, and every
+// load is unconditional (slots past the count are replaced by i and masked).
 template <bool UMASS>
 __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t *__restrict__ start,
-                                               const float4 *__restrict__ posm_p, const BufF4 &rec, float4 mt,
+                                               const float4 *__restrict__ posm_p, const RecBuf &rec, float4 mt,
                                                uint32_t cw, const int4 &cell, const uint4 *__restrict__ grp_p,
                                                uint32_t mstride, uint32_t i, const float4 &pi, SurAcc &a) {
     if (cw & SLF_NONE) return;
     if (cw & SLF_SLOW) {
-        for_each_candidate(s, start, posm_p, cell.x, cell.y, cell.z,
-                           [&](uint32_t j) { sur_cand<UMASS>(s, true, j == i, pi, mt, rec[2 * j], rec[2 * j + 1], a); });
+        for_each_candidate(s, start, posm_p, cell.x, cell.y, cell.z, [&](uint32_t j) {
+            sur_rec_cand<UMASS>(s, true, pi, mt, rec.pos(j), rec.vd(j), a);
+        });
         return;
     }
     const uint32_t n = cw & SLF_COUNT;
@@ -226,28 +248,41 @@ __device__ __forceinline__ void sur_list_sweep(const SceneDev &s, const uint32_t
     // particle count
     const uint32_t off = 16u * i;
     const uint4 *row = grp_p;
-    auto load_row = [&](const uint4 *r) {
+    auto load_row = [&](const uint4 *r, uint32_t o) {
         const __amdgpu_buffer_rsrc_t lr =
             __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(r), (short)0, (int)(16u * mstride), 0x00020000);
-        return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lr, off, 0, 0));
+        return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lr, o, 0, 0));
     };
-    uint4 grp = load_row(row);
+    // rows past the last group read as zero (offset beyond the resource)
+    auto row_off = [&](uint32_t k) { return k < n ? off : 0xfffffff0u; };
+    struct Recs {
+        float4 p0, p1, p2, p3;
+        uint2 v0, v1, v2, v3;
+    };
+    auto fetch = [&](uint4 g, uint32_t k) {
+        const uint32_t j0 = k < n ? g.x : i, j1 = k + 1 < n ? g.y : i, j2 = k + 2 < n ? g.z : i,
+                       j3 = k + 3 < n ? g.w : i;
+        Recs r;
+        r.p0 = rec.pos(j0);
+        r.v0 = rec.vd(j0);
+        r.p1 = rec.pos(j1);
+        r.v1 = rec.vd(j1);
+        r.p2 = rec.pos(j2);
+        r.v2 = rec.vd(j2);
+        r.p3 = rec.pos(j3);
+        r.v3 = rec.vd(j3);
+        return r;
+    };
+    uint4 grp = load_row(row, off);
 #pragma unroll 1
     for (uint32_t k = 0; k < n; k += 4) {
-        const bool o1 = k + 1 < n, o2 = k + 2 < n, o3 = k + 3 < n;
-        const uint32_t j0 = grp.x, j1 = o1 ? grp.y : i, j2 = o2 ? grp.z : i, j3 = o3 ? grp.w : i;
-        const float4 a0 = rec[2 * j0], b0 = rec[2 * j0 + 1];
-        const float4 a1 = rec[2 * j1], b1 = rec[2 * j1 + 1];
-        const float4 a2 = rec[2 * j2], b2 = rec[2 * j2 + 1];
-        const float4 a3 = rec[2 * j3], b3 = rec[2 * j3 + 1];
-        // issued after the records: loads complete in order, so the records
-        // can be consumed while this one is still in flight
+        const Recs cur = fetch(grp, k);
         row += mstride;
-        const uint4 nxt = load_row(row);
-        sur_cand<UMASS>(s, true, j0 == i, pi, mt, a0, b0, a);
-        sur_cand<UMASS>(s, o1, j1 == i, pi, mt, a1, b1, a);
-        sur_cand<UMASS>(s, o2, j2 == i, pi, mt, a2, b2, a);
-        sur_cand<UMASS>(s, o3, j3 == i, pi, mt, a3, b3, a);
+        const uint4 nxt = load_row(row, row_off(k + 4));
+        sur_rec_cand<UMASS>(s, true, pi, mt, cur.p0, cur.v0, a);
+        sur_rec_cand<UMASS>(s, k + 1 < n, pi, mt, cur.p1, cur.v1, a);
+        sur_rec_cand<UMASS>(s, k + 2 < n, pi, mt, cur.p2, cur.v2, a);
+        sur_rec_cand<UMASS>(s, k + 3 < n, pi, mt, cur.p3, cur.v3, a);
         grp = nxt;
     }
 }
@@ -302,7 +337,7 @@ __global__ __launch_bounds__(PH_TILE) void k_sur_collect(SurCollectArgs a) {
 // operand the next one needs there), and the integrate; writes the advanced
 // state, the new history and the records of the next substep.
 template <bool UMASS>
-__global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
+__global__ __launch_bounds__(PH_TILE, 7) void k_sur_step(SurStepArgs a) {
     __shared__ uint16_t xs[PH_TILE][MLP_IN];
     __shared__ float ys[PH_TILE][4];
     __shared__ mlp_bf16x4 w1p[8][64], w2p[8][64];
@@ -339,7 +374,7 @@ __global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
         tg = a.tag[i];
         const uint32_t cw = a.cnt[i];
         const int4 cl = (cw & SLF_SLOW) ? a.cell[i] : make_int4(0, 0, 0, 0);
-        const BufF4 rec(a.rec, 2u * s.part_end);
+        const RecBuf rec(a.rec, s.part_end);
         const float4 mt = make_float4(a.mass.x, a.mass.y, a.mass.z, a.mass.w);
         SurAcc acc;
         sur_list_sweep<UMASS>(s, a.start, a.posm, rec, mt, cw, cl, reinterpret_cast<const uint4 *>(a.idx), a.mstride, i,
@@ -387,8 +422,8 @@ __global__ __launch_bounds__(PH_TILE, 6) void k_sur_step(SurStepArgs a) {
     a.vel_out[i] = vo;
     const float dq = f.q - ai.y;
     a.aux_out[i] = make_float4(f.rho, f.q, dq, 0.0f);
-    a.rec_out[2 * i] = make_float4(po.x, po.y, po.z, pack_pq(f.q, tg >> 24));
-    a.rec_out[2 * i + 1] = make_float4(vo.x, vo.y, vo.z, dq);
+    a.rec_out.pos[i] = make_float4(po.x, po.y, po.z, pack_pq(f.q, tg >> 24));
+    a.rec_out.vd[i] = pack_vd(vo.x, vo.y, vo.z, dq);
 }
 
 // aux_out[d] = aux_in[perm[d]]: history follows its particle through the sort.
@@ -405,7 +440,7 @@ __global__ __launch_bounds__(256) void k_aux_update(const TileDev *__restrict__ 
                                                     const float2 *__restrict__ rq, const float4 *__restrict__ aux_prev,
                                                     const uint32_t *__restrict__ tag, float4 *__restrict__ aux_out,
                                                     float2 *__restrict__ tile_err, const float4 *__restrict__ posm_new,
-                                                    const float4 *__restrict__ vel_new, float4 *__restrict__ rec_out) {
+                                                    const float4 *__restrict__ vel_new, SurRec rec_out) {
     const uint32_t tile = blockIdx.x;
     const TileDev t = tiles[tile];
     const SceneDev &s = scenes[t.scene];
@@ -416,11 +451,11 @@ __global__ __launch_bounds__(256) void k_aux_update(const TileDev *__restrict__ 
         const float4 ap = aux_prev[i];
         const uint32_t tg = tag[i];
         aux_out[i] = make_float4(v.x, v.y, v.y - ap.y, 0.0f);
-        if (rec_out) {
+        if (rec_out.pos) {
             // the surrogate's neighbour records of the new state
             const float4 p = posm_new[i], u = vel_new[i];
-            rec_out[2 * i] = make_float4(p.x, p.y, p.z, pack_pq(v.y, tg >> 24));
-            rec_out[2 * i + 1] = make_float4(u.x, u.y, u.z, v.y - ap.y);
+            rec_out.pos[i] = make_float4(p.x, p.y, p.z, pack_pq(v.y, tg >> 24));
+            rec_out.vd[i] = pack_vd(u.x, u.y, u.z, v.y - ap.y);
         }
         const float rho0 = s.phase_rho0[min(tg >> 24, 3u)];
         const float r = v.x / rho0;
@@ -499,7 +534,7 @@ hipError_t launch_gather_aux(hipStream_t st, uint32_t n, const uint32_t *perm, c
 }
 hipError_t launch_aux_update(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                              const float2 *rq, const float4 *aux_prev, const uint32_t *tag, float4 *aux_out,
-                             float2 *tile_err, const float4 *posm_new, const float4 *vel_new, float4 *rec_out) {
+                             float2 *tile_err, const float4 *posm_new, const float4 *vel_new, SurRec rec_out) {
     hipLaunchKernelGGL(k_aux_update, dim3(ntiles), dim3(256), 0, st, tiles, scenes, rq, aux_prev, tag, aux_out,
                        tile_err, posm_new, vel_new, rec_out);
     return hipGetLastError();

@@ -173,7 +173,16 @@ struct World3 {
     // list radius (1 + skin') h with skin' = max(skin, 4 check_every dt vmax / h):
     // a Verlet skin over the substeps one list serves, at least `skin`
     float skin = 0.01f;
-    ph::DBuf<float4> sl_rec[2];  // neighbour records of generation k (2 float4 per slot)
+    // neighbour records of generation k: rec_m positions (float4), then
+    // rec_m packed velocity words (uint2) in the same allocation
+    ph::DBuf<float4> sl_rec[2];
+    size_t rec_m = 0;
+    SurRec recs(int k) const {
+        SurRec r;
+        r.pos = sl_rec[k].p;
+        r.vd = reinterpret_cast<uint2 *>(sl_rec[k].p + rec_m);
+        return r;
+    }
     uint64_t list_sorts = ~0ull, list_epoch = ~0ull;
     uint32_t list_stride = 0;
     uint64_t n_col_skipped = 0;
@@ -767,9 +776,11 @@ struct World3 {
             ++n_col_skipped;
         }
         if (collect) col->grow_samples((uint64_t)col->nsamples + cnt);
-        if (sur) {
-            sl_rec[0].ensure(2 * e.alloc_particles);
-            sl_rec[1].ensure(2 * e.alloc_particles);
+        if (sur && rec_m != e.alloc_particles) {
+            // both generations are rewritten before a surrogate substep reads them
+            rec_m = e.alloc_particles;
+            sl_rec[0].ensure(rec_m + (rec_m + 1) / 2);
+            sl_rec[1].ensure(rec_m + (rec_m + 1) / 2);
         }
         e.sort_phase(e.perm.p);
         // Without a collection the force pass also writes the new history
@@ -782,7 +793,7 @@ struct World3 {
             af.perm = e.perm.p;
             af.aux_in = e.aux[e.cs].p;
             af.aux_out = e.aux[ns].p;
-            af.rec_out = sur ? sl_rec[e.cs].p : nullptr;
+            if (sur) af.rec_out = recs(e.cs);
             af.tile_err = e.tile_err.p;
             e.plain_phase(nullptr, &af);
             e.aux[0].swap(e.aux[1]);  // the new history is generation cs
@@ -823,7 +834,7 @@ struct World3 {
             e.run(ph::KIND_SUR_AUX, [&] {
                 return ph::launch_aux_update(e.stream, nt, e.d_tiles.p, e.d_scenes.p, e.rq.p, e.aux[ns].p,
                                              e.tag[ntg].p, e.aux[e.cs].p, e.tile_err.p, e.posm[e.cs].p, e.vel[e.cs].p,
-                                             sur ? sl_rec[e.cs].p : nullptr);
+                                             sur ? recs(e.cs) : SurRec());
             });
         if (judge) {
             apply_judge();  // an older verdict still pending (check_every < 2)
@@ -859,8 +870,8 @@ struct World3 {
         a.cnt = e.nl_cnt.p;
         a.idx = e.nl_idx.p;
         a.mstride = list_stride;
-        a.rec = sl_rec[e.cs].p;
-        a.rec_out = sl_rec[1 - e.cs].p;
+        a.rec = recs(e.cs);
+        a.rec_out = recs(1 - e.cs);
         a.mass = mass4();
         a.mlp = sur->dev();
         a.qs = qs();
