@@ -310,14 +310,22 @@ int32_t phieng_surrogate3_eval(Surrogate3 *s, float *out_loss, float *out_loss_b
 const char *phieng_surrogate3_last_error(void);
 
 /* Step `world` with the surrogate in the loop (null disables it): every
- * check_every-th substep is a plain substep that also judges the state —
- * the mean |rho/rho0 - 1| over particles with rho >= 0.9 rho0.  The verdict
- * is read back without stalling the stream and applied two substeps after
- * the judging one: if it exceeds max_density_error, the check_every
- * substeps from there on stay plain (a fallback), so while judges keep
- * failing only the one substep after each judge can be a surrogate one.  use_network = 0 runs the same substep with the
- * network's coefficients fixed at zero (the method without the network).
- * The world must not be in a batch; 1 / 0. */
+ * check_every-th substep is a plain substep that also judges the state on
+ * two measures — the mean |rho/rho0 - 1| over particles with
+ * rho >= 0.9 rho0, and the surrogate's acceleration error: before the plain
+ * substep the surrogate is evaluated on the same state for every 16th
+ * particle (the offset rotating from judge to judge), and the error is
+ * sum |a_plain - a_sur| / sum |a_plain| over those particles (pressure and
+ * viscous accelerations, gravity excluded; not evaluated, -1, when no
+ * surrogate substep could have run from that state or the judge also
+ * collects samples).  The verdict is read back without stalling the stream
+ * and applied two substeps after the judging one: if the density error
+ * exceeds max_density_error or the acceleration error exceeds the limit of
+ * phieng_world3_set_surrogate_accel_limit, the check_every substeps from
+ * there on stay plain (a fallback), so while judges keep failing only the
+ * one substep after each judge can be a surrogate one.  use_network = 0
+ * runs the same substep with the network's coefficients fixed at zero (the
+ * method without the network).  The world must not be in a batch; 1 / 0. */
 int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check_every,
                                     float max_density_error, int32_t use_network);
 /* While set (null clears), every `every`-th substep of `world` is plain and
@@ -329,6 +337,13 @@ int32_t phieng_world3_collect_samples(World3 *world, Surrogate3 *s, uint32_t eve
  * 1 / 0. */
 int32_t phieng_world3_surrogate_stats(const World3 *world, uint64_t *sur_steps, uint64_t *plain_steps,
                                       uint64_t *fallbacks, float *last_density_error);
+/* Limit of the judge's acceleration error (default 0.2; > 0, +inf turns
+ * the measure off).  No counterpart in the reference.  1 / 0. */
+int32_t phieng_world3_set_surrogate_accel_limit(World3 *world, float max_accel_error);
+/* The newest judged density and acceleration errors (-1 before any / when
+ * not evaluated), including a verdict not yet applied.  No counterpart in
+ * the reference.  1 / 0. */
+int32_t phieng_world3_surrogate_judge(const World3 *world, float *density_error, float *accel_error);
 /* Minimum skin of the surrogate's neighbour list: surrogate substeps
  * between two plain substeps see the candidates that were within
  * (1 + s) h at the plain one, with the Verlet skin

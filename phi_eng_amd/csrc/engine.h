@@ -258,7 +258,7 @@ struct Engine {
     bool aux_on = false;
     DBuf<uint32_t> perm;
     DBuf<float4> acc_sorted;
-    DBuf<float2> tile_err;
+    DBuf<float4> tile_err;
     DBuf<float> err_out;
     // Boundary ghosts of scene 0 (single-world engines; fluid3.rs:716-920):
     // cell-sorted positions + mass, per-substep state, cell table.

@@ -1,10 +1,14 @@
 // mlp.h — the surrogate's 2-layer MLP (16 -> 128 GELU -> 16) on MFMA.
 //
-// BF16 weights and activations feed v_mfma_f32_16x16x16bf16_1k tiles with
-// FP32 accumulation; biases, GELU and the loss are FP32.  A block of 256
-// threads evaluates 256 rows: wave w owns rows [64 w, 64 w + 64) as four
-// 16-row tiles.  Lane layout of one 16x16x16 tile (lane l, r = l % 16,
-// q = l / 16): A[r][4q + i], B[4q + i][r], D[4q + i][r], i = 0..3.
+// BF16 weights and activations feed MFMA tiles with FP32 accumulation;
+// biases, GELU and the loss are FP32.  A block of 256 threads evaluates 256
+// rows: wave w owns rows [64 w, 64 w + 64) as four 16-row tiles.  Products
+// that reduce over the 128 hidden units (the forward's second layer) use
+// gfx950's v_mfma_f32_16x16x32_bf16; those that reduce over 16 (the first
+// layer's 16 inputs, and the backward's products over 16 outputs or a
+// 16-row tile) use v_mfma_f32_16x16x16_bf16.  Lane layouts (lane l,
+// r = l % 16, q = l / 16): K = 16: A[r][4q + i], B[4q + i][r], i = 0..3;
+// K = 32: A[r][8q + e], B[8q + e][r], e = 0..7; both: D[4q + i][r].
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -22,6 +26,7 @@
 #define MLP_NPARAM (MLP_OFF_B2 + MLP_OUT)
 
 typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef short mlp_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
 
 struct MlpDev {
@@ -63,20 +68,27 @@ __device__ __forceinline__ float mlp_gelu(float z, float *dgelu) {
 __device__ __forceinline__ mlp_f32x4 mlp_mfma(mlp_bf16x4 a, mlp_bf16x4 b, mlp_f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+typedef __bf16 mlp_bf16v8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ mlp_f32x4 mlp_mfma32(mlp_bf16x8 a, mlp_bf16x8 b, mlp_f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mlp_bf16v8, a),
+                                                   __builtin_bit_cast(mlp_bf16v8, b), c, 0, 0, 0);
+}
 
 // Forward of the block's 256 rows: inputs in xs (bf16), outputs to ys
 // (fp32).  All 256 threads must call it.
 __device__ __forceinline__ void mlp_forward_block(const MlpDev &m, uint16_t (*xs)[MLP_IN],
                                                   uint16_t (*hs)[16][MLP_HPAD], float (*ys)[MLP_OUT]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
-    mlp_bf16x4 b1v[8], b2v[8];
+    mlp_bf16x4 b1v[8];
+    mlp_bf16x8 b2v[4];
 #pragma unroll
     for (int nt = 0; nt < 8; ++nt)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            b1v[nt][i] = (short)m.w1[(4 * q + i) * MLP_HID + nt * 16 + r];
-            b2v[nt][i] = (short)m.w2[(nt * 16 + 4 * q + i) * MLP_OUT + r];
-        }
+        for (int i = 0; i < 4; ++i) b1v[nt][i] = (short)m.w1[(4 * q + i) * MLP_HID + nt * 16 + r];
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) b2v[kt][e] = (short)m.w2[(kt * 32 + 8 * q + e) * MLP_OUT + r];
     __syncthreads();
     for (int mt = 0; mt < 4; ++mt) {
         const int r0 = w * 64 + mt * 16;
@@ -94,11 +106,11 @@ __device__ __forceinline__ void mlp_forward_block(const MlpDev &m, uint16_t (*xs
         __syncthreads();
         mlp_f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int kt = 0; kt < 8; ++kt) {
-            mlp_bf16x4 a2;
+        for (int kt = 0; kt < 4; ++kt) {
+            mlp_bf16x8 a2;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a2[i] = (short)hs[w][r][kt * 16 + 4 * q + i];
-            y = mlp_mfma(a2, b2v[kt], y);
+            for (int e = 0; e < 8; ++e) a2[e] = (short)hs[w][r][kt * 32 + 8 * q + e];
+            y = mlp_mfma32(a2, b2v[kt], y);
         }
         const float bias2 = m.b2[r];
 #pragma unroll

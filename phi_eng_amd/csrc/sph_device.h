@@ -606,14 +606,20 @@ struct SurRec {
 // force pass (one pass instead of a history gather and an update kernel):
 // the new history (rho, P, P - P_prev) of each sorted slot, with P_prev read
 // through the sort's permutation from the previous order, the neighbour
-// records of the advanced state, and per tile the bulk density deviation
-// sum |rho/rho0 - 1| over rho >= 0.9 rho0 and its count.
+// records of the advanced state, and the judge's per-tile sums.
 struct AuxFuse {
     const uint32_t *perm = nullptr;   // sorted slot -> slot before the sort
     const float4 *aux_in = nullptr;   // history in the order before the sort
     float4 *aux_out = nullptr;        // null: nothing fused
     SurRec rec_out;                   // pos may be null: no records
-    float2 *tile_err = nullptr;
+    // per tile: (sum |rho/rho0 - 1|, count) of the bulk, and over the
+    // particles the surrogate was evaluated on before this substep
+    // (previous slot o with o % sur_stride == sur_off, its acceleration at
+    // sur_acc[o / sur_stride]; sur_acc null: none) the sums of
+    // |a_plain - a_sur| and |a_plain| (accelerations without gravity)
+    float4 *tile_err = nullptr;
+    const float4 *sur_acc = nullptr;
+    uint32_t sur_stride = 16u, sur_off = 0u;
     // particles in contact with a dynamic body: defer[0] counts them,
     // defer[1 + k] (k < defer_cap) holds their slots; their state is left
     // as before the boundary clamp (null: no dynamic bodies)

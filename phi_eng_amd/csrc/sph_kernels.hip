@@ -523,7 +523,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
     const uint32_t i = t.first + threadIdx.x;
     const bool active = i < s.part_end;
     int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bz0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN, bz1 = INT32_MIN;
-    float jerr = 0.0f, jcnt = 0.0f;  // judge sums (AuxFuse)
+    float jerr = 0.0f, jcnt = 0.0f, jae = 0.0f, jam = 0.0f;  // judge sums (AuxFuse)
     float dmax2 = 0.0f;               // largest squared displacement (VL)
     if (active) {
         const float4 pi = posm[i];
@@ -625,7 +625,8 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             rec_acc[3 * id + 2] = az;
         }
         if (af.aux_out) {
-            const float4 ap = af.aux_in[af.perm[i]];
+            const uint32_t o = af.perm[i];
+            const float4 ap = af.aux_in[o];
             const uint32_t tg = tag[i];
             const float dq = rqi.y - ap.y;
             af.aux_out[i] = make_float4(rqi.x, rqi.y, dq, 0.0f);
@@ -637,6 +638,12 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             if (r >= 0.9f) {
                 jerr = fabsf(r - 1.0f);
                 jcnt = 1.0f;
+            }
+            if (af.sur_acc && o % af.sur_stride == af.sur_off) {
+                const float4 as = af.sur_acc[o / af.sur_stride];
+                const float px = ax - s.gx, py = ay - s.gy, pz = az - s.gz;
+                jae = sqrtf(dot3(px - as.x, py - as.y, pz - as.z));
+                jam = sqrtf(dot3(px, py, pz));
             }
         }
         if (finite3(x, y, z)) {
@@ -667,9 +674,11 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         for (int d = 32; d >= 1; d >>= 1) {
             jerr += __shfl_xor(jerr, d, 64);
             jcnt += __shfl_xor(jcnt, d, 64);
+            jae += __shfl_xor(jae, d, 64);
+            jam += __shfl_xor(jam, d, 64);
         }
     __shared__ int32_t wb[4][6];
-    __shared__ float wj[4][2];
+    __shared__ float wj[4][4];
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         wb[w][0] = bx0;
@@ -680,6 +689,8 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         wb[w][5] = bz1;
         wj[w][0] = jerr;
         wj[w][1] = jcnt;
+        wj[w][2] = jae;
+        wj[w][3] = jam;
         if (VL) wj[w][0] = dmax2;
     }
     __syncthreads();
@@ -692,7 +703,9 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         bbox[6 * (size_t)tile + c] = v;
     }
     if (judge && threadIdx.x == 0)
-        af.tile_err[tile] = make_float2(wj[0][0] + wj[1][0] + wj[2][0] + wj[3][0], wj[0][1] + wj[1][1] + wj[2][1] + wj[3][1]);
+        af.tile_err[tile] =
+            make_float4(wj[0][0] + wj[1][0] + wj[2][0] + wj[3][0], wj[0][1] + wj[1][1] + wj[2][1] + wj[3][1],
+                        wj[0][2] + wj[1][2] + wj[2][2] + wj[3][2], wj[0][3] + wj[1][3] + wj[2][3] + wj[3][3]);
 }
 
 // Largest speed of each tile's particles (the list radius's input).

@@ -43,16 +43,21 @@ struct SurStepArgs {
     float qs;
     float4 *posm_out, *vel_out, *aux_out;
     SurRec rec_out;
+    // the judge's evaluation (launch_sur_judge): slots judge_off +
+    // judge_stride k, accelerations without gravity to judge_acc[k]
+    float4 *judge_acc;
+    uint32_t judge_stride, judge_off;
 };
 
 namespace ph {
 hipError_t launch_sur_collect(hipStream_t st, uint32_t ntiles, const SurCollectArgs &a);
 hipError_t launch_sur_step(hipStream_t st, uint32_t ntiles, const SurStepArgs &a);
+hipError_t launch_sur_judge(hipStream_t st, uint32_t nsamples, const SurStepArgs &a);
 hipError_t launch_gather_aux(hipStream_t st, uint32_t n, const uint32_t *perm, const float4 *in, float4 *out);
 hipError_t launch_aux_update(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                              const float2 *rq, const float4 *aux_prev, const uint32_t *tag, float4 *aux_out,
-                             float2 *tile_err, const float4 *posm_new = nullptr, const float4 *vel_new = nullptr,
+                             float4 *tile_err, const float4 *posm_new = nullptr, const float4 *vel_new = nullptr,
                              SurRec rec_out = SurRec());
-hipError_t launch_err_reduce(hipStream_t st, const float2 *tile_err, uint32_t ntiles, float *out);
+hipError_t launch_err_reduce(hipStream_t st, const float4 *tile_err, uint32_t ntiles, float *out);
 hipError_t launch_aux_reset(hipStream_t st, uint32_t n, const uint32_t *tag, float4 rho0, float4 *aux);
 }  // namespace ph

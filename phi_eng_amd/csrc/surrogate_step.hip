@@ -333,32 +333,43 @@ __global__ __launch_bounds__(PH_TILE) void k_sur_collect(SurCollectArgs a) {
 
 // One surrogate substep: the skin-list sweep over the neighbour records,
 // the network on MFMA (computed transposed, H^T = W1^T X^T and
-// Y^T = W2^T H^T, so the hidden tile that one MFMA leaves in a lane is the
-// operand the next one needs there), and the integrate; writes the advanced
-// state, the new history and the records of the next substep.
-template <bool UMASS>
+// Y^T = W2^T H^T, so the hidden tiles that two K = 16 products leave in a
+// lane are the operand the K = 32 product needs there), and the integrate;
+// writes the advanced state, the new history and the records of the next
+// substep.  JUDGE:
+// the same evaluation for every judge_stride-th slot from judge_off (the
+// slots of sample k = blockIdx.x * PH_TILE + threadIdx.x), writing only the
+// acceleration without gravity to judge_acc[k] (one scene).
+template <bool UMASS, bool JUDGE>
 __global__ __launch_bounds__(PH_TILE, 7) void k_sur_step(SurStepArgs a) {
     __shared__ uint16_t xs[PH_TILE][MLP_IN];
     __shared__ float ys[PH_TILE][4];
-    __shared__ mlp_bf16x4 w1p[8][64], w2p[8][64];
+    __shared__ mlp_bf16x4 w1p[8][64];
+    __shared__ mlp_bf16x8 w2q[4][64];
     __shared__ __align__(16) float b1s[MLP_HID];
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const TileDev t = a.tiles[tile];
-    const SceneDev s = a.scenes[t.scene];
-    const uint32_t i = t.first + threadIdx.x;
+    const uint32_t tile = JUDGE ? 0u : xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t scene = JUDGE ? 0u : a.tiles[tile].scene;
+    const SceneDev s = a.scenes[scene];
+    const uint32_t sample = blockIdx.x * PH_TILE + threadIdx.x;
+    const uint32_t i = JUDGE ? sample * a.judge_stride + a.judge_off : a.tiles[tile].first + threadIdx.x;
     const bool active = i < s.part_end;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
     if (a.use_net) {
-        // lane-ready weight tiles: w1p[nt][l] = W1[4q+i][16 nt + r], w2p[nt][l] = W2[16 nt + 4q + i][r]
+        // lane-ready weight tiles: w1p[nt][l] = W1[4q+i][16 nt + r] (K = 16);
+        // w2q[p][l] = W2[hid(p, q, e)][r] (K = 32), hid(p, q, e) =
+        // 16 (2p + e / 4) + 4q + e % 4: the hidden units that the first
+        // layer's tiles 2p and 2p + 1 leave in lane l
         for (int e = threadIdx.x; e < 8 * 64; e += PH_TILE) {
             const int nt = e >> 6, l = e & 63, lr = l & 15, lq = l >> 4;
-            mlp_bf16x4 v1, v2;
-            for (int k = 0; k < 4; ++k) {
-                v1[k] = (short)a.mlp.w1[(4 * lq + k) * MLP_HID + nt * 16 + lr];
-                v2[k] = (short)a.mlp.w2[(nt * 16 + 4 * lq + k) * MLP_OUT + lr];
-            }
+            mlp_bf16x4 v1;
+            for (int k = 0; k < 4; ++k) v1[k] = (short)a.mlp.w1[(4 * lq + k) * MLP_HID + nt * 16 + lr];
             w1p[nt][l] = v1;
-            w2p[nt][l] = v2;
+            if (nt < 4) {
+                mlp_bf16x8 v2;
+                for (int k = 0; k < 8; ++k)
+                    v2[k] = (short)a.mlp.w2[(16 * (2 * nt + (k >> 2)) + 4 * lq + (k & 3)) * MLP_OUT + lr];
+                w2q[nt][l] = v2;
+            }
         }
         if (threadIdx.x < MLP_HID) b1s[threadIdx.x] = a.mlp.b1[threadIdx.x];
         __syncthreads();  // the only block-wide wait: each wave's MLP below uses its own LDS rows
@@ -379,7 +390,7 @@ __global__ __launch_bounds__(PH_TILE, 7) void k_sur_step(SurStepArgs a) {
         SurAcc acc;
         sur_list_sweep<UMASS>(s, a.start, a.posm, rec, mt, cw, cl, reinterpret_cast<const uint4 *>(a.idx), a.mstride, i,
                               pi, acc);
-        const float rho0 = a.scenes[t.scene].phase_rho0[min(tg >> 24, 3u)];
+        const float rho0 = a.scenes[scene].phase_rho0[min(tg >> 24, 3u)];
         sur_features(s, rho0, a.qs, acc, ai.x, ai.z, f);
         const float irl = __builtin_amdgcn_rcpf(rmaxf(ai.x, PH_EPS));
         xsx = fmaf(-vi.x, acc.rho, acc.sx) * irl;
@@ -397,14 +408,20 @@ __global__ __launch_bounds__(PH_TILE, 7) void k_sur_step(SurStepArgs a) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) xb[k] = (short)xs[row][4 * q + k];
             mlp_f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 2
-            for (int nt = 0; nt < 8; ++nt) {
-                const float4 bv = *reinterpret_cast<const float4 *>(&b1s[nt * 16 + 4 * q]);
-                const mlp_f32x4 z = mlp_mfma(w1p[nt][lane], xb, mlp_f32x4{bv.x, bv.y, bv.z, bv.w});  // + b1
-                mlp_bf16x4 hb;
+#pragma unroll 1
+            for (int p = 0; p < 4; ++p) {
+                // hidden tiles 2p and 2p + 1 (+ b1), then one K = 32 product over both
+                const float4 bv0 = *reinterpret_cast<const float4 *>(&b1s[32 * p + 4 * q]);
+                const float4 bv1 = *reinterpret_cast<const float4 *>(&b1s[32 * p + 16 + 4 * q]);
+                const mlp_f32x4 z0 = mlp_mfma(w1p[2 * p][lane], xb, mlp_f32x4{bv0.x, bv0.y, bv0.z, bv0.w});
+                const mlp_f32x4 z1 = mlp_mfma(w1p[2 * p + 1][lane], xb, mlp_f32x4{bv1.x, bv1.y, bv1.z, bv1.w});
+                mlp_bf16x8 hb;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) hb[k] = (short)mlp_bf16_hw(mlp_gelu(z[k], nullptr));
-                y = mlp_mfma(w2p[nt][lane], hb, y);
+                for (int k = 0; k < 4; ++k) {
+                    hb[k] = (short)mlp_bf16_hw(mlp_gelu(z0[k], nullptr));
+                    hb[4 + k] = (short)mlp_bf16_hw(mlp_gelu(z1[k], nullptr));
+                }
+                y = mlp_mfma32(w2q[p][lane], hb, y);
             }
             if (q == 0)
                 *reinterpret_cast<float4 *>(ys[row]) =
@@ -416,6 +433,10 @@ __global__ __launch_bounds__(PH_TILE, 7) void k_sur_step(SurStepArgs a) {
     if (!active) return;
     float ap[3];
     for (int d = 0; d < 3; ++d) ap[d] = f.a0[d] + c[0] * f.b[0][d] + c[1] * f.b[1][d] + c[2] * f.b[2][d];
+    if (JUDGE) {
+        a.judge_acc[sample] = make_float4(ap[0], ap[1], ap[2], 0.0f);
+        return;
+    }
     float4 po, vo;
     sur_integrate(s, a.bodies, pi, vi, ap, xsx, xsy, xsz, po, vo);
     a.posm_out[i] = po;
@@ -435,11 +456,12 @@ __global__ __launch_bounds__(256) void k_gather_aux(uint32_t n, const uint32_t *
 
 // After a plain substep: new history (rho, P, P - P_prev) and, per tile,
 // the bulk density deviation sum |rho/rho0 - 1| over particles with
-// rho >= 0.9 rho0, and their count (the judge's measure).
+// rho >= 0.9 rho0, and their count (the judge's density measure; this path
+// leaves the acceleration sums at 0).
 __global__ __launch_bounds__(256) void k_aux_update(const TileDev *__restrict__ tiles, const SceneDev *__restrict__ scenes,
                                                     const float2 *__restrict__ rq, const float4 *__restrict__ aux_prev,
                                                     const uint32_t *__restrict__ tag, float4 *__restrict__ aux_out,
-                                                    float2 *__restrict__ tile_err, const float4 *__restrict__ posm_new,
+                                                    float4 *__restrict__ tile_err, const float4 *__restrict__ posm_new,
                                                     const float4 *__restrict__ vel_new, SurRec rec_out) {
     const uint32_t tile = blockIdx.x;
     const TileDev t = tiles[tile];
@@ -475,31 +497,38 @@ __global__ __launch_bounds__(256) void k_aux_update(const TileDev *__restrict__ 
     }
     __syncthreads();
     if (threadIdx.x == 0)
-        tile_err[tile] = make_float2(red[0][0] + red[1][0] + red[2][0] + red[3][0],
-                                     red[0][1] + red[1][1] + red[2][1] + red[3][1]);
+        tile_err[tile] = make_float4(red[0][0] + red[1][0] + red[2][0] + red[3][0],
+                                     red[0][1] + red[1][1] + red[2][1] + red[3][1], 0.0f, 0.0f);
 }
 
-__global__ __launch_bounds__(256) void k_err_reduce(const float2 *__restrict__ tile_err, uint32_t ntiles,
-                                                    float *__restrict__ out) {
-    double e = 0.0, c = 0.0;
-    for (uint32_t k = threadIdx.x; k < ntiles; k += 256) {
-        e += tile_err[k].x;
-        c += tile_err[k].y;
+// The judge's verdict from the per-tile sums: out[0] = mean bulk density
+// deviation, out[1] = its particle count, out[2] = sum |a_plain - a_sur| /
+// sum |a_plain| over the evaluated particles (-1 without any), out[3] = the
+// denominator.  One block, sums in double in a fixed order.
+__global__ __launch_bounds__(1024) void k_err_reduce(const float4 *__restrict__ tile_err, uint32_t ntiles,
+                                                     float *__restrict__ out) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k = threadIdx.x; k < ntiles; k += 1024) {
+        const float4 t = tile_err[k];
+        v[0] += t.x;
+        v[1] += t.y;
+        v[2] += t.z;
+        v[3] += t.w;
     }
-    __shared__ double re[256], rc[256];
-    re[threadIdx.x] = e;
-    rc[threadIdx.x] = c;
+    for (int d = 32; d >= 1; d >>= 1)
+        for (int c = 0; c < 4; ++c) v[c] += __shfl_xor(v[c], d, 64);
+    __shared__ double red[16][4];
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 4; ++c) red[threadIdx.x >> 6][c] = v[c];
     __syncthreads();
-    for (int d = 128; d >= 1; d >>= 1) {
-        if ((int)threadIdx.x < d) {
-            re[threadIdx.x] += re[threadIdx.x + d];
-            rc[threadIdx.x] += rc[threadIdx.x + d];
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        out[0] = (float)(rc[0] > 0 ? re[0] / rc[0] : 0.0);
-        out[1] = (float)rc[0];
+        double t[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int w = 0; w < 16; ++w)
+            for (int c = 0; c < 4; ++c) t[c] += red[w][c];
+        out[0] = (float)(t[1] > 0 ? t[0] / t[1] : 0.0);
+        out[1] = (float)t[1];
+        out[2] = (float)(t[3] > 0 ? t[2] / t[3] : -1.0);
+        out[3] = (float)t[3];
     }
 }
 
@@ -523,9 +552,19 @@ hipError_t launch_sur_step(hipStream_t st, uint32_t ntiles, const SurStepArgs &a
     // skips the per-candidate phase lookup
     const bool umass = a.mass.x == a.mass.y && a.mass.x == a.mass.z && a.mass.x == a.mass.w;
     if (umass)
-        hipLaunchKernelGGL(k_sur_step<true>, dim3(ntiles), dim3(PH_TILE), 0, st, a);
+        hipLaunchKernelGGL((k_sur_step<true, false>), dim3(ntiles), dim3(PH_TILE), 0, st, a);
     else
-        hipLaunchKernelGGL(k_sur_step<false>, dim3(ntiles), dim3(PH_TILE), 0, st, a);
+        hipLaunchKernelGGL((k_sur_step<false, false>), dim3(ntiles), dim3(PH_TILE), 0, st, a);
+    return hipGetLastError();
+}
+hipError_t launch_sur_judge(hipStream_t st, uint32_t nsamples, const SurStepArgs &a) {
+    if (nsamples == 0) return hipSuccess;
+    const bool umass = a.mass.x == a.mass.y && a.mass.x == a.mass.z && a.mass.x == a.mass.w;
+    const dim3 grid((nsamples + PH_TILE - 1) / PH_TILE);
+    if (umass)
+        hipLaunchKernelGGL((k_sur_step<true, true>), grid, dim3(PH_TILE), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_sur_step<false, true>), grid, dim3(PH_TILE), 0, st, a);
     return hipGetLastError();
 }
 hipError_t launch_gather_aux(hipStream_t st, uint32_t n, const uint32_t *perm, const float4 *in, float4 *out) {
@@ -534,13 +573,13 @@ hipError_t launch_gather_aux(hipStream_t st, uint32_t n, const uint32_t *perm, c
 }
 hipError_t launch_aux_update(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                              const float2 *rq, const float4 *aux_prev, const uint32_t *tag, float4 *aux_out,
-                             float2 *tile_err, const float4 *posm_new, const float4 *vel_new, SurRec rec_out) {
+                             float4 *tile_err, const float4 *posm_new, const float4 *vel_new, SurRec rec_out) {
     hipLaunchKernelGGL(k_aux_update, dim3(ntiles), dim3(256), 0, st, tiles, scenes, rq, aux_prev, tag, aux_out,
                        tile_err, posm_new, vel_new, rec_out);
     return hipGetLastError();
 }
-hipError_t launch_err_reduce(hipStream_t st, const float2 *tile_err, uint32_t ntiles, float *out) {
-    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(256), 0, st, tile_err, ntiles, out);
+hipError_t launch_err_reduce(hipStream_t st, const float4 *tile_err, uint32_t ntiles, float *out) {
+    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, tile_err, ntiles, out);
     return hipGetLastError();
 }
 hipError_t launch_aux_reset(hipStream_t st, uint32_t n, const uint32_t *tag, float4 rho0, float4 *aux) {

@@ -158,6 +158,16 @@ struct World3 {
     uint64_t step_index = 0;
     uint64_t n_sur = 0, n_plain = 0, n_fallback = 0;
     float last_err = -1.0f;
+    // The judge's second measure: before its plain substep the surrogate is
+    // evaluated on the judged state for every kJudgeStride-th slot (the
+    // offset rotating per judge); the plain force pass then sums
+    // |a_plain - a_sur| and |a_plain| over the same particles.  The verdict
+    // fails above max_accel_err of their ratio as well as above max_err.
+    static constexpr uint32_t kJudgeStride = 16;
+    float max_accel_err = 0.2f;
+    float last_accel_err = -1.0f;
+    uint32_t n_judges = 0;
+    ph::DBuf<float4> judge_acc;
     // training-sample collection from plain substeps
     Surrogate3 *col = nullptr;
     uint32_t col_every = 10, col_stride = 64, col_phase = 0;
@@ -722,10 +732,14 @@ struct World3 {
         PH_CK(hipEventSynchronize(judge_ev));
         judge_pending = false;
         last_err = judge_host[0];
-        if (last_err > max_err) {
+        last_accel_err = judge_host[2];
+        if (judge_fails(last_err, last_accel_err)) {
             fallback_left = check_every;
             ++n_fallback;
         }
+    }
+    bool judge_fails(float dens_err, float accel_err) const {
+        return dens_err > max_err || accel_err > max_accel_err;  // accel_err < 0: not evaluated
     }
 
     float4 mass4() const {
@@ -748,7 +762,7 @@ struct World3 {
         e.perm.ensure(m);
         e.acc_sorted.ensure(m);
         e.tile_err.ensure(std::max<size_t>(e.tiles.size(), 1));
-        e.err_out.ensure(2);
+        e.err_out.ensure(4);
         const float4 r0 = make_float4(cfg.phase_rest_density[0], cfg.phase_rest_density[1],
                                       cfg.phase_rest_density[2], cfg.phase_rest_density[3]);
         e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_aux_reset(e.stream, n, e.tag[e.ct].p, r0, e.aux[e.cs].p); });
@@ -776,6 +790,20 @@ struct World3 {
             ++n_col_skipped;
         }
         if (collect) col->grow_samples((uint64_t)col->nsamples + cnt);
+        // the surrogate on the judged state, from the list and records its
+        // own substep would read (a judge with a collection skips this)
+        const bool accel = judge && sur && !collect && list_usable();
+        const uint32_t joff = n_judges % kJudgeStride;
+        if (accel) {
+            judge_acc.ensure(e.alloc_particles / kJudgeStride + 1);
+            SurStepArgs a = sur_args();
+            a.judge_acc = judge_acc.p;
+            a.judge_stride = kJudgeStride;
+            a.judge_off = joff;
+            const uint32_t ns_ = n > joff ? (n - 1 - joff) / kJudgeStride + 1 : 0;
+            e.run(ph::KIND_SUR_STEP, [&] { return ph::launch_sur_judge(e.stream, ns_, a); });
+        }
+        if (judge) ++n_judges;
         if (sur && rec_m != e.alloc_particles) {
             // both generations are rewritten before a surrogate substep reads them
             rec_m = e.alloc_particles;
@@ -795,6 +823,11 @@ struct World3 {
             af.aux_out = e.aux[ns].p;
             if (sur) af.rec_out = recs(e.cs);
             af.tile_err = e.tile_err.p;
+            if (accel) {
+                af.sur_acc = judge_acc.p;
+                af.sur_stride = kJudgeStride;
+                af.sur_off = joff;
+            }
             e.plain_phase(nullptr, &af);
             e.aux[0].swap(e.aux[1]);  // the new history is generation cs
         } else {
@@ -840,10 +873,10 @@ struct World3 {
             apply_judge();  // an older verdict still pending (check_every < 2)
             if (!judge_ev) {
                 PH_CK(hipEventCreateWithFlags(&judge_ev, hipEventDisableTiming));
-                PH_CK(hipHostMalloc((void **)&judge_host, 2 * sizeof(float), hipHostMallocDefault));
+                PH_CK(hipHostMalloc((void **)&judge_host, 4 * sizeof(float), hipHostMallocDefault));
             }
             e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_err_reduce(e.stream, e.tile_err.p, nt, e.err_out.p); });
-            PH_CK(hipMemcpyAsync(judge_host, e.err_out.p, 2 * sizeof(float), hipMemcpyDeviceToHost, e.stream));
+            PH_CK(hipMemcpyAsync(judge_host, e.err_out.p, 4 * sizeof(float), hipMemcpyDeviceToHost, e.stream));
             PH_CK(hipEventRecord(judge_ev, e.stream));
             judge_pending = true;
             judge_at = step_index;
@@ -853,9 +886,10 @@ struct World3 {
 
     // Surrogate substep on the order of the last sort: generation cs is
     // read, 1 - cs written, then becomes current (tags keep their order).
-    void substep_surrogate() {
+    // Arguments of a surrogate substep on the current generation (reading
+    // generation cs, writing 1 - cs).
+    SurStepArgs sur_args() {
         ph::Engine &e = eng;
-        e.vl_valid = false;  // moves particles without the Verlet displacement bound
         SurStepArgs a{};
         a.use_net = use_net;
         a.tiles = e.d_tiles.p;
@@ -878,6 +912,13 @@ struct World3 {
         a.posm_out = e.posm[1 - e.cs].p;
         a.vel_out = e.vel[1 - e.cs].p;
         a.aux_out = e.aux[1 - e.cs].p;
+        return a;
+    }
+
+    void substep_surrogate() {
+        ph::Engine &e = eng;
+        e.vl_valid = false;  // moves particles without the Verlet displacement bound
+        const SurStepArgs a = sur_args();
         e.run(ph::KIND_SUR_STEP, [&] { return ph::launch_sur_step(e.stream, (uint32_t)e.tiles.size(), a); });
         e.cs = 1 - e.cs;
         e.rec_valid = false;  // the density and force passes did not run
@@ -1501,12 +1542,34 @@ int32_t phieng_world3_surrogate_stats(const World3 *world, uint64_t *sur_steps, 
             return 0;
         }
         err = world->judge_host[0];
-        if (err > world->max_err) ++fb;
+        if (world->judge_fails(err, world->judge_host[2])) ++fb;
     }
     if (sur_steps) *sur_steps = world->n_sur;
     if (plain_steps) *plain_steps = world->n_plain;
     if (fallbacks) *fallbacks = fb;
     if (last_density_error) *last_density_error = err;
+    return 1;
+}
+
+int32_t phieng_world3_set_surrogate_accel_limit(World3 *world, float max_accel_error) {
+    if (!world || !(max_accel_error > 0.0f)) return 0;
+    world->max_accel_err = max_accel_error;
+    return 1;
+}
+
+int32_t phieng_world3_surrogate_judge(const World3 *world, float *density_error, float *accel_error) {
+    if (!world) return 0;
+    float de = world->last_err, ae = world->last_accel_err;
+    if (world->judge_pending) {
+        if (hipEventSynchronize(world->judge_ev) != hipSuccess) {
+            set_error("phieng_world3_surrogate_judge", "waiting for the judge's readback failed");
+            return 0;
+        }
+        de = world->judge_host[0];
+        ae = world->judge_host[2];
+    }
+    if (density_error) *density_error = de;
+    if (accel_error) *accel_error = ae;
     return 1;
 }
 

@@ -240,6 +240,17 @@ class World3:
         if not self._l.phieng_world3_set_surrogate_skin(self.ptr, float(skin)):
             raise RuntimeError("set_surrogate_skin rejected its argument")
 
+    def set_surrogate_accel_limit(self, limit):
+        if not self._l.phieng_world3_set_surrogate_accel_limit(self.ptr, float(limit)):
+            raise RuntimeError("set_surrogate_accel_limit rejected its argument")
+
+    def surrogate_judge(self):
+        """Newest judged (density error, acceleration error); -1 when absent."""
+        d, a = ctypes.c_float(), ctypes.c_float()
+        if not self._l.phieng_world3_surrogate_judge(self.ptr, ctypes.byref(d), ctypes.byref(a)):
+            raise RuntimeError(self._l.phieng_last_error().decode())
+        return d.value, a.value
+
     def surrogate_stats(self):
         a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         e = ctypes.c_float()

@@ -133,6 +133,13 @@ def test_null_tolerance():
         lib.phieng_world3_step(w.ptr, float("nan"))
         lib.phieng_world3_step(w.ptr, -1.0)
         lib.phieng_world3_step(w.ptr, 0.016)  # empty fluid: no-op, no device needed
+        # the surrogate judge's acceleration limit: > 0 (inf turns it off)
+        assert lib.phieng_world3_set_surrogate_accel_limit(None, 0.5) == 0
+        for bad in (0.0, -1.0, float("nan")):
+            assert lib.phieng_world3_set_surrogate_accel_limit(w.ptr, bad) == 0
+        assert lib.phieng_world3_set_surrogate_accel_limit(w.ptr, float("inf")) == 1
+        assert lib.phieng_world3_surrogate_judge(None, fp, fp) == 0
+        assert w.surrogate_judge() == (-1.0, -1.0)
 
 
 def test_add_body_rules():

@@ -148,3 +148,51 @@ def test_network_free_surrogate_tracks_plain(gpu):
     for w in (plain, sw):
         w.close()
     sur.close()
+
+
+def test_accel_judge(gpu):
+    """The judge's second measure, sum |a_plain - a_sur| / sum |a_plain| over
+    every 16th particle: evaluated once a surrogate substep could have run
+    from the judged state (not at the first judge), equal for a zero network
+    and the network-free method (both give coefficients 0), and a limit
+    below it turns judges into fallbacks that the density measure alone
+    would pass."""
+    sc, a, b = _pair()
+    net = Surrogate3(seed=3)
+    net.set_params(np.zeros(net.nparam, dtype=np.float32))
+    a.set_surrogate(net, check_every=3, max_density_error=1.0, use_network=True)
+    b.set_surrogate(net, check_every=3, max_density_error=1.0, use_network=False)
+    a.step(sc.dt)
+    assert a.surrogate_judge()[1] == -1.0  # first judge: no list before it
+    b.step(sc.dt)
+    for k in range(9):
+        a.step(sc.dt)
+        b.step(sc.dt)
+    a.sync()
+    b.sync()
+    _same_state(a, b, "zero network differs from the network-free step")
+    da, ea = a.surrogate_judge()
+    db, eb = b.surrogate_judge()
+    print(f"density err {da:.3e}, accel err {ea:.4f}", flush=True)
+    assert (da, ea) == (db, eb)
+    assert np.isfinite(ea) and 0.0 < ea < 10.0
+    assert a.surrogate_stats()["fallbacks"] == 0 or ea > 0.2
+    for w in (a, b):
+        w.close()
+    # the same run with the limit below the measured error: judges fail on
+    # the acceleration alone (the density limit of 1 always passes)
+    runs = {}
+    for name, lim in (("strict", ea * 1e-3), ("off", float("inf"))):
+        w = scenes.make_world(sc)
+        w.set_surrogate(net, check_every=3, max_density_error=1.0, use_network=False)
+        w.set_surrogate_accel_limit(lim)
+        for k in range(12):
+            w.step(sc.dt)
+        w.sync()
+        runs[name] = w.surrogate_stats()
+        w.close()
+    print(runs, flush=True)
+    assert runs["off"]["fallbacks"] == 0
+    assert runs["strict"]["fallbacks"] >= 2
+    assert runs["strict"]["plain_steps"] > runs["off"]["plain_steps"]
+    net.close()
