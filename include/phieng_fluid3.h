@@ -341,9 +341,17 @@ int32_t phieng_world3_surrogate_stats(const World3 *world, uint64_t *sur_steps, 
  * the measure off).  No counterpart in the reference.  1 / 0. */
 int32_t phieng_world3_set_surrogate_accel_limit(World3 *world, float max_accel_error);
 /* The newest judged density and acceleration errors (-1 before any / when
- * not evaluated), including a verdict not yet applied.  No counterpart in
- * the reference.  1 / 0. */
-int32_t phieng_world3_surrogate_judge(const World3 *world, float *density_error, float *accel_error);
+ * not evaluated), including a verdict not yet applied, and the count of
+ * judges that kept the particle order and the neighbour list of the judge
+ * before.  A judge that builds the list sizes it for two judge intervals
+ * when that costs the surrogate substeps less than the next judge saves
+ * (from the largest particle speed at the previous list's judge); the next
+ * judge, if no particle fell outside that list, then computes the plain
+ * substep over it instead of sorting and building a list (the same
+ * neighbours, summed in list order instead of cell order).  No
+ * counterpart in the reference.  1 / 0. */
+int32_t phieng_world3_surrogate_judge(const World3 *world, float *density_error, float *accel_error,
+                                      uint64_t *kept_judges);
 /* Minimum skin of the surrogate's neighbour list: surrogate substeps
  * between two plain substeps see the candidates that were within
  * (1 + s) h at the plain one, with the Verlet skin

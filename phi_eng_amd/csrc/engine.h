@@ -131,6 +131,10 @@ struct Engine {
     // With list_coef > 0 the list radius is a Verlet skin from the current
     // speeds, at least list_skin (launch_list_radius)
     float list_coef = 0.0f;
+    // where the list radius pass also writes the largest speed, and the
+    // counter of particles whose list did not fit (both may be null)
+    float *list_vmax_out = nullptr;
+    uint32_t *list_slow = nullptr;
     DBuf<float> tile_vmax, list_f2;
     // bucket tables
     DBuf<uint32_t> counts, start, partial;

@@ -602,9 +602,10 @@ void Engine::plain_phase(float4 *acc_sorted, const AuxFuse *af, bool density_don
         list_f2.ensure(1);
         run(KIND_SUR_AUX, [&] {
             return launch_list_radius(stream, nt, d_tiles.p, d_scenes.p, vel[ns].p, tile_vmax.p, list_coef, list_skin,
-                                      list_f2.p);
+                                      list_f2.p, list_vmax_out);
         });
         lo.skin_factor2_dev = list_f2.p;
+        lo.slow = list_slow;
     }
     if (!density_done)
         run(KIND_DENSITY, [&] {

@@ -608,7 +608,7 @@ struct SurRec {
 // through the sort's permutation from the previous order, the neighbour
 // records of the advanced state, and the judge's per-tile sums.
 struct AuxFuse {
-    const uint32_t *perm = nullptr;   // sorted slot -> slot before the sort
+    const uint32_t *perm = nullptr;   // sorted slot -> slot before the sort (null: no sort, the same slot)
     const float4 *aux_in = nullptr;   // history in the order before the sort
     float4 *aux_out = nullptr;        // null: nothing fused
     SurRec rec_out;                   // pos may be null: no records

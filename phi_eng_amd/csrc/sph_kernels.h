@@ -102,14 +102,26 @@ struct ListOut {
     uint32_t *idx = nullptr;  // SL_CAP / 4 groups of 4 slots, group g of particle i at uint4 g * mstride + i
     float skin_factor2 = 1.0f;              // ((1 + skin))^2, or
     const float *skin_factor2_dev = nullptr;  // the same computed on the device (k_list_radius)
+    uint32_t *slow = nullptr;  // non-null: counts the particles whose list did not fit (SLF_SLOW)
 };
 // List radius of a surrogate world from the current speeds (a Verlet skin):
 // out = (1 + max(skin_min, coef * vmax))^2 with vmax the largest speed
 // (per tile into tile_vmax, then over tiles) and coef = 4 x the substeps a
 // list serves x dt / h: two particles approach at most at twice the
 // largest speed, and speeds are allowed to double before the next list.
+// vmax_out (may be null) receives vmax.
 hipError_t launch_list_radius(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
-                              const float4 *vel, float *tile_vmax, float coef, float skin_min, float *out);
+                              const float4 *vel, float *tile_vmax, float coef, float skin_min, float *out,
+                              float *vmax_out = nullptr);
+// Density of every slot from a kept skin list (ListOut cnt/idx of an
+// earlier density pass over the same particle order): the entries within
+// h in list order, then the density pass's clamp, equation of state and
+// records.  A slot whose list did not fit (SLF_SLOW) sweeps the cells of
+// the table in start.
+hipError_t launch_density_skin(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                               const uint32_t *start, const float4 *posm, const uint32_t *tag, const uint32_t *lcnt,
+                               const uint32_t *lidx, uint32_t mstride, float2 *rq, float *rec_rho, float *rec_p,
+                               uint32_t *rec_count);
 hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                                const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
                                uint32_t mstride, float *rec_rho, float *rec_p, uint32_t *rec_count, const ListOut &lo);

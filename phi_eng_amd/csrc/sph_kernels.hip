@@ -399,8 +399,10 @@ __device__ __forceinline__ void dev_density(uint32_t bid, uint32_t nbl, const Ti
             // the last, partly filled group
             if ((lk & 3u) != 0u && lk < SL_CAP) lgrp[(size_t)(lk >> 2) * mstride] = make_uint4(q0, q1, q2, q3);
             lo.cnt[i] = lk <= SL_CAP ? lk : SLF_SLOW;
+            if (lk > SL_CAP && lo.slow) atomicAdd(lo.slow, 1u);
         } else {
             lo.cnt[i] = SLF_SLOW;
+            if (lo.slow) atomicAdd(lo.slow, 1u);
             for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) {
                 const float4 pj = posm[j];
                 const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
@@ -625,7 +627,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             rec_acc[3 * id + 2] = az;
         }
         if (af.aux_out) {
-            const uint32_t o = af.perm[i];
+            const uint32_t o = af.perm ? af.perm[i] : i;
             const float4 ap = af.aux_in[o];
             const uint32_t tg = tag[i];
             const float dq = rqi.y - ap.y;
@@ -733,7 +735,8 @@ __global__ __launch_bounds__(PH_TILE) void k_tile_vmax(const TileDev *__restrict
 
 // One block: (1 + max(skin_min, coef * vmax))^2 from the per-tile speeds.
 __global__ __launch_bounds__(256) void k_list_radius(const float *__restrict__ tile_vmax, uint32_t ntiles, float coef,
-                                                     float skin_min, float *__restrict__ out) {
+                                                     float skin_min, float *__restrict__ out,
+                                                     float *__restrict__ vmax_out) {
     float v = 0.0f;
     for (uint32_t k = threadIdx.x; k < ntiles; k += 256) v = fmaxf(v, tile_vmax[k]);
     for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
@@ -744,6 +747,7 @@ __global__ __launch_bounds__(256) void k_list_radius(const float *__restrict__ t
         const float vm = fmaxf(fmaxf(wv[0], wv[1]), fmaxf(wv[2], wv[3]));
         const float f = 1.0f + fmaxf(skin_min, coef * vm);
         out[0] = f * f;
+        if (vmax_out) vmax_out[0] = vm;
     }
 }
 
@@ -1106,11 +1110,12 @@ hipError_t launch_force_list(hipStream_t st, uint32_t ntiles, const TileDev *til
     return hipGetLastError();
 }
 hipError_t launch_list_radius(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
-                              const float4 *vel, float *tile_vmax, float coef, float skin_min, float *out) {
+                              const float4 *vel, float *tile_vmax, float coef, float skin_min, float *out,
+                              float *vmax_out) {
     hipLaunchKernelGGL(k_tile_vmax, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, vel, tile_vmax);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_list_radius, dim3(1), dim3(256), 0, st, tile_vmax, ntiles, coef, skin_min, out);
+    hipLaunchKernelGGL(k_list_radius, dim3(1), dim3(256), 0, st, tile_vmax, ntiles, coef, skin_min, out, vmax_out);
     return hipGetLastError();
 }
 hipError_t launch_ghost_density(hipStream_t st, const SceneDev *scenes, uint32_t scene, const uint32_t *start,

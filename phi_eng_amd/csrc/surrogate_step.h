@@ -58,6 +58,7 @@ hipError_t launch_aux_update(hipStream_t st, uint32_t ntiles, const TileDev *til
                              const float2 *rq, const float4 *aux_prev, const uint32_t *tag, float4 *aux_out,
                              float4 *tile_err, const float4 *posm_new = nullptr, const float4 *vel_new = nullptr,
                              SurRec rec_out = SurRec());
-hipError_t launch_err_reduce(hipStream_t st, const float4 *tile_err, uint32_t ntiles, float *out);
+hipError_t launch_err_reduce(hipStream_t st, const float4 *tile_err, uint32_t ntiles, float *out, uint32_t *slow,
+                             const float *vmax);
 hipError_t launch_aux_reset(hipStream_t st, uint32_t n, const uint32_t *tag, float4 rho0, float4 *aux);
 }  // namespace ph

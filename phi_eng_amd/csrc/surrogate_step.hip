@@ -504,9 +504,12 @@ __global__ __launch_bounds__(256) void k_aux_update(const TileDev *__restrict__ 
 // The judge's verdict from the per-tile sums: out[0] = mean bulk density
 // deviation, out[1] = its particle count, out[2] = sum |a_plain - a_sur| /
 // sum |a_plain| over the evaluated particles (-1 without any), out[3] = the
-// denominator.  One block, sums in double in a fixed order.
+// denominator, out[4] = the count in slow (which is then cleared), out[5] =
+// vmax[0].  One block, sums in double in a fixed order; out may be pinned
+// host memory.
 __global__ __launch_bounds__(1024) void k_err_reduce(const float4 *__restrict__ tile_err, uint32_t ntiles,
-                                                     float *__restrict__ out) {
+                                                     float *__restrict__ out, uint32_t *__restrict__ slow,
+                                                     const float *__restrict__ vmax) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t k = threadIdx.x; k < ntiles; k += 1024) {
         const float4 t = tile_err[k];
@@ -529,6 +532,9 @@ __global__ __launch_bounds__(1024) void k_err_reduce(const float4 *__restrict__ 
         out[1] = (float)t[1];
         out[2] = (float)(t[3] > 0 ? t[2] / t[3] : -1.0);
         out[3] = (float)t[3];
+        out[4] = (float)slow[0];
+        out[5] = vmax[0];
+        slow[0] = 0u;
     }
 }
 
@@ -578,8 +584,9 @@ hipError_t launch_aux_update(hipStream_t st, uint32_t ntiles, const TileDev *til
                        tile_err, posm_new, vel_new, rec_out);
     return hipGetLastError();
 }
-hipError_t launch_err_reduce(hipStream_t st, const float4 *tile_err, uint32_t ntiles, float *out) {
-    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, tile_err, ntiles, out);
+hipError_t launch_err_reduce(hipStream_t st, const float4 *tile_err, uint32_t ntiles, float *out, uint32_t *slow,
+                             const float *vmax) {
+    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, tile_err, ntiles, out, slow, vmax);
     return hipGetLastError();
 }
 hipError_t launch_aux_reset(hipStream_t st, uint32_t n, const uint32_t *tag, float4 rho0, float4 *aux) {

@@ -593,7 +593,71 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
     }
 }
 
+// Density from a kept skin list: a judging substep of a surrogate world
+// that keeps the particle order and the list of an earlier sort (the list
+// radius covers the displacement since).  Entries within h in list order,
+// which sums the same terms as the cell order up to rounding.
+__global__ __launch_bounds__(PH_TILE) void k_density_skin(const TileDev *__restrict__ tiles,
+                                                          const SceneDev *__restrict__ scenes,
+                                                          const uint32_t *__restrict__ start,
+                                                          const float4 *__restrict__ posm,
+                                                          const uint32_t *__restrict__ tag,
+                                                          const uint32_t *__restrict__ lcnt,
+                                                          const uint4 *__restrict__ lgrp, uint32_t mstride,
+                                                          float2 *__restrict__ rq, float *__restrict__ rec_rho,
+                                                          float *__restrict__ rec_p, uint32_t *__restrict__ rec_count) {
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const TileDev t = tiles[tile];
+    const SceneDev s = scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    if (i >= s.part_end) return;
+    const float4 pi = posm[i];
+    const uint32_t cw = lcnt[i];
+    const float hg2 = s.h_grid2;
+    float rho = 0.0f;
+    uint32_t cnt = 0;
+    auto entry = [&](const float4 &pj) {
+        const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+        if (r2 < hg2) {
+            ++cnt;
+            rho += pj.w * poly6(r2, s);
+        }
+    };
+    if (cw & SLF_NONE) {
+        // non-finite position: no neighbours
+    } else if (cw & SLF_SLOW) {
+        int32_t cx, cy, cz;
+        cell_of(pi.x, pi.y, pi.z, s.h_grid, cx, cy, cz);
+        for_each_candidate(s, start, posm, cx, cy, cz, [&](uint32_t j) { entry(posm[j]); });
+    } else {
+        const uint32_t n = cw & SLF_COUNT;
+        const uint4 *g = lgrp + i;
+#pragma unroll 1
+        for (uint32_t k = 0; k < n; k += 4) {
+            const uint4 e = *g;
+            g += mstride;
+            // slots past the count read the particle itself, then are masked
+            const uint32_t j1 = k + 1 < n ? e.y : i, j2 = k + 2 < n ? e.z : i, j3 = k + 3 < n ? e.w : i;
+            const float4 p0 = posm[e.x], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
+            entry(p0);
+            if (k + 1 < n) entry(p1);
+            if (k + 2 < n) entry(p2);
+            if (k + 3 < n) entry(p3);
+        }
+    }
+    density_out(s, scenes, t.scene, i, rho, cnt, tag, rq, rec_rho, rec_p, rec_count);
+}
+
 namespace ph {
+
+hipError_t launch_density_skin(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
+                               const uint32_t *start, const float4 *posm, const uint32_t *tag, const uint32_t *lcnt,
+                               const uint32_t *lidx, uint32_t mstride, float2 *rq, float *rec_rho, float *rec_p,
+                               uint32_t *rec_count) {
+    hipLaunchKernelGGL(k_density_skin, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, lcnt,
+                       reinterpret_cast<const uint4 *>(lidx), mstride, rq, rec_rho, rec_p, rec_count);
+    return hipGetLastError();
+}
 
 hipError_t launch_vl_decide(hipStream_t st, VerletDev *vs, const float *tile_dmax, uint32_t *tile_q, uint32_t ntiles,
                             uint32_t force, float thr) {

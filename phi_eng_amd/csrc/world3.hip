@@ -168,6 +168,18 @@ struct World3 {
     float last_accel_err = -1.0f;
     uint32_t n_judges = 0;
     ph::DBuf<float4> judge_acc;
+    // A list built at a judge may be sized for two judge intervals; the
+    // next judge then keeps the particle order and the list (no sort, no
+    // list build: substep_kept_judge).  The list's judge, the intervals it
+    // covers, and what its verdict read back: the largest speed and the
+    // count of particles whose list did not fit.
+    int64_t list_judge = -1;
+    uint32_t list_k = 1;
+    bool list_checked = false;
+    uint32_t list_slow = 0;
+    float list_vmax = -1.0f;
+    ph::DBuf<uint32_t> slow_cnt;
+    uint64_t n_kept_judges = 0;
     // training-sample collection from plain substeps
     Surrogate3 *col = nullptr;
     uint32_t col_every = 10, col_stride = 64, col_phase = 0;
@@ -508,9 +520,13 @@ struct World3 {
                 if (sur && !judge && !collect && fallback_left == 0 && list_usable()) {
                     substep_surrogate();
                     ++n_sur;
+                } else if (judge && !collect && fallback_left == 0 && list_kept()) {
+                    substep_kept_judge();
+                    ++n_plain;
+                    ++n_kept_judges;
                 } else {
                     // the next substep may be a surrogate one: leave it a list
-                    substep_plain_aux(judge, collect, sur && fallback_left <= 1);
+                    substep_plain_aux(judge, collect, sur && fallback_left <= 1, sdt);
                     ++n_plain;
                     if (fallback_left) --fallback_left;
                 }
@@ -733,6 +749,11 @@ struct World3 {
         judge_pending = false;
         last_err = judge_host[0];
         last_accel_err = judge_host[2];
+        if ((int64_t)judge_at == list_judge) {
+            list_checked = true;
+            list_slow = (uint32_t)judge_host[4];
+            list_vmax = judge_host[5];
+        }
         if (judge_fails(last_err, last_accel_err)) {
             fallback_left = check_every;
             ++n_fallback;
@@ -762,7 +783,19 @@ struct World3 {
         e.perm.ensure(m);
         e.acc_sorted.ensure(m);
         e.tile_err.ensure(std::max<size_t>(e.tiles.size(), 1));
-        e.err_out.ensure(4);
+        e.err_out.ensure(6);
+        if (!slow_cnt.p) {
+            slow_cnt.ensure(1);
+            e.run(ph::KIND_FILL, [&] { return ph::launch_fill_u32(e.stream, slow_cnt.p, 1u, 0u); });
+            // 0.0f is the all-zero word
+            e.run(ph::KIND_FILL, [&] {
+                return ph::launch_fill_u32(e.stream, reinterpret_cast<uint32_t *>(e.err_out.p), 6u, 0u);
+            });
+        }
+        if (!judge_ev) {
+            PH_CK(hipEventCreateWithFlags(&judge_ev, hipEventDisableTiming));
+            PH_CK(hipHostMalloc((void **)&judge_host, 6 * sizeof(float), hipHostMallocDefault));
+        }
         const float4 r0 = make_float4(cfg.phase_rest_density[0], cfg.phase_rest_density[1],
                                       cfg.phase_rest_density[2], cfg.phase_rest_density[3]);
         e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_aux_reset(e.stream, n, e.tag[e.ct].p, r0, e.aux[e.cs].p); });
@@ -774,10 +807,36 @@ struct World3 {
                list_stride == eng.list_stride();
     }
 
+    // A judge at this substep may keep the order and the list: the list
+    // was built at a judge whose verdict has arrived with no particle
+    // outside its list, and its radius covers the substeps up to the next
+    // judge (whose evaluation reads it too).
+    bool list_kept() const {
+        return list_usable() && list_judge >= 0 && list_checked && list_slow == 0 &&
+               (int64_t)(step_index + check_every) - list_judge <= (int64_t)(list_k * check_every);
+    }
+
+    // Judge intervals the list built at this judge is sized for.  Two when
+    // the list for two, from the speed at the last list's judge, holds few
+    // enough extra candidates: per interval, a full judge plus (c - 1)
+    // surrogate substeps against half of a full and a kept judge plus
+    // (c - 1) substeps over the larger list.  Costs in units of one
+    // surrogate substep, measured on the 1,048,576-particle dam break
+    // (full judge 3.8, kept judge 2.2); a surrogate substep's cost grows
+    // with the candidates, (1 + skin)^3.
+    uint32_t plan_k(float sdt) const {
+        if (list_vmax < 0.0f || check_every < 2) return 1;
+        const float c = 4.0f * (float)check_every * sdt * list_vmax / std::max(cfg.smoothing_length, 1e-6f);
+        const float s1 = std::max(skin, c), s2 = std::max(skin, 2.0f * c);
+        const float g = (1.0f + s2) / (1.0f + s1), infl = g * g * g;
+        const float m = (float)(check_every - 1), full = 3.8f, kept = 2.2f;
+        return 0.5f * (full + kept) + m * infl < full + m ? 2u : 1u;
+    }
+
     // Plain substep that also refreshes the history; optionally judged
     // (bulk density deviation, read back), collecting samples, and/or
     // building the skin list for surrogate substeps.
-    void substep_plain_aux(bool judge, bool collect, bool list) {
+    void substep_plain_aux(bool judge, bool collect, bool list, float sdt) {
         ph::Engine &e = eng;
         const uint32_t nt = (uint32_t)e.tiles.size();
         const int ns = 1 - e.cs, ntg = 1 - e.ct;
@@ -793,17 +852,16 @@ struct World3 {
         // the surrogate on the judged state, from the list and records its
         // own substep would read (a judge with a collection skips this)
         const bool accel = judge && sur && !collect && list_usable();
-        const uint32_t joff = n_judges % kJudgeStride;
-        if (accel) {
-            judge_acc.ensure(e.alloc_particles / kJudgeStride + 1);
-            SurStepArgs a = sur_args();
-            a.judge_acc = judge_acc.p;
-            a.judge_stride = kJudgeStride;
-            a.judge_off = joff;
-            const uint32_t ns_ = n > joff ? (n - 1 - joff) / kJudgeStride + 1 : 0;
-            e.run(ph::KIND_SUR_STEP, [&] { return ph::launch_sur_judge(e.stream, ns_, a); });
-        }
+        const uint32_t joff = accel ? judge_evaluate() : 0u;
         if (judge) ++n_judges;
+        // a judge that builds the list sizes it for one or two intervals and
+        // reads back the largest speed and the count of particles outside it
+        const bool build = list && e.list_cells && e.list_skin == skin;
+        const float coef1 = e.list_coef;
+        const uint32_t k = judge && build && !collect ? plan_k(sdt) : 1u;
+        e.list_coef = coef1 * (float)k;
+        e.list_slow = sur ? slow_cnt.p : nullptr;
+        e.list_vmax_out = judge ? e.err_out.p + 5 : nullptr;
         if (sur && rec_m != e.alloc_particles) {
             // both generations are rewritten before a surrogate substep reads them
             rec_m = e.alloc_particles;
@@ -828,6 +886,7 @@ struct World3 {
                 af.sur_stride = kJudgeStride;
                 af.sur_off = joff;
             }
+
             e.plain_phase(nullptr, &af);
             e.aux[0].swap(e.aux[1]);  // the new history is generation cs
         } else {
@@ -835,11 +894,15 @@ struct World3 {
                   [&] { return ph::launch_gather_aux(e.stream, n, e.perm.p, e.aux[e.cs].p, e.aux[ns].p); });
             e.plain_phase(e.acc_sorted.p);
         }
-        if (list && e.list_cells && e.list_skin == skin) {
+        e.list_coef = coef1;
+        if (build) {
             // the skin list of the order this substep sorted into
             list_sorts = e.sorts_done;
             list_epoch = e.layout_epoch;
             list_stride = e.list_stride();
+            list_judge = judge && !collect ? (int64_t)step_index : -1;
+            list_k = k;
+            list_checked = false;
         }
         if (collect) {
             SurCollectArgs a{};
@@ -869,23 +932,84 @@ struct World3 {
                                              e.tag[ntg].p, e.aux[e.cs].p, e.tile_err.p, e.posm[e.cs].p, e.vel[e.cs].p,
                                              sur ? recs(e.cs) : SurRec());
             });
-        if (judge) {
-            apply_judge();  // an older verdict still pending (check_every < 2)
-            if (!judge_ev) {
-                PH_CK(hipEventCreateWithFlags(&judge_ev, hipEventDisableTiming));
-                PH_CK(hipHostMalloc((void **)&judge_host, 4 * sizeof(float), hipHostMallocDefault));
-            }
-            e.run(ph::KIND_SUR_AUX, [&] { return ph::launch_err_reduce(e.stream, e.tile_err.p, nt, e.err_out.p); });
-            PH_CK(hipMemcpyAsync(judge_host, e.err_out.p, 4 * sizeof(float), hipMemcpyDeviceToHost, e.stream));
-            PH_CK(hipEventRecord(judge_ev, e.stream));
-            judge_pending = true;
-            judge_at = step_index;
-        }
+        if (judge) judge_readback();
         e.finish_substep();
     }
 
-    // Surrogate substep on the order of the last sort: generation cs is
-    // read, 1 - cs written, then becomes current (tags keep their order).
+    // The judge's verdict from the tile sums, written by the reduction
+    // kernel straight into the pinned host words and read after the
+    // judge's event (applied two substeps later, so the stream is never
+    // drained for it).  An older verdict still pending (check_every < 2) is
+    // applied first: its words are reused.
+    void judge_readback() {
+        ph::Engine &e = eng;
+        apply_judge();
+        const uint32_t nt = (uint32_t)e.tiles.size();
+        e.run(ph::KIND_SUR_AUX, [&] {
+            return ph::launch_err_reduce(e.stream, e.tile_err.p, nt, judge_host, slow_cnt.p, e.err_out.p + 5);
+        });
+        PH_CK(hipEventRecord(judge_ev, e.stream));
+        judge_pending = true;
+        judge_at = step_index;
+    }
+
+    // Launch the surrogate's evaluation on the current generation for every
+    // kJudgeStride-th slot; returns the slot offset.
+    uint32_t judge_evaluate() {
+        ph::Engine &e = eng;
+        const uint32_t joff = n_judges % kJudgeStride;
+        judge_acc.ensure(e.alloc_particles / kJudgeStride + 1);
+        SurStepArgs a = sur_args();
+        a.judge_acc = judge_acc.p;
+        a.judge_stride = kJudgeStride;
+        a.judge_off = joff;
+        const uint32_t cnt = n > joff ? (n - 1 - joff) / kJudgeStride + 1 : 0;
+        e.run(ph::KIND_SUR_STEP, [&] { return ph::launch_sur_judge(e.stream, cnt, a); });
+        return joff;
+    }
+
+    // A judging plain substep that keeps the particle order and the skin
+    // list of the judge before (list_kept): the surrogate's evaluation,
+    // the density from the list, the force pass over it with the history,
+    // the records and the judge's sums, then the verdict's readback.  Like
+    // a surrogate substep it reads generation cs and writes 1 - cs, which
+    // becomes current; tags keep their order.
+    void substep_kept_judge() {
+        ph::Engine &e = eng;
+        const uint32_t nt = (uint32_t)e.tiles.size();
+        const int ns = 1 - e.cs;
+        const bool rd = e.rec_flags & PHIENG_RECORD_DENSITY, ra = e.rec_flags & PHIENG_RECORD_ACCEL;
+        e.vl_valid = false;
+        const uint32_t joff = judge_evaluate();
+        ++n_judges;
+        e.run(ph::KIND_DENSITY, [&] {
+            return ph::launch_density_skin(e.stream, nt, e.d_tiles.p, e.d_scenes.p, e.start.p, e.posm[e.cs].p,
+                                           e.tag[e.ct].p, e.nl_cnt.p, e.nl_idx.p, list_stride, e.rq.p,
+                                           rd ? e.rec_rho.p : nullptr, rd ? e.rec_p.p : nullptr,
+                                           rd ? e.rec_count.p : nullptr);
+        });
+        AuxFuse af;
+        af.aux_in = e.aux[e.cs].p;
+        af.aux_out = e.aux[ns].p;
+        af.rec_out = recs(ns);
+        af.tile_err = e.tile_err.p;
+        af.sur_acc = judge_acc.p;
+        af.sur_stride = kJudgeStride;
+        af.sur_off = joff;
+        e.run(ph::KIND_FORCE, [&] {
+            return ph::launch_force_list(e.stream, nt, e.d_tiles.p, e.d_scenes.p, e.d_bodies.p, e.start.p,
+                                         e.posm[e.cs].p, e.vel[e.cs].p, e.rq.p, e.tag[e.ct].p, e.nl_cnt.p,
+                                         e.nl_idx.p, list_stride, e.posm[ns].p, e.vel[ns].p,
+                                         ra ? e.rec_acc.p : nullptr, e.d_bbox.p, nullptr, GhostDev{}, af);
+        });
+        e.cs = ns;
+        judge_readback();
+        // the particle records by index are this substep's; the cell record
+        // belongs to the sort this substep kept
+        e.rec_valid = e.rec_flags != 0 && !(e.rec_flags & PHIENG_RECORD_CELLS);
+        ++e.substeps_done;
+    }
+
     // Arguments of a surrogate substep on the current generation (reading
     // generation cs, writing 1 - cs).
     SurStepArgs sur_args() {
@@ -915,6 +1039,8 @@ struct World3 {
         return a;
     }
 
+    // Surrogate substep on the order of the last sort: generation cs is
+    // read, 1 - cs written, then becomes current (tags keep their order).
     void substep_surrogate() {
         ph::Engine &e = eng;
         e.vl_valid = false;  // moves particles without the Verlet displacement bound
@@ -1501,6 +1627,9 @@ int32_t phieng_world3_set_surrogate(World3 *world, Surrogate3 *s, uint32_t check
     world->fallback_left = 0;
     world->step_index = 0;
     world->judge_pending = false;
+    world->list_judge = -1;
+    world->list_checked = false;
+    world->list_vmax = -1.0f;
     return 1;
 }
 
@@ -1519,6 +1648,8 @@ int32_t phieng_world3_collect_samples(World3 *world, Surrogate3 *s, uint32_t eve
     world->col_stride = stride ? stride : 1;
     world->col_phase = 0;
     world->step_index = 0;
+    world->list_judge = -1;
+    world->list_checked = false;
     return 1;
 }
 
@@ -1557,7 +1688,8 @@ int32_t phieng_world3_set_surrogate_accel_limit(World3 *world, float max_accel_e
     return 1;
 }
 
-int32_t phieng_world3_surrogate_judge(const World3 *world, float *density_error, float *accel_error) {
+int32_t phieng_world3_surrogate_judge(const World3 *world, float *density_error, float *accel_error,
+                                      uint64_t *kept_judges) {
     if (!world) return 0;
     float de = world->last_err, ae = world->last_accel_err;
     if (world->judge_pending) {
@@ -1570,6 +1702,7 @@ int32_t phieng_world3_surrogate_judge(const World3 *world, float *density_error,
     }
     if (density_error) *density_error = de;
     if (accel_error) *accel_error = ae;
+    if (kept_judges) *kept_judges = world->n_kept_judges;
     return 1;
 }
 

@@ -170,7 +170,7 @@ SIGNATURES = {
     "phieng_world3_surrogate_stats": (_I32, [_P, _u64p, _u64p, _u64p, _fp]),
     "phieng_world3_set_surrogate_skin": (_I32, [_P, _F]),
     "phieng_world3_set_surrogate_accel_limit": (_I32, [_P, _F]),
-    "phieng_world3_surrogate_judge": (_I32, [_P, _fp, _fp]),
+    "phieng_world3_surrogate_judge": (_I32, [_P, _fp, _fp, _u64p]),
     "phieng_world3_boundary_info": (_I32, [_P, _u32p, _u64p]),
     "phieng_world3_set_neighbour_list": (_I32, [_P, _I32, _F]),
     "phieng_world3_neighbour_stats": (_I32, [_P, _u64p, _u64p, _u64p]),

@@ -245,11 +245,12 @@ class World3:
             raise RuntimeError("set_surrogate_accel_limit rejected its argument")
 
     def surrogate_judge(self):
-        """Newest judged (density error, acceleration error); -1 when absent."""
-        d, a = ctypes.c_float(), ctypes.c_float()
-        if not self._l.phieng_world3_surrogate_judge(self.ptr, ctypes.byref(d), ctypes.byref(a)):
+        """Newest judged (density error, acceleration error), -1 when absent,
+        and the count of judges that kept the order and list of the one before."""
+        d, a, k = ctypes.c_float(), ctypes.c_float(), ctypes.c_uint64()
+        if not self._l.phieng_world3_surrogate_judge(self.ptr, ctypes.byref(d), ctypes.byref(a), ctypes.byref(k)):
             raise RuntimeError(self._l.phieng_last_error().decode())
-        return d.value, a.value
+        return d.value, a.value, k.value
 
     def surrogate_stats(self):
         a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()

@@ -138,8 +138,8 @@ def test_null_tolerance():
         for bad in (0.0, -1.0, float("nan")):
             assert lib.phieng_world3_set_surrogate_accel_limit(w.ptr, bad) == 0
         assert lib.phieng_world3_set_surrogate_accel_limit(w.ptr, float("inf")) == 1
-        assert lib.phieng_world3_surrogate_judge(None, fp, fp) == 0
-        assert w.surrogate_judge() == (-1.0, -1.0)
+        assert lib.phieng_world3_surrogate_judge(None, fp, fp, None) == 0
+        assert w.surrogate_judge() == (-1.0, -1.0, 0)
 
 
 def test_add_body_rules():

@@ -171,8 +171,8 @@ def test_accel_judge(gpu):
     a.sync()
     b.sync()
     _same_state(a, b, "zero network differs from the network-free step")
-    da, ea = a.surrogate_judge()
-    db, eb = b.surrogate_judge()
+    da, ea, _ = a.surrogate_judge()
+    db, eb, _ = b.surrogate_judge()
     print(f"density err {da:.3e}, accel err {ea:.4f}", flush=True)
     assert (da, ea) == (db, eb)
     assert np.isfinite(ea) and 0.0 < ea < 10.0
@@ -195,4 +195,57 @@ def test_accel_judge(gpu):
     assert runs["off"]["fallbacks"] == 0
     assert runs["strict"]["fallbacks"] >= 2
     assert runs["strict"]["plain_steps"] > runs["off"]["plain_steps"]
+    net.close()
+
+
+def test_kept_judge_is_a_plain_substep(gpu):
+    """A judge that keeps the particle order and the neighbour list of the
+    judge before (that list sized for two judge intervals) computes the
+    plain substep over the list: the same neighbours summed in list order
+    instead of cell order, so its densities and the advanced state agree
+    with a plain substep from the same state up to rounding.  Schedule with
+    check_every 4: judges at substeps 0, 4, 8; the first list is sized for
+    one interval (no speed known yet), the one built at 4 for two (the
+    dam starts at rest), so the judge at 8 keeps it."""
+    import dataclasses
+
+    sc = scenes.small_dam(12)
+    w = scenes.make_world(sc)
+    net = Surrogate3(seed=5)
+    w.set_surrogate(net, check_every=4, max_density_error=1.0, use_network=False)
+    w.set_surrogate_accel_limit(float("inf"))
+    w.set_record(RECORD_DENSITY)
+    for k in range(8):
+        w.step(sc.dt)
+    w.sync()
+    assert w.surrogate_judge()[2] == 0
+    pos, vel = w.get_state()
+    w.step(sc.dt)
+    w.sync()
+    assert w.surrogate_judge()[2] == 1, "the judge at substep 8 did not keep the list"
+    rho_w = w.get_record()["density"]
+    pw, vw = w.get_state()
+    p = scenes.make_world(dataclasses.replace(sc, positions=pos))
+    p.set_velocity(vel)
+    p.set_record(RECORD_DENSITY)
+    p.step(sc.dt)
+    p.sync()
+    rho_p = p.get_record()["density"]
+    pp, vp = p.get_state()
+    print(f"max rel density diff {np.abs(rho_w / rho_p - 1).max():.2e}, max |dv| {np.abs(vw - vp).max():.2e}",
+          flush=True)
+    assert np.allclose(rho_w, rho_p, rtol=1e-5, atol=0.0)
+    assert np.allclose(vw, vp, rtol=1e-4, atol=1e-5)
+    assert np.allclose(pw, pp, rtol=0.0, atol=1e-6)
+    # the run goes on from the kept judge: surrogate substeps, a judge at
+    # 12 that sorts and builds a list again, and one at 16 that keeps it
+    for k in range(8):
+        w.step(sc.dt)
+    w.sync()
+    st = w.surrogate_stats()
+    assert st["fallbacks"] == 0 and st["surrogate_steps"] == 12 and st["plain_steps"] == 5
+    assert w.surrogate_judge()[2] == 2
+    assert np.isfinite(w.get_state()[0]).all()
+    for x in (w, p):
+        x.close()
     net.close()
