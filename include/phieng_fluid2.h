@@ -128,7 +128,8 @@ int32_t phieng_world_boundary_info(const World2 *world, uint32_t *count, uint64_
 /* Extensions: the 2D forms of phieng_world3_set_neighbour_list and
  * phieng_world3_neighbour_stats. */
 int32_t phieng_world_set_neighbour_list(World2 *world, int32_t verlet, float skin);
-int32_t phieng_world_neighbour_stats(World2 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired);
+int32_t phieng_world_neighbour_stats(World2 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired,
+                                     uint64_t *list_substeps);
 uint32_t phieng_world_boundary_get(const World2 *world, float *xy, float *density, float *pressure, uint32_t max);
 
 #ifdef __cplusplus

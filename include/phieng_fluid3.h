@@ -208,9 +208,15 @@ int32_t phieng_world3_boundary_info(const World3 *world, uint32_t *count, uint64
  * 27 cells every substep.  Both give the reference's results bit for bit.
  * skin in [0, 0.5] (default 0.1).  1, or 0 on a bad argument. */
 int32_t phieng_world3_set_neighbour_list(World3 *world, int32_t verlet, float skin);
-/* Counters of the Verlet path: Verlet substeps, list rebuilds, and rows put
- * back in order after a neighbour changed cell.  Synchronises. */
-int32_t phieng_world3_neighbour_stats(World3 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired);
+/* Counters of the Verlet path: Verlet substeps, list rebuilds, rows put
+ * back in order after a neighbour changed cell, and substeps that swept the
+ * cells instead because such rows had become too many (more than
+ * max(1024, n / 512) in a substep: the list form then runs for 16
+ * substeps, twice as many on each return to a flow still that fast, before
+ * the rows are rebuilt; every fourth Verlet substep decides from the one
+ * two before, so the choice is the same on every run).  Synchronises. */
+int32_t phieng_world3_neighbour_stats(World3 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired,
+                                      uint64_t *list_substeps);
 /* The ghosts in generation order: positions (3 floats), and the density
  * and pressure the last substep gave them.  Null outputs are skipped;
  * returns the number written. */

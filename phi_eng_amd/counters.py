@@ -23,12 +23,13 @@ import json
 import os
 
 # kernel function -> the kind name of the engine's timing table; a kind
-# whose substep runs two launches (the Verlet density pass: the rebuild or
-# the walk, the other returning at once) sums them.
+# whose substep runs several launches (the Verlet density pass: rebuild,
+# halo build and walk, those not needed returning at once) sums them.
 KINDS = {
     "k_cell_key": "cell_key", "k_scan_reduce": "scan_reduce", "k_scan_partials": "scan_partials",
     "k_scan_apply": "scan_apply", "k_scatter": "scatter", "k_rank_gather": "rank_gather",
     "k_density_list<false>": "density", "k_density_walk": "density", "k_density_rebuild": "density",
+    "k_halo_build": "density",
     "k_force_list": "force_integrate", "k_force_verlet": "force_integrate", "k_vl_decide": "verlet_decide",
     "k_density_list<true>": "density_list_skin", "k_sur_step": "surrogate_step", "k_tile_vmax": "tile_vmax",
     "k_list_radius": "list_radius", "k_err_reduce": "err_reduce", "k_monitor": "monitor",
@@ -65,7 +66,8 @@ def per_dispatch(d):
 
 def per_kind(d, counters):
     """Per kind: the mean over its launches of each counter, launches of
-    one kind in a substep added up."""
+    one kind in a substep added up (SQ_WAVES: the largest of them, the
+    grid of one pass)."""
     acc = collections.defaultdict(lambda: collections.defaultdict(list))
     for name, grid, vals in per_dispatch(d).values():
         kind = KINDS.get(name)
@@ -79,7 +81,8 @@ def per_kind(d, counters):
         for name, v in names.items():
             n = len(v) // len(counters)
             for i, c in enumerate(counters):
-                out[kind][c] += sum(v[i::len(counters)]) / max(n, 1)
+                mean = sum(v[i::len(counters)]) / max(n, 1)
+                out[kind][c] = max(out[kind][c], mean) if c == "SQ_WAVES" else out[kind][c] + mean
     return out
 
 

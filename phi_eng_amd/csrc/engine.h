@@ -221,6 +221,23 @@ struct Engine {
     DBuf<float> vl_tile_dmax;
     DBuf<VerletDev> vl_state;
     bool vl_state_ready = false;
+    // Rows whose neighbours changed cell cost their wave a re-ordering each
+    // substep; in a fast flow they outnumber what the sort and list passes
+    // of the list form cost (the two forms give the same bits).  Above
+    // vl_repair_limit() such rows per substep the list form runs for
+    // vl_list_left substeps (doubling on every return to a flow still that
+    // fast, reset after a long quiet stretch), then the rows are rebuilt.
+    // Every fourth Verlet substep decides, from what the decision kernel of
+    // the Verlet substep two before left in pinned host memory (slot
+    // vl_k % 4 of a ring of 4; slot 0's substeps record vl_ev), so the
+    // decision is the same on every run and the host waits for the stream
+    // once in four substeps, two substeps behind.
+    uint32_t *vl_host = nullptr;  // [4][2]: rows re-ordered, rebuild flag
+    hipEvent_t vl_ev = nullptr;
+    uint64_t vl_k = 0;  // Verlet substeps since the list form last ran
+    uint32_t vl_list_left = 0, vl_backoff = 16;
+    uint64_t vl_list_substeps = 0;
+    uint32_t vl_repair_limit() const { return std::max<uint32_t>(1024u, n_total / 512u); }
     void sort_phase(uint32_t *perm_out);
     // Density pass (writing the neighbour list), boundary ghosts, force pass
     // (with af, also the surrogate's history, records and judge sums).

@@ -251,6 +251,8 @@ Engine::~Engine() {
         for (auto e : ev_pool) (void)hipEventDestroy(e);
         if (mon_event) (void)hipEventDestroy(mon_event);
         if (h_monitor) (void)hipHostFree(h_monitor);
+        if (vl_ev) (void)hipEventDestroy(vl_ev);
+        if (vl_host) (void)hipHostFree(vl_host);
         if (rs_host) (void)hipHostFree(rs_host);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -456,8 +458,23 @@ void Engine::upload_tables() {
 void Engine::substep() {
     if (n_total == 0 || tiles.empty()) return;
     if (vl_enabled && !(scenes.size() == 1 && scenes[0].ghost_count)) {
-        substep_verlet();
-        return;
+        if (vl_list_left == 0 && vl_k % 4 == 2) {
+            // every fourth Verlet substep: the one two before (slot 0)
+            PH_CK(hipEventSynchronize(vl_ev));
+            if (vl_host[0] > vl_repair_limit()) {
+                vl_list_left = vl_backoff;
+                vl_backoff = std::min(2u * vl_backoff, 1024u);
+            } else if (vl_k >= 512) {
+                vl_backoff = 16;
+            }
+        }
+        if (vl_list_left == 0) {
+            substep_verlet();
+            return;
+        }
+        --vl_list_left;
+        vl_k = 0;
+        ++vl_list_substeps;
     }
     sort_phase(nullptr);
     plain_phase(nullptr);
@@ -492,6 +509,11 @@ void Engine::substep_verlet() {
         });
         vl_state_ready = true;
     }
+    if (!vl_host) {
+        PH_CK(hipHostMalloc((void **)&vl_host, 8 * sizeof(uint32_t), hipHostMallocDefault));
+        PH_CK(hipEventCreateWithFlags(&vl_ev, hipEventDisableTiming));
+    }
+    const uint32_t slot = (uint32_t)(vl_k % 4);
     // Rows built for another particle set or layout cannot be walked; the
     // cell record is the sorted state's, so it sorts every substep.
     const bool force = !vl_valid || vl_epoch != layout_epoch || vl_n != n || rc;
@@ -503,7 +525,8 @@ void Engine::substep_verlet() {
     const uint32_t *gate = reinterpret_cast<const uint32_t *>(vl_state.p);  // VerletDev::rebuild
     vl_valid = false;  // until this substep is queued whole
     run(KIND_VDECIDE, [&] {
-        return launch_vl_decide(stream, vl_state.p, vl_tile_dmax.p, vl_tile_q.p, nt, force ? 1u : 0u, thr);
+        return launch_vl_decide(stream, vl_state.p, vl_tile_dmax.p, vl_tile_q.p, nt, force ? 1u : 0u, thr,
+                                vl_host + 2 * slot);
     });
     if (counts_dirty)
         run(KIND_FILL, [&] { return launch_fill_u32(stream, counts.p, (uint32_t)alloc_buckets, 0u); });
@@ -551,6 +574,8 @@ void Engine::substep_verlet() {
                                    vl_word[1].p);
     });
     finish_substep();
+    if (slot == 0) PH_CK(hipEventRecord(vl_ev, stream));
+    ++vl_k;
     vl_valid = true;
     vl_epoch = layout_epoch;
     vl_n = n;

@@ -89,8 +89,10 @@ struct VerletArgs {
     uint32_t *lidx;
     uint32_t lstride;
 };
+// host_out (pinned host memory): [0] rows re-ordered in the last walk,
+// [1] this substep's rebuild flag.
 hipError_t launch_vl_decide(hipStream_t st, VerletDev *vs, const float *tile_dmax, uint32_t *tile_q, uint32_t ntiles,
-                            uint32_t force, float thr);
+                            uint32_t force, float thr, uint32_t *host_out);
 hipError_t launch_density_verlet(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                                  const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
                                  float *rec_rho, float *rec_p, uint32_t *rec_count, uint32_t *fcnt,

@@ -142,7 +142,8 @@ __device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t &total) 
 // order go to the statistics and restart at zero.
 __global__ __launch_bounds__(kDecideThreads) void k_vl_decide(VerletDev *vs, const float *__restrict__ tile_dmax,
                                                               uint32_t *__restrict__ tile_q, uint32_t nt,
-                                                              uint32_t force, float thr) {
+                                                              uint32_t force, float thr,
+                                                              uint32_t *__restrict__ host_out) {
     // every load of the block issued before any reduction: one round trip
     float dm = 0.0f;
     uint32_t qs = 0;
@@ -183,6 +184,8 @@ __global__ __launch_bounds__(kDecideThreads) void k_vl_decide(VerletDev *vs, con
         vs->rebuilds += rb ? 1ull : 0ull;
         vs->repaired += qs;
         vs->substeps += 1ull;
+        host_out[0] = qs;
+        host_out[1] = rb ? 1u : 0u;
     }
 }
 
@@ -660,8 +663,9 @@ hipError_t launch_density_skin(hipStream_t st, uint32_t ntiles, const TileDev *t
 }
 
 hipError_t launch_vl_decide(hipStream_t st, VerletDev *vs, const float *tile_dmax, uint32_t *tile_q, uint32_t ntiles,
-                            uint32_t force, float thr) {
-    hipLaunchKernelGGL(k_vl_decide, dim3(1), dim3(kDecideThreads), 0, st, vs, tile_dmax, tile_q, ntiles, force, thr);
+                            uint32_t force, float thr, uint32_t *host_out) {
+    hipLaunchKernelGGL(k_vl_decide, dim3(1), dim3(kDecideThreads), 0, st, vs, tile_dmax, tile_q, ntiles, force, thr,
+                       host_out);
     return hipGetLastError();
 }
 

@@ -1714,7 +1714,8 @@ int32_t phieng_world3_set_neighbour_list(World3 *world, int32_t verlet, float sk
     return 1;
 }
 
-int32_t phieng_world3_neighbour_stats(World3 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired) {
+int32_t phieng_world3_neighbour_stats(World3 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired,
+                                      uint64_t *list_substeps) {
     if (!world) return 0;
     try {
         VerletDev v{};
@@ -1726,6 +1727,7 @@ int32_t phieng_world3_neighbour_stats(World3 *world, uint64_t *substeps, uint64_
         if (substeps) *substeps = v.substeps;
         if (rebuilds) *rebuilds = v.rebuilds;
         if (repaired) *repaired = v.repaired;
+        if (list_substeps) *list_substeps = e.vl_list_substeps;
         return 1;
     } catch (const std::exception &ex) {
         set_error("phieng_world3_neighbour_stats", ex.what());
@@ -2006,8 +2008,9 @@ int32_t phieng_world_set_neighbour_list(World2 *world, int32_t verlet, float ski
     return world ? phieng_world3_set_neighbour_list(world2_base(world), verlet, skin) : 0;
 }
 
-int32_t phieng_world_neighbour_stats(World2 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired) {
-    return world ? phieng_world3_neighbour_stats(world2_base(world), substeps, rebuilds, repaired) : 0;
+int32_t phieng_world_neighbour_stats(World2 *world, uint64_t *substeps, uint64_t *rebuilds, uint64_t *repaired,
+                                     uint64_t *list_substeps) {
+    return world ? phieng_world3_neighbour_stats(world2_base(world), substeps, rebuilds, repaired, list_substeps) : 0;
 }
 
 int32_t phieng_world_boundary_info(const World2 *world, uint32_t *count, uint64_t *regens) {

@@ -173,7 +173,7 @@ SIGNATURES = {
     "phieng_world3_surrogate_judge": (_I32, [_P, _fp, _fp, _u64p]),
     "phieng_world3_boundary_info": (_I32, [_P, _u32p, _u64p]),
     "phieng_world3_set_neighbour_list": (_I32, [_P, _I32, _F]),
-    "phieng_world3_neighbour_stats": (_I32, [_P, _u64p, _u64p, _u64p]),
+    "phieng_world3_neighbour_stats": (_I32, [_P, _u64p, _u64p, _u64p, _u64p]),
     "phieng_world3_boundary_get": (_U32, [_P, _fp, _fp, _fp, _U32]),
     # include/phieng_fluid2.h (World2; FluidConfig2Ffi has FluidConfig3Ffi's layout)
     "phieng_world_new": (_P, []),
@@ -199,7 +199,7 @@ SIGNATURES = {
     "phieng_world_sync": (_I32, [_P]),
     "phieng_world_boundary_info": (_I32, [_P, _u32p, _u64p]),
     "phieng_world_set_neighbour_list": (_I32, [_P, _I32, _F]),
-    "phieng_world_neighbour_stats": (_I32, [_P, _u64p, _u64p, _u64p]),
+    "phieng_world_neighbour_stats": (_I32, [_P, _u64p, _u64p, _u64p, _u64p]),
     "phieng_world_boundary_get": (_U32, [_P, _fp, _fp, _fp, _U32]),
 }
 

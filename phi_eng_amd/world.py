@@ -207,10 +207,10 @@ class World3:
             raise ValueError("neighbour list settings rejected")
 
     def neighbour_stats(self):
-        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-        if not self._l.phieng_world3_neighbour_stats(self.ptr, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)):
+        a, b, c, d = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        if not self._l.phieng_world3_neighbour_stats(self.ptr, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)):
             raise RuntimeError(last_error())
-        return {"substeps": a.value, "rebuilds": b.value, "repaired": c.value}
+        return {"substeps": a.value, "rebuilds": b.value, "repaired": c.value, "list_substeps": d.value}
 
     def boundary(self):
         """Ghosts in generation order: dict(pos (G, 3), density, pressure)."""
@@ -417,10 +417,10 @@ class World2:
             raise ValueError("neighbour list settings rejected")
 
     def neighbour_stats(self):
-        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-        if not self._l.phieng_world_neighbour_stats(self.ptr, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)):
+        a, b, c, d = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        if not self._l.phieng_world_neighbour_stats(self.ptr, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)):
             raise RuntimeError(last_error())
-        return {"substeps": a.value, "rebuilds": b.value, "repaired": c.value}
+        return {"substeps": a.value, "rebuilds": b.value, "repaired": c.value, "list_substeps": d.value}
 
     def boundary(self):
         """Ghosts in generation order: dict(pos (G, 2), density, pressure)."""

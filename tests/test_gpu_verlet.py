@@ -50,7 +50,7 @@ def test_walk_trajectory_bit_exact(gpu, oracle_lib):
         assert np.array_equal(_bits(v), _bits(rv)), f"velocities differ at step {k}"
     st = w.neighbour_stats()
     print(f"[verlet walk] stats {st}", flush=True)
-    assert st["substeps"] == 40
+    assert st["substeps"] + st["list_substeps"] == 40
     assert 1 <= st["rebuilds"] < st["substeps"]
     assert st["repaired"] > 0
     w.close()
@@ -100,5 +100,30 @@ def test_settings_do_not_change_results(gpu, verlet, skin):
     assert np.array_equal(_bits(va), _bits(vb))
     if verlet and skin == 0.0:
         assert sb["rebuilds"] == sb["substeps"] == 30
+    a.close()
+    b.close()
+
+
+def test_fast_flow_switches_to_the_list_form(gpu):
+    """A dam stirred fast enough that more than 1,024 rows in a substep need
+    re-ordering: the world sweeps the cells for a stretch of substeps, then
+    rebuilds the rows and walks again, and its trajectory is the list-off
+    world's bit for bit throughout."""
+    dt = 2e-4
+    _, a = _stirred_dam(16, seed=34, speed=6.0)
+    _, b = _stirred_dam(16, seed=34, speed=6.0)
+    b.set_neighbour_list(False, 0.1)
+    for k in range(60):
+        a.step(dt)
+        b.step(dt)
+        if k % 20 == 19:
+            pa, va = a.get_state()
+            pb, vb = b.get_state()
+            print(f"[verlet switch] step {k}: {a.neighbour_stats()}", flush=True)
+            assert np.array_equal(_bits(pa), _bits(pb)), f"positions differ by step {k}"
+            assert np.array_equal(_bits(va), _bits(vb)), f"velocities differ by step {k}"
+    st = a.neighbour_stats()
+    assert st["list_substeps"] > 0 and st["substeps"] > 0
+    assert st["substeps"] + st["list_substeps"] == 60
     a.close()
     b.close()
