@@ -70,7 +70,7 @@ def _args():
 
 # Plain judge every CHECK_EVERY substeps in the surrogate runs (the bench
 # re-measures 15 and 20 beside it).  Each judge also evaluates the surrogate
-# on every 16th particle of the judged state and falls back above an
+# on 1/16 of the particles of the judged state and falls back above an
 # acceleration error sum |a_plain - a_sur| / sum |a_plain| of 0.2, besides
 # the bulk density deviation of 1%.
 CHECK_EVERY = 10
@@ -109,7 +109,7 @@ def bench_surrogate(a, plain_value):
         "plain_ms_per_step_same_steps": round(runs["plain"]["wall"] * 1e3 / S, 4),
         "speedup_vs_plain_headline": round(res["value"] / plain_value, 4),
         "steps": S, "check_every": CHECK_EVERY, "max_density_error": 0.01, "max_accel_error": 0.2,
-        "judge": "every check_every-th substep plain: bulk |rho/rho0 - 1| and, over every 16th particle, "
+        "judge": "every check_every-th substep plain: bulk |rho/rho0 - 1| and, over 1/16 of the particles, "
                  "sum |a_plain - a_sur| / sum |a_plain| of the surrogate evaluated on the judged state",
         "skin": "Verlet: max(0.01, 4 k check_every dt vmax / h), k = 1 or 2 judge intervals per list",
         "network_free": out["network_free"],

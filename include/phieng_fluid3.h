@@ -319,8 +319,9 @@ const char *phieng_surrogate3_last_error(void);
  * check_every-th substep is a plain substep that also judges the state on
  * two measures — the mean |rho/rho0 - 1| over particles with
  * rho >= 0.9 rho0, and the surrogate's acceleration error: before the plain
- * substep the surrogate is evaluated on the same state for every 16th
- * particle (the offset rotating from judge to judge), and the error is
+ * substep the surrogate is evaluated on the same state for 1/16 of the
+ * particles (every 16th run of 64 consecutive slots of the particle order,
+ * the offset rotating from judge to judge), and the error is
  * sum |a_plain - a_sur| / sum |a_plain| over those particles (pressure and
  * viscous accelerations, gravity excluded; not evaluated, -1, when no
  * surrogate substep could have run from that state or the judge also

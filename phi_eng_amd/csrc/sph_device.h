@@ -595,6 +595,20 @@ __device__ __forceinline__ void unpack_vd(uint2 w, float &vx, float &vy, float &
     dq = __uint_as_float(w.y & 0xffff0000u);
 }
 
+// The judge's sample of slots: runs of 64 consecutive slots (one wave's
+// worth, so the sample keeps the neighbour locality of the slot order),
+// every stride-th run from run off.  Sample k is slot judge_slot(k); slot
+// o is in the sample when judge_sample(o) returns true, with its k.
+__host__ __device__ __forceinline__ uint32_t judge_slot(uint32_t k, uint32_t stride, uint32_t off) {
+    return ((k >> 6) * stride + off) * 64u + (k & 63u);
+}
+__host__ __device__ __forceinline__ bool judge_sample(uint32_t o, uint32_t stride, uint32_t off, uint32_t &k) {
+    const uint32_t run = o >> 6;
+    if (run % stride != off) return false;
+    k = (run / stride) * 64u + (o & 63u);
+    return true;
+}
+
 // Neighbour records of one generation of the surrogate's state: per slot
 // j, pos[j] = (x, y, z, P | phase) and vd[j] = pack_vd(v, dP).
 struct SurRec {
@@ -614,9 +628,10 @@ struct AuxFuse {
     SurRec rec_out;                   // pos may be null: no records
     // per tile: (sum |rho/rho0 - 1|, count) of the bulk, and over the
     // particles the surrogate was evaluated on before this substep
-    // (previous slot o with o % sur_stride == sur_off, its acceleration at
-    // sur_acc[o / sur_stride]; sur_acc null: none) the sums of
-    // |a_plain - a_sur| and |a_plain| (accelerations without gravity)
+    // (previous slot o in the judge sample of (sur_stride, sur_off), its
+    // acceleration at sur_acc[k], k its sample index; sur_acc null: none)
+    // the sums of |a_plain - a_sur| and |a_plain| (accelerations without
+    // gravity)
     float4 *tile_err = nullptr;
     const float4 *sur_acc = nullptr;
     uint32_t sur_stride = 16u, sur_off = 0u;

@@ -641,8 +641,9 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
                 jerr = fabsf(r - 1.0f);
                 jcnt = 1.0f;
             }
-            if (af.sur_acc && o % af.sur_stride == af.sur_off) {
-                const float4 as = af.sur_acc[o / af.sur_stride];
+            uint32_t ks;
+            if (af.sur_acc && judge_sample(o, af.sur_stride, af.sur_off, ks)) {
+                const float4 as = af.sur_acc[ks];
                 const float px = ax - s.gx, py = ay - s.gy, pz = az - s.gz;
                 jae = sqrtf(dot3(px - as.x, py - as.y, pz - as.z));
                 jam = sqrtf(dot3(px, py, pz));

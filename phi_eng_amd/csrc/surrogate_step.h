@@ -43,8 +43,9 @@ struct SurStepArgs {
     float qs;
     float4 *posm_out, *vel_out, *aux_out;
     SurRec rec_out;
-    // the judge's evaluation (launch_sur_judge): slots judge_off +
-    // judge_stride k, accelerations without gravity to judge_acc[k]
+    // the judge's evaluation (launch_sur_judge): sample k is slot
+    // judge_slot(k, judge_stride, judge_off); its acceleration without
+    // gravity goes to judge_acc[k]
     float4 *judge_acc;
     uint32_t judge_stride, judge_off;
 };

@@ -336,10 +336,9 @@ __global__ __launch_bounds__(PH_TILE) void k_sur_collect(SurCollectArgs a) {
 // Y^T = W2^T H^T, so the hidden tiles that two K = 16 products leave in a
 // lane are the operand the K = 32 product needs there), and the integrate;
 // writes the advanced state, the new history and the records of the next
-// substep.  JUDGE:
-// the same evaluation for every judge_stride-th slot from judge_off (the
-// slots of sample k = blockIdx.x * PH_TILE + threadIdx.x), writing only the
-// acceleration without gravity to judge_acc[k] (one scene).
+// substep.  JUDGE: the same evaluation for the judge's sample (sample
+// k = blockIdx.x * PH_TILE + threadIdx.x is slot judge_slot(k)), writing
+// only the acceleration without gravity to judge_acc[k] (one scene).
 template <bool UMASS, bool JUDGE>
 __global__ __launch_bounds__(PH_TILE, 7) void k_sur_step(SurStepArgs a) {
     __shared__ uint16_t xs[PH_TILE][MLP_IN];
@@ -351,7 +350,7 @@ __global__ __launch_bounds__(PH_TILE, 7) void k_sur_step(SurStepArgs a) {
     const uint32_t scene = JUDGE ? 0u : a.tiles[tile].scene;
     const SceneDev s = a.scenes[scene];
     const uint32_t sample = blockIdx.x * PH_TILE + threadIdx.x;
-    const uint32_t i = JUDGE ? sample * a.judge_stride + a.judge_off : a.tiles[tile].first + threadIdx.x;
+    const uint32_t i = JUDGE ? judge_slot(sample, a.judge_stride, a.judge_off) : a.tiles[tile].first + threadIdx.x;
     const bool active = i < s.part_end;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
     if (a.use_net) {

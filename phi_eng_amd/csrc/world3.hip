@@ -159,8 +159,8 @@ struct World3 {
     uint64_t n_sur = 0, n_plain = 0, n_fallback = 0;
     float last_err = -1.0f;
     // The judge's second measure: before its plain substep the surrogate is
-    // evaluated on the judged state for every kJudgeStride-th slot (the
-    // offset rotating per judge); the plain force pass then sums
+    // evaluated on the judged state for every kJudgeStride-th run of 64
+    // slots (the offset rotating per judge); the plain force pass then sums
     // |a_plain - a_sur| and |a_plain| over the same particles.  The verdict
     // fails above max_accel_err of their ratio as well as above max_err.
     static constexpr uint32_t kJudgeStride = 16;
@@ -953,17 +953,19 @@ struct World3 {
         judge_at = step_index;
     }
 
-    // Launch the surrogate's evaluation on the current generation for every
-    // kJudgeStride-th slot; returns the slot offset.
+    // Launch the surrogate's evaluation on the current generation for the
+    // judge's sample (judge_slot); returns its offset.
     uint32_t judge_evaluate() {
         ph::Engine &e = eng;
         const uint32_t joff = n_judges % kJudgeStride;
-        judge_acc.ensure(e.alloc_particles / kJudgeStride + 1);
+        judge_acc.ensure((e.alloc_particles / (64 * kJudgeStride) + 1) * 64);
         SurStepArgs a = sur_args();
         a.judge_acc = judge_acc.p;
         a.judge_stride = kJudgeStride;
         a.judge_off = joff;
-        const uint32_t cnt = n > joff ? (n - 1 - joff) / kJudgeStride + 1 : 0;
+        // every run that starts below n (slots past n are idle lanes)
+        const uint32_t runs = n / 64 + (n % 64 ? 1 : 0);
+        const uint32_t cnt = runs > joff ? ((runs - 1 - joff) / kJudgeStride + 1) * 64 : 0;
         e.run(ph::KIND_SUR_STEP, [&] { return ph::launch_sur_judge(e.stream, cnt, a); });
         return joff;
     }

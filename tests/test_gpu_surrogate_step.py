@@ -152,7 +152,7 @@ def test_network_free_surrogate_tracks_plain(gpu):
 
 def test_accel_judge(gpu):
     """The judge's second measure, sum |a_plain - a_sur| / sum |a_plain| over
-    every 16th particle: evaluated once a surrogate substep could have run
+    1/16 of the particles: evaluated once a surrogate substep could have run
     from the judged state (not at the first judge), equal for a zero network
     and the network-free method (both give coefficients 0), and a limit
     below it turns judges into fallbacks that the density measure alone
