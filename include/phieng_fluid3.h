@@ -203,10 +203,12 @@ const char *phieng_kernel_name(int32_t kind);
  * config, fingerprinted as fluid3.rs:660-714).  1, or 0 on null. */
 int32_t phieng_world3_boundary_info(const World3 *world, uint32_t *count, uint64_t *regens);
 /* Extension (no reference counterpart): neighbour search of the plain
- * substep.  verlet = 1 (the default) keeps per particle the particles within
- * (1 + skin) h at the last rebuild and walks that list; verlet = 0 sweeps the
- * 27 cells every substep.  Both give the reference's results bit for bit.
- * skin in [0, 0.5] (default 0.1).  1, or 0 on a bad argument. */
+ * substep.  verlet = 1 keeps per particle the particles within (1 + skin) h
+ * at the last rebuild and walks that list; verlet = 0 sweeps the 27 cells
+ * every substep; verlet = 2 (the default) keeps the lists from 131,072
+ * particles up and sweeps below (where the list form's extra launches cost
+ * more than its walk saves).  All give the reference's results bit for
+ * bit.  skin in [0, 0.5] (default 0.1).  1, or 0 on a bad argument. */
 int32_t phieng_world3_set_neighbour_list(World3 *world, int32_t verlet, float skin);
 /* Counters of the Verlet path: Verlet substeps, list rebuilds, rows put
  * back in order after a neighbour changed cell, and substeps that swept the

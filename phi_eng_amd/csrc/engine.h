@@ -212,6 +212,10 @@ struct Engine {
     // current state; every other way of moving or reordering particles
     // clears it, and the next Verlet substep then rebuilds.
     bool vl_enabled = true;
+    // below this many particles the Verlet form's extra launches (decision,
+    // halo build) cost more than its walk saves: the list form runs
+    uint32_t vl_min_particles = kVlAutoMin;
+    static constexpr uint32_t kVlAutoMin = 131072;
     float vl_skin = 0.1f;
     bool vl_valid = false;
     uint64_t vl_epoch = 0;
@@ -228,12 +232,13 @@ struct Engine {
     // vl_list_left substeps (doubling on every return to a flow still that
     // fast, reset after a long quiet stretch), then the rows are rebuilt.
     // Every fourth Verlet substep decides, from what the decision kernel of
-    // the Verlet substep two before left in pinned host memory (slot
-    // vl_k % 4 of a ring of 4; slot 0's substeps record vl_ev), so the
-    // decision is the same on every run and the host waits for the stream
-    // once in four substeps, two substeps behind.
-    uint32_t *vl_host = nullptr;  // [4][2]: rows re-ordered, rebuild flag
-    hipEvent_t vl_ev = nullptr;
+    // the Verlet substep kVlLag before left in pinned host memory (slot
+    // vl_k % 32 of a ring; every fourth substep records an event of a ring
+    // of 8), so the decision is the same on every run and the host may run
+    // kVlLag substeps ahead of the device.
+    static constexpr uint32_t kVlLag = 2;
+    uint32_t *vl_host = nullptr;  // [32][2]: rows re-ordered, rebuild flag
+    hipEvent_t vl_ev[8] = {};
     uint64_t vl_k = 0;  // Verlet substeps since the list form last ran
     uint32_t vl_list_left = 0, vl_backoff = 16;
     uint64_t vl_list_substeps = 0;

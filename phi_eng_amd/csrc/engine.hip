@@ -251,7 +251,8 @@ Engine::~Engine() {
         for (auto e : ev_pool) (void)hipEventDestroy(e);
         if (mon_event) (void)hipEventDestroy(mon_event);
         if (h_monitor) (void)hipHostFree(h_monitor);
-        if (vl_ev) (void)hipEventDestroy(vl_ev);
+        for (auto ev : vl_ev)
+            if (ev) (void)hipEventDestroy(ev);
         if (vl_host) (void)hipHostFree(vl_host);
         if (rs_host) (void)hipHostFree(rs_host);
         if (stream) (void)hipStreamDestroy(stream);
@@ -457,11 +458,11 @@ void Engine::upload_tables() {
 
 void Engine::substep() {
     if (n_total == 0 || tiles.empty()) return;
-    if (vl_enabled && !(scenes.size() == 1 && scenes[0].ghost_count)) {
-        if (vl_list_left == 0 && vl_k % 4 == 2) {
-            // every fourth Verlet substep: the one two before (slot 0)
-            PH_CK(hipEventSynchronize(vl_ev));
-            if (vl_host[0] > vl_repair_limit()) {
+    if (vl_enabled && n_total >= vl_min_particles && !(scenes.size() == 1 && scenes[0].ghost_count)) {
+        if (vl_list_left == 0 && vl_k >= kVlLag && (vl_k - kVlLag) % 4 == 0) {
+            const uint64_t u = vl_k - kVlLag;  // a multiple of 4: it recorded an event
+            PH_CK(hipEventSynchronize(vl_ev[(u / 4) % 8]));
+            if (vl_host[2 * (u % 32)] > vl_repair_limit()) {
                 vl_list_left = vl_backoff;
                 vl_backoff = std::min(2u * vl_backoff, 1024u);
             } else if (vl_k >= 512) {
@@ -510,10 +511,10 @@ void Engine::substep_verlet() {
         vl_state_ready = true;
     }
     if (!vl_host) {
-        PH_CK(hipHostMalloc((void **)&vl_host, 8 * sizeof(uint32_t), hipHostMallocDefault));
-        PH_CK(hipEventCreateWithFlags(&vl_ev, hipEventDisableTiming));
+        PH_CK(hipHostMalloc((void **)&vl_host, 64 * sizeof(uint32_t), hipHostMallocDefault));
+        for (auto &ev : vl_ev) PH_CK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     }
-    const uint32_t slot = (uint32_t)(vl_k % 4);
+    const uint32_t slot = (uint32_t)(vl_k % 32);
     // Rows built for another particle set or layout cannot be walked; the
     // cell record is the sorted state's, so it sorts every substep.
     const bool force = !vl_valid || vl_epoch != layout_epoch || vl_n != n || rc;
@@ -574,7 +575,7 @@ void Engine::substep_verlet() {
                                    vl_word[1].p);
     });
     finish_substep();
-    if (slot == 0) PH_CK(hipEventRecord(vl_ev, stream));
+    if (vl_k % 4 == 0) PH_CK(hipEventRecord(vl_ev[(vl_k / 4) % 8], stream));
     ++vl_k;
     vl_valid = true;
     vl_epoch = layout_epoch;

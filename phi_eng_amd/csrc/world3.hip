@@ -1709,8 +1709,9 @@ int32_t phieng_world3_surrogate_judge(const World3 *world, float *density_error,
 }
 
 int32_t phieng_world3_set_neighbour_list(World3 *world, int32_t verlet, float skin) {
-    if (!world || !(skin >= 0.0f && skin <= 0.5f)) return 0;
+    if (!world || !(skin >= 0.0f && skin <= 0.5f) || verlet < 0 || verlet > 2) return 0;
     world->eng.vl_enabled = verlet != 0;
+    world->eng.vl_min_particles = verlet == 2 ? ph::Engine::kVlAutoMin : 0u;
     world->eng.vl_skin = skin;
     world->eng.vl_valid = false;  // the next substep rebuilds with the new radius
     return 1;

@@ -203,7 +203,10 @@ class World3:
         return c.value, r.value
 
     def set_neighbour_list(self, verlet=True, skin=0.1):
-        if not self._l.phieng_world3_set_neighbour_list(self.ptr, 1 if verlet else 0, float(skin)):
+        """verlet: True (the row form), False (the cells), or "auto" (the
+        default: rows from 131,072 particles up)."""
+        v = 2 if verlet == "auto" else (1 if verlet else 0)
+        if not self._l.phieng_world3_set_neighbour_list(self.ptr, v, float(skin)):
             raise ValueError("neighbour list settings rejected")
 
     def neighbour_stats(self):
@@ -413,7 +416,8 @@ class World2:
         return c.value, r.value
 
     def set_neighbour_list(self, verlet=True, skin=0.1):
-        if not self._l.phieng_world_set_neighbour_list(self.ptr, 1 if verlet else 0, float(skin)):
+        v = 2 if verlet == "auto" else (1 if verlet else 0)
+        if not self._l.phieng_world_set_neighbour_list(self.ptr, v, float(skin)):
             raise ValueError("neighbour list settings rejected")
 
     def neighbour_stats(self):

@@ -133,6 +133,12 @@ def test_null_tolerance():
         lib.phieng_world3_step(w.ptr, float("nan"))
         lib.phieng_world3_step(w.ptr, -1.0)
         lib.phieng_world3_step(w.ptr, 0.016)  # empty fluid: no-op, no device needed
+        # neighbour search: 0 cells, 1 rows, 2 automatic; skin in [0, 0.5]
+        for v in (0, 1, 2):
+            assert lib.phieng_world3_set_neighbour_list(w.ptr, v, 0.1) == 1
+        assert lib.phieng_world3_set_neighbour_list(w.ptr, 3, 0.1) == 0
+        assert lib.phieng_world3_set_neighbour_list(w.ptr, -1, 0.1) == 0
+        assert lib.phieng_world3_set_neighbour_list(w.ptr, 1, 0.6) == 0
         # the surrogate judge's acceleration limit: > 0 (inf turns it off)
         assert lib.phieng_world3_set_surrogate_accel_limit(None, 0.5) == 0
         for bad in (0.0, -1.0, float("nan")):

@@ -19,6 +19,7 @@ def _stirred_dam(n_side, seed, speed):
     viscosity within a few substeps)."""
     sc = scenes.small_dam(n_side, seed=seed)
     w = scenes.make_world(sc)
+    w.set_neighbour_list(True, 0.1)  # the row form at any size (the default takes it from 131,072 up)
     rng = np.random.default_rng(seed)
     pos, _ = w.get_state()
     d = np.array([2.0, 0.5, 1.0]) / np.linalg.norm([2.0, 0.5, 1.0])
@@ -81,10 +82,11 @@ def test_walk_records_bit_exact(gpu, oracle_lib):
     w.close()
 
 
-@pytest.mark.parametrize("verlet,skin", [(False, 0.1), (True, 0.0), (True, 0.3)])
+@pytest.mark.parametrize("verlet,skin", [(False, 0.1), (True, 0.0), (True, 0.3), ("auto", 0.1)])
 def test_settings_do_not_change_results(gpu, verlet, skin):
-    """The list off, a zero skin (a rebuild every substep) and a wide skin
-    give the default world's trajectory bit for bit."""
+    """The list off, a zero skin (a rebuild every substep), a wide skin and
+    the automatic choice (the cells, at this size) give the row-form
+    world's trajectory bit for bit."""
     dt = 2e-4
     _, a = _stirred_dam(10, seed=33, speed=1.5)
     _, b = _stirred_dam(10, seed=33, speed=1.5)
