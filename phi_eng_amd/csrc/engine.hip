@@ -572,7 +572,7 @@ void Engine::substep_verlet() {
         return launch_force_verlet(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p, vel[ns].p,
                                    rq.p, tag[nt_].p, nl_cnt.p, nl_idx.p, list_stride(), posm[cs].p, vel[cs].p,
                                    ra ? rec_acc.p : nullptr, d_bbox.p, GhostDev{}, vl_tile_dmax.p, vl_state.p,
-                                   vl_word[1].p);
+                                   vl_word[1].p, vl_halo.p, vl_halo_n.p);
     });
     finish_substep();
     if (vl_k % 4 == 0) PH_CK(hipEventRecord(vl_ev[(vl_k / 4) % 8], stream));

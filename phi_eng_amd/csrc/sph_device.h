@@ -88,6 +88,42 @@ PH_HD bool finite3(float x, float y, float z) {
 // glam Vec3::length_squared: (x*x + y*y) + z*z.
 PH_HD float dot3(float x, float y, float z) { return x * x + y * y + z * z; }
 
+// sqrtf(r2) rounded to nearest, for a pair's squared distance (finite,
+// >= 0).  For r2 >= 2^-96 the hardware root is within one ulp, and its two
+// one-ulp neighbours are tested with exact FMA residuals: the correction
+// the compiler emits for an IEEE sqrtf, without its rescaling of inputs
+// below 2^-96 and its select for 0 and infinity.  Below 2^-96 (0 included)
+// the result is some value below 2^-40; every caller tests dist < EPS
+// (1e-6) before using the root, which such a pair fails either way.
+__device__ __forceinline__ float sqrt_pair(float r2) {
+    const float s = __builtin_amdgcn_sqrtf(r2);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = __builtin_fmaf(-sd, s, r2) <= 0.0f ? sd : s;
+    r = __builtin_fmaf(-su, s, r2) > 0.0f ? su : r;
+    return r;
+}
+
+// AMD's IEEE f32 division a / b is: scale both operands (v_div_scale),
+// y = rcp(b) refined by one Newton step, q = a y, two residual corrections
+// q += (a - b q) y, then a fix-up of special operands (v_div_fixup).  The
+// scaling and the fix-up are identities when a != 0, |a| >= 2^-100,
+// |a| <= 2 b, b is normal and 1 / b and a / b are normal; then the middle
+// part alone gives the IEEE quotient bit for bit, and the refined
+// reciprocal can be shared by every numerator over the same b.
+__device__ __forceinline__ float div_recip(float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    return __builtin_fmaf(e, y0, y0);
+}
+__device__ __forceinline__ float div_by_recip(float a, float b, float y) {
+    float q = a * y;
+    float r = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(r, y, q);
+}
+
 // fluid3.rs:220-228 (coefficient precomputed on the host in f32).
 __device__ __forceinline__ float poly6(float r2, const SceneDev &s) {
     if (r2 >= s.h2 || r2 < 0.0f) return 0.0f;

@@ -59,7 +59,8 @@ hipError_t launch_force_verlet(hipStream_t st, uint32_t ntiles, const TileDev *t
                                const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
                                const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
                                uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
-                               const GhostDev &gh, float *tile_dmax, VerletDev *vs, uint32_t *vword_next);
+                               const GhostDev &gh, float *tile_dmax, VerletDev *vs, uint32_t *vword_next,
+                               const uint32_t *vhalo, const uint32_t *vhalo_n);
 // Verlet substep (verlet.hip).  k_vl_decide (one block) decides whether
 // this substep sorts and rebuilds every row: on request (force), after a
 // particle needed the per-cell path, or when the displacement bound (the

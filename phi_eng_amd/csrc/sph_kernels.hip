@@ -459,13 +459,22 @@ __device__ __forceinline__ void pair_terms(const SceneDev &s, const float4 &pi, 
     const float rho_j = rmaxf(rqj.x, PH_EPS);
     // delta = x_i - x_j; |delta|^2 equals the cut-off test's r2 exactly.
     const float ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-    const float dist = sqrtf(r2);
+    const float dist = sqrt_pair(r2);
     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
     if (!(dist >= h || dist < PH_EPS)) {  // spiky_grad, fluid3.rs:232-239
         const float sc = s.spiky_c * (h - dist) * (h - dist);
-        gx = ex / dist * sc;
-        gy = ey / dist * sc;
-        gz = ez / dist * sc;
+        // dist in [1e-6, h) with h <= 1024, and |e| <= dist: the three
+        // quotients share one reciprocal unless a component is 0 or tiny
+        if (h <= 1024.0f && fminf(fminf(fabsf(ex), fabsf(ey)), fabsf(ez)) >= 0x1p-100f) {
+            const float y = div_recip(dist);
+            gx = div_by_recip(ex, dist, y) * sc;
+            gy = div_by_recip(ey, dist, y) * sc;
+            gz = div_by_recip(ez, dist, y) * sc;
+        } else {
+            gx = ex / dist * sc;
+            gy = ey / dist * sc;
+            gz = ez / dist * sc;
+        }
     }
     float term = q_i + rqj.y;
     if (s.av_alpha > 0.0f && dist > PH_EPS) {  // artificial_pressure, fluid3.rs:254-269
@@ -518,12 +527,26 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
     const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox,
     float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}, const AuxFuse &af = AuxFuse{},
-    float *__restrict__ tile_dmax = nullptr, VerletDev *vs = nullptr, uint32_t *__restrict__ vword_next = nullptr) {
+    float *__restrict__ tile_dmax = nullptr, VerletDev *vs = nullptr, uint32_t *__restrict__ vword_next = nullptr,
+    const uint32_t *__restrict__ vhalo = nullptr, const uint32_t *__restrict__ vhalo_n = nullptr) {
     const uint32_t tile = xcd_tile(bid, nbl);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     const bool active = i < s.part_end;
+    // A Verlet walk substep's tile with a halo hands over 16-bit halo
+    // indices (eight to a group); their slots are staged here.
+    __shared__ uint32_t f_s[VL ? VL_HALO_CAP : 1];
+    bool l16 = false;
+    if (VL && vs->rebuild == 0u) {
+        const uint32_t H = vhalo_n[tile];
+        if (H != VL_HALO_NONE) {  // the same for the whole block
+            l16 = true;
+            const uint32_t *hl = vhalo + (size_t)tile * VL_HALO_CAP;
+            for (uint32_t k = threadIdx.x; k < H; k += PH_TILE) f_s[k] = hl[k];
+            __syncthreads();
+        }
+    }
     int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bz0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN, bz1 = INT32_MIN;
     float jerr = 0.0f, jcnt = 0.0f, jae = 0.0f, jam = 0.0f;  // judge sums (AuxFuse)
     float dmax2 = 0.0f;               // largest squared displacement (VL)
@@ -539,9 +562,8 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             const uint32_t cw = lcnt[i];
             // one list entry: the self entry and those outside h add nothing
             auto entry = [&](uint32_t j, const float4 &pj, const float4 &vj, const float2 &qj) {
-                if (j == i) return;
                 const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                if (r2 < hg2) pair_terms(s, pi, vi, rho_i, q_i, pj, vj, qj, r2, acc);
+                if (j != i && r2 < hg2) pair_terms(s, pi, vi, rho_i, q_i, pj, vj, qj, r2, acc);
             };
             if (cw & SLF_NONE) {
                 // non-finite position: no neighbours
@@ -552,20 +574,51 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             } else {
                 const uint32_t n = cw & SLF_COUNT;
                 const uint4 *g = lgrp + i;
-#pragma unroll 1
-                for (uint32_t k = 0; k < n; k += 4) {
-                    const uint4 e = *g;
-                    g += mstride;
-                    // slots past the count are replaced by i (skipped)
-                    const uint32_t j0 = e.x, j1 = k + 1 < n ? e.y : i, j2 = k + 2 < n ? e.z : i,
-                                   j3 = k + 3 < n ? e.w : i;
-                    const float4 p0 = posm[j0], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
-                    const float4 v0 = vel[j0], v1 = vel[j1], v2 = vel[j2], v3 = vel[j3];
-                    const float2 q0 = rq[j0], q1 = rq[j1], q2 = rq[j2], q3 = rq[j3];
+                // byte offsets fit 32 bits (slots < 2^27): one base register
+                // per array and one shift per slot and element size
+                const char *pb = reinterpret_cast<const char *>(posm), *vb = reinterpret_cast<const char *>(vel),
+                           *qb = reinterpret_cast<const char *>(rq);
+                auto p4 = [](const char *b, uint32_t j) { return *reinterpret_cast<const float4 *>(b + (j << 4)); };
+                auto q2l = [](const char *b, uint32_t j) { return *reinterpret_cast<const float2 *>(b + (j << 3)); };
+                // four neighbours (slots past the count are replaced by i,
+                // which is skipped)
+                auto four = [&](uint32_t j0, uint32_t j1, uint32_t j2, uint32_t j3) {
+                    const float4 p0 = p4(pb, j0), p1 = p4(pb, j1), p2 = p4(pb, j2), p3 = p4(pb, j3);
+                    const float4 v0 = p4(vb, j0), v1 = p4(vb, j1), v2 = p4(vb, j2), v3 = p4(vb, j3);
+                    const float2 q0 = q2l(qb, j0), q1 = q2l(qb, j1), q2 = q2l(qb, j2), q3 = q2l(qb, j3);
                     entry(j0, p0, v0, q0);
                     entry(j1, p1, v1, q1);
                     entry(j2, p2, v2, q2);
                     entry(j3, p3, v3, q3);
+                };
+                // the next group is loaded while this one's neighbours are
+                uint4 e = n ? *g : make_uint4(0u, 0u, 0u, 0u);
+                if (VL && l16) {
+#pragma unroll 1
+                    for (uint32_t k = 0; k < n; k += 8) {
+                        g += mstride;
+                        const uint4 en = *(k + 8 < n ? g : lgrp + i);
+                        const uint32_t ws[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+                        for (uint32_t hq = 0; hq < 2; ++hq) {
+                            uint32_t j[4];
+#pragma unroll
+                            for (uint32_t q = 0; q < 4; ++q) {
+                                const uint32_t m = 4u * hq + q;
+                                j[q] = k + m < n ? f_s[(ws[m >> 1] >> (16u * (m & 1u))) & 0xFFFFu] : i;
+                            }
+                            four(j[0], j[1], j[2], j[3]);
+                        }
+                        e = en;
+                    }
+                } else {
+#pragma unroll 1
+                    for (uint32_t k = 0; k < n; k += 4) {
+                        g += mstride;
+                        const uint4 en = *(k + 4 < n ? g : lgrp + i);
+                        four(e.x, k + 1 < n ? e.y : i, k + 2 < n ? e.z : i, k + 3 < n ? e.w : i);
+                        e = en;
+                    }
                 }
             }
         }
@@ -767,10 +820,11 @@ __global__ __launch_bounds__(PH_TILE) void k_force_verlet(const TileDev *__restr
     const float4 *__restrict__ vel, const float2 *__restrict__ rq, const uint32_t *__restrict__ tag,
     const uint32_t *__restrict__ lcnt, const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, GhostDev gh,
-    float *__restrict__ tile_dmax, VerletDev *vs, uint32_t *__restrict__ vword_next) {
+    float *__restrict__ tile_dmax, VerletDev *vs, uint32_t *__restrict__ vword_next,
+    const uint32_t *__restrict__ vhalo, const uint32_t *__restrict__ vhalo_n) {
     dev_force_integrate<true>(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, lcnt, lgrp,
                               mstride, posm_out, vel_out, rec_acc, bbox, nullptr, gh, AuxFuse{}, tile_dmax, vs,
-                              vword_next);
+                              vword_next, vhalo, vhalo_n);
 }
 
 // Boundary ghost density and pressure from the fluid (fluid3.rs:906-920):
@@ -1082,10 +1136,11 @@ hipError_t launch_force_verlet(hipStream_t st, uint32_t ntiles, const TileDev *t
                                const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
                                const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
                                uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
-                               const GhostDev &gh, float *tile_dmax, VerletDev *vs, uint32_t *vword_next) {
+                               const GhostDev &gh, float *tile_dmax, VerletDev *vs, uint32_t *vword_next,
+                               const uint32_t *vhalo, const uint32_t *vhalo_n) {
     hipLaunchKernelGGL(k_force_verlet, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm, vel,
                        rq, tag, lcnt, reinterpret_cast<const uint4 *>(lidx), mstride, posm_out, vel_out, rec_acc,
-                       bbox, gh, tile_dmax, vs, vword_next);
+                       bbox, gh, tile_dmax, vs, vword_next, vhalo, vhalo_n);
     return hipGetLastError();
 }
 hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,

@@ -84,6 +84,38 @@ struct ListPush {
     }
 };
 
+// The same list from a tile that walks its halo: the neighbours' halo
+// indices (16 bits), eight to a 16-byte group.  Each push shifts the group
+// down by one entry and puts the new one on top, so after eight pushes
+// entry m of the group sits at bits 16 m.
+struct ListPush16 {
+    uint4 *g;
+    uint32_t stride;
+    uint32_t k = 0;
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    __device__ __forceinline__ ListPush16(uint32_t *base, uint32_t stride_, uint32_t i)
+        : g(reinterpret_cast<uint4 *>(base) + i), stride(stride_) {}
+    __device__ __forceinline__ void shift_in(uint32_t x) {
+        w0 = __builtin_amdgcn_alignbit(w1, w0, 16u);
+        w1 = __builtin_amdgcn_alignbit(w2, w1, 16u);
+        w2 = __builtin_amdgcn_alignbit(w3, w2, 16u);
+        w3 = __builtin_amdgcn_alignbit(x, w3, 16u);
+    }
+    __device__ __forceinline__ void push(uint32_t x) {
+        shift_in(x);
+        if ((k & 7u) == 7u && k < SL_CAP) g[(size_t)(k >> 3) * stride] = make_uint4(w0, w1, w2, w3);
+        ++k;
+    }
+    // the last, partly filled group moved down to bit 0; the count word
+    __device__ __forceinline__ uint32_t finish() {
+        if ((k & 7u) != 0u && k < SL_CAP) {
+            for (uint32_t r = k & 7u; r < 8u; ++r) shift_in(0u);
+            g[(size_t)(k >> 3) * stride] = make_uint4(w0, w1, w2, w3);
+        }
+        return k <= SL_CAP ? k : SLF_SLOW;
+    }
+};
+
 // The per-cell sum (reference order, outlier cells included).
 __device__ __forceinline__ float density_cells(const SceneDev &s, const uint32_t *__restrict__ start,
                                                const float4 *__restrict__ posm, const float4 &pi, int32_t cx,
@@ -408,8 +440,11 @@ __global__ __launch_bounds__(PH_TILE) void k_halo_build(const TileDev *__restric
 }
 
 // The walk substep's density pass (returns at once on a rebuild substep).
-// A tile with a halo copies its halo's positions, words and slots to LDS
-// and walks 16-bit rows; other tiles walk 32-bit rows of slots.
+// A tile with a halo copies its halo's positions and words to LDS, walks
+// 16-bit rows and hands the force pass a 16-bit list of halo indices;
+// other tiles walk 32-bit rows of slots and hand over slots.  A row group
+// (eight entries) is loaded while the one before is summed, and each
+// group's LDS reads are issued together.
 __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restrict__ tiles,
                                                           const SceneDev *__restrict__ scenes,
                                                           const float4 *__restrict__ posm,
@@ -417,12 +452,12 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
                                                           float *__restrict__ rec_rho, float *__restrict__ rec_p,
                                                           uint32_t *__restrict__ rec_count,
                                                           uint32_t *__restrict__ fcnt, ph::VerletArgs va) {
-    __shared__ float4 h_p[VL_HALO_CAP];                  // position and mass
-    __shared__ uint32_t h_w[VL_HALO_CAP], h_s[VL_HALO_CAP];  // neighbour word, slot
-    // per wave: one row being put in order (entries, keys, contributions)
-    __shared__ uint32_t s_key[PH_TILE / 64][VL_CAP];
-    __shared__ uint32_t s_uj[PH_TILE / 64][VL_CAP], s_slot[PH_TILE / 64][VL_CAP];
-    __shared__ float s_uc[PH_TILE / 64][VL_CAP], s_c[PH_TILE / 64][VL_CAP];
+    __shared__ float4 h_p[VL_HALO_CAP];  // position and mass
+    __shared__ uint32_t h_w[VL_HALO_CAP];  // neighbour word
+    // per wave: one row being put in order (keys; entries with the
+    // within-h bit and contributions, overwritten in rank order)
+    __shared__ uint32_t s_key[PH_TILE / 64][VL_CAP], s_uj[PH_TILE / 64][VL_CAP];
+    __shared__ float s_uc[PH_TILE / 64][VL_CAP];
     if (va.vs->rebuild != 0u) return;
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const TileDev t = tiles[tile];
@@ -437,15 +472,13 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
             const uint32_t j = hl[k];
             h_p[k] = posm[j];
             h_w[k] = va.vword[j];
-            h_s[k] = j;
         }
         __syncthreads();
     }
     const float4 pi = act ? posm[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const bool fin = finite3(pi.x, pi.y, pi.z);
     // Walk (every lane of the wave stays to the end: the repair below is
-    // the whole wave's).  For each entry, the position and the neighbour
-    // word (independent loads).
+    // the whole wave's).
     const float hg2 = s.h_grid2;
     const bool mine = act && fin;
     if (act && !fin) {
@@ -459,50 +492,60 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
     bool bad = false;
     uint32_t cnt = 0, fc = 0;
     float rho = 0.0f;
-    {
-        ListPush lp(va.lidx, va.lstride, i);
-        uint32_t next = 0;  // the next neighbour's key must be at least this
-        auto one = [&](uint32_t j, const float4 &pj, uint32_t wj) {
-            const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-            const uint32_t key = vl_key(wi, wj);
-            if (r2 < hg2 && key != VL_NOT27) {
-                bad |= key < next;
-                next = key + 1u;
-                ++cnt;
-                rho += pj.w * poly6(r2, s);
-                lp.push(j);
-            }
-        };
-        if (halo) {
-            const uint4 *row = va.rows16 + i;
+    uint32_t next = 0;  // the next neighbour's key must be at least this
+    // one row entry: true if it is a neighbour (within h, among the 27 cells)
+    auto one = [&](const float4 &pj, uint32_t wj) {
+        const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+        const uint32_t key = vl_key(wi, wj);
+        const bool in = r2 < hg2 && key != VL_NOT27;
+        if (in) {
+            bad |= key < next;
+            next = key + 1u;
+            ++cnt;
+            rho += pj.w * poly6(r2, s);
+        }
+        return in;
+    };
+    if (halo) {
+        ListPush16 lp(va.lidx, va.lstride, i);
+        const uint4 *row = va.rows16 + i;
+        // entries past the count in a row's last group are 0 (a valid index)
+        uint4 e = n ? *row : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll 1
-            for (uint32_t k = 0; k < n; k += 8) {
-                const uint4 e = *row;
-                row += va.vstride;
-                const uint32_t ws[4] = {e.x, e.y, e.z, e.w};
+        for (uint32_t k = 0; k < n; k += 8) {
+            row += va.vstride;
+            const uint4 en = *(k + 8 < n ? row : va.rows16 + i);
+            const uint32_t ws[4] = {e.x, e.y, e.z, e.w};
+            uint32_t x[8], wj[8];
+            float4 pj[8];
 #pragma unroll
-                for (uint32_t q = 0; q < 8; ++q) {
-                    if (k + q < n) {
-                        const uint32_t x = (ws[q >> 1] >> (16u * (q & 1u))) & 0xFFFFu;
-                        one(h_s[x], h_p[x], h_w[x]);
-                    }
-                }
+            for (uint32_t q = 0; q < 8; ++q) x[q] = (ws[q >> 1] >> (16u * (q & 1u))) & 0xFFFFu;
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+                pj[q] = h_p[x[q]];
+                wj[q] = h_w[x[q]];
             }
-        } else {
-            const uint4 *row = rows + i;
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q)
+                if (k + q < n && one(pj[q], wj[q])) lp.push(x[q]);
+            e = en;
+        }
+        fc = lp.finish();
+    } else {
+        ListPush lp(va.lidx, va.lstride, i);
+        const uint4 *row = rows + i;
 #pragma unroll 1
-            for (uint32_t k = 0; k < n; k += 4) {
-                const uint4 e = *row;
-                row += va.vstride;
-                const bool v1 = k + 1 < n, v2 = k + 2 < n, v3 = k + 3 < n;
-                const uint32_t j0 = e.x, j1 = v1 ? e.y : e.x, j2 = v2 ? e.z : e.x, j3 = v3 ? e.w : e.x;
-                const float4 p0 = posm[j0], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
-                const uint32_t w0 = va.vword[j0], w1 = va.vword[j1], w2 = va.vword[j2], w3 = va.vword[j3];
-                one(j0, p0, w0);
-                if (v1) one(j1, p1, w1);
-                if (v2) one(j2, p2, w2);
-                if (v3) one(j3, p3, w3);
-            }
+        for (uint32_t k = 0; k < n; k += 4) {
+            const uint4 e = *row;
+            row += va.vstride;
+            const bool v1 = k + 1 < n, v2 = k + 2 < n, v3 = k + 3 < n;
+            const uint32_t j0 = e.x, j1 = v1 ? e.y : e.x, j2 = v2 ? e.z : e.x, j3 = v3 ? e.w : e.x;
+            const float4 p0 = posm[j0], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
+            const uint32_t w0 = va.vword[j0], w1 = va.vword[j1], w2 = va.vword[j2], w3 = va.vword[j3];
+            if (one(p0, w0)) lp.push(j0);
+            if (v1 && one(p1, w1)) lp.push(j1);
+            if (v2 && one(p2, w2)) lp.push(j2);
+            if (v3 && one(p3, w3)) lp.push(j3);
         }
         fc = lp.finish();
     }
@@ -519,40 +562,61 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
         const uint32_t ri = __shfl(i, b, 64), rn = __shfl(n, b, 64), rw = __shfl(wi, b, 64);
         const float px = __shfl(pi.x, b, 64), py = __shfl(pi.y, b, 64), pz = __shfl(pi.z, b, 64);
         const uint32_t *ru = reinterpret_cast<const uint32_t *>(halo ? va.rows16 : rows);
-        // entries lane and lane + 64: the entry (slot, or halo index) with
-        // bit 31 set for a neighbour within h, its contribution, its key
-#pragma unroll 1
-        for (uint32_t e = lane; e < rn; e += 64u) {
-            uint32_t x;
-            float4 pj;
-            uint32_t wj;
-            if (halo) {
-                x = (ru[((size_t)(e >> 3) * va.vstride + ri) * 4u + ((e & 7u) >> 1)] >> (16u * (e & 1u))) & 0xFFFFu;
-                pj = h_p[x];
-                wj = h_w[x];
-            } else {
-                x = ru[((size_t)(e >> 2) * va.vstride + ri) * 4u + (e & 3u)];
-                pj = posm[x];
-                wj = va.vword[x];
+        // entries lane and lane + 64 (rn <= VL_CAP < 128): the entry (slot,
+        // or halo index) with bit 31 set for a neighbour within h, its
+        // contribution, its key
+#pragma unroll
+        for (uint32_t h2 = 0; h2 < 2; ++h2) {
+            const uint32_t e = lane + 64u * h2;
+            if (e < rn) {
+                uint32_t x;
+                float4 pj;
+                uint32_t wj;
+                if (halo) {
+                    x = (ru[((size_t)(e >> 3) * va.vstride + ri) * 4u + ((e & 7u) >> 1)] >> (16u * (e & 1u))) &
+                        0xFFFFu;
+                    pj = h_p[x];
+                    wj = h_w[x];
+                } else {
+                    x = ru[((size_t)(e >> 2) * va.vstride + ri) * 4u + (e & 3u)];
+                    pj = posm[x];
+                    wj = va.vword[x];
+                }
+                const uint32_t key = vl_key(rw, wj);
+                const float r2 = dot3(pj.x - px, pj.y - py, pj.z - pz);
+                const bool in = r2 < hg2 && key != VL_NOT27;
+                s_key[w][e] = key;
+                s_uj[w][e] = x | (in ? 0x80000000u : 0u);
+                s_uc[w][e] = in ? pj.w * poly6(r2, s) : 0.0f;
             }
-            const uint32_t key = vl_key(rw, wj);
-            const float r2 = dot3(pj.x - px, pj.y - py, pj.z - pz);
-            const bool in = r2 < hg2 && key != VL_NOT27;
-            s_key[w][e] = key;
-            s_uj[w][e] = x | (in ? 0x80000000u : 0u);
-            s_uc[w][e] = in ? pj.w * poly6(r2, s) : 0.0f;
         }
         wave_sync();
-#pragma unroll 1
-        for (uint32_t e = lane; e < rn; e += 64u) {
-            const uint32_t ke = s_key[w][e];
-            uint32_t r = 0;
-            for (uint32_t m = 0; m < rn; ++m) {
-                const uint32_t km = s_key[w][m];
-                r += (km < ke || (km == ke && m < e)) ? 1u : 0u;
+        // each lane's two entries: rank, entry and contribution read before
+        // any lane writes them back in rank order
+        uint32_t rk[2] = {0u, 0u}, uj[2] = {0u, 0u};
+        float uc[2] = {0.0f, 0.0f};
+#pragma unroll
+        for (uint32_t h2 = 0; h2 < 2; ++h2) {
+            const uint32_t e = lane + 64u * h2;
+            if (e < rn) {
+                const uint32_t ke = s_key[w][e];
+                uint32_t r = 0;
+                for (uint32_t m = 0; m < rn; ++m) {
+                    const uint32_t km = s_key[w][m];
+                    r += (km < ke || (km == ke && m < e)) ? 1u : 0u;
+                }
+                rk[h2] = r;
+                uj[h2] = s_uj[w][e];
+                uc[h2] = s_uc[w][e];
             }
-            s_slot[w][r] = s_uj[w][e];
-            s_c[w][r] = s_uc[w][e];
+        }
+        wave_sync();
+#pragma unroll
+        for (uint32_t h2 = 0; h2 < 2; ++h2) {
+            if (lane + 64u * h2 < rn) {
+                s_uj[w][rk[h2]] = uj[h2];
+                s_uc[w][rk[h2]] = uc[h2];
+            }
         }
         wave_sync();
         // the row in visiting order, one 16-byte group per lane
@@ -561,32 +625,45 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
             if (g0 < rn) {
                 uint32_t v[4] = {0u, 0u, 0u, 0u};
                 for (uint32_t q = 0; q < 8 && g0 + q < rn; ++q)
-                    v[q >> 1] |= (s_slot[w][g0 + q] & 0xFFFFu) << (16u * (q & 1u));
+                    v[q >> 1] |= (s_uj[w][g0 + q] & 0xFFFFu) << (16u * (q & 1u));
                 va.rows16[(size_t)lane * va.vstride + ri] = make_uint4(v[0], v[1], v[2], v[3]);
             }
         } else {
             const uint32_t g0 = lane * 4u;
             if (g0 < rn) {
-                const uint32_t e0 = s_slot[w][g0] & 0x7FFFFFFFu;
-                const uint32_t e1 = g0 + 1 < rn ? s_slot[w][g0 + 1] & 0x7FFFFFFFu : 0u,
-                               e2 = g0 + 2 < rn ? s_slot[w][g0 + 2] & 0x7FFFFFFFu : 0u,
-                               e3 = g0 + 3 < rn ? s_slot[w][g0 + 3] & 0x7FFFFFFFu : 0u;
+                const uint32_t e0 = s_uj[w][g0] & 0x7FFFFFFFu;
+                const uint32_t e1 = g0 + 1 < rn ? s_uj[w][g0 + 1] & 0x7FFFFFFFu : 0u,
+                               e2 = g0 + 2 < rn ? s_uj[w][g0 + 2] & 0x7FFFFFFFu : 0u,
+                               e3 = g0 + 3 < rn ? s_uj[w][g0 + 3] & 0x7FFFFFFFu : 0u;
                 rows[(size_t)lane * va.vstride + ri] = make_uint4(e0, e1, e2, e3);
             }
         }
         if (lane == (uint32_t)b) {
             rho = 0.0f;
             cnt = 0;
-            ListPush lp(va.lidx, va.lstride, ri);
-            for (uint32_t m = 0; m < rn; ++m) {
-                const uint32_t e = s_slot[w][m];
-                if (e & 0x80000000u) {
-                    ++cnt;
-                    rho += s_c[w][m];
-                    lp.push(halo ? h_s[e & 0xFFFFu] : (e & 0x7FFFFFFFu));
+            if (halo) {
+                ListPush16 lp(va.lidx, va.lstride, ri);
+                for (uint32_t m = 0; m < rn; ++m) {
+                    const uint32_t e = s_uj[w][m];
+                    if (e & 0x80000000u) {
+                        ++cnt;
+                        rho += s_uc[w][m];
+                        lp.push(e & 0xFFFFu);
+                    }
                 }
+                fc = lp.finish();
+            } else {
+                ListPush lp(va.lidx, va.lstride, ri);
+                for (uint32_t m = 0; m < rn; ++m) {
+                    const uint32_t e = s_uj[w][m];
+                    if (e & 0x80000000u) {
+                        ++cnt;
+                        rho += s_uc[w][m];
+                        lp.push(e & 0x7FFFFFFFu);
+                    }
+                }
+                fc = lp.finish();
             }
-            fc = lp.finish();
         }
         wave_sync();
     }
