@@ -497,19 +497,29 @@ struct VerletDev {
     unsigned long long rebuilds, repaired, substeps;  // statistics
 };
 
+// Neighbour word of a particle: its original index (24 bits) and its cell
+// coordinates mod 4, x at bit 24, y at bit 27, z at bit 30, a zero bit
+// above the x and y fields.
 __device__ __forceinline__ uint32_t vl_word(uint32_t id, int32_t cx, int32_t cy, int32_t cz) {
-    return (id & PH_ID_MASK) | (((uint32_t)cx & 3u) << 24) | (((uint32_t)cy & 3u) << 26) |
-           (((uint32_t)cz & 3u) << 28);
+    return (id & PH_ID_MASK) | (((uint32_t)cx & 3u) << 24) | (((uint32_t)cy & 3u) << 27) |
+           (((uint32_t)cz & 3u) << 30);
 }
-// Visiting key of j among i's neighbours: (cell offset rank, index), or
+#define VL_KEY_MASK 0xDBFFFFFFu  // the index and the three fields (bits 26 and 29 clear)
+#define VL_FIELD_LO 0x49000000u  // the low bit of each field
+// Particle i's addend: (1 - c_i) mod 4 in each field.  Added to j's word it
+// leaves (c_j - c_i + 1) mod 4 in each field (a field's carry goes to the
+// zero bit above it, or out of the word): 0, 1, 2 for the cell offsets -1,
+// 0, +1, and 3 for an offset of 2 (two particles within h lie at most two
+// cells apart per axis).
+__device__ __forceinline__ uint32_t vl_addend(uint32_t wi) {
+    return (((1u - (wi >> 24)) & 3u) << 24) | (((1u - (wi >> 27)) & 3u) << 27) | (((1u - (wi >> 30)) & 3u) << 30);
+}
+// Visiting key of j among i's neighbours: the offsets (z, y, x) ranked 0..2,
+// then the index, so keys increase in the reference order (fluid3.rs:326-338);
 // VL_NOT27 when j's cell is not one of i's 27.  Valid for pairs within h.
-__device__ __forceinline__ uint32_t vl_key(uint32_t wi, uint32_t wj) {
-    const uint32_t dx = ((wj >> 24) - (wi >> 24)) & 3u, dy = ((wj >> 26) - (wi >> 26)) & 3u,
-                   dz = ((wj >> 28) - (wi >> 28)) & 3u;
-    if (dx == 2u || dy == 2u || dz == 2u) return VL_NOT27;
-    // offsets -1, 0, 1 are stored as 3, 0, 1; rank them 0, 1, 2
-    const uint32_t rx = (dx + 1u) & 3u, ry = (dy + 1u) & 3u, rz = (dz + 1u) & 3u;
-    return (((rz * 3u + ry) * 3u + rx) << 24) | (wj & PH_ID_MASK);
+__device__ __forceinline__ uint32_t vl_key(uint32_t ai, uint32_t wj) {
+    const uint32_t t = (wj + ai) & VL_KEY_MASK;
+    return (t & (t >> 1) & VL_FIELD_LO) ? VL_NOT27 : t;
 }
 __device__ __forceinline__ bool vl_far(float x, float y, float z, float h) {
     const float f = VL_FAR * h;

@@ -489,6 +489,7 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
     // never VLF_SLOW here (such a state sorts and rebuilds); bounded anyway
     const uint32_t n = mine ? min(va.vcnt[i], VL_CAP) : 0u;
     const uint32_t wi = mine ? va.vword[i] : 0u;
+    const uint32_t ai = vl_addend(wi);
     bool bad = false;
     uint32_t cnt = 0, fc = 0;
     float rho = 0.0f;
@@ -496,7 +497,7 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
     // one row entry: true if it is a neighbour (within h, among the 27 cells)
     auto one = [&](const float4 &pj, uint32_t wj) {
         const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-        const uint32_t key = vl_key(wi, wj);
+        const uint32_t key = vl_key(ai, wj);
         const bool in = r2 < hg2 && key != VL_NOT27;
         if (in) {
             bad |= key < next;
@@ -582,7 +583,7 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
                     pj = posm[x];
                     wj = va.vword[x];
                 }
-                const uint32_t key = vl_key(rw, wj);
+                const uint32_t key = vl_key(vl_addend(rw), wj);
                 const float r2 = dot3(pj.x - px, pj.y - py, pj.z - pz);
                 const bool in = r2 < hg2 && key != VL_NOT27;
                 s_key[w][e] = key;
