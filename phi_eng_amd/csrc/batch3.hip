@@ -42,6 +42,7 @@ struct Batch3 {
     ph::DBuf<uint32_t> bar;
     hipGraphExec_t exec[2] = {nullptr, nullptr};  // one per tag generation (ct = 0 / 1)
     uint32_t exec_nbuckets[2] = {0, 0};
+    int exec_cs[2] = {0, 0};  // the state generation a graph was captured on (Verlet substeps alternate it)
     bool abort_seen = false;
     float dt_uploaded = -1.0f;
 
@@ -132,7 +133,7 @@ struct Batch3 {
                 e.counts_dirty = false;
             }
             const int k = e.ct;
-            if (!exec[k] || exec_nbuckets[k] != e.nbuckets) {
+            if (!exec[k] || exec_nbuckets[k] != e.nbuckets || exec_cs[k] != e.cs) {
                 if (exec[k]) {
                     (void)hipGraphExecDestroy(exec[k]);
                     exec[k] = nullptr;
@@ -153,6 +154,7 @@ struct Batch3 {
                 (void)hipGraphDestroy(g);
                 if (ie != hipSuccess) throw ph::HipError("hipGraphInstantiate", ie);
                 exec_nbuckets[k] = e.nbuckets;
+                exec_cs[k] = e.cs;
             }
             PH_CK(hipGraphLaunch(exec[k], e.stream));
             e.counts_dirty = false;
@@ -168,7 +170,7 @@ struct Batch3 {
             launch_persistent();
         } else {
             const int k = e.ct;
-            if (!exec[k] || exec_nbuckets[k] != e.nbuckets) {
+            if (!exec[k] || exec_nbuckets[k] != e.nbuckets || exec_cs[k] != e.cs) {
                 if (exec[k]) {
                     (void)hipGraphExecDestroy(exec[k]);
                     exec[k] = nullptr;
@@ -184,6 +186,7 @@ struct Batch3 {
                 (void)hipGraphDestroy(g);
                 if (ie != hipSuccess) throw ph::HipError("hipGraphInstantiate", ie);
                 exec_nbuckets[k] = e.nbuckets;
+                exec_cs[k] = e.cs;
             }
             PH_CK(hipGraphLaunch(exec[k], e.stream));
         }

@@ -562,18 +562,26 @@ void Engine::substep_verlet() {
     va.rows16 = vl_row16.p;
     va.lidx = nl_idx.p;
     va.lstride = list_stride();
+    // The walk and the force pass read generation cs: between rebuilds it
+    // holds the state in the kept order (the gather leaves it), and a
+    // rebuild's density pass copies the sorted state (generation ns) there.
+    // The force pass writes generation ns, which becomes the state.
+    va.keep_posm = posm[cs].p;
+    va.sorted_vel = vel[ns].p;
+    va.keep_vel = vel[cs].p;
     run(KIND_DENSITY, [&] {
-        return launch_density_verlet(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, tag[nt_].p, rq.p,
-                                     rd ? rec_rho.p : nullptr, rd ? rec_p.p : nullptr, rd ? rec_count.p : nullptr,
-                                     nl_cnt.p, va);
+        return launch_density_verlet(stream, nt, d_tiles.p, d_scenes.p, start.p, posm[ns].p, posm[cs].p,
+                                     tag[nt_].p, rq.p, rd ? rec_rho.p : nullptr, rd ? rec_p.p : nullptr,
+                                     rd ? rec_count.p : nullptr, nl_cnt.p, va);
     });
     gen_open = true;
     run(KIND_FORCE, [&] {
-        return launch_force_verlet(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[ns].p, vel[ns].p,
-                                   rq.p, tag[nt_].p, nl_cnt.p, nl_idx.p, list_stride(), posm[cs].p, vel[cs].p,
+        return launch_force_verlet(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[cs].p, vel[cs].p,
+                                   rq.p, tag[nt_].p, nl_cnt.p, nl_idx.p, list_stride(), posm[ns].p, vel[ns].p,
                                    ra ? rec_acc.p : nullptr, d_bbox.p, GhostDev{}, vl_tile_dmax.p, vl_state.p,
                                    vl_word[1].p, vl_halo.p, vl_halo_n.p);
     });
+    cs = ns;  // the force pass was queued: its output is the state
     finish_substep();
     if (vl_k % 4 == 0) PH_CK(hipEventRecord(vl_ev[(vl_k / 4) % 8], stream));
     ++vl_k;

@@ -89,15 +89,22 @@ struct VerletArgs {
     // of slot i at uint4 g * lstride + i; the count word in fcnt)
     uint32_t *lidx;
     uint32_t lstride;
+    // a rebuild's density pass copies the sorted state (the pass's posm, and
+    // sorted_vel) into the generation the walk and the force pass read
+    float4 *keep_posm;
+    const float4 *sorted_vel;
+    float4 *keep_vel;
 };
 // host_out (pinned host memory): [0] rows re-ordered in the last walk,
 // [1] this substep's rebuild flag.
 hipError_t launch_vl_decide(hipStream_t st, VerletDev *vs, const float *tile_dmax, uint32_t *tile_q, uint32_t ntiles,
                             uint32_t force, float thr, uint32_t *host_out);
+// posm_sorted: the sort's output (the rebuild reads it); posm: the state
+// the walk reads (va.keep_posm, where the rebuild copies the sorted state).
 hipError_t launch_density_verlet(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
-                                 const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
-                                 float *rec_rho, float *rec_p, uint32_t *rec_count, uint32_t *fcnt,
-                                 const VerletArgs &va);
+                                 const uint32_t *start, const float4 *posm_sorted, const float4 *posm,
+                                 const uint32_t *tag, float2 *rq, float *rec_rho, float *rec_p,
+                                 uint32_t *rec_count, uint32_t *fcnt, const VerletArgs &va);
 // Skin list written by the density pass for surrogate substeps.
 struct ListOut {
     int4 *cell = nullptr;

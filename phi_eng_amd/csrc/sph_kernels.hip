@@ -241,12 +241,10 @@ __device__ __forceinline__ void dev_rank_gather(uint32_t bid, uint32_t nbl, cons
     const uint32_t q = t.first + threadIdx.x;
     if (q >= s.part_end) return;
     if (vs && !vs->rebuild) {
-        // A Verlet substep that keeps the order: the same slots.  Positions
-        // and velocities are copied; both tag generations already hold the
-        // sorted tags (the rebuild's density pass mirrors them) and the
-        // force pass has written the next words.
-        posm_out[q] = posm_in[q];
-        vel_out[q] = vel_in[q];
+        // A Verlet substep that keeps the order: the same slots, and the
+        // passes read the state where it is (both tag generations already
+        // hold the sorted tags, the rebuild's density pass mirrors them, and
+        // the force pass has written the next words).
         if (perm_out) perm_out[q] = q;
         return;
     }

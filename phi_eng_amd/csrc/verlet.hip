@@ -330,9 +330,12 @@ __global__ __launch_bounds__(PH_TILE) void k_density_rebuild(const TileDev *__re
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     if (i >= s.part_end) return;
-    // both tag generations hold the sorted tags until the next rebuild
+    // both tag generations hold the sorted tags until the next rebuild, and
+    // the walk's generation the sorted state
     va.tag_mirror[i] = tag[i];
     const float4 pi = posm[i];
+    va.keep_posm[i] = pi;
+    va.keep_vel[i] = va.sorted_vel[i];
     if (!finite3(pi.x, pi.y, pi.z)) {
         // fluid3.rs:302-304: no neighbours, density 0; an empty row
         va.vcnt[i] = 0u;
@@ -748,11 +751,11 @@ hipError_t launch_vl_decide(hipStream_t st, VerletDev *vs, const float *tile_dma
 }
 
 hipError_t launch_density_verlet(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
-                                 const uint32_t *start, const float4 *posm, const uint32_t *tag, float2 *rq,
-                                 float *rec_rho, float *rec_p, uint32_t *rec_count, uint32_t *fcnt,
-                                 const VerletArgs &va) {
-    hipLaunchKernelGGL(k_density_rebuild, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, rq,
-                       rec_rho, rec_p, rec_count, fcnt, va);
+                                 const uint32_t *start, const float4 *posm_sorted, const float4 *posm,
+                                 const uint32_t *tag, float2 *rq, float *rec_rho, float *rec_p,
+                                 uint32_t *rec_count, uint32_t *fcnt, const VerletArgs &va) {
+    hipLaunchKernelGGL(k_density_rebuild, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm_sorted, tag,
+                       rq, rec_rho, rec_p, rec_count, fcnt, va);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_halo_build, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, va);
