@@ -69,6 +69,9 @@ PH_HD float rmaxf(float a, float b) {
     if (b != b) return a;
     return a > b ? a : b;
 }
+// rmaxf(a, b) for b > 0 (not NaN): IEEE maxNum, which takes b for a NaN a
+// and max(a, b) otherwise, is the same function.
+PH_HD float rmax_pos(float a, float b) { return fmaxf(a, b); }
 // Rust f32::clamp: NaN passes through.
 PH_HD float rclampf(float v, float lo, float hi) {
     if (v < lo) v = lo;
@@ -127,6 +130,13 @@ __device__ __forceinline__ float div_by_recip(float a, float b, float y) {
 // fluid3.rs:220-228 (coefficient precomputed on the host in f32).
 __device__ __forceinline__ float poly6(float r2, const SceneDev &s) {
     if (r2 >= s.h2 || r2 < 0.0f) return 0.0f;
+    float t = s.h2 - r2;
+    return s.poly6_c * t * t * t;
+}
+
+// poly6 for 0 <= r2 < h_grid^2 <= h^2 (a neighbour of the cut-off test),
+// where the kernel's range test cannot fail.
+__device__ __forceinline__ float poly6_within(float r2, const SceneDev &s) {
     float t = s.h2 - r2;
     return s.poly6_c * t * t * t;
 }

@@ -450,13 +450,14 @@ struct PairAcc {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;  // XSPH smoothing
 };
 
+// ex, ey, ez: x_i - x_j; r2 = |x_i - x_j|^2 (the same bits as the cut-off
+// test's |x_j - x_i|^2: the differences are exact negations).  The caller
+// has tested r2 < h_grid^2 <= h^2.
 __device__ __forceinline__ void pair_terms(const SceneDev &s, const float4 &pi, const float4 &vi, float rho_i,
                                            float q_i, const float4 &pj, const float4 &vj, const float2 &rqj,
-                                           float r2, PairAcc &a) {
+                                           float ex, float ey, float ez, float r2, PairAcc &a) {
     const float h = s.h;
-    const float rho_j = rmaxf(rqj.x, PH_EPS);
-    // delta = x_i - x_j; |delta|^2 equals the cut-off test's r2 exactly.
-    const float ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+    const float rho_j = rmax_pos(rqj.x, PH_EPS);
     const float dist = sqrt_pair(r2);
     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
     if (!(dist >= h || dist < PH_EPS)) {  // spiky_grad, fluid3.rs:232-239
@@ -501,7 +502,7 @@ __device__ __forceinline__ void pair_terms(const SceneDev &s, const float4 &pi, 
     if (s.viscosity > 0.0f) {
         const float rho_bar = 0.5f * (rho_i + rho_j);
         const float qm = pj.w / rho_bar;
-        const float w = poly6(r2, s);
+        const float w = poly6_within(r2, s);
         a.sx = a.sx + (vj.x - vi.x) * qm * w;
         a.sy = a.sy + (vj.y - vi.y) * qm * w;
         a.sz = a.sz + (vj.z - vi.z) * qm * w;
@@ -560,8 +561,9 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             const uint32_t cw = lcnt[i];
             // one list entry: the self entry and those outside h add nothing
             auto entry = [&](uint32_t j, const float4 &pj, const float4 &vj, const float2 &qj) {
-                const float r2 = dot3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                if (j != i && r2 < hg2) pair_terms(s, pi, vi, rho_i, q_i, pj, vj, qj, r2, acc);
+                const float ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+                const float r2 = dot3(ex, ey, ez);
+                if (j != i && r2 < hg2) pair_terms(s, pi, vi, rho_i, q_i, pj, vj, qj, ex, ey, ez, r2, acc);
             };
             if (cw & SLF_NONE) {
                 // non-finite position: no neighbours
