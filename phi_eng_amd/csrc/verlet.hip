@@ -346,34 +346,35 @@ __global__ __launch_bounds__(PH_TILE) void k_density_rebuild(const TileDev *__re
     density_rebuild(s, scenes, t.scene, start, posm, tag, rq, rec_rho, rec_p, rec_count, fcnt, va, i, pi);
 }
 
-// The walk substep's density pass (returns at once on a rebuild substep).
-// After a rebuild, one block per tile: the sorted union of the tile's rows
-// (its halo) from a bitmap over the rows' slot span, and every row entry
-// rewritten as its rank in the halo (16 bits).  A tile whose span or halo
-// is too large, or with a row that takes the per-cell path, keeps the
-// global rows (VL_HALO_NONE).
-__global__ __launch_bounds__(PH_TILE) void k_halo_build(const TileDev *__restrict__ tiles,
-                                                        const SceneDev *__restrict__ scenes, ph::VerletArgs va) {
+// One block per tile: the sorted union of the tile's lists (its halo, at
+// most cap slots) from a bitmap over the lists' slot span, and every entry
+// rewritten as its rank in the halo (16 bits, eight to a 16-byte group:
+// group g of slot i at out16[g * stride + i]).  A list is a count word per
+// slot (bit 24: the slot takes the per-cell path; bit 25: no entries) and
+// groups of four slots (group g of slot i at idx[g * stride + i]).  A tile
+// whose span or halo is too large, or with a slot on the per-cell path,
+// keeps the 32-bit lists (halo_n = VL_HALO_NONE).
+__device__ __forceinline__ void halo_build_tile(uint32_t tile, const TileDev &t, const SceneDev &s,
+                                                const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ idx,
+                                                uint32_t stride, uint32_t cap, uint32_t *__restrict__ halo,
+                                                uint32_t *__restrict__ halo_n, uint4 *__restrict__ out16) {
     constexpr uint32_t kWords = VL_HALO_SPAN / 32u, kPer = kWords / PH_TILE;
     __shared__ uint32_t bm[kWords];  // bit (slot - lo)
     __shared__ uint32_t pw[kWords];  // halo rank of the word's first set bit
     __shared__ uint32_t red[3][PH_TILE / 64];
-    if (va.vs->rebuild == 0u) return;
-    const uint32_t tile = blockIdx.x;
-    const TileDev t = tiles[tile];
-    const SceneDev &s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    const uint32_t *ru = va.vidx;
-    auto entry = [&](uint32_t m) { return ru[((size_t)(m >> 2) * va.vstride + i) * 4u + (m & 3u)]; };
-    const uint32_t raw = i < s.part_end ? va.vcnt[i] : 0u;
-    const bool slow = (raw & VLF_SLOW) != 0u;
-    const uint32_t n = slow ? 0u : min(raw, VL_CAP);
+    const uint4 *grp = reinterpret_cast<const uint4 *>(idx) + i;
+    const uint32_t raw = i < s.part_end ? cnt[i] : 0u;
+    const bool slow = (raw & 0x01000000u) != 0u;
+    const uint32_t n = (slow || (raw & 0x02000000u)) ? 0u : min(raw & 0x00FFFFFFu, VL_CAP);
+    auto group = [&](uint32_t g) { return grp[(size_t)g * stride]; };
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-    for (uint32_t m = 0; m < n; ++m) {
-        const uint32_t j = entry(m);
-        lo = min(lo, j);
-        hi = max(hi, j);
+    for (uint32_t m = 0; m < n; m += 4) {
+        const uint4 e = group(m >> 2);
+        const uint32_t e1 = m + 1 < n ? e.y : e.x, e2 = m + 2 < n ? e.z : e.x, e3 = m + 3 < n ? e.w : e.x;
+        lo = min(lo, min(min(e.x, e1), min(e2, e3)));
+        hi = max(hi, max(max(e.x, e1), max(e2, e3)));
     }
     uint32_t bad = slow ? 1u : 0u;
     for (int d = 32; d >= 1; d >>= 1) {
@@ -393,19 +394,23 @@ __global__ __launch_bounds__(PH_TILE) void k_halo_build(const TileDev *__restric
         bad |= red[2][k];
     }
     if (bad || (hi >= lo && hi - lo >= VL_HALO_SPAN)) {
-        if (threadIdx.x == 0) va.halo_n[tile] = VL_HALO_NONE;
+        if (threadIdx.x == 0) halo_n[tile] = VL_HALO_NONE;
         return;
     }
     if (hi < lo) {  // no entries at all
-        if (threadIdx.x == 0) va.halo_n[tile] = 0u;
+        if (threadIdx.x == 0) halo_n[tile] = 0u;
         return;
     }
     const uint32_t nw = ((hi - lo) >> 5) + 1u;
     for (uint32_t k = threadIdx.x; k < nw; k += PH_TILE) bm[k] = 0u;
     __syncthreads();
-    for (uint32_t m = 0; m < n; ++m) {
-        const uint32_t d = entry(m) - lo;
-        atomicOr(&bm[d >> 5], 1u << (d & 31u));
+    for (uint32_t m = 0; m < n; m += 4) {
+        const uint4 e = group(m >> 2);
+        const uint32_t es[4] = {e.x, e.y, e.z, e.w};
+        for (uint32_t q = 0; q < 4 && m + q < n; ++q) {
+            const uint32_t d = es[q] - lo;
+            atomicOr(&bm[d >> 5], 1u << (d & 31u));
+        }
     }
     __syncthreads();
     // ranks: thread k owns words [k kPer, (k + 1) kPer)
@@ -416,12 +421,12 @@ __global__ __launch_bounds__(PH_TILE) void k_halo_build(const TileDev *__restric
     }
     uint32_t total;
     const uint32_t base = block_scan_u32(c, total);
-    if (total > VL_HALO_CAP) {
-        if (threadIdx.x == 0) va.halo_n[tile] = VL_HALO_NONE;
+    if (total > cap) {
+        if (threadIdx.x == 0) halo_n[tile] = VL_HALO_NONE;
         return;
     }
     uint32_t r = base;
-    uint32_t *hl = va.halo + (size_t)tile * VL_HALO_CAP;
+    uint32_t *hl = halo + (size_t)tile * VL_HALO_CAP;
     for (uint32_t q = 0; q < kPer; ++q) {
         const uint32_t k = threadIdx.x * kPer + q;
         if (k >= nw) break;
@@ -429,17 +434,33 @@ __global__ __launch_bounds__(PH_TILE) void k_halo_build(const TileDev *__restric
         for (uint32_t bits = bm[k]; bits; bits &= bits - 1u) hl[r++] = lo + k * 32u + (uint32_t)(__ffs(bits) - 1);
     }
     __syncthreads();
-    uint4 *r16 = va.rows16 + i;
+    uint4 *r16 = out16 + i;
     for (uint32_t m = 0; m < n; m += 8) {
+        const uint4 e0 = group(m >> 2), e1 = m + 4 < n ? group((m >> 2) + 1) : e0;
+        const uint32_t es[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
         uint32_t v[4] = {0u, 0u, 0u, 0u};
-        for (uint32_t q = 0; q < 8 && m + q < n; ++q) {
-            const uint32_t d = entry(m + q) - lo;
-            const uint32_t rank = pw[d >> 5] + __popc(bm[d >> 5] & ((1u << (d & 31u)) - 1u));
-            v[q >> 1] |= rank << (16u * (q & 1u));
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q) {
+            if (m + q < n) {
+                const uint32_t d = es[q] - lo;
+                const uint32_t rank = pw[d >> 5] + __popc(bm[d >> 5] & ((1u << (d & 31u)) - 1u));
+                v[q >> 1] |= rank << (16u * (q & 1u));
+            }
         }
-        r16[(size_t)(m >> 3) * va.vstride] = make_uint4(v[0], v[1], v[2], v[3]);
+        r16[(size_t)(m >> 3) * stride] = make_uint4(v[0], v[1], v[2], v[3]);
     }
-    if (threadIdx.x == 0) va.halo_n[tile] = total;
+    if (threadIdx.x == 0) halo_n[tile] = total;
+}
+
+// The walk substep's density pass reads its rows from a tile's halo: built
+// after a rebuild (returns at once otherwise).
+__global__ __launch_bounds__(PH_TILE) void k_halo_build(const TileDev *__restrict__ tiles,
+                                                        const SceneDev *__restrict__ scenes, ph::VerletArgs va) {
+    if (va.vs->rebuild == 0u) return;
+    const uint32_t tile = blockIdx.x;
+    const TileDev t = tiles[tile];
+    halo_build_tile(tile, t, scenes[t.scene], va.vcnt, va.vidx, va.vstride, VL_HALO_CAP, va.halo, va.halo_n,
+                    va.rows16);
 }
 
 // The walk substep's density pass (returns at once on a rebuild substep).
