@@ -538,8 +538,7 @@ void Engine::substep_verlet() {
                                rc ? rec_cells.p : nullptr, vl_state.p, vl_word[0].p);
     });
     run(KIND_SCAN_REDUCE, [&] { return launch_scan_reduce(stream, counts.p, nbuckets, partial.p, gate); });
-    run(KIND_SCAN_PARTIALS, [&] { return launch_scan_partials(stream, partial.p, nbuckets, start.p, gate); });
-    run(KIND_SCAN_APPLY, [&] { return launch_scan_apply(stream, counts.p, nbuckets, partial.p, start.p, gate); });
+    run(KIND_SCAN_APPLY, [&] { return launch_scan_apply_sum(stream, counts.p, nbuckets, partial.p, start.p, gate); });
     counts_dirty = false;
     run(KIND_SCATTER, [&] { return launch_scatter(stream, n, keys.p, slots.p, start.p, tmp.p, gate); });
     run(KIND_RANK_GATHER, [&] {
@@ -608,10 +607,7 @@ void Engine::sort_phase(uint32_t *perm_out) {
                                rc ? rec_cells.p : nullptr);
     });
     run(KIND_SCAN_REDUCE, [&] { return launch_scan_reduce(stream, counts.p, nbuckets, partial.p); });
-    run(KIND_SCAN_PARTIALS, [&] {
-        return launch_scan_partials(stream, partial.p, nbuckets, start.p);
-    });
-    run(KIND_SCAN_APPLY, [&] { return launch_scan_apply(stream, counts.p, nbuckets, partial.p, start.p); });
+    run(KIND_SCAN_APPLY, [&] { return launch_scan_apply_sum(stream, counts.p, nbuckets, partial.p, start.p); });
     counts_dirty = false;
     run(KIND_SCATTER, [&] { return launch_scatter(stream, n, keys.p, slots.p, start.p, tmp.p); });
     run(KIND_RANK_GATHER, [&] {

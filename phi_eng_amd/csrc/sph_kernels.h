@@ -42,6 +42,10 @@ hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, 
                                 const uint32_t *gate = nullptr);
 hipError_t launch_scan_apply(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
                              uint32_t *start, const uint32_t *gate = nullptr);
+// scan_apply with the block offsets summed from the raw partials (no
+// scan_partials launch), start[nb] = total.
+hipError_t launch_scan_apply_sum(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
+                                 uint32_t *start, const uint32_t *gate = nullptr);
 hipError_t launch_scatter(hipStream_t st, uint32_t n, const uint32_t *keys, const uint32_t *slots,
                           const uint32_t *start, uint32_t *tmp, const uint32_t *gate = nullptr);
 hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,

@@ -205,6 +205,39 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t *__restrict_
     dev_scan_apply(blockIdx.x, gridDim.x, counts, nb, partial, start);
 }
 
+// The same with the block's offset summed here from the raw block
+// partials (a few hundred words: no separate scan of them), and the last
+// block writing start[nb] = total.
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply_sum(uint32_t *__restrict__ counts, uint32_t nb,
+                                                               const uint32_t *__restrict__ partial,
+                                                               uint32_t *__restrict__ start, const uint32_t *gate) {
+    if (gate && !*gate) return;
+    const uint32_t bid = blockIdx.x;
+    uint32_t s = 0;
+    for (uint32_t k = threadIdx.x; k < bid; k += kScanBlock) s += partial[k];
+    uint32_t off0;
+    block_excl_scan(s, off0);
+    const uint32_t base = bid * (kScanBlock * kScanItems) + threadIdx.x * kScanItems;
+    uint32_t c[kScanItems];
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        c[k] = base + k < nb ? counts[base + k] : 0u;
+        v += c[k];
+    }
+    uint32_t tot;
+    uint32_t off = block_excl_scan(v, tot) + off0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < nb) {
+            start[base + k] = off;
+            counts[base + k] = 0u;
+        }
+        off += c[k];
+    }
+    if (bid + 1 == gridDim.x && threadIdx.x == 0) start[nb] = off0 + tot;
+}
+
 // ---------------------------------------------------------------- K3
 __device__ __forceinline__ void dev_scatter(uint32_t bid, uint32_t nbl, uint32_t n, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ slots,
@@ -1113,6 +1146,12 @@ hipError_t launch_scan_partials(hipStream_t st, uint32_t *partial, uint32_t nb, 
 hipError_t launch_scan_apply(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
                              uint32_t *start, const uint32_t *gate) {
     hipLaunchKernelGGL(k_scan_apply, dim3(scan_partials_needed(nb)), dim3(kScanBlock), 0, st, counts, nb,
+                       partial, start, gate);
+    return hipGetLastError();
+}
+hipError_t launch_scan_apply_sum(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
+                                 uint32_t *start, const uint32_t *gate) {
+    hipLaunchKernelGGL(k_scan_apply_sum, dim3(scan_partials_needed(nb)), dim3(kScanBlock), 0, st, counts, nb,
                        partial, start, gate);
     return hipGetLastError();
 }

@@ -314,38 +314,6 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The rebuild substep's density pass (returns at once on a walk substep).
-__global__ __launch_bounds__(PH_TILE) void k_density_rebuild(const TileDev *__restrict__ tiles,
-                                                             const SceneDev *__restrict__ scenes,
-                                                             const uint32_t *__restrict__ start,
-                                                             const float4 *__restrict__ posm,
-                                                             const uint32_t *__restrict__ tag,
-                                                             float2 *__restrict__ rq, float *__restrict__ rec_rho,
-                                                             float *__restrict__ rec_p,
-                                                             uint32_t *__restrict__ rec_count,
-                                                             uint32_t *__restrict__ fcnt, ph::VerletArgs va) {
-    if (va.vs->rebuild == 0u) return;
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const TileDev t = tiles[tile];
-    const SceneDev s = scenes[t.scene];
-    const uint32_t i = t.first + threadIdx.x;
-    if (i >= s.part_end) return;
-    // both tag generations hold the sorted tags until the next rebuild, and
-    // the walk's generation the sorted state
-    va.tag_mirror[i] = tag[i];
-    const float4 pi = posm[i];
-    va.keep_posm[i] = pi;
-    va.keep_vel[i] = va.sorted_vel[i];
-    if (!finite3(pi.x, pi.y, pi.z)) {
-        // fluid3.rs:302-304: no neighbours, density 0; an empty row
-        va.vcnt[i] = 0u;
-        fcnt[i] = SLF_NONE;
-        density_out(s, scenes, t.scene, i, 0.0f, 0u, tag, rq, rec_rho, rec_p, rec_count);
-        return;
-    }
-    density_rebuild(s, scenes, t.scene, start, posm, tag, rq, rec_rho, rec_p, rec_count, fcnt, va, i, pi);
-}
-
 // One block per tile: the sorted union of the tile's lists (its halo, at
 // most cap slots) from a bitmap over the lists' slot span, and every entry
 // rewritten as its rank in the halo (16 bits, eight to a 16-byte group:
@@ -452,15 +420,42 @@ __device__ __forceinline__ void halo_build_tile(uint32_t tile, const TileDev &t,
     if (threadIdx.x == 0) halo_n[tile] = total;
 }
 
-// The walk substep's density pass reads its rows from a tile's halo: built
-// after a rebuild (returns at once otherwise).
-__global__ __launch_bounds__(PH_TILE) void k_halo_build(const TileDev *__restrict__ tiles,
-                                                        const SceneDev *__restrict__ scenes, ph::VerletArgs va) {
+// The rebuild substep's density pass (returns at once on a walk substep):
+// each particle sweeps the cells for its row and density, then the block
+// builds its tile's halo from those rows (they are the block's own), which
+// the walk substeps until the next rebuild read.
+__global__ __launch_bounds__(PH_TILE) void k_density_rebuild(const TileDev *__restrict__ tiles,
+                                                             const SceneDev *__restrict__ scenes,
+                                                             const uint32_t *__restrict__ start,
+                                                             const float4 *__restrict__ posm,
+                                                             const uint32_t *__restrict__ tag,
+                                                             float2 *__restrict__ rq, float *__restrict__ rec_rho,
+                                                             float *__restrict__ rec_p,
+                                                             uint32_t *__restrict__ rec_count,
+                                                             uint32_t *__restrict__ fcnt, ph::VerletArgs va) {
     if (va.vs->rebuild == 0u) return;
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const TileDev t = tiles[tile];
-    halo_build_tile(tile, t, scenes[t.scene], va.vcnt, va.vidx, va.vstride, VL_HALO_CAP, va.halo, va.halo_n,
-                    va.rows16);
+    const SceneDev s = scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    if (i < s.part_end) {
+        // both tag generations hold the sorted tags until the next rebuild,
+        // and the walk's generation the sorted state
+        va.tag_mirror[i] = tag[i];
+        const float4 pi = posm[i];
+        va.keep_posm[i] = pi;
+        va.keep_vel[i] = va.sorted_vel[i];
+        if (!finite3(pi.x, pi.y, pi.z)) {
+            // fluid3.rs:302-304: no neighbours, density 0; an empty row
+            va.vcnt[i] = 0u;
+            fcnt[i] = SLF_NONE;
+            density_out(s, scenes, t.scene, i, 0.0f, 0u, tag, rq, rec_rho, rec_p, rec_count);
+        } else {
+            density_rebuild(s, scenes, t.scene, start, posm, tag, rq, rec_rho, rec_p, rec_count, fcnt, va, i, pi);
+        }
+    }
+    __syncthreads();  // the tile's rows and counts are written
+    halo_build_tile(tile, t, s, va.vcnt, va.vidx, va.vstride, VL_HALO_CAP, va.halo, va.halo_n, va.rows16);
 }
 
 // The walk substep's density pass (returns at once on a rebuild substep).
@@ -778,9 +773,6 @@ hipError_t launch_density_verlet(hipStream_t st, uint32_t ntiles, const TileDev 
     hipLaunchKernelGGL(k_density_rebuild, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm_sorted, tag,
                        rq, rec_rho, rec_p, rec_count, fcnt, va);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_halo_build, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, va);
-    e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_density_walk, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, posm, tag, rq, rec_rho,
                        rec_p, rec_count, fcnt, va);
