@@ -88,6 +88,7 @@ struct ListPush {
 // indices (16 bits), eight to a 16-byte group.  Each push shifts the group
 // down by one entry and puts the new one on top, so after eight pushes
 // entry m of the group sits at bits 16 m.
+static_assert(VL_CAP <= SL_CAP, "a walked row's neighbours fit the force pass's list");
 struct ListPush16 {
     uint4 *g;
     uint32_t stride;
@@ -101,9 +102,10 @@ struct ListPush16 {
         w2 = __builtin_amdgcn_alignbit(w3, w2, 16u);
         w3 = __builtin_amdgcn_alignbit(x, w3, 16u);
     }
+    // (a walked row has at most VL_CAP = SL_CAP entries, so every push fits)
     __device__ __forceinline__ void push(uint32_t x) {
         shift_in(x);
-        if ((k & 7u) == 7u && k < SL_CAP) g[(size_t)(k >> 3) * stride] = make_uint4(w0, w1, w2, w3);
+        if ((k & 7u) == 7u) g[(size_t)(k >> 3) * stride] = make_uint4(w0, w1, w2, w3);
         ++k;
     }
     // the last, partly filled group moved down to bit 0; the count word
@@ -522,7 +524,7 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
             bad |= key < next;
             next = key + 1u;
             ++cnt;
-            rho += pj.w * poly6(r2, s);
+            rho += pj.w * poly6_within(r2, s);
         }
         return in;
     };
