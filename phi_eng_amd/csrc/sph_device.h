@@ -496,7 +496,7 @@ __device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__r
 // VL_HALO_CAP particles over a slot span of at most VL_HALO_SPAN; a tile
 // beyond either walks from global memory (VL_HALO_NONE).
 #define VL_HALO_CAP 1792u
-#define VL_HALO_SPAN 65536u
+#define VL_HALO_SPAN 262144u  // 2^18 slots: cell layers of up to ~2^17 particles
 #define VL_HALO_NONE 0xFFFFFFFFu
 
 struct VerletDev {

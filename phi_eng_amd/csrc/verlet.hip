@@ -330,7 +330,7 @@ __device__ __forceinline__ void halo_build_tile(uint32_t tile, const TileDev &t,
                                                 uint32_t *__restrict__ halo_n, uint4 *__restrict__ out16) {
     constexpr uint32_t kWords = VL_HALO_SPAN / 32u, kPer = kWords / PH_TILE;
     __shared__ uint32_t bm[kWords];  // bit (slot - lo)
-    __shared__ uint32_t pw[kWords];  // halo rank of the word's first set bit
+    __shared__ uint16_t pw[kWords];  // halo rank of the word's first set bit (< VL_HALO_CAP)
     __shared__ uint32_t red[3][PH_TILE / 64];
     const uint32_t i = t.first + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
@@ -400,7 +400,7 @@ __device__ __forceinline__ void halo_build_tile(uint32_t tile, const TileDev &t,
     for (uint32_t q = 0; q < kPer; ++q) {
         const uint32_t k = threadIdx.x * kPer + q;
         if (k >= nw) break;
-        pw[k] = r;
+        pw[k] = (uint16_t)r;
         for (uint32_t bits = bm[k]; bits; bits &= bits - 1u) hl[r++] = lo + k * 32u + (uint32_t)(__ffs(bits) - 1);
     }
     __syncthreads();
@@ -423,9 +423,7 @@ __device__ __forceinline__ void halo_build_tile(uint32_t tile, const TileDev &t,
 }
 
 // The rebuild substep's density pass (returns at once on a walk substep):
-// each particle sweeps the cells for its row and density, then the block
-// builds its tile's halo from those rows (they are the block's own), which
-// the walk substeps until the next rebuild read.
+// each particle sweeps the cells for its row and density.
 __global__ __launch_bounds__(PH_TILE) void k_density_rebuild(const TileDev *__restrict__ tiles,
                                                              const SceneDev *__restrict__ scenes,
                                                              const uint32_t *__restrict__ start,
@@ -456,8 +454,19 @@ __global__ __launch_bounds__(PH_TILE) void k_density_rebuild(const TileDev *__re
             density_rebuild(s, scenes, t.scene, start, posm, tag, rq, rec_rho, rec_p, rec_count, fcnt, va, i, pi);
         }
     }
-    __syncthreads();  // the tile's rows and counts are written
-    halo_build_tile(tile, t, s, va.vcnt, va.vidx, va.vstride, VL_HALO_CAP, va.halo, va.halo_n, va.rows16);
+}
+
+// After a rebuild (returns at once otherwise): each tile's halo from its
+// rows, which the walk substeps until the next rebuild read.  A kernel of
+// its own: the rebuild's cell sweep keeps its occupancy (this one takes
+// 48 KB of LDS per block).
+__global__ __launch_bounds__(PH_TILE) void k_halo_build(const TileDev *__restrict__ tiles,
+                                                        const SceneDev *__restrict__ scenes, ph::VerletArgs va) {
+    if (va.vs->rebuild == 0u) return;
+    const uint32_t tile = blockIdx.x;
+    const TileDev t = tiles[tile];
+    halo_build_tile(tile, t, scenes[t.scene], va.vcnt, va.vidx, va.vstride, VL_HALO_CAP, va.halo, va.halo_n,
+                    va.rows16);
 }
 
 // The walk substep's density pass (returns at once on a rebuild substep).
@@ -775,6 +784,9 @@ hipError_t launch_density_verlet(hipStream_t st, uint32_t ntiles, const TileDev 
     hipLaunchKernelGGL(k_density_rebuild, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm_sorted, tag,
                        rq, rec_rho, rec_p, rec_count, fcnt, va);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_halo_build, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, va);
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_density_walk, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, posm, tag, rq, rec_rho,
                        rec_p, rec_count, fcnt, va);
