@@ -575,7 +575,18 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         if (H != VL_HALO_NONE) {  // the same for the whole block
             l16 = true;
             const uint32_t *hl = vhalo + (size_t)tile * VL_HALO_CAP;
-            for (uint32_t k = threadIdx.x; k < H; k += PH_TILE) f_s[k] = hl[k];
+            constexpr uint32_t kR = (VL_HALO_CAP + PH_TILE - 1) / PH_TILE;
+            uint32_t js[kR];  // every load issued before the stores wait
+#pragma unroll
+            for (uint32_t q = 0; q < kR; ++q) {
+                const uint32_t k = threadIdx.x + q * PH_TILE;
+                js[q] = k < H ? hl[k] : 0u;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < kR; ++q) {
+                const uint32_t k = threadIdx.x + q * PH_TILE;
+                if (k < H) f_s[k] = js[q];
+            }
             __syncthreads();
         }
     }

@@ -497,11 +497,30 @@ __global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restr
     const uint32_t H = va.halo_n[tile];
     const bool halo = H != VL_HALO_NONE;  // the same for the whole block
     if (halo) {
+        // every load of a stage issued before the next stage waits: the
+        // slots, then the positions and words, then the LDS stores
         const uint32_t *hl = va.halo + (size_t)tile * VL_HALO_CAP;
-        for (uint32_t k = threadIdx.x; k < H; k += PH_TILE) {
-            const uint32_t j = hl[k];
-            h_p[k] = posm[j];
-            h_w[k] = va.vword[j];
+        constexpr uint32_t kR = (VL_HALO_CAP + PH_TILE - 1) / PH_TILE;
+        uint32_t js[kR];
+#pragma unroll
+        for (uint32_t q = 0; q < kR; ++q) {
+            const uint32_t k = threadIdx.x + q * PH_TILE;
+            js[q] = k < H ? hl[k] : 0u;
+        }
+        float4 ps[kR];
+        uint32_t ws[kR];
+#pragma unroll
+        for (uint32_t q = 0; q < kR; ++q) {  // past H: slot 0, read and dropped
+            ps[q] = posm[js[q]];
+            ws[q] = va.vword[js[q]];
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kR; ++q) {
+            const uint32_t k = threadIdx.x + q * PH_TILE;
+            if (k < H) {
+                h_p[k] = ps[q];
+                h_w[k] = ws[q];
+            }
         }
         __syncthreads();
     }
