@@ -607,6 +607,10 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             auto entry = [&](uint32_t j, const float4 &pj, const float4 &vj, const float2 &qj) {
                 const float ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
                 const float r2 = dot3(ex, ey, ez);
+                // the neighbour's velocity and density in registers before the
+                // branch (no instruction): the compiler would otherwise issue
+                // their loads inside it, a second memory wait per group
+                asm volatile("" ::"v"(vj.x), "v"(vj.y), "v"(vj.z), "v"(qj.x), "v"(qj.y));
                 if (j != i && r2 < hg2) pair_terms(s, pi, vi, rho_i, q_i, pj, vj, qj, ex, ey, ez, r2, acc);
             };
             if (cw & SLF_NONE) {
