@@ -324,14 +324,21 @@ __device__ __forceinline__ void wave_sync() {
 // groups of four slots (group g of slot i at idx[g * stride + i]).  A tile
 // whose span or halo is too large, or with a slot on the per-cell path,
 // keeps the 32-bit lists (halo_n = VL_HALO_NONE).
+constexpr uint32_t kHaloWords = VL_HALO_SPAN / 32u;
+// LDS of the halo build: the span's bitmap, the halo rank of every fourth
+// word's first bit (16-bit: ranks stay below VL_HALO_CAP), a reduction.
+constexpr uint32_t kHaloLds = 4u * kHaloWords + 2u * (kHaloWords / 4u) + 4u * 3u * (PH_TILE / 64u);
+
 __device__ __forceinline__ void halo_build_tile(uint32_t tile, const TileDev &t, const SceneDev &s,
                                                 const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ idx,
                                                 uint32_t stride, uint32_t cap, uint32_t *__restrict__ halo,
-                                                uint32_t *__restrict__ halo_n, uint4 *__restrict__ out16) {
-    constexpr uint32_t kWords = VL_HALO_SPAN / 32u, kPer = kWords / PH_TILE;
-    __shared__ uint32_t bm[kWords];  // bit (slot - lo)
-    __shared__ uint16_t pw[kWords];  // halo rank of the word's first set bit (< VL_HALO_CAP)
-    __shared__ uint32_t red[3][PH_TILE / 64];
+                                                uint32_t *__restrict__ halo_n, uint4 *__restrict__ out16,
+                                                uint8_t *lds) {
+    constexpr uint32_t kWords = kHaloWords, kPer = kWords / PH_TILE;
+    static_assert(kPer % 4u == 0u, "a thread's words start a group of four");
+    uint32_t *bm = reinterpret_cast<uint32_t *>(lds);                    // bit (slot - lo)
+    uint16_t *pw = reinterpret_cast<uint16_t *>(lds + 4u * kWords);     // rank at word 4k
+    uint32_t(*red)[PH_TILE / 64] = reinterpret_cast<uint32_t(*)[PH_TILE / 64]>(lds + 4u * kWords + 2u * (kWords / 4u));
     const uint32_t i = t.first + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const uint4 *grp = reinterpret_cast<const uint4 *>(idx) + i;
@@ -400,7 +407,7 @@ __device__ __forceinline__ void halo_build_tile(uint32_t tile, const TileDev &t,
     for (uint32_t q = 0; q < kPer; ++q) {
         const uint32_t k = threadIdx.x * kPer + q;
         if (k >= nw) break;
-        pw[k] = (uint16_t)r;
+        if ((k & 3u) == 0u) pw[k >> 2] = (uint16_t)r;
         for (uint32_t bits = bm[k]; bits; bits &= bits - 1u) hl[r++] = lo + k * 32u + (uint32_t)(__ffs(bits) - 1);
     }
     __syncthreads();
@@ -412,8 +419,13 @@ __device__ __forceinline__ void halo_build_tile(uint32_t tile, const TileDev &t,
 #pragma unroll
         for (uint32_t q = 0; q < 8; ++q) {
             if (m + q < n) {
-                const uint32_t d = es[q] - lo;
-                const uint32_t rank = pw[d >> 5] + __popc(bm[d >> 5] & ((1u << (d & 31u)) - 1u));
+                // the words before d's in its group of four (never past nw
+                // when used), then d's own bits below it
+                const uint32_t d = es[q] - lo, wd = d >> 5, sub = wd & 3u;
+                const uint4 b4 = *reinterpret_cast<const uint4 *>(bm + (wd & ~3u));
+                const uint32_t own = sub == 0u ? b4.x : sub == 1u ? b4.y : sub == 2u ? b4.z : b4.w;
+                const uint32_t rank = pw[wd >> 2] + (sub > 0u ? __popc(b4.x) : 0u) + (sub > 1u ? __popc(b4.y) : 0u) +
+                                      (sub > 2u ? __popc(b4.z) : 0u) + __popc(own & ((1u << (d & 31u)) - 1u));
                 v[q >> 1] |= rank << (16u * (q & 1u));
             }
         }
@@ -422,78 +434,75 @@ __device__ __forceinline__ void halo_build_tile(uint32_t tile, const TileDev &t,
     if (threadIdx.x == 0) halo_n[tile] = total;
 }
 
-// The rebuild substep's density pass (returns at once on a walk substep):
-// each particle sweeps the cells for its row and density.
-__global__ __launch_bounds__(PH_TILE) void k_density_rebuild(const TileDev *__restrict__ tiles,
-                                                             const SceneDev *__restrict__ scenes,
-                                                             const uint32_t *__restrict__ start,
-                                                             const float4 *__restrict__ posm,
-                                                             const uint32_t *__restrict__ tag,
-                                                             float2 *__restrict__ rq, float *__restrict__ rec_rho,
-                                                             float *__restrict__ rec_p,
-                                                             uint32_t *__restrict__ rec_count,
-                                                             uint32_t *__restrict__ fcnt, ph::VerletArgs va) {
-    if (va.vs->rebuild == 0u) return;
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const TileDev t = tiles[tile];
-    const SceneDev s = scenes[t.scene];
-    const uint32_t i = t.first + threadIdx.x;
-    if (i < s.part_end) {
-        // both tag generations hold the sorted tags until the next rebuild,
-        // and the walk's generation the sorted state
-        va.tag_mirror[i] = tag[i];
-        const float4 pi = posm[i];
-        va.keep_posm[i] = pi;
-        va.keep_vel[i] = va.sorted_vel[i];
-        if (!finite3(pi.x, pi.y, pi.z)) {
-            // fluid3.rs:302-304: no neighbours, density 0; an empty row
-            va.vcnt[i] = 0u;
-            fcnt[i] = SLF_NONE;
-            density_out(s, scenes, t.scene, i, 0.0f, 0u, tag, rq, rec_rho, rec_p, rec_count);
-        } else {
-            density_rebuild(s, scenes, t.scene, start, posm, tag, rq, rec_rho, rec_p, rec_count, fcnt, va, i, pi);
-        }
+// The rebuild substep's density pass for one particle of a tile: the
+// particle sweeps the cells for its row and density.
+__device__ __forceinline__ void rebuild_particle(const SceneDev &s, const SceneDev *__restrict__ scenes,
+                                                 uint32_t scene, const uint32_t *__restrict__ start,
+                                                 const float4 *__restrict__ posm, const uint32_t *__restrict__ tag,
+                                                 float2 *__restrict__ rq, float *__restrict__ rec_rho,
+                                                 float *__restrict__ rec_p, uint32_t *__restrict__ rec_count,
+                                                 uint32_t *__restrict__ fcnt, const ph::VerletArgs &va, uint32_t i) {
+    // both tag generations hold the sorted tags until the next rebuild,
+    // and the walk's generation the sorted state
+    va.tag_mirror[i] = tag[i];
+    const float4 pi = posm[i];
+    va.keep_posm[i] = pi;
+    va.keep_vel[i] = va.sorted_vel[i];
+    if (!finite3(pi.x, pi.y, pi.z)) {
+        // fluid3.rs:302-304: no neighbours, density 0; an empty row
+        va.vcnt[i] = 0u;
+        fcnt[i] = SLF_NONE;
+        density_out(s, scenes, scene, i, 0.0f, 0u, tag, rq, rec_rho, rec_p, rec_count);
+    } else {
+        density_rebuild(s, scenes, scene, start, posm, tag, rq, rec_rho, rec_p, rec_count, fcnt, va, i, pi);
     }
 }
 
-// After a rebuild (returns at once otherwise): each tile's halo from its
-// rows, which the walk substeps until the next rebuild read.  A kernel of
-// its own: the rebuild's cell sweep keeps its occupancy (this one takes
-// 48 KB of LDS per block).
-__global__ __launch_bounds__(PH_TILE) void k_halo_build(const TileDev *__restrict__ tiles,
-                                                        const SceneDev *__restrict__ scenes, ph::VerletArgs va) {
-    if (va.vs->rebuild == 0u) return;
-    const uint32_t tile = blockIdx.x;
-    const TileDev t = tiles[tile];
-    halo_build_tile(tile, t, scenes[t.scene], va.vcnt, va.vidx, va.vstride, VL_HALO_CAP, va.halo, va.halo_n,
-                    va.rows16);
-}
+// LDS of the walk: the halo's positions and words, and per wave one row
+// being put in order (keys; entries with the within-h bit and
+// contributions, overwritten in rank order).
+constexpr uint32_t kWalkLds = (16u + 4u) * VL_HALO_CAP + 3u * 4u * (PH_TILE / 64u) * VL_CAP;
+constexpr uint32_t kVerletLds = kWalkLds > kHaloLds ? kWalkLds : kHaloLds;
 
-// The walk substep's density pass (returns at once on a rebuild substep).
-// A tile with a halo copies its halo's positions and words to LDS, walks
-// 16-bit rows and hands the force pass a 16-bit list of halo indices;
-// other tiles walk 32-bit rows of slots and hand over slots.  A row group
-// (eight entries) is loaded while the one before is summed, and each
-// group's LDS reads are issued together.
-__global__ __launch_bounds__(PH_TILE) void k_density_walk(const TileDev *__restrict__ tiles,
-                                                          const SceneDev *__restrict__ scenes,
-                                                          const float4 *__restrict__ posm,
-                                                          const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
-                                                          float *__restrict__ rec_rho, float *__restrict__ rec_p,
-                                                          uint32_t *__restrict__ rec_count,
-                                                          uint32_t *__restrict__ fcnt, ph::VerletArgs va) {
-    __shared__ float4 h_p[VL_HALO_CAP];  // position and mass
-    __shared__ uint32_t h_w[VL_HALO_CAP];  // neighbour word
-    // per wave: one row being put in order (keys; entries with the
-    // within-h bit and contributions, overwritten in rank order)
-    __shared__ uint32_t s_key[PH_TILE / 64][VL_CAP], s_uj[PH_TILE / 64][VL_CAP];
-    __shared__ float s_uc[PH_TILE / 64][VL_CAP];
-    if (va.vs->rebuild != 0u) return;
+// The Verlet substep's density pass, one launch whichever the device
+// decided:
+// - a rebuild substep sweeps the cells for every particle of the tile
+//   (rows, density, the force pass's list), then builds the tile's halo
+//   from those rows, in the same block (a tile's halo reads only its own
+//   rows);
+// - a walk substep: a tile with a halo copies its halo's positions and
+//   words to LDS, walks 16-bit rows and hands the force pass a 16-bit list
+//   of halo indices; other tiles walk 32-bit rows of slots and hand over
+//   slots.  A row group (eight entries) is loaded while the one before is
+//   summed, and each group's LDS reads are issued together.
+// The two forms share one LDS block.
+__global__ __launch_bounds__(PH_TILE) void k_density_verlet(const TileDev *__restrict__ tiles,
+                                                            const SceneDev *__restrict__ scenes,
+                                                            const uint32_t *__restrict__ start,
+                                                            const float4 *__restrict__ posm_sorted,
+                                                            const float4 *__restrict__ posm,
+                                                            const uint32_t *__restrict__ tag, float2 *__restrict__ rq,
+                                                            float *__restrict__ rec_rho, float *__restrict__ rec_p,
+                                                            uint32_t *__restrict__ rec_count,
+                                                            uint32_t *__restrict__ fcnt, ph::VerletArgs va) {
+    __shared__ __align__(16) uint8_t lds[kVerletLds];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
     const bool act = i < s.part_end;
+    if (va.vs->rebuild != 0u) {
+        if (act) rebuild_particle(s, scenes, t.scene, start, posm_sorted, tag, rq, rec_rho, rec_p, rec_count, fcnt,
+                                  va, i);
+        __syncthreads();  // the tile's rows, written above, are read below
+        halo_build_tile(tile, t, s, va.vcnt, va.vidx, va.vstride, VL_HALO_CAP, va.halo, va.halo_n, va.rows16, lds);
+        return;
+    }
+    float4 *h_p = reinterpret_cast<float4 *>(lds);                           // position and mass
+    uint32_t *h_w = reinterpret_cast<uint32_t *>(lds + 16u * VL_HALO_CAP);  // neighbour word
+    uint32_t(*s_key)[VL_CAP] = reinterpret_cast<uint32_t(*)[VL_CAP]>(lds + 20u * VL_HALO_CAP);
+    uint32_t(*s_uj)[VL_CAP] = s_key + PH_TILE / 64u;
+    float(*s_uc)[VL_CAP] = reinterpret_cast<float(*)[VL_CAP]>(s_uj + PH_TILE / 64u);
     const uint32_t H = va.halo_n[tile];
     const bool halo = H != VL_HALO_NONE;  // the same for the whole block
     if (halo) {
@@ -800,15 +809,8 @@ hipError_t launch_density_verlet(hipStream_t st, uint32_t ntiles, const TileDev 
                                  const uint32_t *start, const float4 *posm_sorted, const float4 *posm,
                                  const uint32_t *tag, float2 *rq, float *rec_rho, float *rec_p,
                                  uint32_t *rec_count, uint32_t *fcnt, const VerletArgs &va) {
-    hipLaunchKernelGGL(k_density_rebuild, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm_sorted, tag,
-                       rq, rec_rho, rec_p, rec_count, fcnt, va);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_halo_build, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, va);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_density_walk, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, posm, tag, rq, rec_rho,
-                       rec_p, rec_count, fcnt, va);
+    hipLaunchKernelGGL(k_density_verlet, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm_sorted,
+                       posm, tag, rq, rec_rho, rec_p, rec_count, fcnt, va);
     return hipGetLastError();
 }
 
