@@ -29,7 +29,7 @@ KINDS = {
     "k_cell_key": "cell_key", "k_scan_reduce": "scan_reduce", "k_scan_partials": "scan_partials",
     "k_scan_apply": "scan_apply", "k_scan_apply_sum": "scan_apply", "k_scatter": "scatter", "k_rank_gather": "rank_gather",
     "k_density_list<false>": "density", "k_density_verlet": "density",
-    "k_force_list": "force_integrate", "k_force_verlet": "force_integrate", "k_vl_decide": "verlet_decide",
+    "k_force_list": "force_integrate", "k_force_verlet": "force_integrate",
     "k_density_list<true>": "density_list_skin", "k_sur_step": "surrogate_step", "k_tile_vmax": "tile_vmax",
     "k_list_radius": "list_radius", "k_err_reduce": "err_reduce", "k_monitor": "monitor",
 }

@@ -37,7 +37,6 @@ enum Kind : int {
     KIND_SUR_COLLECT,
     KIND_GHOST,
     KIND_RESAMPLE,
-    KIND_VDECIDE,
     KIND_COUNT
 };
 const char *kind_name(int k);
@@ -220,9 +219,8 @@ struct Engine {
     bool vl_valid = false;
     uint64_t vl_epoch = 0;
     uint32_t vl_n = 0;
-    DBuf<uint32_t> vl_cnt, vl_idx, vl_word[2], vl_tile_q, vl_halo, vl_halo_n;
+    DBuf<uint32_t> vl_cnt, vl_idx, vl_word[2], vl_halo, vl_halo_n;
     DBuf<uint4> vl_row16;
-    DBuf<float> vl_tile_dmax;
     DBuf<VerletDev> vl_state;
     bool vl_state_ready = false;
     // Rows whose neighbours changed cell cost their wave a re-ordering each

@@ -40,7 +40,7 @@ const char *kind_name(int k) {
         "cell_key", "scan_reduce", "scan_partials", "scan_apply", "scatter", "rank_gather",
         "density",  "force_integrate", "vmax", "export", "monitor", "append", "cell_bbox",
         "fill", "set_velocity", "batch_persistent", "surrogate_step", "surrogate_history",
-        "surrogate_collect", "boundary_ghosts", "resample", "verlet_decide"};
+        "surrogate_collect", "boundary_ghosts", "resample"};
     return (k >= 0 && k < KIND_COUNT) ? names[k] : "";
 }
 
@@ -496,13 +496,8 @@ void Engine::substep_verlet() {
     vl_row16.ensure((size_t)(VL_CAP / 8) * m);
     vl_state.ensure(1);
     const size_t ntl = std::max<size_t>(nt, 1);
-    if (vl_tile_q.n < ntl || !vl_state_ready) {
-        vl_tile_q.ensure(ntl);
-        vl_tile_dmax.ensure(ntl);
-        vl_halo.ensure(ntl * VL_HALO_CAP);
-        vl_halo_n.ensure(ntl);
-        run(KIND_FILL, [&] { return launch_fill_u32(stream, vl_tile_q.p, (uint32_t)vl_tile_q.n, 0u); });
-    }
+    vl_halo.ensure(ntl * VL_HALO_CAP);
+    vl_halo_n.ensure(ntl);
     if (!vl_state_ready) {
         run(KIND_FILL, [&] {
             return launch_fill_u32(stream, reinterpret_cast<uint32_t *>(vl_state.p),
@@ -515,27 +510,25 @@ void Engine::substep_verlet() {
         for (auto &ev : vl_ev) PH_CK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     }
     const uint32_t slot = (uint32_t)(vl_k % 32);
-    // Rows built for another particle set or layout cannot be walked; the
-    // cell record is the sorted state's, so it sorts every substep.
-    const bool force = !vl_valid || vl_epoch != layout_epoch || vl_n != n || rc;
     float hmin = INFINITY;
     for (const SceneDev &s : scenes) hmin = std::min(hmin, s.h_grid);
     const float skin = std::min(std::max(vl_skin, 0.0f), 0.5f);
     const float thr = 0.45f * skin * hmin;
+    // Rows built for another particle set or layout cannot be walked; the
+    // cell record is the sorted state's, so it sorts every substep.
+    const bool force = !vl_valid || vl_epoch != layout_epoch || vl_n != n || rc;
+    const uint32_t par = (uint32_t)(vl_k & 1u);
     const int ns = 1 - cs, nt_ = 1 - ct;
     const uint32_t *gate = reinterpret_cast<const uint32_t *>(vl_state.p);  // VerletDev::rebuild
     vl_valid = false;  // until this substep is queued whole
-    run(KIND_VDECIDE, [&] {
-        return launch_vl_decide(stream, vl_state.p, vl_tile_dmax.p, vl_tile_q.p, nt, force ? 1u : 0u, thr,
-                                vl_host + 2 * slot);
-    });
     if (counts_dirty)
         run(KIND_FILL, [&] { return launch_fill_u32(stream, counts.p, (uint32_t)alloc_buckets, 0u); });
     counts_dirty = true;
     ++sorts_done;
     run(KIND_CELL_KEY, [&] {
         return launch_cell_key(stream, nt, d_tiles.p, d_scenes.p, posm[cs].p, tag[ct].p, keys.p, slots.p, counts.p,
-                               rc ? rec_cells.p : nullptr, vl_state.p, vl_word[0].p);
+                               rc ? rec_cells.p : nullptr, vl_state.p, VlStep{force ? 1u : 0u, par, thr, vl_host + 2 * slot},
+                               vl_word[0].p);
     });
     run(KIND_SCAN_REDUCE, [&] { return launch_scan_reduce(stream, counts.p, nbuckets, partial.p, gate); });
     run(KIND_SCAN_APPLY, [&] { return launch_scan_apply_sum(stream, counts.p, nbuckets, partial.p, start.p, gate); });
@@ -554,7 +547,7 @@ void Engine::substep_verlet() {
     va.f2 = (1.0f + skin) * (1.0f + skin);
     va.skin = skin;
     va.vs = vl_state.p;
-    va.tile_q = vl_tile_q.p;
+    va.par = par;
     va.tag_mirror = tag[ct].p;
     va.halo = vl_halo.p;
     va.halo_n = vl_halo_n.p;
@@ -577,8 +570,8 @@ void Engine::substep_verlet() {
     run(KIND_FORCE, [&] {
         return launch_force_verlet(stream, nt, d_tiles.p, d_scenes.p, d_bodies.p, start.p, posm[cs].p, vel[cs].p,
                                    rq.p, tag[nt_].p, nl_cnt.p, nl_idx.p, list_stride(), posm[ns].p, vel[ns].p,
-                                   ra ? rec_acc.p : nullptr, d_bbox.p, GhostDev{}, vl_tile_dmax.p, vl_state.p,
-                                   vl_word[1].p, vl_halo.p, vl_halo_n.p);
+                                   ra ? rec_acc.p : nullptr, d_bbox.p, GhostDev{}, vl_state.p, vl_word[1].p,
+                                   vl_halo.p, vl_halo_n.p, par);
     });
     cs = ns;  // the force pass was queued: its output is the state
     finish_substep();

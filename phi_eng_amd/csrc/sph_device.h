@@ -499,13 +499,59 @@ __device__ __forceinline__ void row_range(const SceneDev &s, const uint32_t *__r
 #define VL_HALO_SPAN 262144u  // 2^18 slots: cell layers of up to ~2^17 particles
 #define VL_HALO_NONE 0xFFFFFFFFu
 
+// State of the Verlet rows.  A field with two entries is indexed by the
+// substep's parity p: substep t writes entry t & 1 and substep t + 1 reads
+// it, so each substep's decision reads what no kernel of that substep
+// writes.
 struct VerletDev {
     uint32_t rebuild;         // this substep sorts and rebuilds every row
-    float dacc;               // bound on any particle's displacement since the last rebuild
-    uint32_t slow;            // a particle needs the per-cell path: rebuild next substep too
     uint32_t pad;
+    float dacc[2];            // bound on any particle's displacement since the last rebuild, before substep t moved
+    uint32_t slow[2];         // a particle needs the per-cell path: substep t + 1 rebuilds too
+    uint32_t dmax2[2];        // largest squared displacement of substep t (float bits, >= 0)
+    uint32_t qsum[2];         // rows put back in order in substep t
     unsigned long long rebuilds, repaired, substeps;  // statistics
 };
+
+// What the host hands a Verlet substep's first kernel: whether it must
+// rebuild (the first of a run of Verlet substeps, or after the particles
+// changed), its parity, the displacement bound, and where the statistics
+// go (pinned host memory: [0] rows put back in order in the last substep,
+// [1] this substep's rebuild flag).
+struct VlStep {
+    uint32_t forced;
+    uint32_t par;
+    float thr;
+    uint32_t *host_out;
+};
+
+// This substep's decision, the same in every block that takes it: rebuild
+// when forced, when a particle needed the per-cell path, or when the bound
+// on any particle's displacement since the last rebuild (the sum over the
+// substeps of the largest displacement) passes thr (a NaN bound rebuilds).
+__device__ __forceinline__ bool vl_decide(const VerletDev *vs, const VlStep &st, float &bound) {
+    const uint32_t q = st.par ^ 1u;
+    bound = vs->dacc[q] + sqrtf(__uint_as_float(vs->dmax2[q]));
+    return st.forced != 0u || vs->slow[q] != 0u || !(bound <= st.thr);
+}
+
+// One thread of the substep's first kernel: the decision for the later
+// kernels, the entries this substep accumulates into cleared, the
+// statistics.
+__device__ __forceinline__ void vl_commit(VerletDev *vs, const VlStep &st, bool rb, float bound) {
+    const uint32_t p = st.par, q = p ^ 1u;
+    const uint32_t qs = st.forced ? 0u : vs->qsum[q];
+    vs->rebuild = rb ? 1u : 0u;
+    vs->dacc[p] = rb ? 0.0f : bound;
+    vs->slow[p] = 0u;
+    vs->dmax2[p] = 0u;
+    vs->qsum[p] = 0u;
+    vs->rebuilds += rb ? 1ull : 0ull;
+    vs->repaired += qs;
+    vs->substeps += 1ull;
+    st.host_out[0] = qs;
+    st.host_out[1] = rb ? 1u : 0u;
+}
 
 // Neighbour word of a particle: its original index (24 bits) and its cell
 // coordinates mod 4, x at bit 24, y at bit 27, z at bit 30, a zero bit

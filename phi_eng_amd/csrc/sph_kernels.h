@@ -29,12 +29,14 @@ struct PersistArgs {
 
 uint32_t scan_partials_needed(uint32_t nbuckets);
 
-// vs (may be null): a Verlet substep; the neighbour words go to vword and
-// the bucket keys are made only when vs->rebuild.
+// vs (may be null): a Verlet substep, whose rebuild decision the kernel
+// takes (vst) and records in vs->rebuild for the substep's later kernels;
+// the neighbour words go to vword and the bucket keys are made only when
+// it rebuilds.
 hipError_t launch_cell_key(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                            const float4 *posm, const uint32_t *tag, uint32_t *keys, uint32_t *slots,
-                           uint32_t *counts, int32_t *rec_cells, const VerletDev *vs = nullptr,
-                           uint32_t *vword = nullptr);
+                           uint32_t *counts, int32_t *rec_cells, VerletDev *vs = nullptr,
+                           const VlStep &vst = VlStep{}, uint32_t *vword = nullptr);
 // gate (may be null): the kernel runs only when *gate is nonzero.
 hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t nb, uint32_t *partial,
                               const uint32_t *gate = nullptr);
@@ -54,21 +56,21 @@ hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *ti
                               float4 *posm_out, float4 *vel_out, uint32_t *tag_out, uint32_t *rec_sorted,
                               uint32_t *perm_out = nullptr, const VerletDev *vs = nullptr,
                               const uint32_t *vword_in = nullptr, uint32_t *vword_out = nullptr);
-// The force pass of a Verlet substep: the list-based pass over the
-// neighbour list the Verlet density pass wrote, also recording each tile's
-// largest squared displacement (tile_dmax), flagging positions beyond the
-// reach of the mod-4 cell words (vs->slow), and writing each slot's next
-// neighbour word (vword_next).
+// The force pass of a Verlet substep (parity vl_par): the list-based pass
+// over the neighbour list the Verlet density pass wrote, also taking the
+// largest squared displacement (vs->dmax2[vl_par]), flagging positions
+// beyond the reach of the mod-4 cell words (vs->slow[vl_par]), and writing
+// each slot's next neighbour word (vword_next).
 hipError_t launch_force_verlet(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                                const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
                                const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
                                uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
-                               const GhostDev &gh, float *tile_dmax, VerletDev *vs, uint32_t *vword_next,
-                               const uint32_t *vhalo, const uint32_t *vhalo_n);
-// Verlet substep (verlet.hip).  k_vl_decide (one block) decides whether
-// this substep sorts and rebuilds every row: on request (force), after a
-// particle needed the per-cell path, or when the displacement bound (the
-// last substep's largest per-tile displacement added up) reaches thr.  The
+                               const GhostDev &gh, VerletDev *vs, uint32_t *vword_next, const uint32_t *vhalo,
+                               const uint32_t *vhalo_n, uint32_t vl_par);
+// Verlet substep (verlet.hip).  Its first kernel (k_cell_key) decides
+// whether this substep sorts and rebuilds every row: on request (forced),
+// after a particle needed the per-cell path, or when the displacement
+// bound (the substeps' largest displacements added up) passes thr.  The
 // density pass then either sweeps the cells within (1 + skin) h of every
 // particle, writing its row, or walks its row; a wave puts rows whose
 // neighbours within h are out of visiting order back in order itself.
@@ -79,8 +81,8 @@ struct VerletArgs {
     uint32_t vstride;
     float f2;                // (1 + skin)^2
     float skin;
-    VerletDev *vs;
-    uint32_t *tile_q;        // per tile: rows put in order (statistics; zeroed by k_vl_decide)
+    VerletDev *vs;           // the walk adds the rows it put in order to vs->qsum[par]
+    uint32_t par;            // the substep's parity (VerletDev)
     uint32_t *tag_mirror;    // a rebuild copies the sorted tags here (the other tag generation)
     // halo form of the rows: per tile the sorted slots of its rows' union
     // (halo[tile * VL_HALO_CAP + k], count halo_n[tile] or VL_HALO_NONE),
@@ -99,10 +101,6 @@ struct VerletArgs {
     const float4 *sorted_vel;
     float4 *keep_vel;
 };
-// host_out (pinned host memory): [0] rows re-ordered in the last walk,
-// [1] this substep's rebuild flag.
-hipError_t launch_vl_decide(hipStream_t st, VerletDev *vs, const float *tile_dmax, uint32_t *tile_q, uint32_t ntiles,
-                            uint32_t force, float thr, uint32_t *host_out);
 // posm_sorted: the sort's output (the rebuild reads it); posm: the state
 // the walk reads (va.keep_posm, where the rebuild copies the sorted state).
 hipError_t launch_density_verlet(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,

@@ -100,10 +100,12 @@ __device__ __forceinline__ void dev_cell_key(uint32_t bid, uint32_t nbl, const T
     }
 }
 
-// With a Verlet list (vs non-null) the kernel works only on a substep that
-// sorts (vs->rebuild, decided by k_vl_decide): every particle's neighbour
-// word (original index, cell mod 4) into vword, then the bucket keys and
-// counts.  (The cell record forces a sort on every substep.)
+// With a Verlet list (vs non-null) this is the substep's first kernel: every
+// block takes the substep's rebuild decision (vl_decide), one thread
+// records it for the later kernels, and the kernel works only on a substep
+// that sorts: every particle's neighbour word (original index, cell mod 4)
+// into vword, then the bucket keys and counts.  (The cell record forces a
+// sort on every substep.)
 __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict__ tiles,
                                                       const SceneDev *__restrict__ scenes,
                                                       const float4 *__restrict__ posm,
@@ -111,11 +113,14 @@ __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict_
                                                       uint32_t *__restrict__ keys,
                                                       uint32_t *__restrict__ slots,
                                                       uint32_t *__restrict__ counts,
-                                                      int32_t *__restrict__ rec_cells, const VerletDev *vs,
+                                                      int32_t *__restrict__ rec_cells, VerletDev *vs, VlStep vst,
                                                       uint32_t *__restrict__ vword) {
     if (vs) {
+        float bound;
+        const bool rb = vl_decide(vs, vst, bound);
+        if (blockIdx.x == 0 && threadIdx.x == 0) vl_commit(vs, vst, rb, bound);
         // between rebuilds the force pass has written the words already
-        if (!vs->rebuild) return;
+        if (!rb) return;
         const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
         const SceneDev &s = scenes[t.scene];
         const uint32_t i = t.first + threadIdx.x;
@@ -559,8 +564,8 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
     const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox,
     float4 *__restrict__ acc_sorted, const GhostDev &gh = GhostDev{}, const AuxFuse &af = AuxFuse{},
-    float *__restrict__ tile_dmax = nullptr, VerletDev *vs = nullptr, uint32_t *__restrict__ vword_next = nullptr,
-    const uint32_t *__restrict__ vhalo = nullptr, const uint32_t *__restrict__ vhalo_n = nullptr) {
+    VerletDev *vs = nullptr, uint32_t *__restrict__ vword_next = nullptr, const uint32_t *__restrict__ vhalo = nullptr,
+    const uint32_t *__restrict__ vhalo_n = nullptr, uint32_t vl_par = 0u) {
     const uint32_t tile = xcd_tile(bid, nbl);
     const TileDev t = tiles[tile];
     const SceneDev s = scenes[t.scene];
@@ -716,7 +721,7 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
             const float d2 = dot3(x - pi.x, y - pi.y, z - pi.z);
             dmax2 = d2 == d2 ? d2 : 0.0f;  // a non-finite particle is nobody's neighbour
             // cells this far out saturate; the next substep sweeps cells
-            if (finite3(x, y, z) && vl_far(x, y, z, s.h_grid)) atomicOr(&vs->slow, 1u);
+            if (finite3(x, y, z) && vl_far(x, y, z, s.h_grid)) atomicOr(&vs->slow[vl_par], 1u);
         }
         posm_out[i] = make_float4(x, y, z, pi.w);
         vel_out[i] = make_float4(ux, uy, uz, 0.0f);
@@ -798,8 +803,8 @@ __device__ __forceinline__ void dev_force_integrate(uint32_t bid, uint32_t nbl, 
         if (VL) wj[w][0] = dmax2;
     }
     __syncthreads();
-    if (VL && threadIdx.x == 0)
-        tile_dmax[tile] = fmaxf(fmaxf(wj[0][0], wj[1][0]), fmaxf(wj[2][0], wj[3][0]));
+    if (VL && threadIdx.x == 0)  // non-negative: integer order is float order
+        atomicMax(&vs->dmax2[vl_par], __float_as_uint(fmaxf(fmaxf(wj[0][0], wj[1][0]), fmaxf(wj[2][0], wj[3][0]))));
     if (threadIdx.x < 6) {
         const int c = threadIdx.x;
         int32_t v = wb[0][c];
@@ -868,11 +873,11 @@ __global__ __launch_bounds__(PH_TILE) void k_force_verlet(const TileDev *__restr
     const float4 *__restrict__ vel, const float2 *__restrict__ rq, const uint32_t *__restrict__ tag,
     const uint32_t *__restrict__ lcnt, const uint4 *__restrict__ lgrp, uint32_t mstride, float4 *__restrict__ posm_out,
     float4 *__restrict__ vel_out, float *__restrict__ rec_acc, int32_t *__restrict__ bbox, GhostDev gh,
-    float *__restrict__ tile_dmax, VerletDev *vs, uint32_t *__restrict__ vword_next,
-    const uint32_t *__restrict__ vhalo, const uint32_t *__restrict__ vhalo_n) {
+    VerletDev *vs, uint32_t *__restrict__ vword_next, const uint32_t *__restrict__ vhalo,
+    const uint32_t *__restrict__ vhalo_n, uint32_t vl_par) {
     dev_force_integrate<true>(blockIdx.x, gridDim.x, tiles, scenes, bodies, start, posm, vel, rq, tag, lcnt, lgrp,
-                              mstride, posm_out, vel_out, rec_acc, bbox, nullptr, gh, AuxFuse{}, tile_dmax, vs,
-                              vword_next, vhalo, vhalo_n);
+                              mstride, posm_out, vel_out, rec_acc, bbox, nullptr, gh, AuxFuse{}, vs, vword_next,
+                              vhalo, vhalo_n, vl_par);
 }
 
 // Boundary ghost density and pressure from the fluid (fluid3.rs:906-920):
@@ -1136,9 +1141,10 @@ static inline uint32_t blocks_for(uint32_t n) { return (n + PH_TILE - 1) / PH_TI
 
 hipError_t launch_cell_key(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                            const float4 *posm, const uint32_t *tag, uint32_t *keys, uint32_t *slots,
-                           uint32_t *counts, int32_t *rec_cells, const VerletDev *vs, uint32_t *vword) {
+                           uint32_t *counts, int32_t *rec_cells, VerletDev *vs, const VlStep &vst,
+                           uint32_t *vword) {
     hipLaunchKernelGGL(k_cell_key, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, posm, tag, keys, slots,
-                       counts, rec_cells, vs, vword);
+                       counts, rec_cells, vs, vst, vword);
     return hipGetLastError();
 }
 
@@ -1190,11 +1196,11 @@ hipError_t launch_force_verlet(hipStream_t st, uint32_t ntiles, const TileDev *t
                                const BodyDev *bodies, const uint32_t *start, const float4 *posm, const float4 *vel,
                                const float2 *rq, const uint32_t *tag, const uint32_t *lcnt, const uint32_t *lidx,
                                uint32_t mstride, float4 *posm_out, float4 *vel_out, float *rec_acc, int32_t *bbox,
-                               const GhostDev &gh, float *tile_dmax, VerletDev *vs, uint32_t *vword_next,
-                               const uint32_t *vhalo, const uint32_t *vhalo_n) {
+                               const GhostDev &gh, VerletDev *vs, uint32_t *vword_next, const uint32_t *vhalo,
+                               const uint32_t *vhalo_n, uint32_t vl_par) {
     hipLaunchKernelGGL(k_force_verlet, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, bodies, start, posm, vel,
                        rq, tag, lcnt, reinterpret_cast<const uint4 *>(lidx), mstride, posm_out, vel_out, rec_acc,
-                       bbox, gh, tile_dmax, vs, vword_next, vhalo, vhalo_n);
+                       bbox, gh, vs, vword_next, vhalo, vhalo_n, vl_par);
     return hipGetLastError();
 }
 hipError_t launch_density_list(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,

@@ -4,7 +4,7 @@
 // every substep (fluid3.rs:319-340): ~216 candidates for ~33 neighbours.
 // Here each particle keeps a row of every particle that was within
 // (1 + skin) h of it at the last rebuild.  While no particle has moved more
-// than 0.45 skin h since then (the bound k_vl_decide keeps), every particle
+// than 0.45 skin h since then (the bound vl_decide keeps), every particle
 // within h of it is in its row, and the row is walked instead of the cells.
 //
 // Between rebuilds the particles keep their slots: the sort's kernels are
@@ -34,8 +34,6 @@
 #include "sph_kernels.h"
 
 namespace {
-
-constexpr uint32_t kDecideThreads = 1024, kDecideUnroll = 8;  // 8,192 tiles in one round
 
 // fluid3.rs:646-655 after the sum: clamp, equation of state, the force
 // pass's pressure quotient, and the parity records (same operations as the
@@ -170,59 +168,6 @@ __device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t &total) 
 
 }  // namespace
 
-// One block: this substep's decision, from the last substep's per-tile
-// largest squared displacements (per-tile stores, so no same-address
-// atomics contend across the XCDs); the per-tile counts of rows put in
-// order go to the statistics and restart at zero.
-__global__ __launch_bounds__(kDecideThreads) void k_vl_decide(VerletDev *vs, const float *__restrict__ tile_dmax,
-                                                              uint32_t *__restrict__ tile_q, uint32_t nt,
-                                                              uint32_t force, float thr,
-                                                              uint32_t *__restrict__ host_out) {
-    // every load of the block issued before any reduction: one round trip
-    float dm = 0.0f;
-    uint32_t qs = 0;
-#pragma unroll
-    for (uint32_t r = 0; r < kDecideUnroll; ++r) {
-        const uint32_t k = r * kDecideThreads + threadIdx.x;
-        if (k < nt) {
-            if (!force) dm = fmaxf(dm, tile_dmax[k]);
-            qs += tile_q[k];
-        }
-    }
-    for (uint32_t k = kDecideUnroll * kDecideThreads + threadIdx.x; k < nt; k += kDecideThreads) {
-        if (!force) dm = fmaxf(dm, tile_dmax[k]);
-        qs += tile_q[k];
-    }
-    for (uint32_t k = threadIdx.x; k < nt; k += kDecideThreads) tile_q[k] = 0u;
-    for (int d = 32; d >= 1; d >>= 1) {
-        dm = fmaxf(dm, __shfl_xor(dm, d, 64));
-        qs += __shfl_xor(qs, d, 64);
-    }
-    __shared__ float wdm[kDecideThreads / 64];
-    __shared__ uint32_t wqs[kDecideThreads / 64];
-    if ((threadIdx.x & 63) == 0) {
-        wdm[threadIdx.x >> 6] = dm;
-        wqs[threadIdx.x >> 6] = qs;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (uint32_t w = 1; w < kDecideThreads / 64; ++w) {
-            dm = fmaxf(dm, wdm[w]);
-            qs += wqs[w];
-        }
-        const float d = force ? 0.0f : vs->dacc + sqrtf(dm);
-        const bool rb = force || vs->slow || !(d <= thr);  // a NaN bound rebuilds
-        vs->rebuild = rb ? 1u : 0u;
-        vs->dacc = rb ? 0.0f : d;
-        vs->slow = 0u;
-        vs->rebuilds += rb ? 1ull : 0ull;
-        vs->repaired += qs;
-        vs->substeps += 1ull;
-        host_out[0] = qs;
-        host_out[1] = rb ? 1u : 0u;
-    }
-}
-
 // The rebuild form of a particle: sweep the cells within (1 + skin) h,
 // writing the row (every particle within (1 + skin) h, ascending slot) and
 // summing the density over the 27 cells in the same sweep.
@@ -250,7 +195,7 @@ __device__ __forceinline__ void density_rebuild(const SceneDev &s, const SceneDe
         // saturate: the per-cell path, and the next substep sorts again
         va.vcnt[i] = VLF_SLOW;
         fcnt[i] = SLF_SLOW;
-        atomicOr(&va.vs->slow, 1u);
+        atomicOr(&va.vs->slow[va.par], 1u);
         rho = density_cells(s, start, posm, pi, cx, cy, cz, cnt);
         density_out(s, scenes, scene, i, rho, cnt, tag, rq, rec_rho, rec_p, rec_count);
         return;
@@ -300,7 +245,7 @@ __device__ __forceinline__ void density_rebuild(const SceneDev &s, const SceneDe
     const uint32_t fc = lp.finish();
     if (lk > VL_CAP || fc == SLF_SLOW) {
         va.vcnt[i] = VLF_SLOW;
-        atomicOr(&va.vs->slow, 1u);
+        atomicOr(&va.vs->slow[va.par], 1u);
     } else {
         va.vcnt[i] = lk;
     }
@@ -615,7 +560,7 @@ __global__ __launch_bounds__(PH_TILE) void k_density_verlet(const TileDev *__res
     // in that order; then the owner sums in that order.
     const unsigned long long badm = __ballot(mine && bad);
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    if (badm != 0ull && lane == 0) atomicAdd(&va.tile_q[tile], (uint32_t)__popcll(badm));  // statistics
+    if (badm != 0ull && lane == 0) atomicAdd(&va.vs->qsum[va.par], (uint32_t)__popcll(badm));  // statistics
     for (unsigned long long mm = badm; mm; mm &= mm - 1ull) {
         const int b = __ffsll((long long)mm) - 1;
         const uint32_t ri = __shfl(i, b, 64), rn = __shfl(n, b, 64), rw = __shfl(wi, b, 64);
@@ -795,13 +740,6 @@ hipError_t launch_density_skin(hipStream_t st, uint32_t ntiles, const TileDev *t
                                uint32_t *rec_count) {
     hipLaunchKernelGGL(k_density_skin, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, start, posm, tag, lcnt,
                        reinterpret_cast<const uint4 *>(lidx), mstride, rq, rec_rho, rec_p, rec_count);
-    return hipGetLastError();
-}
-
-hipError_t launch_vl_decide(hipStream_t st, VerletDev *vs, const float *tile_dmax, uint32_t *tile_q, uint32_t ntiles,
-                            uint32_t force, float thr, uint32_t *host_out) {
-    hipLaunchKernelGGL(k_vl_decide, dim3(1), dim3(kDecideThreads), 0, st, vs, tile_dmax, tile_q, ntiles, force, thr,
-                       host_out);
     return hipGetLastError();
 }
 
