@@ -220,6 +220,8 @@ struct Engine {
     uint64_t vl_epoch = 0;
     uint32_t vl_n = 0;
     DBuf<uint32_t> vl_cnt, vl_idx, vl_word[2], vl_halo, vl_halo_n;
+    DBuf<uint32_t> vl_partial;  // the sort's block sums, added up by the cell-key pass; zero between sorts
+    bool vl_partial_dirty = true;  // not known to be zero (new, or a sort cut short)
     DBuf<uint4> vl_row16;
     DBuf<VerletDev> vl_state;
     bool vl_state_ready = false;

@@ -498,6 +498,11 @@ void Engine::substep_verlet() {
     const size_t ntl = std::max<size_t>(nt, 1);
     vl_halo.ensure(ntl * VL_HALO_CAP);
     vl_halo_n.ensure(ntl);
+    const uint32_t np = scan_partials_needed(nbuckets);
+    if (vl_partial.n < np) {
+        vl_partial.ensure(np);
+        vl_partial_dirty = true;
+    }
     if (!vl_state_ready) {
         run(KIND_FILL, [&] {
             return launch_fill_u32(stream, reinterpret_cast<uint32_t *>(vl_state.p),
@@ -523,17 +528,24 @@ void Engine::substep_verlet() {
     vl_valid = false;  // until this substep is queued whole
     if (counts_dirty)
         run(KIND_FILL, [&] { return launch_fill_u32(stream, counts.p, (uint32_t)alloc_buckets, 0u); });
+    if (vl_partial_dirty)
+        run(KIND_FILL, [&] { return launch_fill_u32(stream, vl_partial.p, (uint32_t)vl_partial.n, 0u); });
     counts_dirty = true;
+    vl_partial_dirty = true;
     ++sorts_done;
+    // the cell-key pass adds up the scan's block sums (no reduce launch);
+    // the scatter zeroes them again
     run(KIND_CELL_KEY, [&] {
         return launch_cell_key(stream, nt, d_tiles.p, d_scenes.p, posm[cs].p, tag[ct].p, keys.p, slots.p, counts.p,
-                               rc ? rec_cells.p : nullptr, vl_state.p, VlStep{force ? 1u : 0u, par, thr, vl_host + 2 * slot},
-                               vl_word[0].p);
+                               rc ? rec_cells.p : nullptr, vl_state.p,
+                               VlStep{force ? 1u : 0u, par, thr, vl_host + 2 * slot}, vl_word[0].p, vl_partial.p);
     });
-    run(KIND_SCAN_REDUCE, [&] { return launch_scan_reduce(stream, counts.p, nbuckets, partial.p, gate); });
-    run(KIND_SCAN_APPLY, [&] { return launch_scan_apply_sum(stream, counts.p, nbuckets, partial.p, start.p, gate); });
+    run(KIND_SCAN_APPLY, [&] { return launch_scan_apply_sum(stream, counts.p, nbuckets, vl_partial.p, start.p, gate); });
     counts_dirty = false;
-    run(KIND_SCATTER, [&] { return launch_scatter(stream, n, keys.p, slots.p, start.p, tmp.p, gate); });
+    run(KIND_SCATTER, [&] {
+        return launch_scatter(stream, n, keys.p, slots.p, start.p, tmp.p, gate, vl_partial.p, np);
+    });
+    vl_partial_dirty = false;
     run(KIND_RANK_GATHER, [&] {
         return launch_rank_gather(stream, nt, d_tiles.p, d_scenes.p, tmp.p, keys.p, start.p, posm[cs].p, vel[cs].p,
                                   tag[ct].p, posm[ns].p, vel[ns].p, tag[nt_].p, rc ? rec_sorted.p : nullptr, nullptr,

@@ -32,11 +32,13 @@ uint32_t scan_partials_needed(uint32_t nbuckets);
 // vs (may be null): a Verlet substep, whose rebuild decision the kernel
 // takes (vst) and records in vs->rebuild for the substep's later kernels;
 // the neighbour words go to vword and the bucket keys are made only when
-// it rebuilds.
+// it rebuilds.  pacc (may be null; zero on entry): the scan's block sums
+// (scan_partials_needed entries) are added up there, in place of
+// launch_scan_reduce.
 hipError_t launch_cell_key(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                            const float4 *posm, const uint32_t *tag, uint32_t *keys, uint32_t *slots,
                            uint32_t *counts, int32_t *rec_cells, VerletDev *vs = nullptr,
-                           const VlStep &vst = VlStep{}, uint32_t *vword = nullptr);
+                           const VlStep &vst = VlStep{}, uint32_t *vword = nullptr, uint32_t *pacc = nullptr);
 // gate (may be null): the kernel runs only when *gate is nonzero.
 hipError_t launch_scan_reduce(hipStream_t st, const uint32_t *counts, uint32_t nb, uint32_t *partial,
                               const uint32_t *gate = nullptr);
@@ -48,8 +50,11 @@ hipError_t launch_scan_apply(hipStream_t st, uint32_t *counts, uint32_t nb, cons
 // scan_partials launch), start[nb] = total.
 hipError_t launch_scan_apply_sum(hipStream_t st, uint32_t *counts, uint32_t nb, const uint32_t *partial,
                                  uint32_t *start, const uint32_t *gate = nullptr);
+// zero (may be null): nzero words set to zero when the kernel runs (the
+// block sums the cell-key pass accumulated, once the scan has read them).
 hipError_t launch_scatter(hipStream_t st, uint32_t n, const uint32_t *keys, const uint32_t *slots,
-                          const uint32_t *start, uint32_t *tmp, const uint32_t *gate = nullptr);
+                          const uint32_t *start, uint32_t *tmp, const uint32_t *gate = nullptr,
+                          uint32_t *zero = nullptr, uint32_t nzero = 0);
 hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                               const uint32_t *tmp, const uint32_t *keys, const uint32_t *start,
                               const float4 *posm_in, const float4 *vel_in, const uint32_t *tag_in,

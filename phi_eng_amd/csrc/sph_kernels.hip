@@ -58,7 +58,8 @@ __device__ __forceinline__ void dev_cell_key(uint32_t bid, uint32_t nbl, const T
                                                       uint32_t *__restrict__ keys,
                                                       uint32_t *__restrict__ slots,
                                                       uint32_t *__restrict__ counts,
-                                                      int32_t *__restrict__ rec_cells) {
+                                                      int32_t *__restrict__ rec_cells,
+                                                      uint32_t *__restrict__ pacc = nullptr) {
     const TileDev t = tiles[xcd_tile(bid, nbl)];
     const SceneDev s = scenes[t.scene];
     const uint32_t i = t.first + threadIdx.x;
@@ -89,6 +90,18 @@ __device__ __forceinline__ void dev_cell_key(uint32_t bid, uint32_t nbl, const T
     uint32_t base = 0;
     if (active && head) base = atomicAdd(&counts[key], (uint32_t)(run1 - run0));
     base = __shfl(base, run0, 64);
+    if (pacc) {
+        // the scan's block sums (kScanBlock * kScanItems buckets each),
+        // one atomic per run of lanes in one scan block (the wave's keys
+        // are nearly sorted: one or two runs)
+        const uint32_t pb = key / (uint32_t)(kScanBlock * kScanItems);
+        const bool bhead = lane == 0 || pb != __shfl_up(pb, 1, 64);
+        const unsigned long long bheads = __ballot(bhead);
+        const int b0 = 63 - __clzll(bheads & upto);
+        const unsigned long long bafter = bheads & ~upto;
+        const int b1 = bafter ? __ffsll((long long)bafter) - 1 : 64;
+        if (active && bhead) atomicAdd(&pacc[pb], (uint32_t)(b1 - b0));
+    }
     if (!active) return;
     keys[i] = key;
     slots[i] = base + (uint32_t)(lane - run0);
@@ -114,7 +127,7 @@ __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict_
                                                       uint32_t *__restrict__ slots,
                                                       uint32_t *__restrict__ counts,
                                                       int32_t *__restrict__ rec_cells, VerletDev *vs, VlStep vst,
-                                                      uint32_t *__restrict__ vword) {
+                                                      uint32_t *__restrict__ vword, uint32_t *__restrict__ pacc) {
     if (vs) {
         float bound;
         const bool rb = vl_decide(vs, vst, bound);
@@ -131,7 +144,7 @@ __global__ __launch_bounds__(PH_TILE) void k_cell_key(const TileDev *__restrict_
             vword[i] = vl_word(tag[i], cx, cy, cz);
         }
     }
-    dev_cell_key(blockIdx.x, gridDim.x, tiles, scenes, posm, tag, keys, slots, counts, rec_cells);
+    dev_cell_key(blockIdx.x, gridDim.x, tiles, scenes, posm, tag, keys, slots, counts, rec_cells, pacc);
 }
 
 // ---------------------------------------------------------------- K2
@@ -259,8 +272,13 @@ __device__ __forceinline__ void dev_scatter(uint32_t bid, uint32_t nbl, uint32_t
 __global__ __launch_bounds__(PH_TILE) void k_scatter(uint32_t n, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ slots,
                                                      const uint32_t *__restrict__ start,
-                                                     uint32_t *__restrict__ tmp, const uint32_t *gate) {
+                                                     uint32_t *__restrict__ tmp, const uint32_t *gate,
+                                                     uint32_t *__restrict__ zero, uint32_t nzero) {
     if (gate && !*gate) return;
+    // block sums the cell-key pass accumulated, read by the scan before
+    // this kernel: zero again for the next sort
+    if (zero && blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < nzero; k += PH_TILE) zero[k] = 0u;
     dev_scatter(blockIdx.x, gridDim.x, n, keys, slots, start, tmp);
 }
 
@@ -1142,9 +1160,9 @@ static inline uint32_t blocks_for(uint32_t n) { return (n + PH_TILE - 1) / PH_TI
 hipError_t launch_cell_key(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
                            const float4 *posm, const uint32_t *tag, uint32_t *keys, uint32_t *slots,
                            uint32_t *counts, int32_t *rec_cells, VerletDev *vs, const VlStep &vst,
-                           uint32_t *vword) {
+                           uint32_t *vword, uint32_t *pacc) {
     hipLaunchKernelGGL(k_cell_key, dim3(ntiles), dim3(PH_TILE), 0, st, tiles, scenes, posm, tag, keys, slots,
-                       counts, rec_cells, vs, vst, vword);
+                       counts, rec_cells, vs, vst, vword, pacc);
     return hipGetLastError();
 }
 
@@ -1177,8 +1195,10 @@ hipError_t launch_scan_apply_sum(hipStream_t st, uint32_t *counts, uint32_t nb, 
     return hipGetLastError();
 }
 hipError_t launch_scatter(hipStream_t st, uint32_t n, const uint32_t *keys, const uint32_t *slots,
-                          const uint32_t *start, uint32_t *tmp, const uint32_t *gate) {
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, n, keys, slots, start, tmp, gate);
+                          const uint32_t *start, uint32_t *tmp, const uint32_t *gate, uint32_t *zero,
+                          uint32_t nzero) {
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(PH_TILE), 0, st, n, keys, slots, start, tmp, gate, zero,
+                       nzero);
     return hipGetLastError();
 }
 hipError_t launch_rank_gather(hipStream_t st, uint32_t ntiles, const TileDev *tiles, const SceneDev *scenes,
