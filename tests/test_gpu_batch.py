@@ -134,3 +134,31 @@ def test_barrier_timeout_is_reported_and_recovered(gpu):
     b.close()
     for w in ws:
         w.close()
+
+
+def test_mode_switches_after_verlet_substeps(gpu):
+    """A batch above the automatic Verlet threshold (131,072 particles), so
+    its eager substeps walk Verlet rows and alternate the state generation:
+    eager for an odd number of steps, then hipGraph replays (captured on the
+    generation the eager steps left), then eager again.  After every phase
+    each scene equals the world stepped alone, bit for bit."""
+    scs = scenes.batch_scenes(count=6, seed=12, nmin=24000, nmax=32000)
+    assert sum(sc.n for sc in scs) >= 131072
+    alone = [scenes.make_world(sc) for sc in scs]
+    ws = [scenes.make_world(sc) for sc in scs]
+    b = Batch3(ws)
+    for phase, (mode, steps) in enumerate([(BATCH_KERNELS, 7), (BATCH_KERNELS_GRAPH, 6), (BATCH_KERNELS, 5)]):
+        b.set_mode(mode, 0)
+        for _ in range(steps):
+            b.step(DT)
+            for w in alone:
+                w.step(DT)
+        b.sync()
+        b.sync_worlds()
+        got = [w.get_state() for w in ws]
+        ref = [w.get_state() for w in alone]
+        print(f"[batch switch] phase {phase} mode {mode}: {sum(len(g[0]) for g in got)} particles", flush=True)
+        assert _same(got, ref), f"phase {phase} (mode {mode}) differs"
+    b.close()
+    for w in alone:
+        w.close()
