@@ -211,8 +211,9 @@ struct Engine {
     // current state; every other way of moving or reordering particles
     // clears it, and the next Verlet substep then rebuilds.
     bool vl_enabled = true;
-    // below this many particles the Verlet form's extra launches (decision,
-    // halo build) cost more than its walk saves: the list form runs
+    // below this many particles the Verlet form's extra work (the rows'
+    // rebuild sweeps and halo builds) costs more than its walk saves: the
+    // list form runs
     uint32_t vl_min_particles = kVlAutoMin;
     static constexpr uint32_t kVlAutoMin = 131072;
     float vl_skin = 0.1f;
